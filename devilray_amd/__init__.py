@@ -1,0 +1,190 @@
+"""DevilRay render core for MI355X (gfx950).
+
+Python-side host helpers over the C ABI (include/dr_api.h): a device context,
+scene upload + on-device BVH build, and batched closest/any-hit traversal.
+Every call goes to libdevilray.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (  # noqa: F401
+    DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrMaterial,
+    DrSceneDesc, DrSceneInfo, check, lib, ptr,
+)
+
+__all__ = [
+    "Context", "Scene", "DeviceArray", "gen_icosphere", "gen_rays_surface", "gen_rays_camera",
+    "RAY_DTYPE", "HIT_DTYPE", "DevilRayError",
+]
+
+
+# ---------------------------------------------------------------- generators
+def gen_icosphere(subdiv: int, sigma: float = 0.0, seed: int = 0):
+    """Displaced icosphere: (vertices float32[V,3], indices uint32[T,3])."""
+    nv, nt = C.c_uint32(), C.c_uint32()
+    check(lib().dr_gen_icosphere(subdiv, sigma, seed, None, C.byref(nv), None, C.byref(nt)))
+    v = np.empty((nv.value, 3), np.float32)
+    f = np.empty((nt.value, 3), np.uint32)
+    check(lib().dr_gen_icosphere(subdiv, sigma, seed, ptr(v), C.byref(nv), ptr(f), C.byref(nt)))
+    return v, f
+
+
+def gen_rays_surface(vertices, indices, n_rays: int, seed: int = 0, inward: bool = True,
+                     offset: float = 1e-5) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(indices, np.uint32)
+    rays = np.empty(n_rays, RAY_DTYPE)
+    check(lib().dr_gen_rays_surface(ptr(v), ptr(f), f.shape[0], n_rays, seed, int(inward), offset, ptr(rays)))
+    return rays
+
+
+def gen_rays_camera(eye, target, up, fov_y_deg: float, width: int, height: int) -> np.ndarray:
+    e = np.asarray(eye, np.float32)
+    t = np.asarray(target, np.float32)
+    u = np.asarray(up, np.float32)
+    rays = np.empty(width * height, RAY_DTYPE)
+    check(lib().dr_gen_rays_camera(ptr(e), ptr(t), ptr(u), fov_y_deg, width, height, ptr(rays)))
+    return rays
+
+
+# ---------------------------------------------------------------- device side
+class Context:
+    """One HIP device (dr_context)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().dr_context_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().dr_context_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(lib().dr_synchronize(self.handle))
+
+    def alloc(self, nbytes: int) -> "DeviceArray":
+        return DeviceArray(self, nbytes)
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.nbytes)
+        d.copy_from(a)
+        return d
+
+
+class DeviceArray:
+    """A raw device allocation owned by a Context."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        check(lib().dr_device_alloc(ctx.handle, self.nbytes, C.byref(h)))
+        self.ptr = h.value
+
+    def free(self):
+        if self.ptr:
+            check(lib().dr_device_free(self.ctx.handle, C.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def copy_from(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().dr_memcpy_htod(self.ctx.handle, C.c_void_p(self.ptr), ptr(a), a.nbytes))
+
+    def to_numpy(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().dr_memcpy_dtoh(self.ctx.handle, ptr(out), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+
+class Scene:
+    """Triangle scene on the device (dr_scene) with its BVH."""
+
+    def __init__(self, ctx: Context, vertices, indices, tri_material=None, materials=None):
+        self.ctx = ctx
+        self._v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self._f = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        d = DrSceneDesc()
+        d.vertices = self._v.ctypes.data_as(C.POINTER(C.c_float))
+        d.n_vertices = self._v.shape[0]
+        d.indices = self._f.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.n_triangles = self._f.shape[0]
+        if tri_material is not None:
+            self._m = np.ascontiguousarray(tri_material, np.uint32)
+            d.tri_material = self._m.ctypes.data_as(C.POINTER(C.c_uint32))
+        if materials:
+            arr = (DrMaterial * len(materials))(*materials)
+            self._mats = arr
+            d.materials = C.cast(arr, C.POINTER(DrMaterial))
+            d.n_materials = len(materials)
+        h = C.c_void_p()
+        check(lib().dr_scene_create(ctx.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().dr_scene_destroy(self.handle))
+            self.handle = None
+
+    def build_bvh(self, leaf_max: int = 4, stream=None):
+        p = DrBvhParams(leaf_max, 0)
+        check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
+
+    def info(self) -> DrSceneInfo:
+        i = DrSceneInfo()
+        check(lib().dr_scene_get_info(self.handle, C.byref(i)))
+        return i
+
+    # device-pointer entry points
+    def trace_closest_dev(self, rays_ptr: int, n: int, hits_ptr: int, stream=None):
+        check(lib().dr_trace_closest(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr), stream))
+
+    def trace_any_dev(self, rays_ptr: int, n: int, occ_ptr: int, stream=None):
+        check(lib().dr_trace_any(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(occ_ptr), stream))
+
+    # host convenience wrappers (copy in, trace, copy out)
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        dr = self.ctx.upload(rays)
+        dh = self.ctx.alloc(max(n, 1) * HIT_DTYPE.itemsize)
+        self.trace_closest_dev(dr.ptr, n, dh.ptr)
+        self.ctx.synchronize()
+        return dh.to_numpy(HIT_DTYPE, n)
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        dr = self.ctx.upload(rays)
+        do = self.ctx.alloc(max(n, 1) * 4)
+        self.trace_any_dev(dr.ptr, n, do.ptr)
+        self.ctx.synchronize()
+        return do.to_numpy(np.uint32, n)
+
+    def trace_closest_counted(self, rays: np.ndarray):
+        n = rays.shape[0]
+        dr = self.ctx.upload(rays)
+        dh = self.ctx.alloc(max(n, 1) * HIT_DTYPE.itemsize)
+        db = self.ctx.alloc(max(n, 1) * 4)
+        dt = self.ctx.alloc(max(n, 1) * 4)
+        check(lib().dr_trace_closest_counted(self.handle, C.c_void_p(dr.ptr), n, C.c_void_p(dh.ptr),
+                                             C.c_void_p(db.ptr), C.c_void_p(dt.ptr), None))
+        self.ctx.synchronize()
+        return dh.to_numpy(HIT_DTYPE, n), db.to_numpy(np.uint32, n), dt.to_numpy(np.uint32, n)

@@ -1,0 +1,116 @@
+"""ctypes binding of libdevilray.so (the C ABI declared in include/dr_api.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdevilray.so")
+
+DR_OK, DR_E_INVALID, DR_E_OOM, DR_E_DEVICE, DR_E_UNSUPPORTED, DR_E_OVERFLOW = range(6)
+DR_MAT_LAMBERT, DR_MAT_DIELECTRIC = 0, 1
+
+
+class DevilRayError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"devilray status {status}: {msg}")
+        self.status = status
+
+
+class DrRay(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class DrHit(C.Structure):
+    _fields_ = [("id", C.c_int32), ("t", C.c_float), ("u", C.c_float), ("v", C.c_float)]
+
+
+class DrMaterial(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("albedo", C.c_float * 3), ("emission", C.c_float * 3), ("ior", C.c_float)]
+
+
+class DrSceneDesc(C.Structure):
+    _fields_ = [
+        ("vertices", C.POINTER(C.c_float)), ("n_vertices", C.c_uint32),
+        ("indices", C.POINTER(C.c_uint32)), ("n_triangles", C.c_uint32),
+        ("tri_material", C.POINTER(C.c_uint32)),
+        ("materials", C.POINTER(DrMaterial)), ("n_materials", C.c_uint32),
+    ]
+
+
+class DrBvhParams(C.Structure):
+    _fields_ = [("leaf_max", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class DrSceneInfo(C.Structure):
+    _fields_ = [
+        ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
+        ("depth", C.c_uint32), ("overflow_flag", C.c_uint32),
+        ("build_ms", C.c_double), ("sah_cost", C.c_double),
+    ]
+
+
+RAY_DTYPE = np.dtype([(n, np.float32) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")])
+HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("v", np.float32)])
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_i32 = C.c_int32
+_st = C.c_int
+
+# name -> (restype, argtypes); every symbol include/dr_api.h declares
+PROTOTYPES = {
+    "dr_status_string": (C.c_char_p, [_st]),
+    "dr_last_error": (C.c_char_p, []),
+    "dr_device_count": (_st, [C.POINTER(_i32)]),
+    "dr_context_create": (_st, [_i32, C.POINTER(_vp)]),
+    "dr_context_destroy": (_st, [_vp]),
+    "dr_device_alloc": (_st, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "dr_device_free": (_st, [_vp, _vp]),
+    "dr_memcpy_htod": (_st, [_vp, _vp, _vp, C.c_size_t]),
+    "dr_memcpy_dtoh": (_st, [_vp, _vp, _vp, C.c_size_t]),
+    "dr_synchronize": (_st, [_vp]),
+    "dr_scene_create": (_st, [_vp, C.POINTER(DrSceneDesc), C.POINTER(_vp)]),
+    "dr_scene_destroy": (_st, [_vp]),
+    "dr_scene_get_info": (_st, [_vp, C.POINTER(DrSceneInfo)]),
+    "dr_scene_build_bvh": (_st, [_vp, C.POINTER(DrBvhParams), _vp]),
+    "dr_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
+    "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
+    "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libdevilray.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `python -m devilray_amd.build` (no CPU fallback exists)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != DR_OK:
+        msg = lib().dr_last_error()
+        raise DevilRayError(status, msg.decode() if msg else "")
+
+
+def ptr(a: np.ndarray) -> int:
+    return a.ctypes.data

@@ -1,0 +1,479 @@
+// On-device BVH build: Morton-code LBVH, SAH leaf collapse, 4-wide nodes.
+//
+//  1. per-triangle boxes and the centroid bounds (one atomic per block)
+//  2. 30-bit Morton codes of the centroids, radix-sorted with their ids
+//  3. binary radix tree over the sorted codes (Karras, "Maximizing Parallelism
+//     in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012); equal
+//     codes are ordered by their position
+//  4. bottom-up pass (atomic arrival flags): boxes, triangle counts and the
+//     SAH cost C(n) = min(Ci*A(n)*N(n), Ct*A(n) + C(l) + C(r)); a subtree is
+//     collapsed into one leaf when the leaf term wins and N(n) <= leaf_max
+//     (the collapse rule of Karras & Aila, "Fast Parallel Construction of
+//     High-Quality BVHs", HPG 2013)
+//  5. top-down level-synchronous collapse into 4-wide nodes: each wide node
+//     opens its largest-area interior child until it has four children
+//  6. triangles are written leaf-ordered (sorted order) as 3 float4 with the
+//     caller's triangle id in the first w.
+// Child boxes are rounded outward by one ulp so every vertex lies strictly
+// inside its leaf box.
+#include "dr_internal.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace dr {
+
+namespace {
+
+constexpr int kT = 256;
+
+struct BuildTmp {
+    DevBuf tri_lo, tri_hi;          // float4 per triangle (input order)
+    DevBuf keys, vals;              // sorted codes / ids
+    DevBuf leaf_lo, leaf_hi;        // float4 per leaf (sorted order)
+    DevBuf child_l, child_r;        // int per internal node (<0: ~leaf)
+    DevBuf parent_i, parent_l;      // parent of internal nodes / leaves
+    DevBuf range_first, count;      // per internal node
+    DevBuf node_lo, node_hi;        // float4 per internal node
+    DevBuf cost;                    // float per internal node
+    DevBuf visit;                   // u32 arrival flags
+    DevBuf collapse;                // u8-in-u32 per internal node
+    DevBuf queue_a, queue_b;        // int2 (binary node, wide node)
+    DevBuf counters;                // u32[8] + ordered-int bounds[6]
+};
+
+__device__ __forceinline__ int32_t f2ord(float f)
+{
+    const int32_t i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord2f(int32_t i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__global__ __launch_bounds__(kT) void k_tri_bounds(const float *__restrict__ verts,
+                                                   const uint32_t *__restrict__ idx, uint32_t n,
+                                                   float4 *__restrict__ lo, float4 *__restrict__ hi,
+                                                   int32_t *__restrict__ cbounds)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    float c[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, C[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (i < n) {
+        float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = idx[3 * (size_t)i + k];
+            for (int a = 0; a < 3; ++a) {
+                const float x = verts[3 * (size_t)vi + a];
+                l[a] = fminf(l[a], x);
+                h[a] = fmaxf(h[a], x);
+            }
+        }
+        lo[i] = make_float4(l[0], l[1], l[2], 0.0f);
+        hi[i] = make_float4(h[0], h[1], h[2], 0.0f);
+        for (int a = 0; a < 3; ++a) {
+            const float m = 0.5f * (l[a] + h[a]);
+            c[a] = m; C[a] = m;
+        }
+    }
+    // wave reduction then one atomic per wave
+    for (int a = 0; a < 3; ++a) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            c[a] = fminf(c[a], __shfl_xor(c[a], off, 64));
+            C[a] = fmaxf(C[a], __shfl_xor(C[a], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&cbounds[a], f2ord(c[a]));
+            atomicMax(&cbounds[3 + a], f2ord(C[a]));
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(kT) void k_morton(const float4 *__restrict__ lo, const float4 *__restrict__ hi,
+                                               uint32_t n, const int32_t *__restrict__ cbounds,
+                                               uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const float4 l = lo[i], h = hi[i];
+    const float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+        const float bl = ord2f(cbounds[a]), bh = ord2f(cbounds[3 + a]);
+        const float ext = bh - bl;
+        float x = ext > 0.0f ? (c[a] - bl) / ext : 0.0f;
+        x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+        q[a] = (uint32_t)x;
+    }
+    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[i] = i;
+}
+
+// leaf boxes and leaf-ordered triangle vertices
+__global__ __launch_bounds__(kT) void k_gather(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                                               const uint32_t *__restrict__ vals, uint32_t n,
+                                               const float4 *__restrict__ tlo, const float4 *__restrict__ thi,
+                                               float4 *__restrict__ llo, float4 *__restrict__ lhi,
+                                               float4 *__restrict__ tri_v)
+{
+    const uint32_t k = blockIdx.x * kT + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t id = vals[k];
+    llo[k] = tlo[id];
+    lhi[k] = thi[id];
+    for (int j = 0; j < 3; ++j) {
+        const uint32_t vi = idx[3 * (size_t)id + j];
+        const float w = j == 0 ? __int_as_float((int32_t)id) : 0.0f;
+        tri_v[3 * (size_t)k + j] = make_float4(verts[3 * (size_t)vi], verts[3 * (size_t)vi + 1],
+                                               verts[3 * (size_t)vi + 2], w);
+    }
+}
+
+__device__ __forceinline__ int delta(const uint32_t *__restrict__ codes, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = codes[i], b = codes[j];
+    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(kT) void k_karras(const uint32_t *__restrict__ codes, int n,
+                                               int32_t *__restrict__ cl, int32_t *__restrict__ cr,
+                                               int32_t *__restrict__ pi, int32_t *__restrict__ pl,
+                                               uint32_t *__restrict__ rfirst, uint32_t *__restrict__ cnt)
+{
+    const int i = (int)(blockIdx.x * kT + threadIdx.x);
+    if (i >= n - 1) return;
+    const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(codes, n, i, i - d);
+    int lmax = 2;
+    while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(codes, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int first = min(i, j), last = max(i, j);
+    const int32_t left = (first == gamma) ? ~gamma : gamma;
+    const int32_t right = (last == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    cl[i] = left;
+    cr[i] = right;
+    if (left < 0) pl[~left] = i; else pi[left] = i;
+    if (right < 0) pl[~right] = i; else pi[right] = i;
+    rfirst[i] = (uint32_t)first;
+    cnt[i] = (uint32_t)(last - first + 1);
+    if (i == 0) pi[0] = -1;
+}
+
+__device__ __forceinline__ float half_area(float4 l, float4 h)
+{
+    const float dx = h.x - l.x, dy = h.y - l.y, dz = h.z - l.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ __launch_bounds__(kT) void k_fit(int n, const int32_t *__restrict__ cl, const int32_t *__restrict__ cr,
+                                            const int32_t *__restrict__ pi, const int32_t *__restrict__ pl,
+                                            const uint32_t *__restrict__ cnt, const float4 *__restrict__ llo,
+                                            const float4 *__restrict__ lhi, float4 *node_lo, float4 *node_hi,
+                                            float *cost, uint32_t *visit, uint32_t *collapse,
+                                            uint32_t leaf_max, float ct, float ci)
+{
+    const int k = (int)(blockIdx.x * kT + threadIdx.x);
+    if (k >= n) return;
+    int node = pl[k];
+    while (node >= 0) {
+        __threadfence();
+        if (atomicAdd(&visit[node], 1u) == 0u) return;  // first arrival: sibling not ready
+        __threadfence();
+        const int32_t a = cl[node], b = cr[node];
+        float4 alo, ahi, blo, bhi;
+        float ac, bc;
+        if (a < 0) { alo = llo[~a]; ahi = lhi[~a]; ac = ci * half_area(alo, ahi); }
+        else { alo = node_lo[a]; ahi = node_hi[a]; ac = cost[a]; }
+        if (b < 0) { blo = llo[~b]; bhi = lhi[~b]; bc = ci * half_area(blo, bhi); }
+        else { blo = node_lo[b]; bhi = node_hi[b]; bc = cost[b]; }
+        const float4 lo = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), 0.0f);
+        const float4 hi = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), 0.0f);
+        const float area = half_area(lo, hi);
+        const uint32_t N = cnt[node];
+        const float c_split = ct * area + ac + bc;
+        const float c_leaf = (N <= leaf_max) ? ci * area * (float)N : INFINITY;
+        const bool col = c_leaf <= c_split;
+        node_lo[node] = lo;
+        node_hi[node] = hi;
+        cost[node] = col ? c_leaf : c_split;
+        collapse[node] = col ? 1u : 0u;
+        node = pi[node];
+    }
+}
+
+__device__ __forceinline__ float down(float x) { return nextafterf(x, -INFINITY); }
+__device__ __forceinline__ float up(float x) { return nextafterf(x, INFINITY); }
+
+__global__ __launch_bounds__(kT) void k_collapse(
+    const int2 *__restrict__ qin, uint32_t nin, int2 *__restrict__ qout, uint32_t *counters,
+    const int32_t *__restrict__ cl, const int32_t *__restrict__ cr, const uint32_t *__restrict__ rfirst,
+    const uint32_t *__restrict__ cnt, const float4 *__restrict__ node_lo, const float4 *__restrict__ node_hi,
+    const float4 *__restrict__ llo, const float4 *__restrict__ lhi, const uint32_t *__restrict__ collapse,
+    WideNode *__restrict__ wide, uint32_t wide_cap)
+{
+    const uint32_t qi = blockIdx.x * kT + threadIdx.x;
+    if (qi >= nin) return;
+    const int2 e = qin[qi];
+    int32_t kids[4];
+    int nk = 2;
+    kids[0] = cl[e.x];
+    kids[1] = cr[e.x];
+    while (nk < 4) {
+        int best = -1;
+        float best_a = -1.0f;
+        for (int c = 0; c < nk; ++c) {
+            const int32_t k = kids[c];
+            if (k < 0 || collapse[k]) continue;
+            const float a = half_area(node_lo[k], node_hi[k]);
+            if (a > best_a) { best_a = a; best = c; }
+        }
+        if (best < 0) break;
+        const int32_t k = kids[best];
+        kids[best] = cl[k];
+        kids[nk++] = cr[k];
+    }
+    WideNode wn;
+    for (int c = 0; c < 4; ++c) {
+        if (c >= nk) {
+            wn.lox[c] = wn.hix[c] = wn.loy[c] = wn.hiy[c] = wn.loz[c] = wn.hiz[c] = INFINITY;
+            wn.child[c] = DR_EMPTY_REF;
+            continue;
+        }
+        const int32_t k = kids[c];
+        float4 lo, hi;
+        int32_t ref;
+        if (k < 0) {
+            lo = llo[~k]; hi = lhi[~k];
+            ref = make_leaf_ref((uint32_t)~k, 1u);
+            atomicAdd(&counters[2], 1u);
+        } else {
+            lo = node_lo[k]; hi = node_hi[k];
+            if (collapse[k]) {
+                ref = make_leaf_ref(rfirst[k], cnt[k]);
+                atomicAdd(&counters[2], 1u);
+            } else {
+                const uint32_t w = atomicAdd(&counters[0], 1u);
+                if (w >= wide_cap) {  // cannot happen by the bound above; flag instead of writing
+                    atomicOr(&counters[3], 1u);
+                    ref = DR_EMPTY_REF;
+                } else {
+                    const uint32_t q = atomicAdd(&counters[1], 1u);
+                    qout[q] = make_int2(k, (int)w);
+                    ref = (int32_t)w;
+                }
+            }
+        }
+        wn.lox[c] = down(lo.x); wn.loy[c] = down(lo.y); wn.loz[c] = down(lo.z);
+        wn.hix[c] = up(hi.x); wn.hiy[c] = up(hi.y); wn.hiz[c] = up(hi.z);
+        wn.child[c] = ref;
+    }
+    wn.pad[0] = nk; wn.pad[1] = wn.pad[2] = wn.pad[3] = 0;
+    wide[e.y] = wn;
+}
+
+// Root wide node with a single leaf child covering all triangles (n <= leaf
+// cap, or a collapsed root).
+__global__ void k_single_leaf_root(const float4 *__restrict__ lo, const float4 *__restrict__ hi, uint32_t n,
+                                   WideNode *__restrict__ wide, const float4 *__restrict__ root_lo,
+                                   const float4 *__restrict__ root_hi)
+{
+    WideNode wn;
+    const float4 l = root_lo ? *root_lo : lo[0];
+    const float4 h = root_hi ? *root_hi : hi[0];
+    for (int c = 0; c < 4; ++c) {
+        wn.lox[c] = wn.hix[c] = wn.loy[c] = wn.hiy[c] = wn.loz[c] = wn.hiz[c] = INFINITY;
+        wn.child[c] = DR_EMPTY_REF;
+        wn.pad[c] = 0;
+    }
+    wn.lox[0] = down(l.x); wn.loy[0] = down(l.y); wn.loz[0] = down(l.z);
+    wn.hix[0] = up(h.x); wn.hiy[0] = up(h.y); wn.hiz[0] = up(h.z);
+    wn.child[0] = make_leaf_ref(0u, n);
+    wn.pad[0] = 1;
+    *wide = wn;
+}
+
+__global__ void k_init_counters(uint32_t *c, int32_t *cb)
+{
+    const int t = threadIdx.x;
+    if (t < 8) c[t] = 0;
+    if (t < 3) { cb[t] = 0x7FFFFFFF; cb[3 + t] = (int32_t)0x80000000; }
+}
+
+// counters: [0] wide nodes allocated (the root), [1] queue out, [2] leaves, [3] error
+__global__ void k_collapse_init(uint32_t *c, int2 *queue)
+{
+    c[0] = 1u; c[1] = 0u; c[2] = 0u; c[3] = 0u;
+    queue[0] = make_int2(0, 0);
+}
+
+__global__ void k_reset_queue_count(uint32_t *c) { c[1] = 0u; }
+
+__global__ __launch_bounds__(kT) void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
+
+}  // namespace
+
+dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
+{
+    const uint32_t n = s->n_tris;
+    uint32_t leaf_max = p ? p->leaf_max : 4u;
+    if (leaf_max < 1 || leaf_max > (uint32_t)kLeafMaxCap) return fail(DR_E_INVALID, "leaf_max must be 1..8");
+    if (n == 0) return fail(DR_E_INVALID, "scene has no triangles");
+    if (n >= (1u << 27)) return fail(DR_E_UNSUPPORTED, "more than 2^27 triangles per scene");
+    Context *ctx = s->ctx;
+
+    hipEvent_t e0, e1;
+    DR_HIP_TRY(hipEventCreate(&e0));
+    DR_HIP_TRY(hipEventCreate(&e1));
+    DR_HIP_TRY(hipEventRecord(e0, stream));
+
+    BuildTmp t;
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    dr_status st = DR_OK;
+    auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
+    need(t.tri_lo, sizeof(float4) * n);
+    need(t.tri_hi, sizeof(float4) * n);
+    need(t.keys, sizeof(uint32_t) * n);
+    need(t.vals, sizeof(uint32_t) * n);
+    need(t.leaf_lo, sizeof(float4) * n);
+    need(t.leaf_hi, sizeof(float4) * n);
+    need(t.child_l, sizeof(int32_t) * ni);
+    need(t.child_r, sizeof(int32_t) * ni);
+    need(t.parent_i, sizeof(int32_t) * ni);
+    need(t.parent_l, sizeof(int32_t) * n);
+    need(t.range_first, sizeof(uint32_t) * ni);
+    need(t.count, sizeof(uint32_t) * ni);
+    need(t.node_lo, sizeof(float4) * ni);
+    need(t.node_hi, sizeof(float4) * ni);
+    need(t.cost, sizeof(float) * ni);
+    need(t.visit, sizeof(uint32_t) * ni);
+    need(t.collapse, sizeof(uint32_t) * ni);
+    need(t.queue_a, sizeof(int2) * ni);
+    need(t.queue_b, sizeof(int2) * ni);
+    need(t.counters, 64);
+    need(s->tri_v, sizeof(float4) * 3 * (size_t)n);
+    // Wide nodes with a wide child have 4 children, the others only leaf
+    // children (>= 2), so W <= 2L/3 + 1 <= 2n/3 + 1.
+    const uint32_t wide_cap = (uint32_t)(2ull * n / 3 + 4);
+    need(s->nodes, sizeof(WideNode) * (size_t)wide_cap);
+    if (st) return st;
+
+    uint32_t *counters = t.counters.as<uint32_t>();
+    int32_t *cbounds = reinterpret_cast<int32_t *>(counters + 8);
+    k_init_counters<<<1, 64, 0, stream>>>(counters, cbounds);
+    k_tri_bounds<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), n,
+                                               t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), cbounds);
+    k_morton<<<blocks(n), kT, 0, stream>>>(t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), n, cbounds,
+                                           t.keys.as<uint32_t>(), t.vals.as<uint32_t>());
+    DR_HIP_TRY(hipGetLastError());
+    if ((st = radix_sort_pairs(ctx, t.keys.as<uint32_t>(), t.vals.as<uint32_t>(), n, 30, stream))) return st;
+    k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), t.vals.as<uint32_t>(), n,
+                                           t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.leaf_lo.as<float4>(),
+                                           t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
+    DR_HIP_TRY(hipGetLastError());
+
+    const float ct = 1.0f, ci = 1.0f;
+    uint32_t n_nodes = 1, n_leaves = 1, depth = 1;
+    if (n == 1) {
+        k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), 1, s->nodes.as<WideNode>(),
+                                                nullptr, nullptr);
+        DR_HIP_TRY(hipGetLastError());
+    } else {
+        k_karras<<<blocks(n - 1), kT, 0, stream>>>(t.keys.as<uint32_t>(), (int)n, t.child_l.as<int32_t>(),
+                                                   t.child_r.as<int32_t>(), t.parent_i.as<int32_t>(),
+                                                   t.parent_l.as<int32_t>(), t.range_first.as<uint32_t>(),
+                                                   t.count.as<uint32_t>());
+        k_fill_u32<<<blocks(n - 1), kT, 0, stream>>>(t.visit.as<uint32_t>(), n - 1, 0u);
+        k_fit<<<blocks(n), kT, 0, stream>>>((int)n, t.child_l.as<int32_t>(), t.child_r.as<int32_t>(),
+                                            t.parent_i.as<int32_t>(), t.parent_l.as<int32_t>(), t.count.as<uint32_t>(),
+                                            t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), t.node_lo.as<float4>(),
+                                            t.node_hi.as<float4>(), t.cost.as<float>(), t.visit.as<uint32_t>(),
+                                            t.collapse.as<uint32_t>(), leaf_max, ct, ci);
+        DR_HIP_TRY(hipGetLastError());
+        uint32_t root_collapse = 0;
+        float root_cost = 0.0f;
+        float4 root_lo, root_hi;
+        DR_HIP_TRY(hipMemcpyAsync(&root_collapse, t.collapse.as<uint32_t>(), 4, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipMemcpyAsync(&root_cost, t.cost.as<float>(), 4, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipMemcpyAsync(&root_lo, t.node_lo.as<float4>(), 16, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipMemcpyAsync(&root_hi, t.node_hi.as<float4>(), 16, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipStreamSynchronize(stream));
+        const float ra = (root_hi.x - root_lo.x) * (root_hi.y - root_lo.y) + (root_hi.y - root_lo.y) * (root_hi.z - root_lo.z) +
+                         (root_hi.z - root_lo.z) * (root_hi.x - root_lo.x);
+        s->sah_cost = ra > 0 ? (double)root_cost / ra : 0.0;
+        if (root_collapse) {
+            k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), n,
+                                                    s->nodes.as<WideNode>(), t.node_lo.as<float4>(), t.node_hi.as<float4>());
+            DR_HIP_TRY(hipGetLastError());
+        } else {
+            // level-synchronous collapse; counters: [0] wide nodes, [1] queue out, [2] leaves
+            k_collapse_init<<<1, 1, 0, stream>>>(counters, t.queue_a.as<int2>());
+            uint32_t nin = 1;
+            int2 *qa = t.queue_a.as<int2>(), *qb = t.queue_b.as<int2>();
+            depth = 0;
+            while (nin > 0) {
+                k_collapse<<<blocks(nin), kT, 0, stream>>>(qa, nin, qb, counters, t.child_l.as<int32_t>(),
+                                                          t.child_r.as<int32_t>(), t.range_first.as<uint32_t>(),
+                                                          t.count.as<uint32_t>(), t.node_lo.as<float4>(),
+                                                          t.node_hi.as<float4>(), t.leaf_lo.as<float4>(),
+                                                          t.leaf_hi.as<float4>(), t.collapse.as<uint32_t>(),
+                                                          s->nodes.as<WideNode>(), wide_cap);
+                DR_HIP_TRY(hipGetLastError());
+                uint32_t c[4];
+                DR_HIP_TRY(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, stream));
+                DR_HIP_TRY(hipStreamSynchronize(stream));
+                k_reset_queue_count<<<1, 1, 0, stream>>>(counters);
+                nin = c[1];
+                n_nodes = c[0];
+                n_leaves = c[2];
+                ++depth;
+                if (c[3]) return fail(DR_E_DEVICE, "BVH collapse exceeded its node bound");
+                int2 *tmp = qa; qa = qb; qb = tmp;
+                if (depth > 128) return fail(DR_E_DEVICE, "BVH collapse did not terminate");
+            }
+        }
+    }
+    DR_HIP_TRY(hipEventRecord(e1, stream));
+    DR_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    DR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->n_nodes = n_nodes;
+    s->n_leaves = n_leaves;
+    s->depth = depth;
+    s->build_ms = ms;
+    s->built = true;
+    for (DevBuf *b : {&t.tri_lo, &t.tri_hi, &t.keys, &t.vals, &t.leaf_lo, &t.leaf_hi, &t.child_l, &t.child_r,
+                      &t.parent_i, &t.parent_l, &t.range_first, &t.count, &t.node_lo, &t.node_hi, &t.cost,
+                      &t.visit, &t.collapse, &t.queue_a, &t.queue_b, &t.counters})
+        b->release();
+    return DR_OK;
+}
+
+}  // namespace dr

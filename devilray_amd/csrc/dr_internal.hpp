@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP translation units of libdevilray.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dr_api.h"
+
+namespace dr {
+
+// ---- error plumbing --------------------------------------------------------
+void set_error(const std::string &msg);
+dr_status fail(dr_status s, const std::string &msg);
+
+#define DR_HIP_TRY(expr)                                                              \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::dr::fail(e_ == hipErrorOutOfMemory ? DR_E_OOM : DR_E_DEVICE,     \
+                              std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// ---- device buffer ---------------------------------------------------------
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    dr_status ensure(size_t n);  // grow-only, contents not preserved
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ---- wide BVH node (128 B): SoA child boxes, child references --------------
+// child ref >= 0: index of an interior wide node
+// child ref <  0: leaf, bit 31 set, bits 27..30 = count-1, bits 0..26 = first
+// empty slot: DR_EMPTY_REF with a box at +inf in all axes (never hit)
+struct alignas(16) WideNode {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(WideNode) == 128, "wide node is 128 B");
+
+constexpr int32_t DR_EMPTY_REF = -1;  // 0xFFFFFFFF
+constexpr int kLeafMaxCap = 8;
+
+__host__ __device__ inline int32_t make_leaf_ref(uint32_t first, uint32_t count)
+{
+    return (int32_t)(0x80000000u | ((count - 1u) << 27) | first);
+}
+__host__ __device__ inline uint32_t leaf_first(int32_t ref) { return (uint32_t)ref & 0x07FFFFFFu; }
+__host__ __device__ inline uint32_t leaf_count(int32_t ref) { return (((uint32_t)ref >> 27) & 0xFu) + 1u; }
+
+struct Context {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;  // owned helper stream (unused by the async API)
+    DevBuf scratch_sort[6];         // radix-sort / scan temporaries
+};
+
+struct Scene {
+    Context *ctx = nullptr;
+    uint32_t n_tris = 0, n_verts = 0, n_mats = 0;
+    // input data on device (input order)
+    DevBuf verts;     // float3 * n_verts
+    DevBuf indices;   // uint3 * n_tris
+    DevBuf tri_mat;   // uint32 * n_tris
+    DevBuf materials; // dr_material * n_mats
+    std::vector<dr_material> host_materials;
+    // acceleration structure
+    DevBuf tri_v;     // 3 float4 per triangle, leaf order; w of the first = id bits
+    DevBuf nodes;     // WideNode
+    uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
+    double build_ms = 0.0, sah_cost = 0.0;
+    bool built = false;
+    // traversal overflow stack + flag
+    DevBuf ovf_stack;
+    DevBuf flags;     // uint32[4]: [0] overflow
+};
+
+// bvh_build.hip
+dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);
+
+// trace.hip
+dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
+                       uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
+
+// sort.hip
+dr_status radix_sort_pairs(Context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,
+                           hipStream_t stream);
+dr_status exclusive_scan_u32(Context *ctx, uint32_t *data, uint32_t n, hipStream_t stream);
+
+}  // namespace dr

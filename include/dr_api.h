@@ -1,0 +1,154 @@
+/*
+ * DevilRay MI355X render core -- C ABI.
+ *
+ * Plain C types and pointers only (no HIP or torch types): a caller binds it
+ * with ctypes, cgo, JNI or a C++ facade. hipStream_t is passed as an opaque
+ * pointer (`dr_stream`, NULL = the default stream).
+ *
+ * Conventions
+ *   - Every entry point returns a dr_status; dr_last_error() gives the message
+ *     of the last failure on the calling thread.
+ *   - Scene inputs are host pointers and are copied. Ray, hit, occlusion and
+ *     image buffers are device pointers (allocate them with dr_device_alloc or
+ *     any HIP allocator on the context's device).
+ *   - All trace/render entry points are asynchronous on the given stream;
+ *     *_create / *_destroy / dr_scene_build_bvh synchronise.
+ *   - A context or scene is externally synchronised (not thread-safe).
+ *   - Triangle id = index into the caller's input triangle array.
+ *
+ * Reference counterparts (Frontier789/DevilRay) are cited per entry point.
+ */
+#ifndef DEVILRAY_DR_API_H
+#define DEVILRAY_DR_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DR_OK = 0,
+    DR_E_INVALID = 1,     /* bad argument */
+    DR_E_OOM = 2,         /* device allocation failed */
+    DR_E_DEVICE = 3,      /* HIP runtime error (no device, launch failure) */
+    DR_E_UNSUPPORTED = 4, /* valid request this build does not implement */
+    DR_E_OVERFLOW = 5     /* a bounded structure (traversal stack) overflowed */
+} dr_status;
+
+typedef struct dr_context dr_context;
+typedef struct dr_scene dr_scene;
+typedef void *dr_stream;
+
+/* 32-byte ray: origin, tmin, direction, tmax. A hit is reported for
+ * tmin <= t <= tmax; tmin must be >= 0. The direction need not be unit. */
+typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } dr_ray;
+
+/* 16-byte hit: triangle id (-1 on miss), distance t, barycentric weights of
+ * the triangle's 2nd (u) and 3rd (v) vertex. Ties on t resolve to the lower id. */
+typedef struct { int32_t id; float t, u, v; } dr_hit;
+
+/* Materials: the reference's Material variant (include/tracing/Material.hpp:9-25):
+ * DiffuseMaterial {emission, diffuse_reflectance} and TransparentMaterial {ior}. */
+enum { DR_MAT_LAMBERT = 0, DR_MAT_DIELECTRIC = 1 };
+typedef struct {
+    int32_t kind;
+    float albedo[3];
+    float emission[3];
+    float ior;
+} dr_material;
+
+/* Scene description (host memory, copied). The reference builds one
+ * TriangleMesh per object (include/tracing/TriangleMesh.hpp:9-27) with one
+ * material each; here geometry is flattened to world space with a material
+ * index per triangle (tri_material == NULL means material 0 everywhere). */
+typedef struct {
+    const float *vertices;        /* 3 floats per vertex */
+    uint32_t n_vertices;
+    const uint32_t *indices;      /* 3 vertex indices per triangle */
+    uint32_t n_triangles;
+    const uint32_t *tri_material; /* n_triangles entries or NULL */
+    const dr_material *materials;
+    uint32_t n_materials;
+} dr_scene_desc;
+
+/* BVH build parameters: max triangles per leaf (1..8). flags reserved (0). */
+typedef struct {
+    uint32_t leaf_max;
+    uint32_t flags;
+} dr_bvh_params;
+
+typedef struct {
+    uint32_t n_triangles;
+    uint32_t n_nodes;        /* wide (4-ary) nodes */
+    uint32_t n_leaves;
+    uint32_t depth;          /* wide levels */
+    uint32_t overflow_flag;  /* non-zero if a traversal overflowed its stack */
+    double build_ms;         /* device time of the last build */
+    double sah_cost;         /* SAH cost of the binary tree before collapse */
+} dr_scene_info;
+
+const char *dr_status_string(dr_status s);
+const char *dr_last_error(void);
+
+/* Device / context. Replaces the reference's device runtime
+ * (src/device/DeviceMemory.cu:7-69, src/device/DevUtils.cu:12-74). */
+dr_status dr_device_count(int32_t *count);
+dr_status dr_context_create(int32_t device, dr_context **out);
+dr_status dr_context_destroy(dr_context *ctx);
+dr_status dr_device_alloc(dr_context *ctx, size_t bytes, void **out);
+dr_status dr_device_free(dr_context *ctx, void *ptr);
+dr_status dr_memcpy_htod(dr_context *ctx, void *dst, const void *src, size_t bytes);
+dr_status dr_memcpy_dtoh(dr_context *ctx, void *dst, const void *src, size_t bytes);
+dr_status dr_synchronize(dr_context *ctx);
+
+/* Scene upload. Replaces convertMeshToTris / viewGpuTris
+ * (src/GpuTris.cpp:5-85) and Scene (include/tracing/Scene.hpp:15-26). */
+dr_status dr_scene_create(dr_context *ctx, const dr_scene_desc *desc, dr_scene **out);
+dr_status dr_scene_destroy(dr_scene *scene);
+dr_status dr_scene_get_info(dr_scene *scene, dr_scene_info *info);
+
+/* On-device BVH build: Morton-code LBVH + SAH leaf collapse + 4-wide nodes.
+ * Replaces generateSimpleBBH (src/models/BBH.cpp:101-115), which is a host
+ * median split with one triangle per leaf. */
+dr_status dr_scene_build_bvh(dr_scene *scene, const dr_bvh_params *params, dr_stream stream);
+
+/* Closest hit over a ray batch. Replaces getIntersection / cast
+ * (include/tracing/IntersectionTestsImpl.hpp:222-251) for n rays at once. */
+dr_status dr_trace_closest(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_hit *hits,
+                           dr_stream stream);
+
+/* Any hit (occlusion): occluded[i] = 1 if some triangle has tmin <= t <= tmax.
+ * Replaces visible() (include/tracing/PathGeneration.hpp:213-224), which the
+ * reference answers with a closest-hit cast. */
+dr_status dr_trace_any(dr_scene *scene, const dr_ray *rays, uint32_t n, uint32_t *occluded,
+                       dr_stream stream);
+
+/* Closest hit with the reference's HitTests counters
+ * (include/tracing/Benchmark.hpp:5-14, IntersectionTestsImpl.hpp:228-231):
+ * per ray, the number of AABB tests and triangle tests. */
+dr_status dr_trace_closest_counted(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_hit *hits,
+                                   uint32_t *box_tests, uint32_t *tri_tests, dr_stream stream);
+
+/* ---- workload generators (host, deterministic; no reference counterpart:
+ * the reference's tests build their meshes by hand) ---- */
+/* Icosphere with 20*4^subdiv triangles, shared vertices, outward winding;
+ * vertex radius 1 + sigma*N(0,1) from Philox(seed, vertex index). Query sizes
+ * with NULL outputs. */
+dr_status dr_gen_icosphere(uint32_t subdiv, float sigma, uint64_t seed, float *vertices,
+                           uint32_t *n_vertices, uint32_t *indices, uint32_t *n_triangles);
+/* Incoherent rays: origins area-uniform on the mesh, directions cosine-
+ * distributed about the inward (inward != 0) or outward face normal, origin
+ * offset by `offset` along the direction; tmin 0, tmax inf. */
+dr_status dr_gen_rays_surface(const float *vertices, const uint32_t *indices, uint32_t n_triangles,
+                              uint32_t n_rays, uint64_t seed, int32_t inward, float offset,
+                              dr_ray *rays);
+/* Pinhole primary rays, width x height, pixel centres, row-major from the top. */
+dr_status dr_gen_rays_camera(const float eye[3], const float target[3], const float up[3],
+                             float fov_y_deg, uint32_t width, uint32_t height, dr_ray *rays);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker or the baseline, never as
+the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+RAY_DTYPE = np.dtype([(n, np.float32) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")])
+HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("v", np.float32)])
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+PROTOTYPES = {
+    "or_philox4x32_10": (None, [_vp, _vp, _vp]),
+    "or_u01": (C.c_float, [C.c_uint32]),
+    "or_tri_watertight": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "or_tri_reference": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "or_box_interval_ref": (None, [C.c_float, C.c_float, C.c_float, C.c_float, _vp, _vp]),
+    "or_box_ref": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "or_bvh_build": (_vp, [_vp, _i32, _vp, _i32, _i32, _i32]),
+    "or_bvh_free": (None, [_vp]),
+    "or_bvh_node_count": (_i32, [_vp]),
+    "or_bvh_validate": (_i32, [_vp, _vp, _vp]),
+    "or_trace_bvh": (None, [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _i32]),
+    "or_trace_brute": (None, [_vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32]),
+    "or_refbbh_build": (_vp, [_vp, _i32, _vp, _i32]),
+    "or_refbbh_free": (None, [_vp]),
+    "or_refbbh_depth": (_i32, [_vp]),
+    "or_refbbh_trace": (None, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def threads() -> int:
+    return max(1, min(os.cpu_count() or 1, 64))
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def tri_watertight(ray, v0, v1, v2):
+    r = np.asarray(ray, np.float32)
+    a, b, c = (np.asarray(x, np.float32) for x in (v0, v1, v2))
+    t, u, v = (np.zeros(1, np.float32) for _ in range(3))
+    hit = lib().or_tri_watertight(_p(r), _p(a), _p(b), _p(c), _p(t), _p(u), _p(v))
+    return (float(t[0]), float(u[0]), float(v[0])) if hit else None
+
+
+def tri_reference(p, d, a, b, c):
+    arrs = [np.asarray(x, np.float32) for x in (p, d, a, b, c)]
+    t = np.zeros(1, np.float32)
+    bari = np.zeros(3, np.float32)
+    hit = lib().or_tri_reference(*[_p(x) for x in arrs], _p(t), _p(bari))
+    return (float(t[0]), bari.copy()) if hit else None
+
+
+def box_interval_ref(lo, hi, p, v):
+    e, x = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib().or_box_interval_ref(lo, hi, p, v, _p(e), _p(x))
+    return float(e[0]), float(x[0])
+
+
+def box_ref(lo, hi, p, d):
+    arrs = [np.asarray(x, np.float32) for x in (lo, hi, p, d)]
+    t = np.zeros(1, np.float32)
+    hit = lib().or_box_ref(*[_p(x) for x in arrs], _p(t))
+    return float(t[0]) if hit else None
+
+
+class BVH:
+    """Canonical binary binned-SAH BVH (32 bins, <= max_leaf tris per leaf)."""
+
+    def __init__(self, vertices, indices, max_leaf: int = 4, bins: int = 32):
+        self.v = np.ascontiguousarray(vertices, np.float32)
+        self.f = np.ascontiguousarray(indices, np.int32)
+        self.h = lib().or_bvh_build(_p(self.v), self.v.shape[0], _p(self.f), self.f.shape[0], max_leaf, bins)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_bvh_free(self.h)
+            self.h = None
+
+    def node_count(self) -> int:
+        return lib().or_bvh_node_count(self.h)
+
+    def validate(self) -> int:
+        return lib().or_bvh_validate(self.h, _p(self.v), _p(self.f))
+
+    def closest(self, rays: np.ndarray, counts: bool = False, nthreads: int | None = None):
+        n = rays.shape[0]
+        hits = np.empty(n, HIT_DTYPE)
+        nt = np.empty(n, np.int32) if counts else None
+        tt = np.empty(n, np.int32) if counts else None
+        lib().or_trace_bvh(self.h, _p(self.v), _p(self.f), _p(rays), n, 0, _p(hits), None, _p(nt), _p(tt),
+                           nthreads or threads())
+        return (hits, nt, tt) if counts else hits
+
+    def any(self, rays: np.ndarray, counts: bool = False, nthreads: int | None = None):
+        n = rays.shape[0]
+        occ = np.empty(n, np.uint32)
+        nt = np.empty(n, np.int32) if counts else None
+        tt = np.empty(n, np.int32) if counts else None
+        lib().or_trace_bvh(self.h, _p(self.v), _p(self.f), _p(rays), n, 1, None, _p(occ), _p(nt), _p(tt),
+                           nthreads or threads())
+        return (occ, nt, tt) if counts else occ
+
+
+def brute_closest(vertices, indices, rays, nthreads=None):
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(indices, np.int32)
+    n = rays.shape[0]
+    hits = np.empty(n, HIT_DTYPE)
+    lib().or_trace_brute(_p(v), _p(f), f.shape[0], _p(rays), n, 0, _p(hits), None, nthreads or threads())
+    return hits
+
+
+def brute_any(vertices, indices, rays, nthreads=None):
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(indices, np.int32)
+    n = rays.shape[0]
+    occ = np.empty(n, np.uint32)
+    lib().or_trace_brute(_p(v), _p(f), f.shape[0], _p(rays), n, 1, None, _p(occ), nthreads or threads())
+    return occ
+
+
+class RefBBH:
+    """Reference-semantics median-split BBH with the stackless bit-trail walk."""
+
+    def __init__(self, vertices, indices):
+        self.v = np.ascontiguousarray(vertices, np.float32)
+        self.f = np.ascontiguousarray(indices, np.int32)
+        self.h = lib().or_refbbh_build(_p(self.v), self.v.shape[0], _p(self.f), self.f.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_refbbh_free(self.h)
+            self.h = None
+
+    def depth(self) -> int:
+        return lib().or_refbbh_depth(self.h)
+
+    def closest(self, rays, counts: bool = False, nthreads: int | None = None):
+        n = rays.shape[0]
+        hits = np.empty(n, HIT_DTYPE)
+        bt = np.empty(n, np.int64) if counts else None
+        tt = np.empty(n, np.int64) if counts else None
+        lib().or_refbbh_trace(self.h, _p(self.v), _p(self.f), _p(rays), n, _p(hits), _p(bt), _p(tt),
+                              nthreads or threads())
+        return (hits, bt, tt) if counts else hits

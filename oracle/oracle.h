@@ -1,0 +1,102 @@
+/*
+ * DevilRay CPU oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C, the reference's render-path algorithms
+ * (Frontier789/DevilRay) and the build's own well-defined variants of them, so
+ * that the HIP product path can be checked against it. Only the test suite,
+ * the smoke check and the CPU-baseline leg of the benchmark may load it; the
+ * product library never links, loads or calls it.
+ *
+ * Two semantics live here:
+ *   - "reference semantics": restatements of the reference's functions
+ *     (plane-based triangle test, slab box test, median-split BBH with the
+ *     stackless bit-trail walk, heap-paired alias table). Cited per function.
+ *   - "build semantics": the exact arithmetic the GPU kernels use (watertight
+ *     triangle test, robust slab test, (t, id) tie-break, Philox RNG), so that
+ *     hit ids are bit-exact and t/u/v identical between CPU and GPU.
+ *
+ * Every function is pinned by a test under tests/ (known-answer vectors taken
+ * from the reference's own GoogleTest cases, published Philox vectors,
+ * closed forms, brute force on small inputs). A function without such a check
+ * would say "parity unpinned" here; none is at present except where noted.
+ */
+#ifndef DEVILRAY_ORACLE_H
+#define DEVILRAY_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 32-byte ray and 16-byte hit, same layout as the product's dr_ray / dr_hit. */
+typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } or_ray;
+typedef struct { int32_t id; float t, u, v; } or_hit;
+
+/* ---- RNG ---------------------------------------------------------------- */
+/* Philox-4x32-10 (Salmon et al., SC'11). */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* u32 -> float in [0,1): top 24 bits * 2^-24. */
+float or_u01(uint32_t x);
+
+/* ---- triangle / box tests ----------------------------------------------- */
+/* Watertight test (Woop, Benthin, Wald, JCGT 2013), two-sided, build semantics.
+ * Returns 1 and t, u (weight of v1), v (weight of v2) when the ray's line
+ * crosses the triangle with det != 0; the caller applies the [tmin, tmax]
+ * window. Arithmetic order is fixed and must match the device kernel. */
+int or_tri_watertight(const or_ray *r, const float *v0, const float *v1, const float *v2,
+                      float *t, float *u, float *v);
+
+/* Reference semantics: testTriangleIntersection
+ * (include/tracing/IntersectionTestsImpl.hpp:253-297). bari = (wA, wB, wC). */
+int or_tri_reference(const float p[3], const float d[3], const float a[3], const float b[3],
+                     const float c[3], float *t, float bari[3]);
+
+/* Reference semantics: findBoxInterval / testBoxIntersection
+ * (include/tracing/IntersectionTestsImpl.hpp:80-105). */
+void or_box_interval_ref(float lo, float hi, float p, float v, float *enter, float *exit_);
+int or_box_ref(const float lo[3], const float hi[3], const float p[3], const float d[3], float *t);
+
+/* ---- canonical binary SAH BVH (build semantics) -------------------------- */
+/* Binned SAH, `bins` bins, C_trav = C_isect = 1, at most `max_leaf` triangles
+ * per leaf. Deterministic. Returns an opaque handle. */
+void *or_bvh_build(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris,
+                   int32_t max_leaf, int32_t bins);
+void or_bvh_free(void *bvh);
+int32_t or_bvh_node_count(const void *bvh);
+/* Checks structural invariants (leaves tile [0,n), child boxes inside parent,
+ * triangles inside leaf boxes). Returns 0 when all hold. */
+int32_t or_bvh_validate(const void *bvh, const float *verts, const int32_t *tris);
+
+/* Closest-hit (any_hit = 0) or any-hit (any_hit = 1) over a batch, threaded.
+ * Closest: hits[i] = min (t, id) over triangles with tmin <= t <= tmax, id -1
+ * on miss. Any-hit: occluded[i] = 1 if such a triangle exists.
+ * node_tests / tri_tests (may be NULL): per-ray counts of AABB tests and
+ * triangle tests of the front-to-back traversal (32 B and 40 B units). */
+void or_trace_bvh(const void *bvh, const float *verts, const int32_t *tris, const or_ray *rays,
+                  int32_t n, int32_t any_hit, or_hit *hits, uint32_t *occluded,
+                  int32_t *node_tests, int32_t *tri_tests, int32_t threads);
+
+/* Brute force over all triangles (small inputs only). */
+void or_trace_brute(const float *verts, const int32_t *tris, int32_t ntris, const or_ray *rays,
+                    int32_t n, int32_t any_hit, or_hit *hits, uint32_t *occluded, int32_t threads);
+
+/* ---- reference-semantics BBH (median split, 1 tri / leaf) ----------------- */
+/* generateSimpleBBH / generateBBHLayer (src/models/BBH.cpp:38-115) and the
+ * stackless bit-trail walk getIntersectionImpl
+ * (include/tracing/IntersectionTestsImpl.hpp:141-220) with the reference
+ * triangle test. The triangle permutation is returned so that ids can be
+ * reported in input order. */
+void *or_refbbh_build(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris);
+void or_refbbh_free(void *bbh);
+int32_t or_refbbh_depth(const void *bbh);
+/* hits: id = input triangle index or -1, t, u = wB, v = wC. box_tests /
+ * tri_tests: the reference HitTests counters (include/tracing/Benchmark.hpp). */
+void or_refbbh_trace(const void *bbh, const float *verts, const int32_t *tris, const or_ray *rays,
+                     int32_t n, or_hit *hits, int64_t *box_tests, int64_t *tri_tests,
+                     int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
