@@ -149,6 +149,9 @@ class Scene:
         p = DrBvhParams(leaf_max, 0)
         check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
 
+    def set_schedule(self, closest: int, any_hit: int | None = None):
+        check(lib().dr_scene_set_trace_schedule(self.handle, closest, closest if any_hit is None else any_hit))
+
     def info(self) -> DrSceneInfo:
         i = DrSceneInfo()
         check(lib().dr_scene_get_info(self.handle, C.byref(i)))

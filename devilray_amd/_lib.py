@@ -83,6 +83,7 @@ PROTOTYPES = {
     "dr_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "dr_scene_set_trace_schedule": (_st, [_vp, _u32, _u32]),
     "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
     "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),

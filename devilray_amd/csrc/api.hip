@@ -170,7 +170,7 @@ dr_status dr_scene_create(dr_context *c, const dr_scene_desc *d, dr_scene **out)
     if (!st) st = s->indices.ensure(ib);
     if (!st) st = s->tri_mat.ensure(mb);
     if (!st) st = s->materials.ensure(sizeof(dr_material) * s->n_mats);
-    if (!st) st = s->flags.ensure(16);
+    if (!st) st = s->flags.ensure(2048);
     if (st) { delete s; return st; }
     hipError_t e = hipSuccess;
     if (vb) e = hipMemcpy(s->verts.ptr, d->vertices, vb, hipMemcpyHostToDevice);
@@ -190,7 +190,7 @@ dr_status dr_scene_create(dr_context *c, const dr_scene_desc *d, dr_scene **out)
     }
     if (e == hipSuccess)
         e = hipMemcpy(s->materials.ptr, s->host_materials.data(), sizeof(dr_material) * s->n_mats, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(s->flags.ptr, 0, 16);
+    if (e == hipSuccess) e = hipMemset(s->flags.ptr, 0, 2048);
     if (e != hipSuccess) {
         delete s;
         return fail(DR_E_DEVICE, std::string("scene upload: ") + hipGetErrorString(e));
@@ -234,6 +234,16 @@ dr_status dr_scene_build_bvh(dr_scene *sc, const dr_bvh_params *p, dr_stream str
     auto *s = reinterpret_cast<Scene *>(sc);
     DR_HIP_TRY(hipSetDevice(s->ctx->device));
     return build_bvh(s, p, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_scene_set_trace_schedule(dr_scene *sc, uint32_t closest_sched, uint32_t any_sched)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    if (closest_sched > 7 || any_sched > 7) return fail(DR_E_INVALID, "schedule must be 0..7");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    s->sched_closest = closest_sched;
+    s->sched_any = any_sched;
+    return DR_OK;
 }
 
 dr_status dr_trace_closest(dr_scene *sc, const dr_ray *rays, uint32_t n, dr_hit *hits, dr_stream stream)

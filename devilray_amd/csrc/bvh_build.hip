@@ -403,6 +403,12 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), 1, s->nodes.as<WideNode>(),
                                                 nullptr, nullptr);
         DR_HIP_TRY(hipGetLastError());
+        float4 lo, hi;
+        DR_HIP_TRY(hipMemcpyAsync(&lo, t.leaf_lo.as<float4>(), 16, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipMemcpyAsync(&hi, t.leaf_hi.as<float4>(), 16, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipStreamSynchronize(stream));
+        const float b[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+        for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
     } else {
         k_karras<<<blocks(n - 1), kT, 0, stream>>>(t.keys.as<uint32_t>(), (int)n, t.child_l.as<int32_t>(),
                                                    t.child_r.as<int32_t>(), t.parent_i.as<int32_t>(),
@@ -426,6 +432,8 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         const float ra = (root_hi.x - root_lo.x) * (root_hi.y - root_lo.y) + (root_hi.y - root_lo.y) * (root_hi.z - root_lo.z) +
                          (root_hi.z - root_lo.z) * (root_hi.x - root_lo.x);
         s->sah_cost = ra > 0 ? (double)root_cost / ra : 0.0;
+        const float b[6] = {root_lo.x, root_lo.y, root_lo.z, root_hi.x, root_hi.y, root_hi.z};
+        for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
         if (root_collapse) {
             k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), n,
                                                     s->nodes.as<WideNode>(), t.node_lo.as<float4>(), t.node_hi.as<float4>());

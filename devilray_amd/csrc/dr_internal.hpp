@@ -81,7 +81,12 @@ struct Scene {
     bool built = false;
     // traversal overflow stack + flag
     DevBuf ovf_stack;
-    DevBuf flags;     // uint32[4]: [0] overflow
+    DevBuf flags;     // uint32[512]: [0] overflow flag, [16..271] per-launch ray counters
+    uint64_t launch_seq = 0;
+    uint32_t sched_closest = DR_SCHED_PER_RAY_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
+    uint32_t sched_any = DR_SCHED_PER_RAY_SPECULATIVE | DR_SCHED_REORDER;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (lo, hi), set by the BVH build
+    DevBuf ray_keys, ray_perm;             // ray reordering scratch
 };
 
 // bvh_build.hip

@@ -1,12 +1,26 @@
 // Closest-hit and any-hit traversal of the 4-wide BVH.
 //
-// One lane per ray. Each lane keeps a short stack of child references in LDS
-// (kLdsStack entries, laid out [entry][lane] so a wave's pushes at equal
-// depth hit distinct banks); deeper entries spill to a per-lane region in
-// global memory, up to kMaxStack in total. At an interior node all four child
-// boxes are tested with the robust slab test, the hit children are sorted by
-// entry distance, the nearest is taken next and the rest are pushed far to
-// near. Leaves hold up to 8 triangles stored contiguously as 3 float4.
+// Ray order. Optionally (DR_SCHED_REORDER) the batch is first sorted by the
+// key (dominant direction axis, 30-bit Morton code of the origin inside the
+// scene box): neighbouring lanes then start in the same part of the tree and
+// a wave's rays share the dominant axis, so the triangle test runs a
+// compile-time specialisation without per-lane component selects. Results are
+// written back in input order.
+//
+// Schedules. PER_RAY: one lane per ray. SPECULATIVE: the "while-while" loop
+// with postponed leaves (Aila & Laine, "Understanding the Efficiency of Ray
+// Traversal on GPUs", HPG 2009): a lane that reaches a leaf parks it and keeps
+// descending interior nodes until every lane of the wave holds a leaf, then
+// all lanes intersect their leaves together. PERSISTENT: resident waves
+// refill finished lanes from a global counter (one atomic per wave).
+//
+// Each lane keeps a short stack of child references in LDS (kLdsStack
+// entries, laid out [entry][lane]); deeper entries spill to a per-lane region
+// in global memory, up to kMaxStack in total. At an interior node all four
+// child boxes are tested with the robust slab test; near and far planes are
+// loaded by the ray's octant so no per-slab min/max is needed. Hit children
+// are sorted by entry distance, the nearest is taken next and the rest are
+// pushed far to near. Leaves hold up to 8 triangles stored as 3 float4.
 #include "dr_device.hpp"
 #include "dr_internal.hpp"
 
@@ -17,11 +31,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kLdsStack = 16;
 constexpr int kMaxStack = 64;
-
-struct Best {
-    float t, u, v;
-    uint32_t id;
-};
+constexpr int32_t kNone = DR_EMPTY_REF;
+constexpr uint32_t kCounterSlots = 256;
 
 __device__ __forceinline__ void cswap(float &ta, int32_t &ra, float &tb, int32_t &rb)
 {
@@ -31,116 +42,247 @@ __device__ __forceinline__ void cswap(float &ta, int32_t &ra, float &tb, int32_t
     }
 }
 
-template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool is_leaf(int32_t r) { return r < 0 && r != kNone; }
+
+struct LeafState {
+    float bt, bu, bv;
+    uint32_t bid;
+    bool occluded;
+    uint32_t ntri;
+};
+
+template <bool ANY, bool COUNT, int KZ>
+__device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v, int32_t leaf, const RayPre &p,
+                                               LeafState &st)
+{
+    const uint32_t first = leaf_first(leaf), cnt = leaf_count(leaf);
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 *tv = tri_v + 3 * (size_t)(first + k);
+        const float4 a = tv[0], b = tv[1], c = tv[2];
+        float t, u, v;
+        if (COUNT) ++st.ntri;
+        if (!tri_watertight<KZ>(p, a, b, c, t, u, v)) continue;
+        const uint32_t id = (uint32_t)__float_as_int(a.w);
+        if (ANY) {
+            if (t >= p.tmin && t <= p.tmax) { st.occluded = true; return; }
+        } else if (accept_hit(t, id, p.tmin, st.bt, st.bid)) {
+            st.bt = t; st.bu = u; st.bv = v; st.bid = id;
+        }
+    }
+}
+
+template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE>
 __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ nodes,
                                                   const float4 *__restrict__ tri_v,
-                                                  const float4 *__restrict__ rays, uint32_t n,
-                                                  dr_hit *__restrict__ hits, uint32_t *__restrict__ occ,
+                                                  const float4 *__restrict__ rays, const uint32_t *__restrict__ perm,
+                                                  uint32_t n, dr_hit *__restrict__ hits, uint32_t *__restrict__ occ,
                                                   uint32_t *__restrict__ box_tests,
                                                   uint32_t *__restrict__ tri_tests,
-                                                  int32_t *__restrict__ ovf, uint32_t *__restrict__ flags)
+                                                  int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
+                                                  uint32_t *__restrict__ ray_counter)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
     const uint32_t tid = threadIdx.x;
-    const uint32_t i = blockIdx.x * kBlock + tid;
-    if (i >= n) return;
-    const float4 r0 = rays[2 * (size_t)i];
-    const float4 r1 = rays[2 * (size_t)i + 1];
-    RayPre p;
-    ray_prepare(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, p);
+    const uint32_t lane = tid & 63u;
+    const size_t gl = (size_t)blockIdx.x * kBlock + tid;
+    const size_t nlanes = (size_t)gridDim.x * kBlock;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
 
-    Best best{p.tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
-    bool occluded = false;
-    uint32_t nbox = 0, ntri = 0;
+    int32_t ray = -1;
+    bool exhausted = false;
+    RayPre p{};
+    LeafState st{0.0f, 0.0f, 0.0f, 0xFFFFFFFFu, false, 0u};
     int sp = 0;
-    int32_t ref = 0;  // the root is always an interior wide node
+    int32_t node = kNone, leaf = kNone;
+    uint32_t nbox = 0;
+    uint32_t nx = 0, ny = 32, nz = 64;
     bool overflow = false;
-    // A NaN anywhere in the ray hits nothing (the triangle test rejects it
-    // too); skipping traversal also keeps NaN slabs from admitting empty slots.
-    const bool nan_ray = (r0.x != r0.x) || (r0.y != r0.y) || (r0.z != r0.z) || (r0.w != r0.w) ||
-                         (r1.x != r1.x) || (r1.y != r1.y) || (r1.z != r1.z) || (r1.w != r1.w);
-    if (nan_ray) ref = DR_EMPTY_REF;
+    const char *nbase = reinterpret_cast<const char *>(nodes);
 
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kNone;
+        --sp;
+        return sp < kLdsStack ? stack[sp][tid] : ovf[(size_t)(sp - kLdsStack) * nlanes + gl];
+    };
     auto push = [&](int32_t r) {
         if (sp < kLdsStack) stack[sp][tid] = r;
-        else if (sp < kMaxStack) ovf[(size_t)(sp - kLdsStack) * n + i] = r;
+        else if (sp < kMaxStack) ovf[(size_t)(sp - kLdsStack) * nlanes + gl] = r;
         else { overflow = true; return; }
         ++sp;
     };
 
     for (;;) {
-        if (ref == DR_EMPTY_REF) {
-            // nothing to visit (NaN ray)
-        } else if (ref >= 0) {
-            const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-            const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-            const int4 ch = reinterpret_cast<const int4 *>(np)[6];
-            const float lim = (ANY ? p.tmax : best.t) * kBestSlack;
-            float tn[4];
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float t0x = (lx[c] - p.ox) * p.ix, t1x = (hx[c] - p.ox) * p.ix;
-                const float t0y = (ly[c] - p.oy) * p.iy, t1y = (hy[c] - p.oy) * p.iy;
-                const float t0z = (lz[c] - p.oz) * p.iz, t1z = (hz[c] - p.oz) * p.iz;
-                const float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), p.tmin));
-                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kFarScale;
-                far = fminf(far, lim);
-                tn[c] = (near <= far) ? near : INFINITY;
+        // ---- fetch: the lane's own slot, or (persistent) the next free slots
+        const bool need = ray < 0 && !exhausted;
+        const uint64_t m = __ballot(need);
+        if (m) {
+            uint32_t base = 0;
+            if (PERSISTENT) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                if ((int)lane == leader) base = atomicAdd(ray_counter, (uint32_t)__popcll(m));
+                base = __shfl(base, leader, 64);
             }
-            int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
-            if (COUNT) nbox += (ch.x != DR_EMPTY_REF) + (ch.y != DR_EMPTY_REF) + (ch.z != DR_EMPTY_REF) + (ch.w != DR_EMPTY_REF);
-            // sorting network on (distance, ref)
-            cswap(tn[0], r[0], tn[1], r[1]);
-            cswap(tn[2], r[2], tn[3], r[3]);
-            cswap(tn[0], r[0], tn[2], r[2]);
-            cswap(tn[1], r[1], tn[3], r[3]);
-            cswap(tn[1], r[1], tn[2], r[2]);
-            // empty slots carry boxes at +inf and never pass the slab test;
-            // the ref check keeps that true for any ray
-            if (tn[3] != INFINITY && r[3] != DR_EMPTY_REF) push(r[3]);
-            if (tn[2] != INFINITY && r[2] != DR_EMPTY_REF) push(r[2]);
-            if (tn[1] != INFINITY && r[1] != DR_EMPTY_REF) push(r[1]);
-            if (overflow) break;
-            if (tn[0] != INFINITY && r[0] != DR_EMPTY_REF) { ref = r[0]; continue; }
-        } else {
-            const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tv = tri_v + 3 * (size_t)(first + k);
-                const float4 a = tv[0], b = tv[1], c = tv[2];
-                float t, u, v;
-                if (COUNT) ++ntri;
-                if (!tri_watertight(p, a, b, c, t, u, v)) continue;
-                const uint32_t id = (uint32_t)__float_as_int(a.w);
-                if (ANY) {
-                    if (t >= p.tmin && t <= p.tmax) { occluded = true; break; }
-                } else if (accept_hit(t, id, p.tmin, best.t, best.id)) {
-                    best.t = t; best.u = u; best.v = v; best.id = id;
+            if (need) {
+                const uint32_t slot = PERSISTENT ? base + (uint32_t)__popcll(m & lt) : (uint32_t)gl;
+                if (!PERSISTENT) exhausted = true;
+                if (slot < n) {
+                    const uint32_t r = perm ? perm[slot] : slot;
+                    ray = (int32_t)r;
+                    const float4 r0 = rays[2 * (size_t)r];
+                    const float4 r1 = rays[2 * (size_t)r + 1];
+                    ray_prepare(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, p);
+                    st.bt = p.tmax; st.bu = 0.0f; st.bv = 0.0f; st.bid = 0xFFFFFFFFu;
+                    st.occluded = false; st.ntri = 0;
+                    sp = 0; leaf = kNone; nbox = 0;
+                    // a NaN anywhere in the ray hits nothing (the triangle test
+                    // rejects it too), and NaN slabs would admit empty slots
+                    const bool nan_ray = (r0.x != r0.x) || (r0.y != r0.y) || (r0.z != r0.z) || (r0.w != r0.w) ||
+                                         (r1.x != r1.x) || (r1.y != r1.y) || (r1.z != r1.z) || (r1.w != r1.w);
+                    node = nan_ray ? kNone : 0;  // the root is always an interior node
+                    // byte offsets of the near slab planes for this ray's octant
+                    nx = signbit(p.ix) ? 16u : 0u;
+                    ny = signbit(p.iy) ? 48u : 32u;
+                    nz = signbit(p.iz) ? 80u : 64u;
+                } else {
+                    exhausted = true;
                 }
             }
-            if (ANY && occluded) break;
         }
-        if (sp == 0) break;
-        --sp;
-        ref = sp < kLdsStack ? stack[sp][tid] : ovf[(size_t)(sp - kLdsStack) * n + i];
+        if (__ballot(ray >= 0) == 0ull) break;
+
+        if (ray >= 0) {
+            // ---- interior nodes, leaves postponed
+            while (node >= 0) {
+                const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(WideNode);
+                const float4 nxv = *reinterpret_cast<const float4 *>(nbase + nb + nx);
+                const float4 fxv = *reinterpret_cast<const float4 *>(nbase + nb + (nx ^ 16u));
+                const float4 nyv = *reinterpret_cast<const float4 *>(nbase + nb + ny);
+                const float4 fyv = *reinterpret_cast<const float4 *>(nbase + nb + (ny ^ 16u));
+                const float4 nzv = *reinterpret_cast<const float4 *>(nbase + nb + nz);
+                const float4 fzv = *reinterpret_cast<const float4 *>(nbase + nb + (nz ^ 16u));
+                const int4 ch = *reinterpret_cast<const int4 *>(nbase + nb + 96u);
+                const float lim = (ANY ? p.tmax : st.bt) * kBestSlack;
+                const float ax[4] = {nxv.x, nxv.y, nxv.z, nxv.w}, bx[4] = {fxv.x, fxv.y, fxv.z, fxv.w};
+                const float ay[4] = {nyv.x, nyv.y, nyv.z, nyv.w}, by[4] = {fyv.x, fyv.y, fyv.z, fyv.w};
+                const float az[4] = {nzv.x, nzv.y, nzv.z, nzv.w}, bz[4] = {fzv.x, fzv.y, fzv.z, fzv.w};
+                float tn[4];
+                int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float t0x = (ax[c] - p.ox) * p.ix, t1x = (bx[c] - p.ox) * p.ix;
+                    const float t0y = (ay[c] - p.oy) * p.iy, t1y = (by[c] - p.oy) * p.iy;
+                    const float t0z = (az[c] - p.oz) * p.iz, t1z = (bz[c] - p.oz) * p.iz;
+                    const float nr = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, p.tmin));
+                    float fr = fminf(fminf(t1x, t1y), t1z) * kFarScale;
+                    fr = fminf(fr, lim);
+                    // empty slots carry boxes at +inf and never pass; the ref
+                    // check keeps that true for any ray
+                    tn[c] = (nr <= fr && r[c] != kNone) ? nr : INFINITY;
+                }
+                if (COUNT) nbox += (ch.x != kNone) + (ch.y != kNone) + (ch.z != kNone) + (ch.w != kNone);
+                cswap(tn[0], r[0], tn[1], r[1]);
+                cswap(tn[2], r[2], tn[3], r[3]);
+                cswap(tn[0], r[0], tn[2], r[2]);
+                cswap(tn[1], r[1], tn[3], r[3]);
+                cswap(tn[1], r[1], tn[2], r[2]);
+                if (tn[3] != INFINITY) push(r[3]);
+                if (tn[2] != INFINITY) push(r[2]);
+                if (tn[1] != INFINITY) push(r[1]);
+                node = (tn[0] != INFINITY) ? r[0] : pop();
+                if (is_leaf(node) && leaf == kNone) {
+                    leaf = node;
+                    node = pop();
+                }
+                if (SPECULATIVE) {
+                    if (__ballot(leaf == kNone) == 0ull) break;  // every active lane holds a leaf
+                } else if (leaf != kNone) {
+                    break;
+                }
+            }
+            // ---- leaves; a wave whose lanes share the dominant axis runs the
+            // specialised triangle test
+            const int kz0 = __builtin_amdgcn_readfirstlane(p.kz);
+            const bool uniform = __ballot(p.kz != kz0) == 0ull;
+            while (is_leaf(leaf)) {
+                if (uniform) {
+                    if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, leaf, p, st);
+                    else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, leaf, p, st);
+                    else intersect_leaf<ANY, COUNT, 2>(tri_v, leaf, p, st);
+                } else {
+                    intersect_leaf<ANY, COUNT, -1>(tri_v, leaf, p, st);
+                }
+                if (ANY && st.occluded) { leaf = kNone; node = kNone; sp = 0; break; }
+                leaf = kNone;
+                if (is_leaf(node)) {
+                    leaf = node;
+                    node = pop();
+                }
+            }
+            if (overflow) { node = kNone; leaf = kNone; sp = 0; }
+            // ---- finished: write the result, free the lane
+            if (node == kNone && leaf == kNone) {
+                const uint32_t rr = (uint32_t)ray;
+                if (ANY) {
+                    occ[rr] = st.occluded ? 1u : 0u;
+                } else {
+                    const bool hit = st.bid != 0xFFFFFFFFu;
+                    dr_hit h;
+                    h.id = hit ? (int32_t)st.bid : -1;
+                    h.t = hit ? st.bt : INFINITY;
+                    h.u = hit ? st.bu : 0.0f;
+                    h.v = hit ? st.bv : 0.0f;
+                    hits[rr] = h;
+                }
+                if (COUNT) { box_tests[rr] = nbox; tri_tests[rr] = st.ntri; }
+                ray = -1;
+            }
+        }
     }
     if (overflow) atomicOr(&flags[0], 1u);
-    if (ANY) {
-        occ[i] = occluded ? 1u : 0u;
-    } else {
-        dr_hit h;
-        const bool hit = best.id != 0xFFFFFFFFu;
-        h.id = hit ? (int32_t)best.id : -1;
-        h.t = hit ? best.t : INFINITY;
-        h.u = hit ? best.u : 0.0f;
-        h.v = hit ? best.v : 0.0f;
-        hits[i] = h;
-    }
-    if (COUNT) {
-        box_tests[i] = nbox;
-        tri_tests[i] = ntri;
+}
+
+__global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// key = dominant axis (2 bits) | Morton code of the origin (30 bits)
+__global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ rays, uint32_t n, float lx, float ly,
+                                                     float lz, float sx, float sy, float sz,
+                                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 o = rays[2 * (size_t)i];
+    const float4 d = rays[2 * (size_t)i + 1];
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const uint32_t kz = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    // NaN origins map to cell 0 (fmaxf drops the NaN)
+    const float qx = fminf(fmaxf((o.x - lx) * sx, 0.0f), 1023.0f);
+    const float qy = fminf(fmaxf((o.y - ly) * sy, 0.0f), 1023.0f);
+    const float qz = fminf(fmaxf((o.z - lz) * sz, 0.0f), 1023.0f);
+    const uint32_t code = (expand10((uint32_t)qx) << 2) | (expand10((uint32_t)qy) << 1) | expand10((uint32_t)qz);
+    keys[i] = (kz << 30) | code;
+    idx[i] = i;
+}
+
+using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t, dr_hit *,
+                            uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *);
+
+template <bool ANY, bool COUNT>
+TraceKernel pick(uint32_t sched)
+{
+    switch (sched & 3u) {
+    case DR_SCHED_PER_RAY: return k_trace<ANY, COUNT, false, false>;
+    case DR_SCHED_PER_RAY_SPECULATIVE: return k_trace<ANY, COUNT, false, true>;
+    case DR_SCHED_PERSISTENT: return k_trace<ANY, COUNT, true, false>;
+    default: return k_trace<ANY, COUNT, true, true>;
     }
 }
 
@@ -152,21 +294,46 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
     if (n == 0) return DR_OK;
     if (!rays || (any_hit ? !occluded : !hits)) return fail(DR_E_INVALID, "null ray or output buffer");
-    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * n)) return st;
-    if (dr_status st = s->flags.ensure(16)) return st;
-    const dim3 grid((n + kBlock - 1) / kBlock);
-    const auto *nodes = s->nodes.as<WideNode>();
-    const auto *tv = s->tri_v.as<float4>();
-    const auto *r4 = reinterpret_cast<const float4 *>(rays);
-    int32_t *ovf = s->ovf_stack.as<int32_t>();
-    uint32_t *flags = s->flags.as<uint32_t>();
     const bool count = box_tests && tri_tests;
-    if (any_hit)
-        k_trace<true, false><<<grid, kBlock, 0, stream>>>(nodes, tv, r4, n, nullptr, occluded, nullptr, nullptr, ovf, flags);
-    else if (count)
-        k_trace<false, true><<<grid, kBlock, 0, stream>>>(nodes, tv, r4, n, hits, nullptr, box_tests, tri_tests, ovf, flags);
-    else
-        k_trace<false, false><<<grid, kBlock, 0, stream>>>(nodes, tv, r4, n, hits, nullptr, nullptr, nullptr, ovf, flags);
+    const uint32_t sched = any_hit ? s->sched_any : s->sched_closest;
+    const uint32_t loop = sched & 3u;
+    const TraceKernel kern = any_hit ? pick<true, false>(loop) : (count ? pick<false, true>(loop) : pick<false, false>(loop));
+    const bool persistent = loop == DR_SCHED_PERSISTENT || loop == DR_SCHED_PERSISTENT_SPECULATIVE;
+    static int occ_cache[16] = {0};
+    const int variant = (any_hit ? 0 : (count ? 1 : 2)) * 4 + (int)loop;
+    if (!occ_cache[variant]) {
+        int bpc = 0;
+        DR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kBlock, 0));
+        occ_cache[variant] = bpc > 0 ? bpc : 1;
+    }
+    const uint32_t need = (n + kBlock - 1) / kBlock;
+    const uint32_t resident = (uint32_t)(s->ctx->cu_count * occ_cache[variant]);
+    const uint32_t grid = persistent ? (need < resident ? need : resident) : need;
+    const size_t nlanes = (size_t)grid * kBlock;
+    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * nlanes)) return st;
+    const auto *r4 = reinterpret_cast<const float4 *>(rays);
+    const uint32_t *perm = nullptr;
+    if ((sched & DR_SCHED_REORDER) && n >= 4096) {
+        if (dr_status st = s->ray_keys.ensure(sizeof(uint32_t) * (size_t)n)) return st;
+        if (dr_status st = s->ray_perm.ensure(sizeof(uint32_t) * (size_t)n)) return st;
+        const float *lo = s->bounds, *hi = s->bounds + 3;
+        float sc[3];
+        for (int a = 0; a < 3; ++a) sc[a] = hi[a] > lo[a] ? 1024.0f / (hi[a] - lo[a]) : 0.0f;
+        k_ray_keys<<<need, kBlock, 0, stream>>>(r4, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2],
+                                                s->ray_keys.as<uint32_t>(), s->ray_perm.as<uint32_t>());
+        DR_HIP_TRY(hipGetLastError());
+        if (dr_status st = radix_sort_pairs(s->ctx, s->ray_keys.as<uint32_t>(), s->ray_perm.as<uint32_t>(), n, 32,
+                                            stream))
+            return st;
+        perm = s->ray_perm.as<uint32_t>();
+    }
+    // flags[0]: overflow; flags[16 ..]: per-launch ray counters (ring)
+    uint32_t *flags = s->flags.as<uint32_t>();
+    uint32_t *counter = flags + 16 + (s->launch_seq++ % kCounterSlots);
+    if (persistent) k_zero_counter<<<1, 1, 0, stream>>>(counter);
+    int32_t *ovf = s->ovf_stack.as<int32_t>();
+    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, hits, occluded,
+                                      box_tests, tri_tests, ovf, flags, counter);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

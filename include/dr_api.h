@@ -125,6 +125,21 @@ dr_status dr_trace_closest(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_h
 dr_status dr_trace_any(dr_scene *scene, const dr_ray *rays, uint32_t n, uint32_t *occluded,
                        dr_stream stream);
 
+/* Traversal schedules (how lanes take rays and order interior/leaf work):
+ * PER_RAY: one lane per ray for the launch; SPECULATIVE: leaves are parked
+ * until every lane of the wave holds one (Aila & Laine 2009); PERSISTENT:
+ * resident waves refill finished lanes from a global counter. The default is
+ * chosen by measurement; dr_scene_set_trace_schedule overrides it. */
+enum {
+    DR_SCHED_PER_RAY = 0,
+    DR_SCHED_PER_RAY_SPECULATIVE = 1,
+    DR_SCHED_PERSISTENT = 2,
+    DR_SCHED_PERSISTENT_SPECULATIVE = 3,
+    /* flag: sort the batch by (dominant axis, origin Morton code) first */
+    DR_SCHED_REORDER = 4
+};
+dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
+
 /* Closest hit with the reference's HitTests counters
  * (include/tracing/Benchmark.hpp:5-14, IntersectionTestsImpl.hpp:228-231):
  * per ray, the number of AABB tests and triangle tests. */
