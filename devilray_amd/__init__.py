@@ -11,9 +11,48 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrMaterial,
-    DrSceneDesc, DrSceneInfo, check, lib, ptr,
+    ALIAS_DTYPE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera,
+    DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
 )
+
+
+def camera_lookat(eye, target, up, fov_y_deg: float, width: int, height: int,
+                  pixel_sampling: int = 1) -> DrCamera:
+    """Reference-model pinhole camera (Camera / cameraRay) looking from eye
+    to target: rows of the image go down the screen, so camera +y maps to
+    world 'down'."""
+    e = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - e
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    down = np.cross(fwd, right)
+    m = np.zeros(12, np.float32)
+    for r in range(3):
+        m[4 * r + 0] = right[r]
+        m[4 * r + 1] = down[r]
+        m[4 * r + 2] = fwd[r]
+        m[4 * r + 3] = e[r]
+    cam = DrCamera()
+    cam.transform[:] = list(m)
+    cam.focal_length = float((height / 2.0) / np.tan(np.radians(fov_y_deg) / 2.0))
+    cam.center[:] = [width / 2.0, height / 2.0]
+    cam.pixel_size[:] = [1.0, 1.0]
+    cam.width, cam.height, cam.pixel_sampling = width, height, pixel_sampling
+    return cam
+
+
+def camera_reference(width: int, height: int, focal_length: float, pixel_size: float,
+                     pixel_sampling: int = 1) -> DrCamera:
+    """The reference tests' camera (test/src/test_render.cpp:21-36): identity
+    transform, principal point at the image centre."""
+    cam = DrCamera()
+    cam.transform[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    cam.focal_length = focal_length
+    cam.center[:] = [width / 2.0 * pixel_size, height / 2.0 * pixel_size]
+    cam.pixel_size[:] = [pixel_size, pixel_size]
+    cam.width, cam.height, cam.pixel_sampling = width, height, pixel_sampling
+    return cam
 
 __all__ = [
     "Context", "Scene", "DeviceArray", "gen_icosphere", "gen_rays_surface", "gen_rays_camera",
@@ -151,6 +190,36 @@ class Scene:
 
     def set_schedule(self, closest: int, any_hit: int | None = None):
         check(lib().dr_scene_set_trace_schedule(self.handle, closest, closest if any_hit is None else any_hit))
+
+    def set_refill(self, refill_min: int):
+        check(lib().dr_scene_set_trace_refill(self.handle, refill_min))
+
+    def build_lights(self, stream=None):
+        check(lib().dr_scene_build_lights(self.handle, stream))
+
+    def lights(self):
+        """(alias entries, per-triangle area pdf) copied to the host."""
+        n = C.c_uint32()
+        check(lib().dr_scene_light_count(self.handle, C.byref(n)))
+        ent = np.zeros(n.value, ALIAS_DTYPE)
+        pdf = np.zeros(self._f.shape[0], np.float32)
+        check(lib().dr_scene_copy_lights(self.handle, ptr(ent) if n.value else None, ptr(pdf)))
+        return ent, pdf
+
+    def render_dev(self, camera: DrCamera, params: DrRenderParams, accum_ptr: int, stream=None):
+        check(lib().dr_render(self.handle, C.byref(camera), C.byref(params), C.c_void_p(accum_ptr), stream))
+
+    def render(self, camera: DrCamera, spp: int, max_depth: int = 8, rr_depth: int = 3, seed: int = 0,
+               fresnel: int = 0, sample_offset: int = 0, accum: np.ndarray | None = None) -> np.ndarray:
+        """Render into an FP64 RGBW accumulation image (height, width, 4)."""
+        npix = camera.width * camera.height
+        d = self.ctx.alloc(npix * 32)
+        init = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64)
+        d.copy_from(init)
+        p = DrRenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, 0)
+        self.render_dev(camera, p, d.ptr)
+        self.ctx.synchronize()
+        return d.to_numpy(np.float64, npix * 4).reshape(camera.height, camera.width, 4)
 
     def info(self) -> DrSceneInfo:
         i = DrSceneInfo()

@@ -56,6 +56,20 @@ class DrSceneInfo(C.Structure):
     ]
 
 
+class DrCamera(C.Structure):
+    _fields_ = [("transform", C.c_float * 12), ("focal_length", C.c_float), ("center", C.c_float * 2),
+                ("pixel_size", C.c_float * 2), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("pixel_sampling", C.c_uint32)]
+
+
+class DrRenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("sample_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("fresnel", C.c_uint32), ("flags", C.c_uint32)]
+
+
+ALIAS_DTYPE = np.dtype([("p_a", np.float32), ("pdf_a", np.float32), ("pdf_b", np.float32),
+                        ("a", np.int32), ("b", np.int32)])
+
 RAY_DTYPE = np.dtype([(n, np.float32) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")])
 HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("v", np.float32)])
 
@@ -84,6 +98,11 @@ PROTOTYPES = {
     "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "dr_scene_set_trace_schedule": (_st, [_vp, _u32, _u32]),
+    "dr_scene_set_trace_refill": (_st, [_vp, _u32]),
+    "dr_scene_build_lights": (_st, [_vp, _vp]),
+    "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
+    "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),
+    "dr_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
     "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
     "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),

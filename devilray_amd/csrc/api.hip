@@ -246,6 +246,14 @@ dr_status dr_scene_set_trace_schedule(dr_scene *sc, uint32_t closest_sched, uint
     return DR_OK;
 }
 
+dr_status dr_scene_set_trace_refill(dr_scene *sc, uint32_t refill_min)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    if (refill_min < 1 || refill_min > 64) return fail(DR_E_INVALID, "refill_min must be 1..64");
+    reinterpret_cast<Scene *>(sc)->refill_min = refill_min;
+    return DR_OK;
+}
+
 dr_status dr_trace_closest(dr_scene *sc, const dr_ray *rays, uint32_t n, dr_hit *hits, dr_stream stream)
 {
     if (!sc) return fail(DR_E_INVALID, "scene is null");
@@ -267,6 +275,45 @@ dr_status dr_trace_closest_counted(dr_scene *sc, const dr_ray *rays, uint32_t n,
     if (!box_tests || !tri_tests) return fail(DR_E_INVALID, "null counter buffer");
     return launch_trace(reinterpret_cast<Scene *>(sc), rays, n, hits, nullptr, box_tests, tri_tests, false,
                         reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_scene_build_lights(dr_scene *sc, dr_stream stream)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    DR_HIP_TRY(hipSetDevice(s->ctx->device));
+    return build_lights(s, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_scene_light_count(dr_scene *sc, uint32_t *n_emit)
+{
+    if (!sc || !n_emit) return fail(DR_E_INVALID, "null argument");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    if (!s->lights_built)
+        if (dr_status st = build_lights(s, nullptr)) return st;
+    *n_emit = s->n_emit;
+    return DR_OK;
+}
+
+dr_status dr_scene_copy_lights(dr_scene *sc, void *entries, float *pdf_area)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    if (!s->lights_built)
+        if (dr_status st = build_lights(s, nullptr)) return st;
+    DR_HIP_TRY(hipDeviceSynchronize());
+    if (entries && s->n_emit) DR_HIP_TRY(hipMemcpy(entries, s->alias.ptr, 20ull * s->n_emit, hipMemcpyDeviceToHost));
+    if (pdf_area) DR_HIP_TRY(hipMemcpy(pdf_area, s->light_pdf_area.ptr, 4ull * s->n_tris, hipMemcpyDeviceToHost));
+    return DR_OK;
+}
+
+dr_status dr_render(dr_scene *sc, const dr_camera *camera, const dr_render_params *params, double *accum,
+                    dr_stream stream)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    DR_HIP_TRY(hipSetDevice(s->ctx->device));
+    return render(s, camera, params, accum, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -83,16 +83,41 @@ struct Scene {
     DevBuf ovf_stack;
     DevBuf flags;     // uint32[512]: [0] overflow flag, [16..271] per-launch ray counters
     uint64_t launch_seq = 0;
-    uint32_t sched_closest = DR_SCHED_PER_RAY_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
-    uint32_t sched_any = DR_SCHED_PER_RAY_SPECULATIVE | DR_SCHED_REORDER;
+    uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
+    uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;
+    uint32_t refill_min = 16;  // persistent waves refill when this many lanes are idle
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (lo, hi), set by the BVH build
     DevBuf ray_keys, ray_perm;             // ray reordering scratch
+    // light sampler (light.hip)
+    bool lights_built = false;
+    uint32_t n_emit = 0;
+    uint64_t light_q_total = 0;
+    DevBuf tri_area;        // double per triangle
+    DevBuf light_pdf_area;  // float per triangle: selection pdf / area (0: not an emitter)
+    DevBuf emit_tri;        // u32 per emitter
+    DevBuf alias;           // 20-byte alias entries, one per emitter
+    DevBuf light_scalars;
+    // wavefront render state (render.hip)
+    DevBuf rs_rays, rs_hits, rs_beta, rs_radiance, rs_pdf, rs_list_a, rs_list_b, rs_counters;
+    DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib, rs_shadow_path;
+    uint32_t rs_capacity = 0;
 };
+
+// light.hip
+dr_status build_lights(Scene *s, hipStream_t stream);
+
+// render.hip
+dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, double *accum, hipStream_t stream);
 
 // bvh_build.hip
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);
 
 // trace.hip
+// ray_list (optional): indices of the rays to trace (results go to those
+// indices); n_dev (optional): device-resident count, at most n_max.
+dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
+                          const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
+                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream);
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
 
@@ -100,5 +125,6 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
 dr_status radix_sort_pairs(Context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,
                            hipStream_t stream);
 dr_status exclusive_scan_u32(Context *ctx, uint32_t *data, uint32_t n, hipStream_t stream);
+dr_status exclusive_scan_u64(Context *ctx, uint64_t *data, uint32_t n, hipStream_t stream);
 
 }  // namespace dr

@@ -1,0 +1,380 @@
+// Wavefront path tracer: camera -> [extend -> shade -> shadow -> apply] x
+// max_depth -> resolve, all on one stream with device-resident queue counts.
+//
+// Estimator (the reference's sampleColor, include/tracing/PathGeneration.hpp:
+// 363-515, computed incrementally instead of after the whole path): at every
+// Lambert vertex, emission weighted by the power heuristic against the light
+// pdf (weight 1 after the camera or a specular bounce), then next-event
+// estimation with one alias-table light sample and a shadow ray, weighted by
+// the power heuristic against the BSDF pdf; then a cosine-weighted bounce.
+// Dielectric vertices reflect or refract by Fresnel (exact or Schlick).
+// Russian roulette from vertex rr_depth. Paths are independent work items
+// (pixel, sample); per-path radiance is resolved into the FP64 accumulation
+// buffer in sample order, so results do not depend on queue order.
+#include "dr_internal.hpp"
+#include "shade.hpp"
+
+namespace dr {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr uint32_t kPathCap = 1u << 21;
+
+struct AliasEntry {
+    float p_a, pdf_a, pdf_b;
+    int32_t a, b;
+};
+
+struct CamParams {
+    float m[12];
+    float f, cx, cy, psx, psy;
+    uint32_t w, h, sampling;
+};
+
+struct PathParams {
+    uint64_t seed;
+    uint32_t npix, sample_offset;
+    uint64_t item0;  // first work item of the chunk: item = sample * npix + pixel
+    uint32_t n;      // paths in the chunk
+    uint32_t max_depth, rr_depth, schlick;
+};
+
+__device__ __forceinline__ void item_of(const PathParams &pp, uint32_t path, uint32_t &pixel, uint32_t &sample)
+{
+    const uint64_t item = pp.item0 + path;
+    pixel = (uint32_t)(item % pp.npix);
+    sample = pp.sample_offset + (uint32_t)(item / pp.npix);
+}
+
+__global__ __launch_bounds__(kT) void k_camera(CamParams cam, PathParams pp, dr_ray *__restrict__ rays,
+                                               float4 *__restrict__ beta, float4 *__restrict__ rad,
+                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters)
+{
+    const uint32_t p = blockIdx.x * kT + threadIdx.x;
+    if (p == 0) { counters[0] = pp.n; counters[1] = 0u; counters[2] = 0u; }
+    if (p >= pp.n) return;
+    uint32_t pixel, sample;
+    item_of(pp, p, pixel, sample);
+    const uint32_t x = pixel % cam.w, y = pixel / cam.w;
+    float sx = 0.5f, sy = 0.5f;
+    if (cam.sampling == DR_PIXEL_UNIFORM_RANDOM) {
+        const U4 r = path_rng(pixel, sample, 0xFFFFFFFFu, 0u, pp.seed);
+        sx = u01(r.x);
+        sy = u01(r.y);
+    }
+    const float pcx = ((float)x + sx) * cam.psx - cam.cx;
+    const float pcy = ((float)y + sy) * cam.psy - cam.cy;
+    const V3 v = normalize(V3{pcx / cam.f, pcy / cam.f, 1.0f});
+    const float *m = cam.m;
+    dr_ray r;
+    r.ox = m[3]; r.oy = m[7]; r.oz = m[11];
+    r.dx = m[0] * v.x + m[1] * v.y + m[2] * v.z;
+    r.dy = m[4] * v.x + m[5] * v.y + m[6] * v.z;
+    r.dz = m[8] * v.x + m[9] * v.y + m[10] * v.z;
+    r.tmin = 0.0f;
+    r.tmax = INFINITY;
+    rays[p] = r;
+    beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);       // w: pdf of the last BSDF sample
+    rad[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);        // w: 1 = last bounce specular / camera
+    list[p] = p;
+}
+
+__global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count)
+{
+    *out_count = 0u;
+    *shadow_count = 0u;
+}
+
+struct SceneView {
+    const float *verts;
+    const uint32_t *idx;
+    const uint32_t *tri_mat;
+    const dr_material *mats;
+    const AliasEntry *alias;
+    const float *pdf_area;
+    uint32_t n_emit;
+};
+
+__device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
+{
+    return V3{sv.verts[3 * (size_t)vi], sv.verts[3 * (size_t)vi + 1], sv.verts[3 * (size_t)vi + 2]};
+}
+
+__global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint32_t vertex,
+                                              const uint32_t *__restrict__ in_list,
+                                              const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
+                                              uint32_t *__restrict__ out_count, dr_ray *__restrict__ rays,
+                                              const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
+                                              float4 *__restrict__ rad, dr_ray *__restrict__ srays,
+                                              float4 *__restrict__ scontrib, uint32_t *__restrict__ scount)
+{
+    const uint32_t slot = blockIdx.x * kT + threadIdx.x;
+    if (slot >= *in_count) return;
+    const uint32_t path = in_list[slot];
+    const dr_hit h = hits[path];
+    if (h.id < 0) return;  // escaped: no environment light
+    uint32_t pixel, sample;
+    item_of(pp, path, pixel, sample);
+    const dr_ray ray = rays[path];
+    float4 b4 = beta[path];
+    float4 l4 = rad[path];
+    V3 beta3 = V3{b4.x, b4.y, b4.z};
+    const bool spec_prev = l4.w != 0.0f;
+    const float pdf_prev = b4.w;
+
+    const uint32_t tri = (uint32_t)h.id;
+    const V3 A = vtx(sv, sv.idx[3 * (size_t)tri]);
+    const V3 B = vtx(sv, sv.idx[3 * (size_t)tri + 1]);
+    const V3 C = vtx(sv, sv.idx[3 * (size_t)tri + 2]);
+    const dr_material mat = sv.mats[sv.tri_mat[tri]];
+    const V3 d = V3{ray.dx, ray.dy, ray.dz};
+    const V3 p = bary_point(A, B, C, h.u, h.v);
+    const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
+    const bool back = dot(ng, d) > 0.0f;
+    const V3 n = back ? mul(ng, -1.0f) : ng;
+    const float eps = origin_eps(A, B, C);
+    const U4 r0 = path_rng(pixel, sample, vertex, 0u, pp.seed);
+    const U4 r1 = path_rng(pixel, sample, vertex, 1u, pp.seed);
+
+    V3 L = V3{l4.x, l4.y, l4.z};
+    V3 nd;
+    V3 no;
+    bool spec;
+    float pdf;
+    if (mat.kind == DR_MAT_LAMBERT) {
+        // emission, MIS against light sampling
+        const V3 E = V3{mat.emission[0], mat.emission[1], mat.emission[2]};
+        if (E.x != 0.0f || E.y != 0.0f || E.z != 0.0f) {
+            float w = 1.0f;
+            if (!spec_prev) {
+                const float cl = fabsf(dot(d, ng));
+                const float pl = cl < 1e-6f ? 0.0f : sv.pdf_area[tri] * (h.t * h.t) / cl;
+                w = power_heuristic(pdf_prev, pl);
+            }
+            L = add(L, V3{beta3.x * E.x * w, beta3.y * E.y * w, beta3.z * E.z * w});
+        }
+        // next-event estimation
+        const V3 alb = V3{mat.albedo[0], mat.albedo[1], mat.albedo[2]};
+        if (sv.n_emit > 0) {
+            const uint32_t s = (uint32_t)(((uint64_t)r1.x * sv.n_emit) >> 32);
+            const AliasEntry en = sv.alias[s];
+            const bool take_a = u01(r1.y) < en.p_a;
+            const int32_t lt = take_a ? en.a : en.b;
+            const float psel = take_a ? en.pdf_a : en.pdf_b;
+            if (lt >= 0 && psel > 0.0f) {
+                float a1 = u01(r1.z), a2 = u01(r1.w);
+                if (a1 + a2 > 1.0f) { a1 = 1.0f - a1; a2 = 1.0f - a2; }
+                const V3 LA = vtx(sv, sv.idx[3 * (size_t)lt]);
+                const V3 LB = vtx(sv, sv.idx[3 * (size_t)lt + 1]);
+                const V3 LC = vtx(sv, sv.idx[3 * (size_t)lt + 2]);
+                const V3 x = add(add(LA, mul(sub(LB, LA), a1)), mul(sub(LC, LA), a2));
+                const V3 nl = normalize(cross(sub(LB, LA), sub(LC, LA)));
+                const float pa = sv.pdf_area[lt];
+                const V3 toL = sub(x, p);
+                const float dist2 = dot(toL, toL);
+                const float dist = sqrtf(dist2);
+                const V3 om = divs(toL, dist);
+                const float cs = dot(om, n);
+                const float acl = fabsf(dot(om, nl));
+                if (cs > 0.0f && pa > 0.0f && dist > 0.0f) {
+                    const dr_material lm = sv.mats[sv.tri_mat[lt]];
+                    const float G = cs * acl / dist2;
+                    const float pn = acl < 1e-6f ? 0.0f : pa * dist2 / acl;
+                    const float pb = cs / kPi;
+                    const float w = power_heuristic(pn, pb);
+                    const float tmax = dist - 2.0f * fmaxf(eps, origin_eps(LA, LB, LC));
+                    if (pn > 0.0f && tmax > 0.0f) {
+                        const V3 Ld = V3{lm.emission[0] * (alb.x / kPi) * G / pa, lm.emission[1] * (alb.y / kPi) * G / pa,
+                                         lm.emission[2] * (alb.z / kPi) * G / pa};
+                        const V3 c = V3{beta3.x * Ld.x * w, beta3.y * Ld.y * w, beta3.z * Ld.z * w};
+                        const uint32_t k = atomicAdd(scount, 1u);
+                        const V3 so = add(p, mul(om, eps));
+                        srays[k] = dr_ray{so.x, so.y, so.z, 0.0f, om.x, om.y, om.z, tmax};
+                        scontrib[k] = make_float4(c.x, c.y, c.z, __int_as_float((int32_t)path));
+                    }
+                }
+            }
+        }
+        // cosine-weighted bounce
+        nd = cosine_dir(n, u01(r0.x), u01(r0.y));
+        const float cn = dot(nd, n);
+        pdf = cn > 0.0f ? cn / kPi : 0.0f;
+        beta3 = V3{beta3.x * alb.x, beta3.y * alb.y, beta3.z * alb.z};
+        no = add(p, mul(n, eps));
+        spec = false;
+    } else {
+        // dielectric: n faces the incoming ray; ng (winding) points outside
+        const bool entering = !back;
+        const float n1 = entering ? 1.0f : mat.ior;
+        const float n2 = entering ? mat.ior : 1.0f;
+        const float dd = dot(d, n);
+        const float eta = n1 / n2;
+        const float k = 1.0f - eta * eta * (1.0f - dd * dd);
+        bool reflect_ray = true;
+        if (!(n1 > n2 && k < 0.0f)) {
+            const float ci = -dd;
+            const float F = fresnel(ci, n1, n2, sqrtf(fmaxf(k, 0.0f)), pp.schlick != 0u);
+            reflect_ray = u01(r0.w) < F;
+        }
+        if (reflect_ray) {
+            nd = V3{d.x - n.x * 2.0f * dd, d.y - n.y * 2.0f * dd, d.z - n.z * 2.0f * dd};
+            no = add(p, mul(n, eps));
+        } else {
+            const V3 nn = mul(n, -1.0f);
+            const float ci = -dd;
+            const float sk = sqrtf(k);
+            nd = add(mul(nn, sk - ci * eta), mul(d, eta));
+            no = add(p, mul(nn, eps));
+        }
+        pdf = 0.0f;
+        spec = true;
+    }
+    rad[path] = make_float4(L.x, L.y, L.z, spec ? 1.0f : 0.0f);
+    // continue?
+    bool alive = vertex + 1u < pp.max_depth;
+    if (alive && vertex + 1u >= pp.rr_depth) {
+        const float q = fmaxf(beta3.x, fmaxf(beta3.y, beta3.z));
+        if (q < 1.0f) {
+            if (u01(r0.z) >= q) alive = false;
+            else beta3 = V3{beta3.x / q, beta3.y / q, beta3.z / q};
+        }
+    }
+    if (!(beta3.x > 0.0f || beta3.y > 0.0f || beta3.z > 0.0f)) alive = false;
+    if (!alive) return;
+    beta[path] = make_float4(beta3.x, beta3.y, beta3.z, pdf);
+    rays[path] = dr_ray{no.x, no.y, no.z, 0.0f, nd.x, nd.y, nd.z, INFINITY};
+    out_list[atomicAdd(out_count, 1u)] = path;
+}
+
+__global__ __launch_bounds__(kT) void k_shadow_apply(const uint32_t *__restrict__ scount,
+                                                     const uint32_t *__restrict__ occ,
+                                                     const float4 *__restrict__ scontrib, float4 *__restrict__ rad)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= *scount || occ[i]) return;
+    const float4 c = scontrib[i];
+    const uint32_t path = (uint32_t)__float_as_int(c.w);
+    float4 l = rad[path];
+    l.x = l.x + c.x;
+    l.y = l.y + c.y;
+    l.z = l.z + c.z;
+    rad[path] = l;
+}
+
+// Lane p owns the chunk's first item of pixel (item0 + p) % npix and adds
+// that pixel's samples in increasing sample order.
+__global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__restrict__ rad, double *__restrict__ accum)
+{
+    const uint32_t p = blockIdx.x * kT + threadIdx.x;
+    if (p >= pp.n || p >= pp.npix) return;
+    const uint32_t pixel = (uint32_t)((pp.item0 + p) % pp.npix);
+    double *a = accum + 4 * (size_t)pixel;
+    double r = a[0], g = a[1], b = a[2], w = a[3];
+    for (uint32_t path = p; path < pp.n; path += pp.npix) {
+        const float4 l = rad[path];
+        r = r + (double)l.x;
+        g = g + (double)l.y;
+        b = b + (double)l.z;
+        w = w + 1.0;
+    }
+    a[0] = r; a[1] = g; a[2] = b; a[3] = w;
+}
+
+inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
+
+}  // namespace
+
+dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, double *accum, hipStream_t stream)
+{
+    if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
+    if (!cam || !prm || !accum) return fail(DR_E_INVALID, "null argument");
+    if (cam->width == 0 || cam->height == 0) return fail(DR_E_INVALID, "empty image");
+    if (prm->max_depth == 0 || prm->max_depth > 64) return fail(DR_E_INVALID, "max_depth must be 1..64");
+    if (!s->lights_built)
+        if (dr_status st = build_lights(s, stream)) return st;
+    const uint32_t npix = cam->width * cam->height;
+    const uint64_t items = (uint64_t)npix * prm->spp;
+    if (items == 0) return DR_OK;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(items, kPathCap);
+    if (s->rs_capacity < cap) {
+        dr_status st = DR_OK;
+        auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
+        need(s->rs_rays, sizeof(dr_ray) * (size_t)cap);
+        need(s->rs_hits, sizeof(dr_hit) * (size_t)cap);
+        need(s->rs_beta, sizeof(float4) * (size_t)cap);
+        need(s->rs_radiance, sizeof(float4) * (size_t)cap);
+        need(s->rs_list_a, 4ull * cap);
+        need(s->rs_list_b, 4ull * cap);
+        need(s->rs_counters, 64);
+        need(s->rs_shadow_rays, sizeof(dr_ray) * (size_t)cap);
+        need(s->rs_shadow_occ, 4ull * cap);
+        need(s->rs_shadow_contrib, sizeof(float4) * (size_t)cap);
+        if (st) return st;
+        s->rs_capacity = cap;
+    }
+    CamParams cp;
+    for (int i = 0; i < 12; ++i) cp.m[i] = cam->transform[i];
+    cp.f = cam->focal_length;
+    cp.cx = cam->center[0];
+    cp.cy = cam->center[1];
+    cp.psx = cam->pixel_size[0];
+    cp.psy = cam->pixel_size[1];
+    cp.w = cam->width;
+    cp.h = cam->height;
+    cp.sampling = cam->pixel_sampling;
+    SceneView sv;
+    sv.verts = s->verts.as<float>();
+    sv.idx = s->indices.as<uint32_t>();
+    sv.tri_mat = s->tri_mat.as<uint32_t>();
+    sv.mats = s->materials.as<dr_material>();
+    sv.alias = s->alias.as<AliasEntry>();
+    sv.pdf_area = s->light_pdf_area.as<float>();
+    sv.n_emit = s->n_emit;
+    uint32_t *cnt = s->rs_counters.as<uint32_t>();
+    auto *rays = s->rs_rays.as<dr_ray>();
+    auto *hits = s->rs_hits.as<dr_hit>();
+    auto *beta = s->rs_beta.as<float4>();
+    auto *rad = s->rs_radiance.as<float4>();
+    auto *srays = s->rs_shadow_rays.as<dr_ray>();
+    auto *socc = s->rs_shadow_occ.as<uint32_t>();
+    auto *scon = s->rs_shadow_contrib.as<float4>();
+    const uint32_t sched = DR_SCHED_PER_RAY_SPECULATIVE;
+    for (uint64_t item0 = 0; item0 < items; item0 += cap) {
+        PathParams pp;
+        pp.seed = prm->seed;
+        pp.npix = npix;
+        pp.sample_offset = prm->sample_offset;
+        pp.item0 = item0;
+        pp.n = (uint32_t)std::min<uint64_t>(cap, items - item0);
+        pp.max_depth = prm->max_depth;
+        pp.rr_depth = prm->rr_depth;
+        pp.schlick = prm->fresnel == DR_FRESNEL_SCHLICK ? 1u : 0u;
+        k_camera<<<blocks(pp.n), kT, 0, stream>>>(cp, pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt);
+        DR_HIP_TRY(hipGetLastError());
+        for (uint32_t v = 0; v < prm->max_depth; ++v) {
+            uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
+            uint32_t *lout = (v & 1u) ? s->rs_list_a.as<uint32_t>() : s->rs_list_b.as<uint32_t>();
+            uint32_t *cin = cnt + (v & 1u);
+            uint32_t *cout = cnt + ((v + 1u) & 1u);
+            if (dr_status st = launch_trace_ex(s, rays, lin, pp.n, cin, hits, nullptr, nullptr, nullptr, false, sched, stream))
+                return st;
+            k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2);
+            k_shade<<<blocks(pp.n), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
+                                                     cnt + 2);
+            DR_HIP_TRY(hipGetLastError());
+            if (s->n_emit) {
+                if (dr_status st = launch_trace_ex(s, srays, nullptr, pp.n, cnt + 2, nullptr, socc, nullptr, nullptr, true,
+                                                   sched, stream))
+                    return st;
+                k_shadow_apply<<<blocks(pp.n), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
+                DR_HIP_TRY(hipGetLastError());
+            }
+        }
+        k_resolve<<<blocks(std::min(pp.n, npix)), kT, 0, stream>>>(pp, rad, accum);
+        DR_HIP_TRY(hipGetLastError());
+    }
+    return DR_OK;
+}
+
+}  // namespace dr

@@ -86,37 +86,38 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     }
 }
 
-// ---- exclusive scan --------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane)
+// ---- exclusive scan (u32 / u64, exact integer sums) --------------------------
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x, uint32_t lane)
 {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
+        const T y = __shfl_up(x, off, 64);
         if (lane >= (uint32_t)off) x += y;
     }
     return x;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_block(uint32_t *__restrict__ data, uint32_t n,
-                                                         uint32_t *__restrict__ sums)
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_scan_block(T *__restrict__ data, uint32_t n, T *__restrict__ sums)
 {
-    __shared__ uint32_t wsum[kWaves];
+    __shared__ T wsum[kWaves];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint32_t base = blockIdx.x * kTile + tid * kChunks;
-    uint32_t v[kChunks];
-    uint32_t s = 0;
+    T v[kChunks];
+    T s = 0;
 #pragma unroll
     for (int k = 0; k < kChunks; ++k) {
         const uint32_t idx = base + k;
-        v[k] = idx < n ? data[idx] : 0u;
+        v[k] = idx < n ? data[idx] : T(0);
         s += v[k];
     }
-    const uint32_t incl = wave_incl_scan(s, lane);
+    const T incl = wave_incl_scan<T>(s, lane);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    uint32_t wpre = 0;
+    T wpre = 0;
     for (uint32_t ww = 0; ww < w; ++ww) wpre += wsum[ww];
-    uint32_t run = wpre + incl - s;
+    T run = wpre + incl - s;
 #pragma unroll
     for (int k = 0; k < kChunks; ++k) {
         const uint32_t idx = base + k;
@@ -126,10 +127,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_block(uint32_t *__restrict__ 
     if (tid == kThreads - 1 && sums) sums[blockIdx.x] = wpre + incl;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ data, uint32_t n,
-                                                       const uint32_t *__restrict__ sums)
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_scan_add(T *__restrict__ data, uint32_t n, const T *__restrict__ sums)
 {
-    const uint32_t add = sums[blockIdx.x];
+    const T add = sums[blockIdx.x];
     const uint32_t base = blockIdx.x * kTile;
     for (int k = 0; k < kChunks; ++k) {
         const uint32_t idx = base + k * kThreads + threadIdx.x;
@@ -137,22 +138,23 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ da
     }
 }
 
-dr_status scan_rec(Context *ctx, uint32_t *data, uint32_t n, int level, hipStream_t stream)
+template <class T>
+dr_status scan_rec(Context *ctx, T *data, uint32_t n, int level, hipStream_t stream)
 {
     if (n == 0) return DR_OK;
     const uint32_t nb = (n + kTile - 1) / kTile;
     if (nb == 1) {
-        k_scan_block<<<1, kThreads, 0, stream>>>(data, n, nullptr);
+        k_scan_block<T><<<1, kThreads, 0, stream>>>(data, n, nullptr);
         DR_HIP_TRY(hipGetLastError());
         return DR_OK;
     }
     if (level >= 3) return fail(DR_E_INVALID, "scan: input too large");
     DevBuf &tmp = ctx->scratch_sort[3 + level];
-    if (dr_status st = tmp.ensure(sizeof(uint32_t) * nb)) return st;
-    k_scan_block<<<nb, kThreads, 0, stream>>>(data, n, tmp.as<uint32_t>());
+    if (dr_status st = tmp.ensure(sizeof(T) * nb)) return st;
+    k_scan_block<T><<<nb, kThreads, 0, stream>>>(data, n, tmp.as<T>());
     DR_HIP_TRY(hipGetLastError());
-    if (dr_status st = scan_rec(ctx, tmp.as<uint32_t>(), nb, level + 1, stream)) return st;
-    k_scan_add<<<nb, kThreads, 0, stream>>>(data, n, tmp.as<uint32_t>());
+    if (dr_status st = scan_rec<T>(ctx, tmp.as<T>(), nb, level + 1, stream)) return st;
+    k_scan_add<T><<<nb, kThreads, 0, stream>>>(data, n, tmp.as<T>());
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }
@@ -161,7 +163,12 @@ dr_status scan_rec(Context *ctx, uint32_t *data, uint32_t n, int level, hipStrea
 
 dr_status exclusive_scan_u32(Context *ctx, uint32_t *data, uint32_t n, hipStream_t stream)
 {
-    return scan_rec(ctx, data, n, 0, stream);
+    return scan_rec<uint32_t>(ctx, data, n, 0, stream);
+}
+
+dr_status exclusive_scan_u64(Context *ctx, uint64_t *data, uint32_t n, hipStream_t stream)
+{
+    return scan_rec<uint64_t>(ctx, data, n, 0, stream);
 }
 
 dr_status radix_sort_pairs(Context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,

@@ -75,13 +75,16 @@ template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE>
 __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ nodes,
                                                   const float4 *__restrict__ tri_v,
                                                   const float4 *__restrict__ rays, const uint32_t *__restrict__ perm,
-                                                  uint32_t n, dr_hit *__restrict__ hits, uint32_t *__restrict__ occ,
+                                                  uint32_t n_max, const uint32_t *__restrict__ n_dev,
+                                                  dr_hit *__restrict__ hits, uint32_t *__restrict__ occ,
                                                   uint32_t *__restrict__ box_tests,
                                                   uint32_t *__restrict__ tri_tests,
                                                   int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
-                                                  uint32_t *__restrict__ ray_counter)
+                                                  uint32_t *__restrict__ ray_counter, uint32_t refill_min)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
+    // ray count: fixed at launch, or read from device memory (on-device queues)
+    const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const size_t gl = (size_t)blockIdx.x * kBlock + tid;
@@ -115,7 +118,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
         // ---- fetch: the lane's own slot, or (persistent) the next free slots
         const bool need = ray < 0 && !exhausted;
         const uint64_t m = __ballot(need);
-        if (m) {
+        // persistent waves refill in batches (each refill is an atomic and a
+        // ray load on the wave's critical path): when refill_min lanes are
+        // idle, or when none is busy
+        const bool refill = PERSISTENT ? ((uint32_t)__popcll(m) >= refill_min || __ballot(ray >= 0) == 0ull) : true;
+        if (m && refill) {
             uint32_t base = 0;
             if (PERSISTENT) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
@@ -272,8 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
     idx[i] = i;
 }
 
-using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t, dr_hit *,
-                            uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *);
+using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
+                            const uint32_t *, dr_hit *,
+                            uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t);
 
 template <bool ANY, bool COUNT>
 TraceKernel pick(uint32_t sched)
@@ -291,11 +299,19 @@ TraceKernel pick(uint32_t sched)
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream)
 {
+    return launch_trace_ex(s, rays, nullptr, n, nullptr, hits, occluded, box_tests, tri_tests, any_hit,
+                           any_hit ? s->sched_any : s->sched_closest, stream);
+}
+
+dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
+                          const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
+                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream)
+{
+    const uint32_t n = n_max;
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
     if (n == 0) return DR_OK;
     if (!rays || (any_hit ? !occluded : !hits)) return fail(DR_E_INVALID, "null ray or output buffer");
     const bool count = box_tests && tri_tests;
-    const uint32_t sched = any_hit ? s->sched_any : s->sched_closest;
     const uint32_t loop = sched & 3u;
     const TraceKernel kern = any_hit ? pick<true, false>(loop) : (count ? pick<false, true>(loop) : pick<false, false>(loop));
     const bool persistent = loop == DR_SCHED_PERSISTENT || loop == DR_SCHED_PERSISTENT_SPECULATIVE;
@@ -312,8 +328,8 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
     const size_t nlanes = (size_t)grid * kBlock;
     if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * nlanes)) return st;
     const auto *r4 = reinterpret_cast<const float4 *>(rays);
-    const uint32_t *perm = nullptr;
-    if ((sched & DR_SCHED_REORDER) && n >= 4096) {
+    const uint32_t *perm = ray_list;
+    if (!ray_list && !n_dev && (sched & DR_SCHED_REORDER) && n >= 4096) {
         if (dr_status st = s->ray_keys.ensure(sizeof(uint32_t) * (size_t)n)) return st;
         if (dr_status st = s->ray_perm.ensure(sizeof(uint32_t) * (size_t)n)) return st;
         const float *lo = s->bounds, *hi = s->bounds + 3;
@@ -332,8 +348,8 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
     uint32_t *counter = flags + 16 + (s->launch_seq++ % kCounterSlots);
     if (persistent) k_zero_counter<<<1, 1, 0, stream>>>(counter);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
-    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, hits, occluded,
-                                      box_tests, tri_tests, ovf, flags, counter);
+    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
+                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

@@ -139,12 +139,72 @@ enum {
     DR_SCHED_REORDER = 4
 };
 dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
+/* Persistent schedules: a wave refills its finished lanes once at least
+ * refill_min (1..64) of them are idle. */
+dr_status dr_scene_set_trace_refill(dr_scene *scene, uint32_t refill_min);
 
 /* Closest hit with the reference's HitTests counters
  * (include/tracing/Benchmark.hpp:5-14, IntersectionTestsImpl.hpp:228-231):
  * per ray, the number of AABB tests and triangle tests. */
 dr_status dr_trace_closest_counted(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_hit *hits,
                                    uint32_t *box_tests, uint32_t *tri_tests, dr_stream stream);
+
+/* ---- rendering -------------------------------------------------------- */
+
+/* Pinhole camera of the reference (include/tracing/Camera.hpp:6-19,
+ * include/tracing/CameraRay.hpp:6-34): the camera-space direction of pixel
+ * (x, y) with sub-pixel offset s is normalize(((x+s.x)*pixel_size.x -
+ * center.x) / f, ((y+s.y)*pixel_size.y - center.y) / f, 1), rotated by the
+ * 3x3 part of `transform` (row-major rows 0..2 of the reference's Matrix4x4);
+ * the origin is its translation column. Pixel y grows with row index. */
+enum { DR_PIXEL_CENTER = 0, DR_PIXEL_UNIFORM_RANDOM = 1 };
+typedef struct {
+    float transform[12];
+    float focal_length;
+    float center[2];
+    float pixel_size[2];
+    uint32_t width, height;
+    uint32_t pixel_sampling;
+} dr_camera;
+
+/* Fresnel model of dielectrics: exact unpolarised dielectric Fresnel, or
+ * Schlick's approximation as the reference uses (PathGeneration.hpp:75-86). */
+enum { DR_FRESNEL_EXACT = 0, DR_FRESNEL_SCHLICK = 1 };
+
+/* spp samples per pixel are added to the accumulation buffer; sample indices
+ * start at sample_offset (progressive rendering: the counter-based RNG makes
+ * the result independent of how samples are split over calls). max_depth
+ * counts path vertices (the reference uses 10, no Russian roulette);
+ * Russian roulette applies from vertex rr_depth on (rr_depth >= max_depth
+ * disables it). */
+typedef struct {
+    uint32_t spp;
+    uint32_t sample_offset;
+    uint32_t max_depth;
+    uint32_t rr_depth;
+    uint64_t seed;
+    uint32_t fresnel;
+    uint32_t flags;
+} dr_render_params;
+
+/* Build the emissive-triangle alias table (weights area x luminance(E),
+ * exact integer sweep construction). dr_render builds it on first use. */
+dr_status dr_scene_build_lights(dr_scene *scene, dr_stream stream);
+/* Host copies of the light table: n_emit entries of 20 B
+ * {float p_A, pdf_A, pdf_B; int32 A, B} (the reference's AliasEntry,
+ * include/tracing/DistributionSamplers.hpp:44-51, with triangle ids as items)
+ * and per-triangle area pdf (selection probability / area). Either pointer
+ * may be NULL. */
+dr_status dr_scene_light_count(dr_scene *scene, uint32_t *n_emit);
+dr_status dr_scene_copy_lights(dr_scene *scene, void *entries, float *pdf_area);
+
+/* Path tracing with NEE + MIS (power heuristic) into an FP64 accumulation
+ * buffer accum[4 * pixel] = {sum r, sum g, sum b, sample count} (device
+ * memory, width*height*32 bytes, caller-owned). Replaces Renderer::render's
+ * kernel (src/device/RendererImpl.hpp:22-102, PathGeneration.hpp:363-515):
+ * wavefront kernels (camera, extend, shade, shadow, resolve) on one stream. */
+dr_status dr_render(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, double *accum,
+                    dr_stream stream);
 
 /* ---- workload generators (host, deterministic; no reference counterpart:
  * the reference's tests build their meshes by hand) ---- */

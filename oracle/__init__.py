@@ -37,7 +37,68 @@ PROTOTYPES = {
     "or_refbbh_free": (None, [_vp]),
     "or_refbbh_depth": (_i32, [_vp]),
     "or_refbbh_trace": (None, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32]),
+    "or_lights_build": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "or_sincos_2pi": (None, [C.c_float, _vp, _vp]),
+    "or_render": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32]),
 }
+
+MATERIAL_DTYPE = np.dtype([("kind", np.int32), ("albedo", np.float32, 3), ("emission", np.float32, 3),
+                           ("ior", np.float32)])
+ALIAS_DTYPE = np.dtype([("p_a", np.float32), ("pdf_a", np.float32), ("pdf_b", np.float32),
+                        ("a", np.int32), ("b", np.int32)])
+
+
+class Camera(C.Structure):
+    _fields_ = [("transform", C.c_float * 12), ("focal_length", C.c_float), ("center", C.c_float * 2),
+                ("pixel_size", C.c_float * 2), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("pixel_sampling", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("sample_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("fresnel", C.c_uint32), ("flags", C.c_uint32)]
+
+
+def sincos_2pi(u: float):
+    s, c = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib().or_sincos_2pi(u, _p(s), _p(c))
+    return float(s[0]), float(c[0])
+
+
+def materials_array(mats) -> np.ndarray:
+    """[(kind, albedo3, emission3, ior), ...] -> structured array (dr_material layout)."""
+    a = np.zeros(len(mats), MATERIAL_DTYPE)
+    for i, (k, alb, em, ior) in enumerate(mats):
+        a[i] = (k, alb, em, ior)
+    return a
+
+
+def lights_build(vertices, indices, tri_mat, mats):
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(indices, np.int32)
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    m = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+    ent = np.zeros(max(1, f.shape[0]), ALIAS_DTYPE)
+    pdf = np.zeros(f.shape[0], np.float32)
+    ne = lib().or_lights_build(_p(v), _p(f), f.shape[0], _p(tm), _p(m), _p(ent), _p(pdf))
+    return ent[:ne].copy(), pdf
+
+
+def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int = 8, rr_depth: int = 3,
+           seed: int = 0, fresnel: int = 0, sample_offset: int = 0, pixels=None, accum=None,
+           vertex_hist: bool = False, nthreads: int | None = None):
+    """Build-semantics path tracer; returns (accum[h, w, 4] float64, hist)."""
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    m = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+    ent, pdf = lights_build(bvh.v, bvh.f, tm, m)
+    npix = camera.width * camera.height
+    acc = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64).reshape(npix, 4)
+    prm = RenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, 0)
+    lo, hi = (0, npix) if pixels is None else pixels
+    hist = np.zeros(max_depth, np.int32) if vertex_hist else None
+    lib().or_render(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(m), _p(ent) if len(ent) else None, _p(pdf), len(ent),
+                    C.byref(camera), C.byref(prm), lo, hi, _p(acc), _p(hist), nthreads or threads())
+    return acc.reshape(camera.height, camera.width, 4), hist
 
 _lib = None
 

@@ -477,83 +477,94 @@ typedef struct {
     int32_t *ntest, *ttest;
 } trace_ctx;
 
+/* Front-to-back traversal of one ray; counts AABB tests (nt) and triangle
+ * tests (tt). */
+static void trace_ray(const obvh *b, const float *verts, const int32_t *tris, const or_ray *ray, int any,
+                      or_hit *hit_out, uint32_t *occ_out, int32_t *nt_out, int32_t *tt_out)
+{
+    typedef struct { int32_t node; float t; } entry;
+    entry stack[128];
+    ray_pre p;
+    ray_prepare(ray, &p);
+    float tbest = p.tmax;
+    int32_t best = -1;
+    float bu = 0, bv = 0;
+    int32_t nt = 0, tt = 0;
+    int occluded = 0;
+    int sp = 0;
+    if (b->n_nodes > 0) {
+        float te;
+        ++nt;
+        if (slab(b->nodes[0].lo, b->nodes[0].hi, &p, p.tmin, p.tmax * TBEST_SLACK, &te)) {
+            stack[sp].node = 0; stack[sp].t = te; ++sp;
+        }
+    }
+    while (sp > 0) {
+        --sp;
+        const int32_t ni = stack[sp].node;
+        if (stack[sp].t > tbest * TBEST_SLACK) continue;
+        const bnode *nd = &b->nodes[ni];
+        if (nd->count > 0) {
+            for (int32_t j = nd->first; j < nd->first + nd->count; ++j) {
+                const int32_t id = b->prim[j];
+                const int32_t *tri = tris + 3 * (size_t)id;
+                float t, u, v;
+                ++tt;
+                if (!tri_wt(&p, verts + 3 * (size_t)tri[0], verts + 3 * (size_t)tri[1],
+                            verts + 3 * (size_t)tri[2], &t, &u, &v)) continue;
+                if (any) {
+                    if (t >= p.tmin && t <= p.tmax) { occluded = 1; break; }
+                } else if (accept_hit(t, id, p.tmin, tbest, best)) {
+                    tbest = t; best = id; bu = u; bv = v;
+                }
+            }
+            if (occluded) break;
+            continue;
+        }
+        float t0, t1;
+        const float lim = (any ? p.tmax : tbest) * TBEST_SLACK;
+        nt += 2;
+        const int h0 = slab(b->nodes[nd->first].lo, b->nodes[nd->first].hi, &p, p.tmin, lim, &t0);
+        const int h1 = slab(b->nodes[nd->first + 1].lo, b->nodes[nd->first + 1].hi, &p, p.tmin, lim, &t1);
+        if (h0 && h1) {
+            /* push far first so that the near child is popped next */
+            if (t1 < t0) {
+                stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
+                stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
+            } else {
+                stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
+                stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
+            }
+        } else if (h0) {
+            stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
+        } else if (h1) {
+            stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
+        }
+    }
+    if (any) {
+        if (occ_out) *occ_out = (uint32_t)occluded;
+    } else if (hit_out) {
+        hit_out->id = best;
+        hit_out->t = best >= 0 ? tbest : INFINITY;
+        hit_out->u = best >= 0 ? bu : 0.0f;
+        hit_out->v = best >= 0 ? bv : 0.0f;
+    }
+    if (nt_out) *nt_out = nt;
+    if (tt_out) *tt_out = tt;
+}
+
+void or_trace_one(const void *bvh, const float *verts, const int32_t *tris, const or_ray *ray, int any,
+                  or_hit *hit, uint32_t *occluded)
+{
+    trace_ray((const obvh *)bvh, verts, tris, ray, any, hit, occluded, NULL, NULL);
+}
+
 static void trace_range(void *vc, int32_t begin, int32_t end)
 {
     trace_ctx *c = (trace_ctx *)vc;
-    const obvh *b = c->b;
-    typedef struct { int32_t node; float t; } entry;
-    entry stack[128];
-    for (int32_t i = begin; i < end; ++i) {
-        ray_pre p;
-        ray_prepare(&c->rays[i], &p);
-        float tbest = p.tmax;
-        int32_t best = -1;
-        float bu = 0, bv = 0;
-        int32_t nt = 0, tt = 0;
-        int occluded = 0;
-        int sp = 0;
-        if (b->n_nodes > 0) {
-            float te;
-            ++nt;
-            if (slab(b->nodes[0].lo, b->nodes[0].hi, &p, p.tmin, p.tmax * TBEST_SLACK, &te)) {
-                stack[sp].node = 0; stack[sp].t = te; ++sp;
-            }
-        }
-        while (sp > 0) {
-            --sp;
-            const int32_t ni = stack[sp].node;
-            if (stack[sp].t > tbest * TBEST_SLACK) continue;
-            const bnode *nd = &b->nodes[ni];
-            if (nd->count > 0) {
-                for (int32_t j = nd->first; j < nd->first + nd->count; ++j) {
-                    const int32_t id = b->prim[j];
-                    const int32_t *tri = c->tris + 3 * (size_t)id;
-                    float t, u, v;
-                    ++tt;
-                    if (!tri_wt(&p, c->verts + 3 * (size_t)tri[0], c->verts + 3 * (size_t)tri[1],
-                                c->verts + 3 * (size_t)tri[2], &t, &u, &v)) continue;
-                    if (c->any) {
-                        if (t >= p.tmin && t <= p.tmax) { occluded = 1; break; }
-                    } else if (accept_hit(t, id, p.tmin, tbest, best)) {
-                        tbest = t; best = id; bu = u; bv = v;
-                    }
-                }
-                if (occluded) break;
-                continue;
-            }
-            float t0, t1;
-            const float lim = (c->any ? p.tmax : tbest) * TBEST_SLACK;
-            nt += 2;
-            const int h0 = slab(b->nodes[nd->first].lo, b->nodes[nd->first].hi, &p, p.tmin, lim, &t0);
-            const int h1 = slab(b->nodes[nd->first + 1].lo, b->nodes[nd->first + 1].hi, &p, p.tmin, lim, &t1);
-            if (h0 && h1) {
-                /* push far first so that the near child is popped next */
-                if (t1 < t0) {
-                    stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
-                    stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
-                } else {
-                    stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
-                    stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
-                }
-            } else if (h0) {
-                stack[sp].node = nd->first; stack[sp].t = t0; ++sp;
-            } else if (h1) {
-                stack[sp].node = nd->first + 1; stack[sp].t = t1; ++sp;
-            }
-        }
-        if (c->any) {
-            c->occ[i] = (uint32_t)occluded;
-        } else {
-            or_hit h;
-            h.id = best;
-            h.t = best >= 0 ? tbest : INFINITY;
-            h.u = best >= 0 ? bu : 0.0f;
-            h.v = best >= 0 ? bv : 0.0f;
-            c->hits[i] = h;
-        }
-        if (c->ntest) c->ntest[i] = nt;
-        if (c->ttest) c->ttest[i] = tt;
-    }
+    for (int32_t i = begin; i < end; ++i)
+        trace_ray(c->b, c->verts, c->tris, &c->rays[i], c->any, c->hits ? &c->hits[i] : NULL,
+                  c->occ ? &c->occ[i] : NULL, c->ntest ? &c->ntest[i] : NULL, c->ttest ? &c->ttest[i] : NULL);
 }
 
 void or_trace_bvh(const void *bvh, const float *verts, const int32_t *tris, const or_ray *rays,

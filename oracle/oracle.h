@@ -96,6 +96,40 @@ void or_refbbh_trace(const void *bbh, const float *verts, const int32_t *tris, c
                      int32_t n, or_hit *hits, int64_t *box_tests, int64_t *tri_tests,
                      int32_t threads);
 
+/* ---- light sampler and path tracer (build semantics) --------------------- */
+typedef struct { int32_t kind; float albedo[3]; float emission[3]; float ior; } or_material;  /* = dr_material */
+typedef struct { float p_a, pdf_a, pdf_b; int32_t a, b; } or_alias_entry;                   /* 20 B */
+typedef struct {
+    float transform[12];
+    float focal_length;
+    float center[2];
+    float pixel_size[2];
+    uint32_t width, height, pixel_sampling;
+} or_camera;                                                                               /* = dr_camera */
+typedef struct {
+    uint32_t spp, sample_offset, max_depth, rr_depth;
+    uint64_t seed;
+    uint32_t fresnel, flags;
+} or_render_params;                                                                        /* = dr_render_params */
+
+/* Alias table over emissive triangles, weights area x luminance(E) (FP64),
+ * built by a sequential sweep in exact integer units. entries: one per
+ * emitter (index order); pdf_area: per triangle (0 for non-emitters).
+ * Returns the emitter count. */
+int32_t or_lights_build(const float *verts, const int32_t *tris, int32_t ntris, const int32_t *tri_mat,
+                        const or_material *mats, or_alias_entry *entries, float *pdf_area);
+
+/* sin and cos of 2 pi u (the build's own polynomial). */
+void or_sincos_2pi(float u, float *s, float *c);
+
+/* Path-trace pixels [pixel_begin, pixel_end) adding spp samples each to
+ * accum (4 doubles per pixel, in sample order). vertex_hist (optional,
+ * max_depth ints) counts paths reaching each vertex. */
+void or_render(const void *bvh, const float *verts, const int32_t *tris, const int32_t *tri_mat,
+               const or_material *mats, const or_alias_entry *alias, const float *pdf_area, int32_t n_emit,
+               const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
+               double *accum, int32_t *vertex_hist, int32_t threads);
+
 #ifdef __cplusplus
 }
 #endif
