@@ -51,7 +51,7 @@ class DrBvhParams(C.Structure):
 class DrSceneInfo(C.Structure):
     _fields_ = [
         ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
-        ("depth", C.c_uint32), ("overflow_flag", C.c_uint32),
+        ("depth", C.c_uint32), ("overflow_flag", C.c_uint32), ("simd_stats", C.c_uint32 * 4),
         ("build_ms", C.c_double), ("sah_cost", C.c_double),
     ]
 
@@ -98,7 +98,7 @@ PROTOTYPES = {
     "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "dr_scene_set_trace_schedule": (_st, [_vp, _u32, _u32]),
-    "dr_scene_set_trace_refill": (_st, [_vp, _u32]),
+    "dr_scene_set_trace_tuning": (_st, [_vp, _u32, _u32, _u32]),
     "dr_scene_build_lights": (_st, [_vp, _vp]),
     "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
     "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),

@@ -221,9 +221,10 @@ dr_status dr_scene_get_info(dr_scene *sc, dr_scene_info *info)
     info->build_ms = s->build_ms;
     info->sah_cost = s->sah_cost;
     if (s->flags.ptr) {
-        uint32_t f[4] = {0, 0, 0, 0};
-        DR_HIP_TRY(hipMemcpy(f, s->flags.ptr, 16, hipMemcpyDeviceToHost));
+        uint32_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        DR_HIP_TRY(hipMemcpy(f, s->flags.ptr, 32, hipMemcpyDeviceToHost));
         info->overflow_flag = f[0];
+        for (int k = 0; k < 4; ++k) info->simd_stats[k] = f[2 + k];
     }
     return DR_OK;
 }
@@ -246,11 +247,15 @@ dr_status dr_scene_set_trace_schedule(dr_scene *sc, uint32_t closest_sched, uint
     return DR_OK;
 }
 
-dr_status dr_scene_set_trace_refill(dr_scene *sc, uint32_t refill_min)
+dr_status dr_scene_set_trace_tuning(dr_scene *sc, uint32_t refill_min, uint32_t blocks_per_cu, uint32_t sort_key)
 {
     if (!sc) return fail(DR_E_INVALID, "scene is null");
     if (refill_min < 1 || refill_min > 64) return fail(DR_E_INVALID, "refill_min must be 1..64");
-    reinterpret_cast<Scene *>(sc)->refill_min = refill_min;
+    if (sort_key > 1) return fail(DR_E_INVALID, "sort_key must be 0 or 1");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    s->refill_min = refill_min;
+    s->blocks_per_cu = blocks_per_cu;
+    s->sort_key = sort_key;
     return DR_OK;
 }
 

@@ -86,6 +86,8 @@ struct Scene {
     uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
     uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;
     uint32_t refill_min = 16;  // persistent waves refill when this many lanes are idle
+    uint32_t blocks_per_cu = 0;  // persistent grid: resident blocks per CU (0 = occupancy limit)
+    uint32_t sort_key = 0;       // ray reorder key: 0 origin, 1 direction + origin
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (lo, hi), set by the BVH build
     DevBuf ray_keys, ray_perm;             // ray reordering scratch
     // light sampler (light.hip)

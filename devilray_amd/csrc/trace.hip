@@ -49,7 +49,15 @@ struct LeafState {
     uint32_t bid;
     bool occluded;
     uint32_t ntri;
+    uint32_t wave_tri_iters;  // counted variant: triangle-loop iterations issued while this lane led the wave
 };
+
+// Counted variant only: true on one active lane per wave, so a lane-local sum
+// of these events over the wave counts wave-level loop iterations (SIMD slots).
+__device__ __forceinline__ bool leads_wave()
+{
+    return (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1) == (threadIdx.x & 63u);
+}
 
 template <bool ANY, bool COUNT, int KZ>
 __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v, int32_t leaf, const RayPre &p,
@@ -60,7 +68,10 @@ __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v,
         const float4 *tv = tri_v + 3 * (size_t)(first + k);
         const float4 a = tv[0], b = tv[1], c = tv[2];
         float t, u, v;
-        if (COUNT) ++st.ntri;
+        if (COUNT) {
+            ++st.ntri;
+            if (leads_wave()) ++st.wave_tri_iters;
+        }
         if (!tri_watertight<KZ>(p, a, b, c, t, u, v)) continue;
         const uint32_t id = (uint32_t)__float_as_int(a.w);
         if (ANY) {
@@ -94,7 +105,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
     int32_t ray = -1;
     bool exhausted = false;
     RayPre p{};
-    LeafState st{0.0f, 0.0f, 0.0f, 0xFFFFFFFFu, false, 0u};
+    LeafState st{0.0f, 0.0f, 0.0f, 0xFFFFFFFFu, false, 0u, 0u};
+    uint32_t wave_node_iters = 0, node_visits = 0, tri_total = 0;  // counted variant
+    uint32_t tri_cur = 0, tri_end = 0;  // single-step schedule: triangles of the current leaf
     int sp = 0;
     int32_t node = kNone, leaf = kNone;
     uint32_t nbox = 0;
@@ -139,8 +152,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     const float4 r1 = rays[2 * (size_t)r + 1];
                     ray_prepare(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, p);
                     st.bt = p.tmax; st.bu = 0.0f; st.bv = 0.0f; st.bid = 0xFFFFFFFFu;
-                    st.occluded = false; st.ntri = 0;
+                    st.occluded = false;
+                    tri_total += st.ntri;
+                    st.ntri = 0;
                     sp = 0; leaf = kNone; nbox = 0;
+                    tri_cur = 0; tri_end = 0;
                     // a NaN anywhere in the ray hits nothing (the triangle test
                     // rejects it too), and NaN slabs would admit empty slots
                     const bool nan_ray = (r0.x != r0.x) || (r0.y != r0.y) || (r0.z != r0.z) || (r0.w != r0.w) ||
@@ -159,8 +175,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
 
         if (ray >= 0) {
             // ---- interior nodes, leaves postponed
-            while (node >= 0) {
-                const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(WideNode);
+            // one interior node: returns the next reference (nearest hit child
+            // or the top of the stack) after pushing the other hit children
+            auto visit = [&](int32_t cur) -> int32_t {
+                if (COUNT) {
+                    ++node_visits;
+                    if (leads_wave()) ++wave_node_iters;
+                }
+                const uint32_t nb = (uint32_t)cur * (uint32_t)sizeof(WideNode);
                 const float4 nxv = *reinterpret_cast<const float4 *>(nbase + nb + nx);
                 const float4 fxv = *reinterpret_cast<const float4 *>(nbase + nb + (nx ^ 16u));
                 const float4 nyv = *reinterpret_cast<const float4 *>(nbase + nb + ny);
@@ -195,39 +217,65 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                 if (tn[3] != INFINITY) push(r[3]);
                 if (tn[2] != INFINITY) push(r[2]);
                 if (tn[1] != INFINITY) push(r[1]);
-                node = (tn[0] != INFINITY) ? r[0] : pop();
-                if (is_leaf(node) && leaf == kNone) {
-                    leaf = node;
-                    node = pop();
-                }
-                if (SPECULATIVE) {
+                return (tn[0] != INFINITY) ? r[0] : pop();
+            };
+            if (SPECULATIVE) {
+                // ---- interior nodes, leaves postponed
+                while (node >= 0) {
+                    node = visit(node);
+                    if (is_leaf(node) && leaf == kNone) {
+                        leaf = node;
+                        node = pop();
+                    }
                     if (__ballot(leaf == kNone) == 0ull) break;  // every active lane holds a leaf
-                } else if (leaf != kNone) {
-                    break;
                 }
-            }
-            // ---- leaves; a wave whose lanes share the dominant axis runs the
-            // specialised triangle test
-            const int kz0 = __builtin_amdgcn_readfirstlane(p.kz);
-            const bool uniform = __ballot(p.kz != kz0) == 0ull;
-            while (is_leaf(leaf)) {
-                if (uniform) {
-                    if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, leaf, p, st);
-                    else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, leaf, p, st);
-                    else intersect_leaf<ANY, COUNT, 2>(tri_v, leaf, p, st);
-                } else {
-                    intersect_leaf<ANY, COUNT, -1>(tri_v, leaf, p, st);
+                // ---- leaves; a wave whose lanes share the dominant axis runs
+                // the specialised triangle test
+                const int kz0 = __builtin_amdgcn_readfirstlane(p.kz);
+                const bool uniform = __ballot(p.kz != kz0) == 0ull;
+                while (is_leaf(leaf)) {
+                    if (uniform) {
+                        if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, leaf, p, st);
+                        else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, leaf, p, st);
+                        else intersect_leaf<ANY, COUNT, 2>(tri_v, leaf, p, st);
+                    } else {
+                        intersect_leaf<ANY, COUNT, -1>(tri_v, leaf, p, st);
+                    }
+                    if (ANY && st.occluded) { leaf = kNone; node = kNone; sp = 0; break; }
+                    leaf = kNone;
+                    if (is_leaf(node)) {
+                        leaf = node;
+                        node = pop();
+                    }
                 }
-                if (ANY && st.occluded) { leaf = kNone; node = kNone; sp = 0; break; }
-                leaf = kNone;
-                if (is_leaf(node)) {
-                    leaf = node;
+            } else {
+                // ---- single step: one triangle test or one node visit per
+                // iteration, so lanes in either phase make progress together
+                if (tri_cur < tri_end) {
+                    const int kz0 = __builtin_amdgcn_readfirstlane(p.kz);
+                    const bool uniform = __ballot(p.kz != kz0) == 0ull;
+                    const int32_t one = make_leaf_ref(tri_cur, 1u);
+                    if (uniform) {
+                        if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, one, p, st);
+                        else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, one, p, st);
+                        else intersect_leaf<ANY, COUNT, 2>(tri_v, one, p, st);
+                    } else {
+                        intersect_leaf<ANY, COUNT, -1>(tri_v, one, p, st);
+                    }
+                    ++tri_cur;
+                    if (ANY && st.occluded) { tri_cur = tri_end; node = kNone; sp = 0; }
+                } else if (node >= 0) {
+                    node = visit(node);
+                }
+                if (tri_cur >= tri_end && is_leaf(node)) {
+                    tri_cur = leaf_first(node);
+                    tri_end = tri_cur + leaf_count(node);
                     node = pop();
                 }
             }
-            if (overflow) { node = kNone; leaf = kNone; sp = 0; }
+            if (overflow) { node = kNone; leaf = kNone; sp = 0; tri_cur = tri_end; }
             // ---- finished: write the result, free the lane
-            if (node == kNone && leaf == kNone) {
+            if (node == kNone && leaf == kNone && tri_cur >= tri_end) {
                 const uint32_t rr = (uint32_t)ray;
                 if (ANY) {
                     occ[rr] = st.occluded ? 1u : 0u;
@@ -246,6 +294,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
         }
     }
     if (overflow) atomicOr(&flags[0], 1u);
+    if (COUNT) {
+        // SIMD utilisation statistics: [2] wave-level node-loop iterations,
+        // [3] node visits, [4] wave-level triangle-loop iterations, [5] tests
+        atomicAdd(&flags[2], wave_node_iters);
+        atomicAdd(&flags[3], node_visits);
+        atomicAdd(&flags[4], st.wave_tri_iters);
+        atomicAdd(&flags[5], tri_total + st.ntri);
+    }
 }
 
 __global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
@@ -260,8 +316,19 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v)
 }
 
 // key = dominant axis (2 bits) | Morton code of the origin (30 bits)
+__device__ __forceinline__ uint32_t spread2(uint32_t v)  // 16 bits -> even bits
+{
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// key mode 0: dominant axis (2 bits) | origin Morton code (30 bits)
+// key mode 1: dominant axis | direction (octahedral map, 3+3 bits) | origin Morton (24 bits)
 __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ rays, uint32_t n, float lx, float ly,
-                                                     float lz, float sx, float sy, float sz,
+                                                     float lz, float sx, float sy, float sz, uint32_t mode,
                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ idx)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -274,8 +341,25 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
     const float qx = fminf(fmaxf((o.x - lx) * sx, 0.0f), 1023.0f);
     const float qy = fminf(fmaxf((o.y - ly) * sy, 0.0f), 1023.0f);
     const float qz = fminf(fmaxf((o.z - lz) * sz, 0.0f), 1023.0f);
-    const uint32_t code = (expand10((uint32_t)qx) << 2) | (expand10((uint32_t)qy) << 1) | expand10((uint32_t)qz);
-    keys[i] = (kz << 30) | code;
+    if (mode == 0) {
+        const uint32_t code = (expand10((uint32_t)qx) << 2) | (expand10((uint32_t)qy) << 1) | expand10((uint32_t)qz);
+        keys[i] = (kz << 30) | code;
+    } else {
+        const float l1 = ax + ay + az;
+        float u = l1 > 0.0f ? d.x / l1 : 0.0f, v = l1 > 0.0f ? d.y / l1 : 0.0f;
+        if (d.z < 0.0f) {
+            const float uu = (1.0f - fabsf(v)) * (u < 0.0f ? -1.0f : 1.0f);
+            const float vv = (1.0f - fabsf(u)) * (v < 0.0f ? -1.0f : 1.0f);
+            u = uu;
+            v = vv;
+        }
+        const uint32_t du = (uint32_t)fminf(fmaxf((u + 1.0f) * 4.0f, 0.0f), 7.0f);
+        const uint32_t dv = (uint32_t)fminf(fmaxf((v + 1.0f) * 4.0f, 0.0f), 7.0f);
+        const uint32_t dir = (spread2(du) << 1) | spread2(dv);
+        const uint32_t code = (expand10((uint32_t)qx >> 2) << 2) | (expand10((uint32_t)qy >> 2) << 1) |
+                              expand10((uint32_t)qz >> 2);
+        keys[i] = (kz << 30) | (dir << 24) | code;
+    }
     idx[i] = i;
 }
 
@@ -323,7 +407,8 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
         occ_cache[variant] = bpc > 0 ? bpc : 1;
     }
     const uint32_t need = (n + kBlock - 1) / kBlock;
-    const uint32_t resident = (uint32_t)(s->ctx->cu_count * occ_cache[variant]);
+    const int bpc = s->blocks_per_cu ? std::min<int>((int)s->blocks_per_cu, occ_cache[variant]) : occ_cache[variant];
+    const uint32_t resident = (uint32_t)(s->ctx->cu_count * bpc);
     const uint32_t grid = persistent ? (need < resident ? need : resident) : need;
     const size_t nlanes = (size_t)grid * kBlock;
     if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * nlanes)) return st;
@@ -335,7 +420,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
         const float *lo = s->bounds, *hi = s->bounds + 3;
         float sc[3];
         for (int a = 0; a < 3; ++a) sc[a] = hi[a] > lo[a] ? 1024.0f / (hi[a] - lo[a]) : 0.0f;
-        k_ray_keys<<<need, kBlock, 0, stream>>>(r4, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2],
+        k_ray_keys<<<need, kBlock, 0, stream>>>(r4, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2], s->sort_key,
                                                 s->ray_keys.as<uint32_t>(), s->ray_perm.as<uint32_t>());
         DR_HIP_TRY(hipGetLastError());
         if (dr_status st = radix_sort_pairs(s->ctx, s->ray_keys.as<uint32_t>(), s->ray_perm.as<uint32_t>(), n, 32,
@@ -347,6 +432,8 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     uint32_t *flags = s->flags.as<uint32_t>();
     uint32_t *counter = flags + 16 + (s->launch_seq++ % kCounterSlots);
     if (persistent) k_zero_counter<<<1, 1, 0, stream>>>(counter);
+    if (count)
+        for (int k = 2; k <= 5; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
                                       box_tests, tri_tests, ovf, flags, counter, s->refill_min);

@@ -85,6 +85,10 @@ typedef struct {
     uint32_t n_leaves;
     uint32_t depth;          /* wide levels */
     uint32_t overflow_flag;  /* non-zero if a traversal overflowed its stack */
+    /* last counted traversal (dr_trace_closest_counted): wave-level node-loop
+     * iterations, node visits, wave-level triangle iterations, triangle tests;
+     * visits / (64 * iterations) is the SIMD utilisation of each loop */
+    uint32_t simd_stats[4];
     double build_ms;         /* device time of the last build */
     double sah_cost;         /* SAH cost of the binary tree before collapse */
 } dr_scene_info;
@@ -139,9 +143,12 @@ enum {
     DR_SCHED_REORDER = 4
 };
 dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
-/* Persistent schedules: a wave refills its finished lanes once at least
- * refill_min (1..64) of them are idle. */
-dr_status dr_scene_set_trace_refill(dr_scene *scene, uint32_t refill_min);
+/* Tuning knobs (chosen by the autotuner): persistent waves refill their
+ * finished lanes once at least refill_min (1..64) are idle; blocks_per_cu
+ * caps the resident grid (0 = occupancy limit); sort_key selects the reorder
+ * key (0: dominant axis + origin Morton code, 1: dominant axis + quantised
+ * direction + origin). */
+dr_status dr_scene_set_trace_tuning(dr_scene *scene, uint32_t refill_min, uint32_t blocks_per_cu, uint32_t sort_key);
 
 /* Closest hit with the reference's HitTests counters
  * (include/tracing/Benchmark.hpp:5-14, IntersectionTestsImpl.hpp:228-231):

@@ -30,6 +30,7 @@ PROTOTYPES = {
     "or_bvh_build": (_vp, [_vp, _i32, _vp, _i32, _i32, _i32]),
     "or_bvh_free": (None, [_vp]),
     "or_bvh_node_count": (_i32, [_vp]),
+    "or_bvh_sah_cost": (C.c_double, [_vp]),
     "or_bvh_validate": (_i32, [_vp, _vp, _vp]),
     "or_trace_bvh": (None, [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _i32]),
     "or_trace_brute": (None, [_vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32]),
@@ -182,6 +183,9 @@ class BVH:
 
     def node_count(self) -> int:
         return lib().or_bvh_node_count(self.h)
+
+    def sah_cost(self) -> float:
+        return lib().or_bvh_sah_cost(self.h)
 
     def validate(self) -> int:
         return lib().or_bvh_validate(self.h, _p(self.v), _p(self.f))

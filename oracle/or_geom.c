@@ -405,6 +405,27 @@ void or_bvh_free(void *h)
 
 int32_t or_bvh_node_count(const void *h) { return ((const obvh *)h)->n_nodes; }
 
+/* SAH cost with C_trav = C_isect = 1, normalised by the root's area:
+ * sum over interior nodes of A(n)/A(root) + sum over leaves of N A(l)/A(root). */
+double or_bvh_sah_cost(const void *h)
+{
+    const obvh *b = (const obvh *)h;
+    if (b->n_nodes == 0) return 0.0;
+    box3 root;
+    memcpy(root.lo, b->nodes[0].lo, sizeof(root.lo));
+    memcpy(root.hi, b->nodes[0].hi, sizeof(root.hi));
+    const double ra = box_half_area(&root);
+    double cost = 0.0;
+    for (int32_t i = 0; i < b->n_nodes; ++i) {
+        box3 bx;
+        memcpy(bx.lo, b->nodes[i].lo, sizeof(bx.lo));
+        memcpy(bx.hi, b->nodes[i].hi, sizeof(bx.hi));
+        const double a = box_half_area(&bx) / ra;
+        cost += b->nodes[i].count > 0 ? a * b->nodes[i].count : a;
+    }
+    return cost;
+}
+
 int32_t or_bvh_validate(const void *h, const float *verts, const int32_t *tris)
 {
     const obvh *b = (const obvh *)h;

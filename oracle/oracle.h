@@ -64,6 +64,8 @@ void *or_bvh_build(const float *verts, int32_t nverts, const int32_t *tris, int3
                    int32_t max_leaf, int32_t bins);
 void or_bvh_free(void *bvh);
 int32_t or_bvh_node_count(const void *bvh);
+/* SAH cost (C_trav = C_isect = 1) normalised by the root box area. */
+double or_bvh_sah_cost(const void *bvh);
 /* Checks structural invariants (leaves tile [0,n), child boxes inside parent,
  * triangles inside leaf boxes). Returns 0 when all hold. */
 int32_t or_bvh_validate(const void *bvh, const float *verts, const int32_t *tris);
