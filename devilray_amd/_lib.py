@@ -89,6 +89,7 @@ PROTOTYPES = {
     "dr_device_free": (_st, [_vp, _vp]),
     "dr_memcpy_htod": (_st, [_vp, _vp, _vp, C.c_size_t]),
     "dr_memcpy_dtoh": (_st, [_vp, _vp, _vp, C.c_size_t]),
+    "dr_memcpy_dtod": (_st, [_vp, _vp, _vp, C.c_size_t, _vp]),
     "dr_synchronize": (_st, [_vp]),
     "dr_scene_create": (_st, [_vp, C.POINTER(DrSceneDesc), C.POINTER(_vp)]),
     "dr_scene_destroy": (_st, [_vp]),

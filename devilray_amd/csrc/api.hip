@@ -134,6 +134,14 @@ dr_status dr_memcpy_dtoh(dr_context *c, void *dst, const void *src, size_t bytes
     return DR_OK;
 }
 
+dr_status dr_memcpy_dtod(dr_context *c, void *dst, const void *src, size_t bytes, dr_stream stream)
+{
+    if (!c || (!dst && bytes) || (!src && bytes)) return fail(DR_E_INVALID, "null argument");
+    if (bytes)
+        DR_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream)));
+    return DR_OK;
+}
+
 dr_status dr_synchronize(dr_context *c)
 {
     if (!c) return fail(DR_E_INVALID, "context is null");

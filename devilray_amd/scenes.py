@@ -93,6 +93,92 @@ def cornell_c0() -> SceneData:
                         width=256, height=256))
 
 
+def _random_rotations(rng, n):
+    """Uniform random rotation matrices (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], 1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1)], axis=1)
+
+
+def many_lights(n_spheres: int = 400, subdiv: int = 4, n_emissive: int = 100_000, seed: int = 0,
+                extent: float = 20.0, partition: bool = False) -> SceneData:
+    """The many-lights scene: n_spheres displaced icospheres (radius
+    1 + 0.05 N(0,1) per vertex; 20*4^subdiv triangles each, 2,048,000 in
+    total at the defaults) with random positions in [-extent, extent]^3,
+    log-uniform scales in [0.5, 2.5] and uniform random rotations. 30% are
+    dielectric with IOR ~ U[1.3, 1.7], the rest Lambert with albedo ~
+    U[0.2, 0.9]. n_emissive triangles of Lambert spheres emit radiance
+    LogNormal(0, 2) (grey). With partition=True all emitters sit behind an
+    opaque wall (z = -extent - 4 .. ) with a gap, so that nearly all light
+    reaching the camera side is indirect (the guided-sampling scene).
+    Camera: 1920x1080, 45 degree FOV, looking at the origin from +z."""
+    rng = np.random.default_rng(seed)
+    base_v, base_f = gen_icosphere(subdiv, 0.0, 0)
+    unit = base_v.astype(np.float64) / np.linalg.norm(base_v.astype(np.float64), axis=1, keepdims=True)
+    nv, nt = unit.shape[0], base_f.shape[0]
+    rot = _random_rotations(rng, n_spheres)
+    pos = rng.uniform(-extent, extent, size=(n_spheres, 3))
+    scale = np.exp(rng.uniform(np.log(0.5), np.log(2.5), n_spheres))
+    dielectric = rng.random(n_spheres) < 0.3
+    ior = rng.uniform(1.3, 1.7, n_spheres)
+    albedo = rng.uniform(0.2, 0.9, n_spheres)
+    if partition:
+        # emitters live in a slab behind the wall
+        pos[:, 2] = np.where(np.arange(n_spheres) % 4 == 0, rng.uniform(-extent - 14, -extent - 8, n_spheres),
+                             pos[:, 2])
+    verts = np.empty((n_spheres * nv, 3), np.float32)
+    faces = np.empty((n_spheres * nt, 3), np.uint32)
+    tri_mat = np.empty(n_spheres * nt, np.uint32)
+    mats = []
+    for s in range(n_spheres):
+        r = 1.0 + 0.05 * rng.standard_normal(nv)
+        v = (unit * r[:, None]) @ rot[s].T * scale[s] + pos[s]
+        verts[s * nv:(s + 1) * nv] = v
+        faces[s * nt:(s + 1) * nt] = base_f + s * nv
+        if dielectric[s]:
+            mats.append((DR_MAT_DIELECTRIC, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), float(ior[s])))
+        else:
+            a = float(albedo[s])
+            mats.append((DR_MAT_LAMBERT, (a, a, a), (0.0, 0.0, 0.0), 1.0))
+        tri_mat[s * nt:(s + 1) * nt] = s
+    # emissive triangles: drawn from Lambert spheres (behind the wall when partitioned)
+    eligible = np.flatnonzero(~dielectric[tri_mat])
+    if partition:
+        behind = pos[tri_mat[eligible], 2] < -extent - 4
+        eligible = eligible[behind]
+    chosen = np.sort(rng.choice(eligible, size=min(n_emissive, eligible.size), replace=False))
+    radiance = rng.lognormal(0.0, 2.0, chosen.size)
+    first = len(mats)
+    for k, t in enumerate(chosen):
+        a = mats[tri_mat[t]][1]
+        mats.append((DR_MAT_LAMBERT, a, (float(radiance[k]),) * 3, 1.0))
+        tri_mat[t] = first + k
+    b_v, b_f, b_m = [verts], [faces], [tri_mat]
+    if partition:
+        # opaque wall at z = -extent - 4 with a 4-unit gap along the top
+        wall = _Builder()
+        wall.nv = verts.shape[0]
+        wm = len(mats)
+        mats.append((DR_MAT_LAMBERT, (0.5, 0.5, 0.5), (0.0, 0.0, 0.0), 1.0))
+        z = -extent - 4.0
+        h = 2 * extent + 20
+        wall.quad((0, -4.0, z), (0, 0, 1), (1, 0, 0), h, h - 8.0, wm)
+        wall.quad((0, -4.0, z - 0.1), (0, 0, -1), (1, 0, 0), h, h - 8.0, wm)
+        for wv, wf, wmat in zip(wall.v, wall.f, wall.m):
+            b_v.append(wv)
+            b_f.append(wf)
+            b_m.append(wmat)
+    cam = dict(eye=(0.0, 0.0, 3.2 * extent), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
+               width=1920, height=1080)
+    return SceneData(np.concatenate(b_v).astype(np.float32), np.concatenate(b_f).astype(np.uint32),
+                     np.concatenate(b_m).astype(np.uint32), mats, cam)
+
+
 def furnace_planes(emittance: float = 0.55) -> SceneData:
     """The reference's analytic diffuse-reflection scene (test/src/test_render.cpp
     :50-165): a white (albedo 1) receiver quad at z = 1 facing the camera and an

@@ -105,6 +105,9 @@ dr_status dr_device_alloc(dr_context *ctx, size_t bytes, void **out);
 dr_status dr_device_free(dr_context *ctx, void *ptr);
 dr_status dr_memcpy_htod(dr_context *ctx, void *dst, const void *src, size_t bytes);
 dr_status dr_memcpy_dtoh(dr_context *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy, asynchronous on `stream` (also the known-size
+ * kernel against which memory-traffic counters are calibrated). */
+dr_status dr_memcpy_dtod(dr_context *ctx, void *dst, const void *src, size_t bytes, dr_stream stream);
 dr_status dr_synchronize(dr_context *ctx);
 
 /* Scene upload. Replaces convertMeshToTris / viewGpuTris
