@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    ALIAS_DTYPE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera,
+    ALIAS_DTYPE, DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, GUIDE_HEAD_FLOATS,
+    GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
 )
 
@@ -221,6 +222,29 @@ class Scene:
         self.ctx.synchronize()
         return d.to_numpy(np.float64, npix * 4).reshape(camera.height, camera.width, 4)
 
+    def render_guided_dev(self, camera: DrCamera, params: DrRenderParams, guide: "Guide", flags: int,
+                          accum_ptr: int, stream=None):
+        check(lib().dr_render_guided(self.handle, C.byref(camera), C.byref(params),
+                                     guide.handle if guide is not None else None, flags, C.c_void_p(accum_ptr),
+                                     stream))
+
+    def render_guided(self, camera: DrCamera, guide: "Guide", flags: int, spp: int, max_depth: int = 8,
+                      rr_depth: int = 3, seed: int = 0, fresnel: int = 0, sample_offset: int = 0,
+                      accum: np.ndarray | None = None) -> np.ndarray:
+        """render() with the learned guide (flags: DR_GUIDE_USE | DR_GUIDE_COLLECT)."""
+        npix = camera.width * camera.height
+        d = self.ctx.alloc(npix * 32)
+        init = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64)
+        d.copy_from(init)
+        p = DrRenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, 0)
+        self.render_guided_dev(camera, p, guide, flags, d.ptr)
+        self.ctx.synchronize()
+        return d.to_numpy(np.float64, npix * 4).reshape(camera.height, camera.width, 4)
+
+    def bounds(self):
+        """Axis-aligned box of the vertices (lo xyz, hi xyz) as float32."""
+        return np.concatenate([self._v.min(0), self._v.max(0)]).astype(np.float32)
+
     def info(self) -> DrSceneInfo:
         i = DrSceneInfo()
         check(lib().dr_scene_get_info(self.handle, C.byref(i)))
@@ -260,3 +284,87 @@ class Scene:
                                              C.c_void_p(db.ptr), C.c_void_p(dt.ptr), None))
         self.ctx.synchronize()
         return dh.to_numpy(HIT_DTYPE, n), db.to_numpy(np.uint32, n), dt.to_numpy(np.uint32, n)
+
+
+class Guide:
+    """Learned path guide (dr_guide): an MLP proposing an 8-lobe vMF mixture
+    per Lambert vertex, trained on the device with BF16 MFMA kernels."""
+
+    def __init__(self, ctx: Context, bounds, capacity: int = 1 << 18, lr: float = 1e-2, beta1: float = 0.9,
+                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0):
+        self.ctx = ctx
+        self.capacity = capacity
+        p = DrGuideParams((C.c_float * 6)(*[float(b) for b in bounds]), capacity, lr, beta1, beta2, eps, seed)
+        h = C.c_void_p()
+        check(lib().dr_guide_create(ctx.handle, C.byref(p), C.byref(h)))
+        self.handle = h
+        self.collect_fraction, self.guided_vertices = 1.0, 64
+
+    def close(self):
+        if self.handle:
+            check(lib().dr_guide_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(GUIDE_N_PARAMS, np.float32)
+        check(lib().dr_guide_get_params(self.handle, ptr(out)))
+        return out
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, np.float32)
+        assert a.size == GUIDE_N_PARAMS
+        check(lib().dr_guide_set_params(self.handle, ptr(a)))
+
+    def eval(self, samples: np.ndarray) -> np.ndarray:
+        """Decoded heads (n x 44) for host samples (GUIDE_SAMPLE_DTYPE)."""
+        s = np.ascontiguousarray(samples, GUIDE_SAMPLE_DTYPE)
+        n = len(s)
+        ds = self.ctx.upload(s)
+        dh = self.ctx.alloc(max(n, 1) * GUIDE_HEAD_FLOATS * 4)
+        check(lib().dr_guide_eval(self.handle, C.c_void_p(ds.ptr), n, C.c_void_p(dh.ptr), None))
+        self.ctx.synchronize()
+        return dh.to_numpy(np.float32, n * GUIDE_HEAD_FLOATS).reshape(n, GUIDE_HEAD_FLOATS)
+
+    def gradient(self, samples: np.ndarray):
+        """(gradient of sum w * -log p, sum of weights) from the device kernels."""
+        s = np.ascontiguousarray(samples, GUIDE_SAMPLE_DTYPE)
+        n = len(s)
+        ds = self.ctx.upload(s)
+        dg = self.ctx.alloc((GUIDE_N_PARAMS + 1) * 4)
+        check(lib().dr_guide_gradient(self.handle, C.c_void_p(ds.ptr), n, C.c_void_p(dg.ptr), None))
+        self.ctx.synchronize()
+        g = dg.to_numpy(np.float32, GUIDE_N_PARAMS + 1)
+        return g[:GUIDE_N_PARAMS], float(g[GUIDE_N_PARAMS])
+
+    def train_step_dev(self, samples_ptr: int | None = None, n: int = 0, stream=None):
+        check(lib().dr_guide_train_step(self.handle, C.c_void_p(samples_ptr) if samples_ptr else None, n, stream))
+
+    def train_step(self, samples: np.ndarray | None = None):
+        """One Adam step on host samples, or (None) on the last collection."""
+        if samples is None:
+            self.train_step_dev()
+        else:
+            s = np.ascontiguousarray(samples, GUIDE_SAMPLE_DTYPE)
+            ds = self.ctx.upload(s)
+            self.train_step_dev(ds.ptr, len(s))
+            self.ctx.synchronize()
+
+    def copy_samples(self, max_n: int | None = None):
+        """(samples kept by the last collecting render, number offered)."""
+        kept, total = C.c_uint32(), C.c_uint32()
+        check(lib().dr_guide_copy_samples(self.handle, None, 0, C.byref(kept), C.byref(total)))
+        n = kept.value if max_n is None else min(kept.value, max_n)
+        out = np.zeros(n, GUIDE_SAMPLE_DTYPE)
+        if n:
+            check(lib().dr_guide_copy_samples(self.handle, ptr(out), n, C.byref(kept), C.byref(total)))
+        return out, total.value
+
+    def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64):
+        check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices))
+        self.collect_fraction, self.guided_vertices = collect_fraction, guided_vertices

@@ -67,6 +67,18 @@ class DrRenderParams(C.Structure):
                 ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("fresnel", C.c_uint32), ("flags", C.c_uint32)]
 
 
+class DrGuideParams(C.Structure):
+    _fields_ = [("bounds", C.c_float * 6), ("capacity", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
+
+
+GUIDE_N_PARAMS = 17712
+GUIDE_HEAD_FLOATS = 44
+DR_GUIDE_USE, DR_GUIDE_COLLECT = 1, 2
+GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
+                               ("pad0", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
+                               ("wi", np.float32, 3), ("pad2", np.float32)])
+
 ALIAS_DTYPE = np.dtype([("p_a", np.float32), ("pdf_a", np.float32), ("pdf_b", np.float32),
                         ("a", np.int32), ("b", np.int32)])
 
@@ -104,6 +116,16 @@ PROTOTYPES = {
     "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
     "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),
     "dr_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
+    "dr_render_guided": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _u32, _vp, _vp]),
+    "dr_guide_create": (_st, [_vp, C.POINTER(DrGuideParams), C.POINTER(_vp)]),
+    "dr_guide_destroy": (_st, [_vp]),
+    "dr_guide_get_params": (_st, [_vp, _vp]),
+    "dr_guide_set_params": (_st, [_vp, _vp]),
+    "dr_guide_eval": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_guide_gradient": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_guide_train_step": (_st, [_vp, _vp, _u32, _vp]),
+    "dr_guide_copy_samples": (_st, [_vp, _vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
+    "dr_guide_set_options": (_st, [_vp, C.c_float, _u32]),
     "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
     "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),

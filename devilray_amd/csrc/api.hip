@@ -326,7 +326,18 @@ dr_status dr_render(dr_scene *sc, const dr_camera *camera, const dr_render_param
     if (!sc) return fail(DR_E_INVALID, "scene is null");
     auto *s = reinterpret_cast<Scene *>(sc);
     DR_HIP_TRY(hipSetDevice(s->ctx->device));
-    return render(s, camera, params, accum, reinterpret_cast<hipStream_t>(stream));
+    return render(s, camera, params, nullptr, 0u, accum, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_render_guided(dr_scene *sc, const dr_camera *camera, const dr_render_params *params, dr_guide *guide,
+                           uint32_t guide_flags, double *accum, dr_stream stream)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    auto *g = reinterpret_cast<Guide *>(guide);
+    if (g && g->ctx->device != s->ctx->device) return fail(DR_E_INVALID, "guide and scene on different devices");
+    DR_HIP_TRY(hipSetDevice(s->ctx->device));
+    return render(s, camera, params, g, guide_flags, accum, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -100,16 +100,64 @@ struct Scene {
     DevBuf alias;           // 20-byte alias entries, one per emitter
     DevBuf light_scalars;
     // wavefront render state (render.hip)
-    DevBuf rs_rays, rs_hits, rs_beta, rs_radiance, rs_pdf, rs_list_a, rs_list_b, rs_counters;
-    DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib, rs_shadow_path;
+    DevBuf rs_rays, rs_hits, rs_beta, rs_radiance, rs_list_a, rs_list_b, rs_counters;
+    DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib;
+    DevBuf rs_guide_head;   // DR_GUIDE_HEAD_FLOATS floats per path (guided renders)
+    DevBuf rs_pending;      // u32 per path: training sample awaiting its radiance snapshot
     uint32_t rs_capacity = 0;
 };
+
+// ---- learned path guide (guide.hip) ----------------------------------------
+constexpr uint32_t kGuideNoSample = 0xFFFFFFFFu;
+constexpr int kGuideTrainBlocks = 256;  // partial-gradient slots (deterministic reduction)
+
+// Per training sample, what the render needs to complete it: the path slot
+// (kGuideNoSample once finalised), the pdf the direction was drawn with, the
+// throughput after the bounce and the path radiance before the next vertex.
+struct alignas(16) GuideAux {
+    uint32_t path;
+    float pdf;
+    float beta[3];
+    float lsnap[3];
+};
+
+struct Guide {
+    Context *ctx = nullptr;
+    dr_guide_params prm{};
+    DevBuf master;   // FP32 parameters
+    DevBuf m1, m2;   // Adam moments
+    DevBuf frag_f;   // BF16 weight fragments for the forward products
+    DevBuf frag_b;   // BF16 transposed fragments for the backward products
+    DevBuf partials; // kGuideTrainBlocks x (N_PARAMS + 1) FP32
+    DevBuf samples;  // staged samples of the last collecting render: dr_guide_sample x stage_cap
+    DevBuf aux;      // GuideAux x stage_cap
+    DevBuf ring;     // training ring: the latest `capacity` samples of non-zero weight
+    DevBuf flags;    // u32 x (stage_cap + 1): compaction ranks
+    DevBuf counters; // u32[4]: [0] samples offered to the stage, [1] ring head, [2] ring fill
+    uint32_t stage_cap = 0;
+    uint32_t step = 0;
+    float collect_fraction = 1.0f;
+    uint32_t guided_vertices = 64;  // leading path vertices that sample from the guide
+};
+
+// Network inference for the paths of a wavefront queue: for each listed path
+// whose hit is a Lambert surface, head[DR_GUIDE_HEAD_FLOATS * path] receives
+// the decoded mixture (other paths are left untouched).
+dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const dr_hit *hits, const uint32_t *list,
+                            const uint32_t *count, uint32_t n_max, float *head, hipStream_t stream);
+// Complete the training samples of a finished chunk from the path radiance.
+dr_status guide_finalize(Guide *g, const float4 *radiance, hipStream_t stream);
+// Append the staged samples of non-zero weight to the training ring (scan
+// order, deterministic).
+dr_status guide_compact(Guide *g, hipStream_t stream);
+dr_status guide_reset_counters(Guide *g, hipStream_t stream);
 
 // light.hip
 dr_status build_lights(Scene *s, hipStream_t stream);
 
 // render.hip
-dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, double *accum, hipStream_t stream);
+dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
+                 double *accum, hipStream_t stream);
 
 // bvh_build.hip
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);

@@ -1,0 +1,954 @@
+// Learned path guide on the matrix cores: BF16 MFMA inference for wavefront
+// queues, a fused forward/backward training kernel with per-block partial
+// gradients, a deterministic reduction + Adam kernel, training-sample
+// completion, and the C-ABI entry points.
+//
+// Matrix layout. All products use v_mfma_f32_16x16x32_bf16 with one wave per
+// 16 samples: A (16 x 32) holds weights, B (32 x 16) activations with
+// samples as columns, C (16 x 16) FP32. Lane l holds A[l % 16][8 (l / 16) + j]
+// and B[8 (l / 16) + j][l % 16] for j = 0..7, and C[4 (l / 16) + r][l % 16]
+// for r = 0..3. The K order of a hidden layer is permuted so that the C
+// tiles of one layer are, lane for lane, the B operand of the next: k-step
+// ks, lane group g, element j reads input feature
+//     phi(ks, g, j) = (2 ks + j / 4) * 16 + 4 g + j % 4,
+// and the weight fragments are stored pre-permuted (1 KB per 16 x 32 tile,
+// one 16-byte load per lane). The input layer's K order is the identity.
+#include "dr_internal.hpp"
+#include "guide_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dr {
+
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int kNP = DR_GUIDE_N_PARAMS;
+constexpr int kT = 256;
+constexpr int kFragElems = 512;  // 64 lanes x 8 BF16
+constexpr int kFragF = 34;       // forward fragments: 4 + 3 x 8 + 6
+constexpr int kFragB = 32;       // transposed fragments of layers 1..4: 4 x 8
+constexpr float kLobeShare = 0.5f;  // mixture share of the lobes' training pdf
+
+__host__ __device__ constexpr int layer_in(int l) { return l == 0 ? 32 : 64; }
+__host__ __device__ constexpr int layer_out(int l) { return l == 4 ? 48 : 64; }
+__host__ __device__ constexpr int layer_ks(int l) { return l == 0 ? 1 : 2; }
+__host__ __device__ constexpr int layer_ob(int l) { return l == 4 ? 3 : 4; }
+__host__ __device__ constexpr int w_off(int l)
+{
+    return l == 0 ? 0 : w_off(l - 1) + layer_out(l - 1) * layer_in(l - 1) + layer_out(l - 1);
+}
+__host__ __device__ constexpr int b_off(int l) { return w_off(l) + layer_out(l) * layer_in(l); }
+__host__ __device__ constexpr int f_base(int l) { return l == 0 ? 0 : f_base(l - 1) + layer_ob(l - 1) * layer_ks(l - 1); }
+static_assert(b_off(4) + 48 == kNP, "parameter count");
+static_assert(f_base(4) + 6 == kFragF, "forward fragment count");
+
+// k position (ks, g, j) of input feature c of layer l
+__host__ __device__ inline void k_slot(int l, int c, int &ks, int &g, int &j)
+{
+    if (l == 0) {
+        ks = 0;
+        g = c / 8;
+        j = c % 8;
+        return;
+    }
+    const int blk = c / 16, rem = c % 16;
+    ks = blk / 2;
+    g = rem / 4;
+    j = (blk % 2) * 4 + rem % 4;
+}
+
+// Element offsets of weight W_l[o][c] in the forward fragments and (l >= 1)
+// in the transposed fragments (-1 if none).
+__host__ __device__ inline void frag_pos(int l, int o, int c, int &pf, int &pb)
+{
+    int ks, g, j;
+    k_slot(l, c, ks, g, j);
+    pf = ((f_base(l) + (o / 16) * layer_ks(l) + ks) * 64 + (o % 16) + 16 * g) * 8 + j;
+    pb = -1;
+    if (l >= 1) {
+        k_slot(1, o, ks, g, j);  // K of the transposed product = output rows, hidden order
+        pb = (((l - 1) * 8 + (c / 16) * 2 + ks) * 64 + (c % 16) + 16 * g) * 8 + j;
+    }
+}
+
+// Parameter index -> (layer, row, column); column -1 for a bias.
+__host__ __device__ inline int param_layer(int p, int &o, int &c)
+{
+    int l = 0;
+    while (l < 4 && p >= w_off(l + 1)) ++l;
+    const int q = p - w_off(l);
+    if (q >= layer_out(l) * layer_in(l)) {
+        o = q - layer_out(l) * layer_in(l);
+        c = -1;
+    } else {
+        o = q / layer_in(l);
+        c = q % layer_in(l);
+    }
+    return l;
+}
+
+struct Net {
+    const bf16x8 *ff;      // forward fragments
+    const bf16x8 *fb;      // transposed fragments
+    const float *master;   // FP32 parameters (biases are read from here)
+};
+
+struct Box {
+    float lo[3], ext[3];
+};
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 bias4(const float *__restrict__ master, int l, int ob, int g)
+{
+    const float *b = master + b_off(l) + ob * 16 + 4 * g;
+    return f32x4{b[0], b[1], b[2], b[3]};
+}
+
+__device__ __forceinline__ bf16x8 pack_relu(f32x4 lo, f32x4 hi)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r[i] = (__bf16)fmaxf(lo[i], 0.0f);
+        r[4 + i] = (__bf16)fmaxf(hi[i], 0.0f);
+    }
+    return r;
+}
+
+// Input features 8g .. 8g+7 of one sample: for g < 3, sin and cos of
+// 2^k pi u (k = 0..3) with u the position along axis g normalised to the
+// box; for g = 3 the normal, the outgoing direction, 1 and 0.
+__device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &box)
+{
+    bf16x8 f;
+    if (g < 3) {
+        const float x = g == 0 ? p.x : (g == 1 ? p.y : p.z);
+        const float lo = g == 0 ? box.lo[0] : (g == 1 ? box.lo[1] : box.lo[2]);
+        const float ext = g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2]);
+        float u = (x - lo) / ext;
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        float scale = 0.5f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float a = u * scale;
+            const float t = a - floorf(a);
+            float s, c;
+            sincos_2pi(t, s, c);
+            f[2 * k] = (__bf16)s;
+            f[2 * k + 1] = (__bf16)c;
+            scale *= 2.0f;
+        }
+    } else {
+        f[0] = (__bf16)n.x; f[1] = (__bf16)n.y; f[2] = (__bf16)n.z;
+        f[3] = (__bf16)wo.x; f[4] = (__bf16)wo.y; f[5] = (__bf16)wo.z;
+        f[6] = (__bf16)1.0f; f[7] = (__bf16)0.0f;
+    }
+    return f;
+}
+
+// Forward pass of one wave's 16 samples. xh[l] = hidden activation l+1 as
+// B operands (kept for the backward masks); y = output tiles (rows ob*16 +
+// 4g + r of the lane's sample).
+__device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0, bf16x8 (&xh)[4][2], f32x4 (&y)[3])
+{
+    const int g = lane >> 4;
+    f32x4 acc[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff[(f_base(0) + ob) * 64 + lane], x0, bias4(net.master, 0, ob, g));
+    xh[0][0] = pack_relu(acc[0], acc[1]);
+    xh[0][1] = pack_relu(acc[2], acc[3]);
+#pragma unroll
+    for (int l = 1; l < 4; ++l) {
+#pragma unroll
+        for (int ob = 0; ob < 4; ++ob) {
+            acc[ob] = bias4(net.master, l, ob, g);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) acc[ob] = mfma(net.ff[(f_base(l) + ob * 2 + ks) * 64 + lane], xh[l - 1][ks], acc[ob]);
+        }
+        xh[l][0] = pack_relu(acc[0], acc[1]);
+        xh[l][1] = pack_relu(acc[2], acc[3]);
+    }
+#pragma unroll
+    for (int ob = 0; ob < 3; ++ob) {
+        y[ob] = bias4(net.master, 4, ob, g);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) y[ob] = mfma(net.ff[(f_base(4) + ob * 2 + ks) * 64 + lane], xh[3][ks], y[ob]);
+    }
+}
+
+// Decoded lobes 2g, 2g+1 of the lane's sample (local output slot q = 4 ob + r:
+// lobe 2g uses q = 0..4, lobe 2g+1 q = 5..9, group 0 holds the selection
+// logit at q = 10). Softmax and the selection logit are combined across the
+// four lane groups of the sample.
+struct HeadLane {
+    float mu[2][3], r[2], kr[2], kap[2], pi[2], sig;
+};
+
+__device__ __forceinline__ void head_decode(const f32x4 (&y)[3], int lane, HeadLane &h)
+{
+    float yl[12];
+#pragma unroll
+    for (int ob = 0; ob < 3; ++ob)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) yl[ob * 4 + r] = y[ob][r];
+    float w[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int q0 = 5 * t;
+        const float mx = yl[q0], my = yl[q0 + 1], mz = yl[q0 + 2];
+        const float rr = fmaxf(sqrtf(mx * mx + my * my + mz * mz), kMuMinNorm);
+        h.r[t] = rr;
+        h.mu[t][0] = mx / rr;
+        h.mu[t][1] = my / rr;
+        h.mu[t][2] = mz / rr;
+        h.kr[t] = yl[q0 + 3];
+        h.kap[t] = expf(fminf(fmaxf(h.kr[t], kLogKappaLo), kLogKappaHi));
+        w[t] = yl[q0 + 4];
+    }
+    float m = fmaxf(w[0], w[1]);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    const float e0 = expf(w[0] - m), e1 = expf(w[1] - m);
+    float sum = e0 + e1;
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    h.pi[0] = e0 / sum;
+    h.pi[1] = e1 / sum;
+    const float sl = 1.0f / (1.0f + expf(-yl[10]));
+    h.sig = __shfl(sl, lane & 15);
+}
+
+struct PathIn {
+    const float *verts;
+    const uint32_t *idx;
+    const uint32_t *tri_mat;
+    const dr_material *mats;
+    const dr_ray *rays;
+    const dr_hit *hits;
+    const uint32_t *list;
+    const uint32_t *count;
+    float *head;
+};
+
+__device__ __forceinline__ V3 vload(const float *v, uint32_t i)
+{
+    return V3{v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]};
+}
+
+// One wave per 16 queue slots; every lane of a wave reaches the matrix
+// instructions (invalid slots run on neutral inputs and store nothing).
+__global__ __launch_bounds__(kT) void k_guide_infer(Net net, Box box, PathIn in)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * (kT / 64) + wave) * 16u;
+    const uint32_t n = *in.count;
+    if (base >= n) return;  // wave-uniform
+    const int s = lane & 15, g = lane >> 4;
+    const uint32_t slot = base + (uint32_t)s;
+    bool valid = slot < n;
+    uint32_t path = 0;
+    V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f};
+    if (valid) {
+        path = in.list[slot];
+        const dr_hit h = in.hits[path];
+        valid = h.id >= 0;
+        if (valid) {
+            const uint32_t tri = (uint32_t)h.id;
+            valid = in.mats[in.tri_mat[tri]].kind == DR_MAT_LAMBERT;
+            if (valid) {
+                const V3 A = vload(in.verts, in.idx[3 * (size_t)tri]);
+                const V3 B = vload(in.verts, in.idx[3 * (size_t)tri + 1]);
+                const V3 C = vload(in.verts, in.idx[3 * (size_t)tri + 2]);
+                const dr_ray r = in.rays[path];
+                const V3 d{r.dx, r.dy, r.dz};
+                p = bary_point(A, B, C, h.u, h.v);
+                const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
+                nn = dot(ng, d) > 0.0f ? mul(ng, -1.0f) : ng;
+                wo = mul(d, -1.0f);
+            }
+        }
+    }
+    const bf16x8 x0 = encode8(g, p, nn, wo, box);
+    bf16x8 xh[4][2];
+    f32x4 y[3];
+    net_forward(net, lane, x0, xh, y);
+    HeadLane h;
+    head_decode(y, lane, h);
+    // a non-finite head (diverged parameters) becomes a neutral one that the
+    // shading kernel never selects (alpha = 0), so images stay finite
+    float chk = h.sig;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) chk += h.mu[t][0] + h.mu[t][1] + h.mu[t][2] + h.kap[t] + h.pi[t];
+    int bad = isfinite(chk) ? 0 : 1;
+    bad |= __shfl_xor(bad, 16);
+    bad |= __shfl_xor(bad, 32);
+    if (!valid) return;
+    if (bad) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            h.mu[t][0] = 0.0f;
+            h.mu[t][1] = 0.0f;
+            h.mu[t][2] = 1.0f;
+            h.kap[t] = 1.0f;
+            h.pi[t] = 1.0f / kGuideLobes;
+        }
+    }
+    float *o = in.head + (size_t)kHeadFloats * path;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        float *l = o + 5 * (2 * g + t);
+        l[0] = h.mu[t][0];
+        l[1] = h.mu[t][1];
+        l[2] = h.mu[t][2];
+        l[3] = h.kap[t];
+        l[4] = h.pi[t];
+    }
+    if (g == 0) {
+        o[40] = bad ? 0.0f : kAlphaLo + kAlphaSpan * h.sig;
+        o[41] = 0.0f;
+        o[42] = 0.0f;
+        o[43] = 0.0f;
+    }
+}
+
+// ---- evaluation of caller samples (same network path as the render) -------
+__global__ __launch_bounds__(kT) void k_guide_eval(Net net, Box box, const dr_guide_sample *__restrict__ smp,
+                                                   uint32_t n, float *__restrict__ head)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * (kT / 64) + wave) * 16u;
+    if (base >= n) return;
+    const int s = lane & 15, g = lane >> 4;
+    const uint32_t i = base + (uint32_t)s;
+    const bool valid = i < n;
+    V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f};
+    if (valid) {
+        const dr_guide_sample q = smp[i];
+        p = V3{q.pos[0], q.pos[1], q.pos[2]};
+        nn = V3{q.normal[0], q.normal[1], q.normal[2]};
+        wo = V3{q.wo[0], q.wo[1], q.wo[2]};
+    }
+    bf16x8 xh[4][2];
+    f32x4 y[3];
+    net_forward(net, lane, encode8(g, p, nn, wo, box), xh, y);
+    HeadLane h;
+    head_decode(y, lane, h);
+    if (!valid) return;
+    float *o = head + (size_t)kHeadFloats * i;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        float *l = o + 5 * (2 * g + t);
+        l[0] = h.mu[t][0];
+        l[1] = h.mu[t][1];
+        l[2] = h.mu[t][2];
+        l[3] = h.kap[t];
+        l[4] = h.pi[t];
+    }
+    if (g == 0) {
+        o[40] = kAlphaLo + kAlphaSpan * h.sig;
+        o[41] = o[42] = o[43] = 0.0f;
+    }
+}
+
+// ---- training: fused forward + backward, partial gradients per block -------
+constexpr int kLds = 72;                     // BF16 per LDS row: 64 samples + pad
+__host__ __device__ constexpr int x_row(int l) { return l == 0 ? 0 : 32 + 64 * (l - 1); }
+constexpr int kXRows = 32 + 4 * 64;
+// accumulator tiles per wave: layer l has out-blocks x (in-blocks + bias)
+// tiles dealt round-robin to the 4 waves
+__host__ __device__ constexpr int n_tiles(int l) { return layer_ob(l) * (layer_in(l) / 16 + 1); }
+__host__ __device__ constexpr int n_wtiles(int l) { return (n_tiles(l) + 3) / 4; }
+__host__ __device__ constexpr int acc_base(int l) { return l == 4 ? 0 : acc_base(l + 1) + n_wtiles(l + 1); }
+constexpr int kAcc = acc_base(0) + n_wtiles(0);
+static_assert(kAcc == 22, "accumulator tiles per wave");
+
+struct TrainIn {
+    const dr_guide_sample *smp;
+    const uint32_t *count;  // device count (NULL: n)
+    uint32_t n, cap;
+    float *partials;        // gridDim.x x (kNP + 1)
+};
+
+template <int L>
+__device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dbuf, const __bf16 *X, int wave, int lane)
+{
+    constexpr int IB = layer_in(L) / 16;
+    constexpr int T = n_tiles(L);
+    const int i = lane & 15, g = lane >> 4;
+    bf16x8 ones;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
+#pragma unroll
+    for (int u = 0; u < n_wtiles(L); ++u) {
+        const int t = wave + 4 * u;
+        if (t < T) {
+            const int ob = t / (IB + 1), ib = t % (IB + 1);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Dbuf + (ob * 16 + i) * kLds + 32 * ks + 8 * g);
+                const bf16x8 b = ib < IB
+                                     ? *reinterpret_cast<const bf16x8 *>(X + (x_row(L) + ib * 16 + i) * kLds + 32 * ks + 8 * g)
+                                     : ones;
+                acc[acc_base(L) + u] = mfma(a, b, acc[acc_base(L) + u]);
+            }
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int wave, int lane)
+{
+    constexpr int IB = layer_in(L) / 16;
+    constexpr int T = n_tiles(L);
+    const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < n_wtiles(L); ++u) {
+        const int t = wave + 4 * u;
+        if (t < T) {
+            const int ob = t / (IB + 1), ib = t % (IB + 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ob * 16 + 4 * g + r;
+                if (ib < IB) P[w_off(L) + row * layer_in(L) + ib * 16 + j] = acc[acc_base(L) + u][r];
+                else if (j == 0) P[b_off(L) + row] = acc[acc_base(L) + u][r];
+            }
+        }
+    }
+}
+
+// delta of hidden layer L (1..4) from the delta of layer L+1 (B operands in
+// dB): W_L^T dB, masked by the ReLU of activation L; written to Dout and
+// returned as the next B operands.
+template <int L>
+__device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&dB)[2], const bf16x8 (&xl)[2],
+                                            __bf16 *Dout, int col)
+{
+    const int g = lane >> 4;
+    bf16x8 nb[2];
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) c = mfma(net.fb[((L - 1) * 8 + ib * 2 + ks) * 64 + lane], dB[ks], c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = (ib % 2) * 4 + r;
+            const float xv = (float)xl[ib / 2][j];
+            const __bf16 d = (__bf16)(xv > 0.0f ? c[r] : 0.0f);
+            nb[ib / 2][j] = d;
+            Dout[(ib * 16 + 4 * g + r) * kLds + col] = d;
+        }
+    }
+    dB[0] = nb[0];
+    dB[1] = nb[1];
+}
+
+__global__ __launch_bounds__(kT) void k_guide_train(Net net, Box box, TrainIn in)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 X[kXRows * kLds];
+    __shared__ __attribute__((aligned(16))) __bf16 D[2][64 * kLds];
+    __shared__ float red[kT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & 15, g = lane >> 4;
+    const int col = wave * 16 + s;
+    const uint32_t n = in.count ? min(*in.count, in.cap) : in.n;
+    const uint32_t ntiles = (n + 63u) / 64u;
+    f32x4 acc[kAcc];
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t i = tile * 64u + (uint32_t)col;
+        V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f}, wi{0.0f, 0.0f, 1.0f};
+        float w = 0.0f;
+        if (i < n) {
+            const dr_guide_sample q = in.smp[i];
+            p = V3{q.pos[0], q.pos[1], q.pos[2]};
+            nn = V3{q.normal[0], q.normal[1], q.normal[2]};
+            wo = V3{q.wo[0], q.wo[1], q.wo[2]};
+            wi = V3{q.wi[0], q.wi[1], q.wi[2]};
+            w = q.weight;
+        }
+        if (g == 0) wsum += w;
+        const bf16x8 x0 = encode8(g, p, nn, wo, box);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) X[(8 * g + j) * kLds + col] = x0[j];
+        bf16x8 xh[4][2];
+        f32x4 y[3];
+        net_forward(net, lane, x0, xh, y);
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    X[(x_row(l + 1) + (2 * ks + j / 4) * 16 + 4 * g + j % 4) * kLds + col] = xh[l][ks][j];
+
+        // head: loss w * [-log(q/2 + cos+/(2 pi)) - log(alpha q' + (1 - alpha) cos+/pi)]
+        HeadLane h;
+        head_decode(y, lane, h);
+        const float alpha = kAlphaLo + kAlphaSpan * h.sig;
+        float f[2], c[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            c[t] = h.mu[t][0] * wi.x + h.mu[t][1] * wi.y + h.mu[t][2] * wi.z;
+            f[t] = vmf_density(h.kap[t], c[t]);
+        }
+        float q = h.pi[0] * f[0] + h.pi[1] * f[1];
+        q += __shfl_xor(q, 16);
+        q += __shfl_xor(q, 32);
+        const float cb = dot(nn, wi);
+        const float b = cb > 0.0f ? cb / kPi : 0.0f;
+        // lobes: KL of the fixed half-and-half proposal; selection: KL of the
+        // learned one-sample MIS pdf with the mixture held fixed
+        const float pm = alpha * q + (1.0f - alpha) * b;
+        const float G = w != 0.0f ? -w / pm : 0.0f;
+        const float dq = w != 0.0f ? -w / (kLobeShare * q + (1.0f - kLobeShare) * b) * kLobeShare : 0.0f;
+        float dy[12];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int q0 = 5 * t;
+            const float tt = h.pi[t] * f[t];
+            const float k = h.kap[t];
+            dy[q0 + 4] = dq * (tt - h.pi[t] * q);
+            dy[q0 + 3] = (h.kr[t] > kLogKappaLo && h.kr[t] < kLogKappaHi)
+                             ? dq * tt * (1.0f - 2.0f * k / expm1f(2.0f * k) + k * (c[t] - 1.0f))
+                             : 0.0f;
+            const float gx = dq * tt * k * wi.x, gy = dq * tt * k * wi.y, gz = dq * tt * k * wi.z;
+            const float mg = h.mu[t][0] * gx + h.mu[t][1] * gy + h.mu[t][2] * gz;
+            dy[q0] = (gx - h.mu[t][0] * mg) / h.r[t];
+            dy[q0 + 1] = (gy - h.mu[t][1] * mg) / h.r[t];
+            dy[q0 + 2] = (gz - h.mu[t][2] * mg) / h.r[t];
+        }
+        dy[10] = g == 0 ? G * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
+        dy[11] = 0.0f;
+        bf16x8 dB[2];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            dB[0][j] = (__bf16)dy[j];
+            dB[1][j] = (__bf16)(j < 4 ? dy[8 + j] : 0.0f);
+        }
+#pragma unroll
+        for (int qq = 0; qq < 12; ++qq) D[0][((qq / 4) * 16 + 4 * g + qq % 4) * kLds + col] = (__bf16)dy[qq];
+        __syncthreads();
+
+        // backward, one layer per phase; D[0] / D[1] alternate
+        dw_layer<4>(acc, D[0], X, wave, lane);
+        delta_layer<4>(net, lane, dB, xh[3], D[1], col);
+        __syncthreads();
+        dw_layer<3>(acc, D[1], X, wave, lane);
+        delta_layer<3>(net, lane, dB, xh[2], D[0], col);
+        __syncthreads();
+        dw_layer<2>(acc, D[0], X, wave, lane);
+        delta_layer<2>(net, lane, dB, xh[1], D[1], col);
+        __syncthreads();
+        dw_layer<1>(acc, D[1], X, wave, lane);
+        delta_layer<1>(net, lane, dB, xh[0], D[0], col);
+        __syncthreads();
+        dw_layer<0>(acc, D[0], X, wave, lane);
+        __syncthreads();
+    }
+    float *P = in.partials + (size_t)blockIdx.x * (kNP + 1);
+    dw_store<4>(acc, P, wave, lane);
+    dw_store<3>(acc, P, wave, lane);
+    dw_store<2>(acc, P, wave, lane);
+    dw_store<1>(acc, P, wave, lane);
+    dw_store<0>(acc, P, wave, lane);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) wsum += __shfl_xor(wsum, o);
+    if (lane == 0) red[wave] = wsum;
+    __syncthreads();
+    if (threadIdx.x == 0) P[kNP] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__device__ __forceinline__ void pack_param(int p, float w, __bf16 *ff, __bf16 *fb)
+{
+    int o, c;
+    const int l = param_layer(p, o, c);
+    if (c < 0) return;  // biases are read from the FP32 parameters
+    int pf, pb;
+    frag_pos(l, o, c, pf, pb);
+    const __bf16 v = (__bf16)w;
+    ff[pf] = v;
+    if (pb >= 0) fb[pb] = v;
+}
+
+__global__ __launch_bounds__(kT) void k_guide_pack(const float *__restrict__ master, __bf16 *ff, __bf16 *fb)
+{
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p < kNP) pack_param(p, master[p], ff, fb);
+}
+
+struct AdamIn {
+    float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1 - beta1^t), c2 = 1/(1 - beta2^t)
+};
+
+// Sum of the block partials in block order (deterministic); with grad set,
+// only the sums are written; otherwise one Adam step on the loss normalised
+// by the total sample weight, and the BF16 fragments are refreshed.
+__global__ __launch_bounds__(kT) void k_guide_adam(const float *__restrict__ partials, int nblk, float *master,
+                                                   float *m1, float *m2, __bf16 *ff, __bf16 *fb, AdamIn ad,
+                                                   float *grad)
+{
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p > kNP) return;
+    float sum = 0.0f;
+    for (int b = 0; b < nblk; ++b) sum += partials[(size_t)b * (kNP + 1) + p];
+    if (grad) {
+        grad[p] = sum;
+        return;
+    }
+    if (p == kNP) return;
+    float sw = 0.0f;
+    for (int b = 0; b < nblk; ++b) sw += partials[(size_t)b * (kNP + 1) + kNP];
+    if (!(sw > 0.0f) || !isfinite(sw)) return;
+    const float gr = sum / sw;
+    if (!isfinite(gr)) return;
+    const float m = ad.beta1 * m1[p] + (1.0f - ad.beta1) * gr;
+    const float v = ad.beta2 * m2[p] + (1.0f - ad.beta2) * gr * gr;
+    m1[p] = m;
+    m2[p] = v;
+    const float w = master[p] - ad.lr * (m * ad.c1) / (sqrtf(v * ad.c2) + ad.eps);
+    master[p] = w;
+    pack_param(p, w, ff, fb);
+}
+
+// Starts a collection: the stage count only (the training ring persists).
+__global__ void k_guide_zero_counters(uint32_t *c)
+{
+    if (threadIdx.x == 0) c[0] = 0u;
+}
+
+// weight = luminance((L_end - L_snap) / beta) * max(0, n . wi) / pdf
+__global__ __launch_bounds__(kT) void k_guide_finalize(dr_guide_sample *__restrict__ smp, GuideAux *__restrict__ aux,
+                                                       const uint32_t *__restrict__ counters, uint32_t cap,
+                                                       const float4 *__restrict__ rad)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= min(counters[0], cap)) return;
+    const GuideAux x = aux[i];
+    if (x.path == kGuideNoSample) return;
+    const float4 L = rad[x.path];
+    const float s0 = L.x - x.lsnap[0], s1 = L.y - x.lsnap[1], s2 = L.z - x.lsnap[2];
+    const float l0 = x.beta[0] > 0.0f ? s0 / x.beta[0] : 0.0f;
+    const float l1 = x.beta[1] > 0.0f ? s1 / x.beta[1] : 0.0f;
+    const float l2 = x.beta[2] > 0.0f ? s2 / x.beta[2] : 0.0f;
+    const float lum = fmaxf(0.0f, 0.2126f * l0 + 0.7152f * l1 + 0.0722f * l2);
+    const dr_guide_sample q = smp[i];
+    const float cs = fmaxf(0.0f, q.normal[0] * q.wi[0] + q.normal[1] * q.wi[1] + q.normal[2] * q.wi[2]);
+    float w = x.pdf > 0.0f ? lum * cs / x.pdf : 0.0f;
+    if (!isfinite(w)) w = 0.0f;
+    smp[i].weight = w;
+    aux[i].path = kGuideNoSample;
+}
+
+__global__ __launch_bounds__(kT) void k_guide_flag(const dr_guide_sample *__restrict__ smp,
+                                                   const uint32_t *__restrict__ counters, uint32_t stage_cap,
+                                                   uint32_t *__restrict__ flags)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i > stage_cap) return;
+    const uint32_t n = min(counters[0], stage_cap);
+    flags[i] = (i < n && smp[i].weight > 0.0f) ? 1u : 0u;
+}
+
+// flags holds exclusive ranks (flags[stage_cap] = number of non-zero samples);
+// sample of rank r goes to ring slot (head + r) mod ring_cap. Only the first
+// ring_cap ranks are written, so every slot has one writer.
+__global__ __launch_bounds__(kT) void k_guide_scatter(const dr_guide_sample *__restrict__ smp,
+                                                      const uint32_t *__restrict__ counters, uint32_t stage_cap,
+                                                      const uint32_t *__restrict__ flags, dr_guide_sample *ring,
+                                                      uint32_t ring_cap)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    const uint32_t n = min(counters[0], stage_cap);
+    if (i >= n) return;
+    const uint32_t r = flags[i];
+    if (flags[i + 1] == r || r >= ring_cap) return;  // zero weight, or beyond one ring's worth
+    ring[(counters[1] + r) % ring_cap] = smp[i];
+}
+
+__global__ void k_guide_ring_advance(uint32_t *counters, const uint32_t *__restrict__ flags, uint32_t stage_cap,
+                                     uint32_t ring_cap)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t nnz = min(flags[stage_cap], ring_cap);
+    counters[1] = (counters[1] + nnz) % ring_cap;
+    counters[2] = min(counters[2] + nnz, ring_cap);
+}
+
+inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+Net net_of(const Guide *g)
+{
+    return Net{g->frag_f.as<const bf16x8>(), g->frag_b.as<const bf16x8>(), g->master.as<const float>()};
+}
+
+Box box_of(const Guide *g)
+{
+    Box b;
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = g->prm.bounds[k];
+        b.ext[k] = g->prm.bounds[3 + k] - g->prm.bounds[k];
+    }
+    return b;
+}
+
+// He-uniform hidden weights, output layer scaled by 0.1, zero biases; from
+// Philox(seed, parameter index).
+void init_params(std::vector<float> &P, uint64_t seed)
+{
+    P.assign(kNP, 0.0f);
+    for (int p = 0; p < kNP; ++p) {
+        int o, c;
+        const int l = param_layer(p, o, c);
+        if (c < 0) continue;
+        const U4 r = philox4x32_10(U4{(uint32_t)p, 0x6D1Du, 0u, (uint32_t)(seed >> 32)}, 0x51DEu, (uint32_t)seed);
+        const float lim = std::sqrt(6.0f / (float)layer_in(l)) * (l == 4 ? 0.1f : 1.0f);
+        P[p] = (2.0f * u01(r.x) - 1.0f) * lim;
+    }
+}
+
+dr_status upload_params(Guide *g, const float *host)
+{
+    DR_HIP_TRY(hipMemcpy(g->master.ptr, host, sizeof(float) * kNP, hipMemcpyHostToDevice));
+    DR_HIP_TRY(hipMemset(g->m1.ptr, 0, sizeof(float) * kNP));
+    DR_HIP_TRY(hipMemset(g->m2.ptr, 0, sizeof(float) * kNP));
+    k_guide_pack<<<cdiv(kNP, kT), kT>>>(g->master.as<float>(), g->frag_f.as<__bf16>(), g->frag_b.as<__bf16>());
+    DR_HIP_TRY(hipGetLastError());
+    DR_HIP_TRY(hipDeviceSynchronize());
+    g->step = 0;
+    return DR_OK;
+}
+
+dr_status run_train(Guide *g, const dr_guide_sample *smp, const uint32_t *count, uint32_t n, uint32_t cap,
+                    float *grad, hipStream_t stream)
+{
+    const uint32_t n_max = count ? cap : n;
+    const int nblk = (int)std::max<uint32_t>(1u, std::min<uint32_t>(kGuideTrainBlocks, cdiv(n_max, 64)));
+    TrainIn in{smp, count, n, cap, g->partials.as<float>()};
+    k_guide_train<<<nblk, kT, 0, stream>>>(net_of(g), box_of(g), in);
+    DR_HIP_TRY(hipGetLastError());
+    AdamIn ad{};
+    if (!grad) {
+        g->step += 1;
+        const double t = (double)g->step;
+        ad.lr = g->prm.lr;
+        ad.beta1 = g->prm.beta1;
+        ad.beta2 = g->prm.beta2;
+        ad.eps = g->prm.eps;
+        ad.c1 = (float)(1.0 / (1.0 - std::pow((double)g->prm.beta1, t)));
+        ad.c2 = (float)(1.0 / (1.0 - std::pow((double)g->prm.beta2, t)));
+    }
+    k_guide_adam<<<cdiv(kNP + 1, kT), kT, 0, stream>>>(g->partials.as<float>(), nblk, g->master.as<float>(),
+                                                       g->m1.as<float>(), g->m2.as<float>(), g->frag_f.as<__bf16>(),
+                                                       g->frag_b.as<__bf16>(), ad, grad);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+}  // namespace
+
+dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const dr_hit *hits, const uint32_t *list,
+                            const uint32_t *count, uint32_t n_max, float *head, hipStream_t stream)
+{
+    PathIn in{s->verts.as<float>(), s->indices.as<uint32_t>(), s->tri_mat.as<uint32_t>(),
+              s->materials.as<dr_material>(), rays, hits, list, count, head};
+    k_guide_infer<<<cdiv(n_max, 64), kT, 0, stream>>>(net_of(g), box_of(g), in);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+dr_status guide_compact(Guide *g, hipStream_t stream)
+{
+    const uint32_t S = g->stage_cap;
+    k_guide_flag<<<cdiv(S + 1, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), g->counters.as<uint32_t>(), S,
+                                                    g->flags.as<uint32_t>());
+    DR_HIP_TRY(hipGetLastError());
+    if (dr_status st = exclusive_scan_u32(g->ctx, g->flags.as<uint32_t>(), S + 1, stream)) return st;
+    k_guide_scatter<<<cdiv(S, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), g->counters.as<uint32_t>(), S,
+                                                   g->flags.as<uint32_t>(), g->ring.as<dr_guide_sample>(),
+                                                   g->prm.capacity);
+    DR_HIP_TRY(hipGetLastError());
+    k_guide_ring_advance<<<1, 64, 0, stream>>>(g->counters.as<uint32_t>(), g->flags.as<uint32_t>(), S,
+                                               g->prm.capacity);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+dr_status guide_finalize(Guide *g, const float4 *radiance, hipStream_t stream)
+{
+    const uint32_t cap = g->stage_cap;
+    k_guide_finalize<<<cdiv(cap, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), g->aux.as<GuideAux>(),
+                                                       g->counters.as<uint32_t>(), cap, radiance);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+dr_status guide_reset_counters(Guide *g, hipStream_t stream)
+{
+    k_guide_zero_counters<<<1, 64, 0, stream>>>(g->counters.as<uint32_t>());
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+}  // namespace dr
+
+using namespace dr;
+
+extern "C" {
+
+dr_status dr_guide_create(dr_context *cx, const dr_guide_params *params, dr_guide **out)
+{
+    if (!cx || !params || !out) return fail(DR_E_INVALID, "null argument");
+    *out = nullptr;
+    const dr_guide_params &p = *params;
+    for (int k = 0; k < 3; ++k)
+        if (!(p.bounds[3 + k] > p.bounds[k]) || !std::isfinite(p.bounds[k]) || !std::isfinite(p.bounds[3 + k]))
+            return fail(DR_E_INVALID, "guide bounds must satisfy lo < hi on every axis");
+    if (p.capacity == 0 || p.capacity > (1u << 22)) return fail(DR_E_INVALID, "capacity must be 1..2^22");
+    if (!(p.lr > 0.0f) || !(p.beta1 >= 0.0f && p.beta1 < 1.0f) || !(p.beta2 >= 0.0f && p.beta2 < 1.0f) ||
+        !(p.eps > 0.0f))
+        return fail(DR_E_INVALID, "Adam parameters out of range");
+    auto *ctx = reinterpret_cast<Context *>(cx);
+    DR_HIP_TRY(hipSetDevice(ctx->device));
+    auto *g = new (std::nothrow) Guide();
+    if (!g) return fail(DR_E_OOM, "host allocation");
+    g->ctx = ctx;
+    g->prm = p;
+    dr_status st = DR_OK;
+    auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
+    need(g->master, sizeof(float) * kNP);
+    need(g->m1, sizeof(float) * kNP);
+    need(g->m2, sizeof(float) * kNP);
+    need(g->frag_f, 2ull * kFragElems * kFragF);
+    need(g->frag_b, 2ull * kFragElems * kFragB);
+    need(g->partials, sizeof(float) * (size_t)kGuideTrainBlocks * (kNP + 1));
+    g->stage_cap = p.capacity * 4u;
+    need(g->samples, sizeof(dr_guide_sample) * (size_t)g->stage_cap);
+    need(g->aux, sizeof(GuideAux) * (size_t)g->stage_cap);
+    need(g->ring, sizeof(dr_guide_sample) * (size_t)p.capacity);
+    need(g->flags, 4ull * ((size_t)g->stage_cap + 1));
+    need(g->counters, 16);
+    if (st) {
+        delete g;
+        return st;
+    }
+    hipError_t e = hipMemset(g->frag_f.ptr, 0, g->frag_f.bytes);
+    if (e == hipSuccess) e = hipMemset(g->frag_b.ptr, 0, g->frag_b.bytes);
+    if (e == hipSuccess) e = hipMemset(g->counters.ptr, 0, 16);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(DR_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
+    std::vector<float> P;
+    init_params(P, p.seed);
+    if ((st = upload_params(g, P.data()))) {
+        delete g;
+        return st;
+    }
+    *out = reinterpret_cast<dr_guide *>(g);
+    return DR_OK;
+}
+
+dr_status dr_guide_destroy(dr_guide *gd)
+{
+    if (!gd) return DR_OK;
+    auto *g = reinterpret_cast<Guide *>(gd);
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete g;
+    return DR_OK;
+}
+
+dr_status dr_guide_get_params(dr_guide *gd, float *params)
+{
+    if (!gd || !params) return fail(DR_E_INVALID, "null argument");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    DR_HIP_TRY(hipDeviceSynchronize());
+    DR_HIP_TRY(hipMemcpy(params, g->master.ptr, sizeof(float) * kNP, hipMemcpyDeviceToHost));
+    return DR_OK;
+}
+
+dr_status dr_guide_set_params(dr_guide *gd, const float *params)
+{
+    if (!gd || !params) return fail(DR_E_INVALID, "null argument");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    for (int p = 0; p < kNP; ++p)
+        if (!std::isfinite(params[p])) return fail(DR_E_INVALID, "non-finite parameter");
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    DR_HIP_TRY(hipDeviceSynchronize());
+    return upload_params(g, params);
+}
+
+dr_status dr_guide_eval(dr_guide *gd, const dr_guide_sample *samples, uint32_t n, float *head, dr_stream stream)
+{
+    if (!gd || (n && (!samples || !head))) return fail(DR_E_INVALID, "null argument");
+    if (n == 0) return DR_OK;
+    auto *g = reinterpret_cast<Guide *>(gd);
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    k_guide_eval<<<cdiv(n, 64), kT, 0, reinterpret_cast<hipStream_t>(stream)>>>(net_of(g), box_of(g), samples, n,
+                                                                                 head);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+dr_status dr_guide_gradient(dr_guide *gd, const dr_guide_sample *samples, uint32_t n, float *grad, dr_stream stream)
+{
+    if (!gd || !grad || (n && !samples)) return fail(DR_E_INVALID, "null argument");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    return run_train(g, samples, nullptr, n, n, grad, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_guide_train_step(dr_guide *gd, const dr_guide_sample *samples, uint32_t n, dr_stream stream)
+{
+    if (!gd) return fail(DR_E_INVALID, "guide is null");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (samples) return run_train(g, samples, nullptr, n, n, nullptr, st);
+    return run_train(g, g->ring.as<dr_guide_sample>(), g->counters.as<uint32_t>() + 2, 0, g->prm.capacity, nullptr,
+                     st);
+}
+
+dr_status dr_guide_copy_samples(dr_guide *gd, dr_guide_sample *host, uint32_t max_n, uint32_t *n_kept,
+                                uint32_t *n_total)
+{
+    if (!gd) return fail(DR_E_INVALID, "guide is null");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    DR_HIP_TRY(hipSetDevice(g->ctx->device));
+    DR_HIP_TRY(hipDeviceSynchronize());
+    uint32_t c[4];
+    DR_HIP_TRY(hipMemcpy(c, g->counters.ptr, sizeof c, hipMemcpyDeviceToHost));
+    const uint32_t kept = std::min(c[0], g->stage_cap);
+    if (n_kept) *n_kept = kept;
+    if (n_total) *n_total = c[0];
+    if (host && max_n)
+        DR_HIP_TRY(hipMemcpy(host, g->samples.ptr, sizeof(dr_guide_sample) * std::min(kept, max_n),
+                             hipMemcpyDeviceToHost));
+    return DR_OK;
+}
+
+dr_status dr_guide_set_options(dr_guide *gd, float collect_fraction, uint32_t guided_vertices)
+{
+    if (!gd) return fail(DR_E_INVALID, "guide is null");
+    if (!(collect_fraction >= 0.0f && collect_fraction <= 1.0f)) return fail(DR_E_INVALID, "fraction must be in [0, 1]");
+    auto *g = reinterpret_cast<Guide *>(gd);
+    g->collect_fraction = collect_fraction;
+    g->guided_vertices = guided_vertices;
+    return DR_OK;
+}
+
+}  // extern "C"

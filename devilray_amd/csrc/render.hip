@@ -11,7 +11,14 @@
 // Russian roulette from vertex rr_depth. Paths are independent work items
 // (pixel, sample); per-path radiance is resolved into the FP64 accumulation
 // buffer in sample order, so results do not depend on queue order.
+//
+// With a learned guide, Lambert bounces are drawn from the one-sample MIS of
+// the guide's vMF mixture (probability alpha, from the network) and the
+// cosine BSDF, weighted by f cos / (alpha q + (1 - alpha) cos/pi); next-event
+// estimation and emission hits are MIS-weighted against that combined pdf,
+// so the estimator stays unbiased for any network output.
 #include "dr_internal.hpp"
+#include "guide_device.hpp"
 #include "shade.hpp"
 
 namespace dr {
@@ -49,7 +56,8 @@ __device__ __forceinline__ void item_of(const PathParams &pp, uint32_t path, uin
 
 __global__ __launch_bounds__(kT) void k_camera(CamParams cam, PathParams pp, dr_ray *__restrict__ rays,
                                                float4 *__restrict__ beta, float4 *__restrict__ rad,
-                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters)
+                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
+                                               uint32_t *__restrict__ pending)
 {
     const uint32_t p = blockIdx.x * kT + threadIdx.x;
     if (p == 0) { counters[0] = pp.n; counters[1] = 0u; counters[2] = 0u; }
@@ -78,6 +86,7 @@ __global__ __launch_bounds__(kT) void k_camera(CamParams cam, PathParams pp, dr_
     beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);       // w: pdf of the last BSDF sample
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);        // w: 1 = last bounce specular / camera
     list[p] = p;
+    if (pending) pending[p] = kGuideNoSample;
 }
 
 __global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count)
@@ -96,6 +105,19 @@ struct SceneView {
     uint32_t n_emit;
 };
 
+// Guide state seen by the shading kernel: head != NULL samples from the
+// guide (records written by the inference kernel for this vertex);
+// smp != NULL collects training samples.
+struct GuideView {
+    const float *head;
+    dr_guide_sample *smp;
+    GuideAux *aux;
+    uint32_t *counter;
+    uint32_t *pending;
+    uint32_t cap;
+    float frac;
+};
+
 __device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
 {
     return V3{sv.verts[3 * (size_t)vi], sv.verts[3 * (size_t)vi + 1], sv.verts[3 * (size_t)vi + 2]};
@@ -107,18 +129,30 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                                               uint32_t *__restrict__ out_count, dr_ray *__restrict__ rays,
                                               const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
                                               float4 *__restrict__ rad, dr_ray *__restrict__ srays,
-                                              float4 *__restrict__ scontrib, uint32_t *__restrict__ scount)
+                                              float4 *__restrict__ scontrib, uint32_t *__restrict__ scount,
+                                              GuideView gv)
 {
     const uint32_t slot = blockIdx.x * kT + threadIdx.x;
     if (slot >= *in_count) return;
     const uint32_t path = in_list[slot];
     const dr_hit h = hits[path];
+    float4 l4 = rad[path];
+    if (gv.smp) {
+        // the previous bounce's training sample: radiance before this vertex
+        const uint32_t pk = gv.pending[path];
+        if (pk != kGuideNoSample) {
+            GuideAux &x = gv.aux[pk];
+            x.lsnap[0] = l4.x;
+            x.lsnap[1] = l4.y;
+            x.lsnap[2] = l4.z;
+            gv.pending[path] = kGuideNoSample;
+        }
+    }
     if (h.id < 0) return;  // escaped: no environment light
     uint32_t pixel, sample;
     item_of(pp, path, pixel, sample);
     const dr_ray ray = rays[path];
     float4 b4 = beta[path];
-    float4 l4 = rad[path];
     V3 beta3 = V3{b4.x, b4.y, b4.z};
     const bool spec_prev = l4.w != 0.0f;
     const float pdf_prev = b4.w;
@@ -142,6 +176,21 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
     V3 no;
     bool spec;
     float pdf;
+    float hd[kHeadFloats];
+    float alpha = 0.0f;
+    const bool guided = gv.head != nullptr && mat.kind == DR_MAT_LAMBERT;
+    if (guided) {
+        const float4 *H = reinterpret_cast<const float4 *>(gv.head + (size_t)kHeadFloats * path);
+#pragma unroll
+        for (int k = 0; k < kHeadFloats / 4; ++k) {
+            const float4 v = H[k];
+            hd[4 * k] = v.x;
+            hd[4 * k + 1] = v.y;
+            hd[4 * k + 2] = v.z;
+            hd[4 * k + 3] = v.w;
+        }
+        alpha = hd[40];
+    }
     if (mat.kind == DR_MAT_LAMBERT) {
         // emission, MIS against light sampling
         const V3 E = V3{mat.emission[0], mat.emission[1], mat.emission[2]};
@@ -181,7 +230,8 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                     const dr_material lm = sv.mats[sv.tri_mat[lt]];
                     const float G = cs * acl / dist2;
                     const float pn = acl < 1e-6f ? 0.0f : pa * dist2 / acl;
-                    const float pb = cs / kPi;
+                    float pb = cs / kPi;
+                    if (guided) pb = alpha * head_mixture(hd, om) + (1.0f - alpha) * pb;
                     const float w = power_heuristic(pn, pb);
                     const float tmax = dist - 2.0f * fmaxf(eps, origin_eps(LA, LB, LC));
                     if (pn > 0.0f && tmax > 0.0f) {
@@ -196,11 +246,35 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                 }
             }
         }
-        // cosine-weighted bounce
-        nd = cosine_dir(n, u01(r0.x), u01(r0.y));
-        const float cn = dot(nd, n);
-        pdf = cn > 0.0f ? cn / kPi : 0.0f;
-        beta3 = V3{beta3.x * alb.x, beta3.y * alb.y, beta3.z * alb.z};
+        if (guided) {
+            // one-sample MIS: a guide lobe with probability alpha, else the BSDF
+            const U4 r2 = path_rng(pixel, sample, vertex, 2u, pp.seed);
+            if (u01(r2.x) < alpha) {
+                const float u = u01(r2.y);
+                int j = kGuideLobes - 1;
+                float cdf = 0.0f;
+#pragma unroll
+                for (int k = 0; k < kGuideLobes - 1; ++k) {
+                    cdf += hd[5 * k + 4];
+                    if (u < cdf && j == kGuideLobes - 1) j = k;
+                }
+                const float *lb = hd + 5 * j;
+                nd = vmf_sample(V3{lb[0], lb[1], lb[2]}, lb[3], u01(r2.z), u01(r2.w));
+            } else {
+                nd = cosine_dir(n, u01(r0.x), u01(r0.y));
+            }
+            const float cn = dot(nd, n);
+            const float pbsdf = cn > 0.0f ? cn / kPi : 0.0f;
+            pdf = alpha * head_mixture(hd, nd) + (1.0f - alpha) * pbsdf;
+            const float fw = (cn > 0.0f && pdf > 0.0f) ? pbsdf / pdf : 0.0f;
+            beta3 = V3{beta3.x * alb.x * fw, beta3.y * alb.y * fw, beta3.z * alb.z * fw};
+        } else {
+            // cosine-weighted bounce
+            nd = cosine_dir(n, u01(r0.x), u01(r0.y));
+            const float cn = dot(nd, n);
+            pdf = cn > 0.0f ? cn / kPi : 0.0f;
+            beta3 = V3{beta3.x * alb.x, beta3.y * alb.y, beta3.z * alb.z};
+        }
         no = add(p, mul(n, eps));
         spec = false;
     } else {
@@ -242,6 +316,24 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
     }
     if (!(beta3.x > 0.0f || beta3.y > 0.0f || beta3.z > 0.0f)) alive = false;
     if (!alive) return;
+    if (gv.smp && !spec && u01(r0.w) < gv.frac) {
+        const uint32_t k = atomicAdd(gv.counter, 1u);
+        if (k < gv.cap) {
+            dr_guide_sample q;
+            q.pos[0] = p.x; q.pos[1] = p.y; q.pos[2] = p.z; q.weight = 0.0f;
+            q.normal[0] = n.x; q.normal[1] = n.y; q.normal[2] = n.z; q.pad0 = 0.0f;
+            q.wo[0] = -d.x; q.wo[1] = -d.y; q.wo[2] = -d.z; q.pad1 = 0.0f;
+            q.wi[0] = nd.x; q.wi[1] = nd.y; q.wi[2] = nd.z; q.pad2 = 0.0f;
+            gv.smp[k] = q;
+            GuideAux x;
+            x.path = path;
+            x.pdf = pdf;
+            x.beta[0] = beta3.x; x.beta[1] = beta3.y; x.beta[2] = beta3.z;
+            x.lsnap[0] = x.lsnap[1] = x.lsnap[2] = 0.0f;
+            gv.aux[k] = x;
+            gv.pending[path] = k;
+        }
+    }
     beta[path] = make_float4(beta3.x, beta3.y, beta3.z, pdf);
     rays[path] = dr_ray{no.x, no.y, no.z, 0.0f, nd.x, nd.y, nd.z, INFINITY};
     out_list[atomicAdd(out_count, 1u)] = path;
@@ -285,8 +377,13 @@ inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
 
 }  // namespace
 
-dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, double *accum, hipStream_t stream)
+dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
+                 double *accum, hipStream_t stream)
 {
+    if (guide_flags & ~(uint32_t)(DR_GUIDE_USE | DR_GUIDE_COLLECT)) return fail(DR_E_INVALID, "unknown guide flags");
+    if (guide_flags && !guide) return fail(DR_E_INVALID, "guide flags without a guide");
+    const bool use_guide = (guide_flags & DR_GUIDE_USE) != 0;
+    const bool collect = (guide_flags & DR_GUIDE_COLLECT) != 0;
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
     if (!cam || !prm || !accum) return fail(DR_E_INVALID, "null argument");
     if (cam->width == 0 || cam->height == 0) return fail(DR_E_INVALID, "empty image");
@@ -312,6 +409,22 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, do
         need(s->rs_shadow_contrib, sizeof(float4) * (size_t)cap);
         if (st) return st;
         s->rs_capacity = cap;
+    }
+    if (use_guide)
+        if (dr_status st = s->rs_guide_head.ensure(sizeof(float) * kHeadFloats * (size_t)s->rs_capacity)) return st;
+    if (collect) {
+        if (dr_status st = s->rs_pending.ensure(4ull * s->rs_capacity)) return st;
+        if (dr_status st = guide_reset_counters(guide, stream)) return st;
+    }
+    GuideView gv{};
+    if (use_guide) gv.head = s->rs_guide_head.as<float>();
+    if (collect) {
+        gv.smp = guide->samples.as<dr_guide_sample>();
+        gv.aux = guide->aux.as<GuideAux>();
+        gv.counter = guide->counters.as<uint32_t>();
+        gv.pending = s->rs_pending.as<uint32_t>();
+        gv.cap = guide->stage_cap;
+        gv.frac = guide->collect_fraction;
     }
     CamParams cp;
     for (int i = 0; i < 12; ++i) cp.m[i] = cam->transform[i];
@@ -350,7 +463,8 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, do
         pp.max_depth = prm->max_depth;
         pp.rr_depth = prm->rr_depth;
         pp.schlick = prm->fresnel == DR_FRESNEL_SCHLICK ? 1u : 0u;
-        k_camera<<<blocks(pp.n), kT, 0, stream>>>(cp, pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt);
+        k_camera<<<blocks(pp.n), kT, 0, stream>>>(cp, pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt,
+                                                  collect ? gv.pending : nullptr);
         DR_HIP_TRY(hipGetLastError());
         for (uint32_t v = 0; v < prm->max_depth; ++v) {
             uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
@@ -359,9 +473,13 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, do
             uint32_t *cout = cnt + ((v + 1u) & 1u);
             if (dr_status st = launch_trace_ex(s, rays, lin, pp.n, cin, hits, nullptr, nullptr, nullptr, false, sched, stream))
                 return st;
+            const bool guided_v = use_guide && v < guide->guided_vertices;
+            if (guided_v)
+                if (dr_status st = guide_infer_paths(guide, s, rays, hits, lin, cin, pp.n, s->rs_guide_head.as<float>(), stream)) return st;
+            gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
             k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2);
             k_shade<<<blocks(pp.n), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
-                                                     cnt + 2);
+                                                     cnt + 2, gv);
             DR_HIP_TRY(hipGetLastError());
             if (s->n_emit) {
                 if (dr_status st = launch_trace_ex(s, srays, nullptr, pp.n, cnt + 2, nullptr, socc, nullptr, nullptr, true,
@@ -371,9 +489,13 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, do
                 DR_HIP_TRY(hipGetLastError());
             }
         }
+        if (collect)
+            if (dr_status st = guide_finalize(guide, rad, stream)) return st;
         k_resolve<<<blocks(std::min(pp.n, npix)), kT, 0, stream>>>(pp, rad, accum);
         DR_HIP_TRY(hipGetLastError());
     }
+    if (collect)
+        if (dr_status st = guide_compact(guide, stream)) return st;
     return DR_OK;
 }
 

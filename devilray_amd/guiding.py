@@ -1,0 +1,70 @@
+"""Progressive rendering with online guide training (host driver over the C ABI).
+
+A pass renders `spp_per_pass` samples per pixel into one device-resident FP64
+accumulation buffer; while training, the same pass collects training samples
+(eligible Lambert vertices, thinned to fit the guide's stage; those that
+carried radiance enter the guide's training ring) and `train_steps` Adam
+steps on the ring follow on the device. Every pass's samples are unbiased
+estimates whatever the state of the network, so all passes are accumulated
+into the image. The plain sampler is the same loop with no guide.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import DR_GUIDE_COLLECT, DR_GUIDE_USE, DrCamera, DrRenderParams, Guide, Scene
+
+
+class ProgressiveRenderer:
+    def __init__(self, scene: Scene, camera: DrCamera, guide: Guide | None = None, spp_per_pass: int = 4,
+                 max_depth: int = 12, rr_depth: int = 3, seed: int = 0, train_passes: int | None = None,
+                 train_steps: int = 1, stream=None):
+        self.scene, self.camera, self.guide = scene, camera, guide
+        self.spp_per_pass = spp_per_pass
+        self.max_depth, self.rr_depth, self.seed = max_depth, rr_depth, seed
+        self.train_passes = train_passes  # None: train on every pass
+        self.train_steps = train_steps
+        self.stream = stream
+        self.npix = camera.width * camera.height
+        self.accum = scene.ctx.alloc(self.npix * 32)
+        self.accum.copy_from(np.zeros((self.npix, 4), np.float64))
+        self.passes = 0
+        self.trained = 0
+        self._fraction_set = False
+
+    def _training(self) -> bool:
+        return self.guide is not None and (self.train_passes is None or self.passes < self.train_passes)
+
+    def step(self):
+        """Render one pass (and train once while training). Asynchronous,
+        except for one synchronisation after the first collecting pass that
+        sizes the collection fraction."""
+        p = DrRenderParams(self.spp_per_pass, self.passes * self.spp_per_pass, self.max_depth, self.rr_depth,
+                           self.seed, 0, 0)
+        if self.guide is None:
+            self.scene.render_dev(self.camera, p, self.accum.ptr, self.stream)
+        else:
+            flags = DR_GUIDE_USE if self.trained > 0 else 0
+            train = self._training()
+            if train:
+                flags |= DR_GUIDE_COLLECT
+            self.scene.render_guided_dev(self.camera, p, self.guide, flags, self.accum.ptr, self.stream)
+            if train:
+                if not self._fraction_set:
+                    _, total = self.guide.copy_samples(0)
+                    if total > 0:
+                        self.guide.set_options(min(1.0, 4 * self.guide.capacity / total), self.guide.guided_vertices)
+                    self._fraction_set = True
+                for _ in range(self.train_steps):
+                    self.guide.train_step_dev(stream=self.stream)
+                self.trained += 1
+        self.passes += 1
+
+    def image(self) -> np.ndarray:
+        """Mean radiance per pixel (height, width, 3), FP64."""
+        self.scene.ctx.synchronize()
+        a = self.accum.to_numpy(np.float64, self.npix * 4).reshape(self.camera.height, self.camera.width, 4)
+        return a[..., :3] / np.maximum(a[..., 3:4], 1.0)
+
+    def close(self):
+        self.accum.free()

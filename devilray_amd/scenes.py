@@ -113,9 +113,11 @@ def many_lights(n_spheres: int = 400, subdiv: int = 4, n_emissive: int = 100_000
     log-uniform scales in [0.5, 2.5] and uniform random rotations. 30% are
     dielectric with IOR ~ U[1.3, 1.7], the rest Lambert with albedo ~
     U[0.2, 0.9]. n_emissive triangles of Lambert spheres emit radiance
-    LogNormal(0, 2) (grey). With partition=True all emitters sit behind an
-    opaque wall (z = -extent - 4 .. ) with a gap, so that nearly all light
-    reaching the camera side is indirect (the guided-sampling scene).
+    LogNormal(0, 2) (grey). With partition=True a quarter of the spheres move
+    behind an opaque partition at z = -extent - 4, all emitters are on those,
+    and a closed room encloses the scene with a gap between the partition and
+    the ceiling; the camera looks down at (0, -0.6 extent, 0), so that what it
+    sees is lit indirectly, off the ceiling (the guided-sampling scene).
     Camera: 1920x1080, 45 degree FOV, looking at the origin from +z."""
     rng = np.random.default_rng(seed)
     base_v, base_f = gen_icosphere(subdiv, 0.0, 0)
@@ -160,23 +162,54 @@ def many_lights(n_spheres: int = 400, subdiv: int = 4, n_emissive: int = 100_000
         tri_mat[t] = first + k
     b_v, b_f, b_m = [verts], [faces], [tri_mat]
     if partition:
-        # opaque wall at z = -extent - 4 with a 4-unit gap along the top
-        wall = _Builder()
-        wall.nv = verts.shape[0]
+        # closed grey room around everything (the camera included) and an
+        # opaque two-sided partition at z = -extent - 4 that leaves a gap of
+        # `gap` units under the ceiling: light crossing the gap rises onto the
+        # ceiling, which the camera (tilted down) does not see, so everything
+        # in view is lit indirectly
+        room = _Builder()
+        room.nv = verts.shape[0]
         wm = len(mats)
-        mats.append((DR_MAT_LAMBERT, (0.5, 0.5, 0.5), (0.0, 0.0, 0.0), 1.0))
-        z = -extent - 4.0
-        h = 2 * extent + 20
-        wall.quad((0, -4.0, z), (0, 0, 1), (1, 0, 0), h, h - 8.0, wm)
-        wall.quad((0, -4.0, z - 0.1), (0, 0, -1), (1, 0, 0), h, h - 8.0, wm)
-        for wv, wf, wmat in zip(wall.v, wall.f, wall.m):
+        mats.append((DR_MAT_LAMBERT, (0.7, 0.7, 0.7), (0.0, 0.0, 0.0), 1.0))
+        R = extent + 12.0
+        z_lo, z_hi, z_p, gap = -extent - 18.0, 3.2 * extent + 6.0, -extent - 4.0, 6.0
+        zc, lz = (z_lo + z_hi) / 2, z_hi - z_lo
+        room.quad((0, -R, zc), (0, 1, 0), (1, 0, 0), 2 * R, lz, wm)
+        room.quad((0, R, zc), (0, -1, 0), (1, 0, 0), 2 * R, lz, wm)
+        room.quad((-R, 0, zc), (1, 0, 0), (0, 0, 1), lz, 2 * R, wm)
+        room.quad((R, 0, zc), (-1, 0, 0), (0, 0, 1), lz, 2 * R, wm)
+        room.quad((0, 0, z_lo), (0, 0, 1), (1, 0, 0), 2 * R, 2 * R, wm)
+        room.quad((0, 0, z_hi), (0, 0, -1), (1, 0, 0), 2 * R, 2 * R, wm)
+        room.quad((0, -gap / 2, z_p), (0, 0, 1), (1, 0, 0), 2 * R, 2 * R - gap, wm)
+        room.quad((0, -gap / 2, z_p - 0.1), (0, 0, -1), (1, 0, 0), 2 * R, 2 * R - gap, wm)
+        for wv, wf, wmat in zip(room.v, room.f, room.m):
             b_v.append(wv)
             b_f.append(wf)
             b_m.append(wmat)
-    cam = dict(eye=(0.0, 0.0, 3.2 * extent), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
-               width=1920, height=1080)
+    cam = dict(eye=(0.0, 0.0, 3.2 * extent), target=(0.0, -0.6 * extent if partition else 0.0, 0.0),
+               up=(0.0, 1.0, 0.0), fov=45.0, width=1920, height=1080)
     return SceneData(np.concatenate(b_v).astype(np.float32), np.concatenate(b_f).astype(np.uint32),
                      np.concatenate(b_m).astype(np.uint32), mats, cam)
+
+
+def hidden_light_reflector() -> SceneData:
+    """Guiding test scene: a grey floor (20 x 20, y = 0) lit only by a 3 x 3
+    white reflector facing down at y = 8. The emitter (1 x 1, radiance 50,
+    y = 7.5) sits under the reflector above a black blocker (1.6 x 1.6,
+    y = 7.4), so no floor point sees it and next-event estimation from the
+    floor always fails: the floor's light arrives from the reflector's small
+    solid angle. Camera above the floor looking down at it."""
+    b = _Builder()
+    floor = b.material(DR_MAT_LAMBERT, (0.5, 0.5, 0.5))
+    refl = b.material(DR_MAT_LAMBERT, (0.9, 0.9, 0.9))
+    black = b.material(DR_MAT_LAMBERT, (0.0, 0.0, 0.0))
+    light = b.material(DR_MAT_LAMBERT, (0.0, 0.0, 0.0), (50.0, 50.0, 50.0))
+    b.quad((0, 0, 0), (0, 1, 0), (1, 0, 0), 20.0, 20.0, floor)
+    b.quad((0, 8.0, 0), (0, -1, 0), (1, 0, 0), 3.0, 3.0, refl)
+    b.quad((0, 7.5, 0), (0, 1, 0), (1, 0, 0), 1.0, 1.0, light)
+    b.quad((0, 7.4, 0), (0, -1, 0), (1, 0, 0), 1.6, 1.6, black)
+    cam = dict(eye=(0.0, 14.0, 16.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=50.0, width=256, height=256)
+    return b.build(cam)
 
 
 def furnace_planes(emittance: float = 0.55) -> SceneData:

@@ -216,6 +216,88 @@ dr_status dr_scene_copy_lights(dr_scene *scene, void *entries, float *pdf_area);
 dr_status dr_render(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, double *accum,
                     dr_stream stream);
 
+/* ---- learned path guide (no reference counterpart: the reference samples
+ * Lambert bounces from the BSDF only, PathGeneration.hpp:406-420) ----------
+ *
+ * Network: 32 input features (frequency encoding of the position within the
+ * guide's box, the shading normal and the outgoing direction) -> 4 hidden
+ * ReLU layers of 64 -> 48 outputs, of which 41 are used: an 8-lobe von
+ * Mises-Fisher mixture (per lobe a mean direction, log concentration and
+ * mixture logit) and one selection logit alpha = 0.1 + 0.8 sigmoid(s).
+ * Weights and activations are BF16 on the matrix cores, accumulation FP32;
+ * the FP32 master parameters and Adam moments stay on the device.
+ *
+ * Parameter vector (DR_GUIDE_N_PARAMS floats): for layer l = 0..4 with
+ * in = {32,64,64,64,64}, out = {64,64,64,64,48}: W_l row-major [out][in]
+ * then b_l [out]. Output row r of the last layer means: with g = (r % 16)/4
+ * and q = 4*(r/16) + r%4, lobe 2g+q/5 component q%5 (mu x, mu y, mu z,
+ * log kappa, logit) for q < 10; the selection logit for g = 0, q = 10; unused
+ * otherwise. */
+#define DR_GUIDE_N_PARAMS 17712
+#define DR_GUIDE_N_LOBES 8
+#define DR_GUIDE_HEAD_FLOATS 44 /* 8 x {mu x, mu y, mu z, kappa, weight}, alpha, 3 pad */
+
+typedef struct dr_guide dr_guide;
+
+typedef struct {
+    float bounds[6];     /* box the position encoding is normalised to: lo xyz, hi xyz */
+    uint32_t capacity;   /* training ring: latest samples of non-zero weight (e.g. 1 << 18);
+                          * a collecting render stages up to 4 x capacity vertices */
+    float lr;            /* Adam step size (1e-2) */
+    float beta1, beta2;  /* Adam moment decay (0.9, 0.99) */
+    float eps;           /* Adam epsilon (1e-8) */
+    uint64_t seed;       /* parameter initialisation */
+} dr_guide_params;
+
+/* One training sample (64 B): a Lambert vertex at pos with shading normal,
+ * outgoing direction wo (towards the previous vertex) and sampled incident
+ * direction wi; weight = luminance(L_i(wi)) * max(0, n . wi) / pdf(wi), the
+ * one-sample estimate of the target product over the pdf it was drawn from. */
+typedef struct {
+    float pos[3], weight;
+    float normal[3], pad0;
+    float wo[3], pad1;
+    float wi[3], pad2;
+} dr_guide_sample;
+
+dr_status dr_guide_create(dr_context *ctx, const dr_guide_params *params, dr_guide **out);
+dr_status dr_guide_destroy(dr_guide *guide);
+/* FP32 master parameters (host arrays of DR_GUIDE_N_PARAMS floats). Setting
+ * them resets the Adam moments and step count. */
+dr_status dr_guide_get_params(dr_guide *guide, float *params);
+dr_status dr_guide_set_params(dr_guide *guide, const float *params);
+/* Network evaluation for n samples (device arrays; only pos, normal and wo
+ * are read): head[DR_GUIDE_HEAD_FLOATS * i] = the decoded mixture. */
+dr_status dr_guide_eval(dr_guide *guide, const dr_guide_sample *samples, uint32_t n, float *head, dr_stream stream);
+/* Gradient of the training loss over n samples: with q the mixture density
+ * at wi and b = max(0, n . wi) / pi, the lobes descend
+ * sum_i weight_i * -log(q/2 + b/2) and the selection logit descends
+ * sum_i weight_i * -log(alpha q + (1 - alpha) b) with q held fixed. grad
+ * receives DR_GUIDE_N_PARAMS + 1 floats (device), the last sum_i weight_i. */
+dr_status dr_guide_gradient(dr_guide *guide, const dr_guide_sample *samples, uint32_t n, float *grad,
+                            dr_stream stream);
+/* One Adam step on the weight-normalised loss. samples == NULL trains on the
+ * guide's ring of the latest non-zero-weight samples from collecting renders
+ * (fill read on the device). */
+dr_status dr_guide_train_step(dr_guide *guide, const dr_guide_sample *samples, uint32_t n, dr_stream stream);
+/* Samples staged by the last collecting render, weights included (each such
+ * render starts a new stage of at most 4 x capacity; *n_total counts all
+ * offered). host may be NULL to query the counts. Synchronises. */
+dr_status dr_guide_copy_samples(dr_guide *guide, dr_guide_sample *host, uint32_t max_n, uint32_t *n_kept,
+                                uint32_t *n_total);
+/* Fraction of eligible vertices a collecting render records (default 1),
+ * and the number of leading path vertices that sample from the guide
+ * (default 64 = all; later vertices sample the BSDF alone). */
+dr_status dr_guide_set_options(dr_guide *guide, float collect_fraction, uint32_t guided_vertices);
+
+/* dr_render with the guide: DR_GUIDE_USE samples Lambert bounces from the
+ * one-sample MIS of the guide mixture and the cosine BSDF (with probability
+ * alpha from the guide), and NEE weights against that combined pdf;
+ * DR_GUIDE_COLLECT gathers training samples. guide_flags == 0 is dr_render. */
+enum { DR_GUIDE_USE = 1, DR_GUIDE_COLLECT = 2 };
+dr_status dr_render_guided(dr_scene *scene, const dr_camera *camera, const dr_render_params *params,
+                           dr_guide *guide, uint32_t guide_flags, double *accum, dr_stream stream);
+
 /* ---- workload generators (host, deterministic; no reference counterpart:
  * the reference's tests build their meshes by hand) ---- */
 /* Icosphere with 20*4^subdiv triangles, shared vertices, outward winding;

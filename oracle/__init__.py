@@ -41,6 +41,12 @@ PROTOTYPES = {
     "or_lights_build": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "or_sincos_2pi": (None, [C.c_float, _vp, _vp]),
     "or_render": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32]),
+    "or_guide_encode": (None, [_vp, _vp, _vp]),
+    "or_guide_forward": (None, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "or_guide_loss": (C.c_double, [_vp, _vp, _vp, _vp, _i32, _i32]),
+    "or_guide_grad": (C.c_double, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    "or_vmf_pdf": (C.c_double, [_vp, C.c_double, _vp]),
+    "or_vmf_sample": (None, [_vp, C.c_double, C.c_double, C.c_double, _vp]),
 }
 
 MATERIAL_DTYPE = np.dtype([("kind", np.int32), ("albedo", np.float32, 3), ("emission", np.float32, 3),
@@ -251,3 +257,65 @@ class RefBBH:
         lib().or_refbbh_trace(self.h, _p(self.v), _p(self.f), _p(rays), n, _p(hits), _p(bt), _p(tt),
                               nthreads or threads())
         return (hits, bt, tt) if counts else hits
+
+
+# ---- learned path guide (FP64) ---------------------------------------------
+GUIDE_N_PARAMS = 17712
+GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
+                               ("pad0", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
+                               ("wi", np.float32, 3), ("pad2", np.float32)])
+
+
+def _guide_args(params, bounds, samples):
+    P = np.ascontiguousarray(params, np.float64)
+    assert P.size == GUIDE_N_PARAMS
+    b = np.ascontiguousarray(bounds, np.float32)
+    s = np.ascontiguousarray(samples, GUIDE_SAMPLE_DTYPE)
+    return P, b, s
+
+
+def guide_encode(bounds, sample) -> np.ndarray:
+    b = np.ascontiguousarray(bounds, np.float32)
+    s = np.ascontiguousarray(np.atleast_1d(sample), GUIDE_SAMPLE_DTYPE)
+    out = np.zeros(32, np.float64)
+    lib().or_guide_encode(_p(b), _p(s), _p(out))
+    return out
+
+
+def guide_forward(params, bounds, samples, emulate_bf16: bool = False):
+    """(raw outputs n x 48, decoded head n x 44)."""
+    P, b, s = _guide_args(params, bounds, samples)
+    raw = np.zeros((len(s), 48), np.float64)
+    head = np.zeros((len(s), 44), np.float64)
+    lib().or_guide_forward(_p(P), _p(b), _p(s), len(s), int(emulate_bf16), _p(raw), _p(head))
+    return raw, head
+
+
+def guide_loss(params, bounds, samples, emulate_bf16: bool = False, frozen=None) -> float:
+    """Training loss; `frozen` parameters give the mixture inside the
+    selection term (default: params themselves)."""
+    P, b, s = _guide_args(params, bounds, samples)
+    F = None if frozen is None else np.ascontiguousarray(frozen, np.float64)
+    return float(lib().or_guide_loss(_p(P), _p(F) if F is not None else None, _p(b), _p(s), len(s),
+                                     int(emulate_bf16)))
+
+
+def guide_grad(params, bounds, samples, emulate_bf16: bool = False):
+    """(gradient of the loss, sum of sample weights)."""
+    P, b, s = _guide_args(params, bounds, samples)
+    g = np.zeros(GUIDE_N_PARAMS, np.float64)
+    sw = lib().or_guide_grad(_p(P), _p(b), _p(s), len(s), int(emulate_bf16), _p(g))
+    return g, float(sw)
+
+
+def vmf_pdf(mu, kappa: float, w) -> float:
+    m = np.ascontiguousarray(mu, np.float64)
+    x = np.ascontiguousarray(w, np.float64)
+    return float(lib().or_vmf_pdf(_p(m), float(kappa), _p(x)))
+
+
+def vmf_sample(mu, kappa: float, x1: float, x2: float) -> np.ndarray:
+    m = np.ascontiguousarray(mu, np.float64)
+    out = np.zeros(3, np.float64)
+    lib().or_vmf_sample(_p(m), float(kappa), float(x1), float(x2), _p(out))
+    return out

@@ -132,6 +132,37 @@ void or_render(const void *bvh, const float *verts, const int32_t *tris, const i
                const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
                double *accum, int32_t *vertex_hist, int32_t threads);
 
+/* ---- learned path guide: FP64 network, head, loss and gradient ----------- */
+#define OR_GUIDE_NP 17712
+typedef struct {
+    float pos[3], weight;
+    float normal[3], pad0;
+    float wo[3], pad1;
+    float wi[3], pad2;
+} or_guide_sample; /* = dr_guide_sample */
+
+/* 32 input features of a sample (positions normalised to bounds lo..hi). */
+void or_guide_encode(const float bounds[6], const or_guide_sample *s, double feat[32]);
+/* Network outputs (48 raw) and decoded head (44: 8 x {mu xyz, kappa,
+ * weight}, alpha, pad) for n samples. emulate_bf16 != 0 rounds weights,
+ * features and activations to BF16 (round to nearest even) where the device
+ * kernels do; 0 evaluates the real-valued network. */
+void or_guide_forward(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
+                      int32_t emulate_bf16, double *raw, double *head);
+/* Loss sum_i w_i * [-log(q/2 + cos+/(2 pi)) - log(alpha q0 + (1 - alpha) cos+/pi)]
+ * with q, alpha from params and q0 the mixture of `frozen` (NULL: params). */
+double or_guide_loss(const double *params, const double *frozen, const float bounds[6], const or_guide_sample *s,
+                     int32_t n, int32_t emulate_bf16);
+/* Gradient of the loss (OR_GUIDE_NP doubles); returns sum_i w_i. With
+ * emulate_bf16 the backward pass also rounds the output and hidden deltas
+ * to BF16 as the device does. */
+double or_guide_grad(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
+                     int32_t emulate_bf16, double *grad);
+/* vMF density kappa / (2 pi (1 - e^-2kappa)) e^(kappa (mu.w - 1)) and
+ * W. Jakob's stable sampler (2012): w = 1 + log(x1 + (1 - x1) e^-2kappa) / kappa. */
+double or_vmf_pdf(const double mu[3], double kappa, const double w[3]);
+void or_vmf_sample(const double mu[3], double kappa, double x1, double x2, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif
