@@ -8,7 +8,11 @@ Workload (one "step" = one dr_trace_closest launch over the whole batch):
 Inputs are resident in HBM before the timed region. "extra" reports the
 coherent 1024x1024 camera batch, any-hit, the BVH build, and the two render
 workloads (Cornell box 256x256 x 16 spp; 2,048,000-triangle many-lights scene
-at 1920x1080, timed over a few samples per pixel).
+at 1920x1080, timed over a few samples per pixel), and the learned guide:
+one training step on 2^18 samples (TFLOP/s against the BF16 MFMA peak) and
+the time to a target MSE, guided vs plain, on the indirect-lit partition
+scene (240x135, reference 65,536 spp of the plain path in FP64) and on the
+hidden-light reflector scene.
 
 Prints one JSON line. `roofline.achieved` uses the algorithmic bytes per ray
 48 + 32*N_node + 40*N_tri with N from the canonical binary SAH BVH counts
@@ -115,6 +119,111 @@ def render_rate(ctx, sc, spp, max_depth, rr_depth, reps, width=None, height=None
             "image_mean": round(mean, 6), "resolution": [w, h], "max_depth": max_depth, "rr_depth": rr_depth}
 
 
+GUIDE_FLOP_PER_SAMPLE = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 48) * 2 + 2 * (3 * 64 * 64 + 48 * 64)
+
+
+def guide_train_rate(ctx):
+    """One Adam step (fused forward + backward + reduction + update) on 2^18
+    samples: mean HIP-event time and algorithmic TFLOP/s against the BF16
+    MFMA peak."""
+    n = 1 << 18
+    rng = np.random.default_rng(0)
+    smp = np.zeros(n, dr.GUIDE_SAMPLE_DTYPE)
+    smp["pos"] = rng.uniform(-1, 1, (n, 3))
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    wi = rng.normal(size=(n, 3))
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    wi *= np.sign(np.sum(wi * nrm, 1))[:, None]
+    smp["normal"], smp["wo"], smp["wi"] = nrm, nrm, wi
+    smp["weight"] = rng.exponential(1.0, n)
+    g = dr.Guide(ctx, [-1, -1, -1, 1, 1, 1], capacity=n, seed=0)
+    d = ctx.upload(smp)
+    _, per = timed_launches(lambda: g.train_step_dev(d.ptr, n), 20, 3, ctx)
+    g.close()
+    ms = float(np.mean(per))
+    tflops = GUIDE_FLOP_PER_SAMPLE * n / (ms * 1e-3) / 1e12
+    return {"samples": n, "ms_per_step": round(ms, 4), "flop_per_sample": GUIDE_FLOP_PER_SAMPLE,
+            "tflops": round(tflops, 2), "bf16_peak_tflops": 2500.0, "frac": round(tflops / 2500.0, 4)}
+
+
+def time_to_mse(ctx, sc, cam, ref, guide, spp_per_pass, passes, checkpoint, seed, train_steps=1, max_depth=12,
+                train_passes=None):
+    """Progressive render; (wall seconds, MSE) at every checkpoint, with the
+    image readback outside the timed spans."""
+    from devilray_amd.guiding import ProgressiveRenderer
+    r = ProgressiveRenderer(sc, cam, guide, spp_per_pass=spp_per_pass, max_depth=max_depth, seed=seed,
+                            train_steps=train_steps, train_passes=train_passes)
+    t, curve = 0.0, []
+    for k in range(passes):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r.step()
+        ctx.synchronize()
+        t += time.perf_counter() - t0
+        if (k + 1) % checkpoint == 0:
+            curve.append((t, float(np.mean((r.image() - ref) ** 2))))
+    r.close()
+    return curve
+
+
+def time_for(curve, target):
+    """Seconds until the MSE first reaches target (interpolated between
+    checkpoints; extrapolated as MSE ~ 1/t past the last one)."""
+    prev = None
+    for t, m in curve:
+        if m <= target:
+            if prev is None:
+                return t
+            (t0, m0) = prev
+            return t0 + (t - t0) * (m0 - target) / (m0 - m)
+        prev = (t, m)
+    t, m = curve[-1]
+    return t * m / target
+
+
+def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, checkpoint, train_steps, max_depth,
+                    capacity=1 << 18, train_passes=None, seeds=3):
+    """Equal-quality time: the plain sampler renders passes x spp; the target
+    is its final MSE against a ref_spp reference (the same plain path, FP64
+    accumulated); the guided sampler (trained online while it renders)
+    renders until it reaches that MSE. MSE and time curves are averaged over
+    `seeds` independent runs of each (the MSE of a few hundred spp is
+    dominated by rare paths and varies between seeds)."""
+    s = dr.Scene(ctx, sd.vertices, sd.indices, sd.tri_material, sd.dr_materials())
+    s.build_bvh(leaf_max=4)
+    c = sd.camera
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], width, height)
+    t0 = time.perf_counter()
+    ref = s.render(cam, spp=ref_spp, max_depth=max_depth, rr_depth=3, seed=77)
+    ref_s = time.perf_counter() - t0
+    ref = ref[..., :3] / ref[..., 3:4]
+    def avg(curves):
+        return [(float(np.mean([c[i][0] for c in curves])), float(np.mean([c[i][1] for c in curves])))
+                for i in range(len(curves[0]))]
+
+    plain = avg([time_to_mse(ctx, s, cam, ref, None, spp_per_pass, passes, checkpoint, 900 + k, max_depth=max_depth)
+                 for k in range(seeds)])
+    runs = []
+    for k in range(seeds):
+        g = dr.Guide(ctx, s.bounds(), capacity=capacity, seed=3 + k)
+        runs.append(time_to_mse(ctx, s, cam, ref, g, spp_per_pass, 2 * passes, checkpoint, 950 + k, train_steps,
+                                max_depth, train_passes))
+        g.close()
+    guided = avg(runs)
+    s.close()
+    target = plain[-1][1]
+    tp, tg = plain[-1][0], time_for(guided, target)
+    return {"resolution": [width, height], "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
+            "spp_per_pass": spp_per_pass, "plain_spp": spp_per_pass * passes, "train_steps_per_pass": train_steps,
+            "train_passes": train_passes, "seeds": seeds,
+            "guided_over_plain_mse_equal_spp": round(guided[passes // checkpoint - 1][1] / target, 3),
+            "target_mse": target, "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1),
+            "guided_over_plain_time": round(tg / tp, 3),
+            "mse_plain_curve": [[round(t * 1e3, 1), m] for t, m in plain],
+            "mse_guided_curve": [[round(t * 1e3, 1), m] for t, m in guided]}
+
+
 def probe() -> int:
     """Traversal launches + a calibration copy, for rocprofv3 counter runs."""
     v, f = dr.gen_icosphere(SUBDIV, SIGMA, SEED)
@@ -142,6 +251,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-guide", action="store_true")
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--probe", action="store_true")
     args = ap.parse_args()
@@ -207,12 +317,23 @@ def main() -> int:
         log("[bench] render: many-lights scene 1920x1080")
         renders["many_lights_1080p"] = render_rate(ctx, scenes.many_lights(), 2, 12, 3, 2)
 
+    guide = {}
+    if not args.no_guide:
+        log("[bench] guide: training step throughput")
+        guide["train_step"] = guide_train_rate(ctx)
+        log("[bench] guide: indirect-lit partition scene (2,048,000 sphere tris), guided vs plain")
+        guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
+                                                   8, 64, 8, 4, 12)
+        log("[bench] guide: hidden-light reflector scene, guided vs plain")
+        guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 256, 256, 65536,
+                                                      16, 64, 8, 16, 8, capacity=1 << 16, train_passes=8)
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(v, f, rays, args.cpu_sample)
 
     out = {
-        "metric": "Mrays/s incoherent closest-hit",
+        "metric": "Mrays/s incoherent closest-hit; ms to target MSE, guided vs plain",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -246,6 +367,7 @@ def main() -> int:
             "occluded_fraction": round(float(np.mean(occ)), 6),
             "stack_overflow": int(info2.overflow_flag),
             "render": renders,
+            "guide": guide,
         },
     }
     print(json.dumps(out), flush=True)

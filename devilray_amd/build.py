@@ -59,7 +59,31 @@ def build_library(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host(verbose)
     return LIB
+
+
+CPP = os.path.join(PKG, "cpp")
+HOST_LIB = os.path.join(PKG, "libdevilray_cpp.so")
+FACADE_TEST = os.path.join(CPP, "test_facade")
+
+
+def build_host(verbose: bool = True) -> None:
+    """The C++ facade (Renderer / Scene / cast over the C ABI) as
+    libdevilray_cpp.so, and its test program; plain g++, no HIP headers."""
+    common = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}"]
+    link = [f"-L{PKG}", "-ldevilray", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    steps = [
+        common + ["-shared", "-o", HOST_LIB, os.path.join(CPP, "devilray.cpp")] + link,
+        common + ["-o", FACADE_TEST, os.path.join(CPP, "test_facade.cpp"), f"-L{PKG}", "-ldevilray_cpp",
+                  "-ldevilray", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"],
+    ]
+    for cmd in steps:
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"g++ failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {HOST_LIB} and {FACADE_TEST}")
 
 
 if __name__ == "__main__":

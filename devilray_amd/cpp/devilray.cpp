@@ -1,0 +1,304 @@
+// C++ host facade over the C ABI: see devilray.hpp.
+#include "devilray.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace devilray {
+
+void check(dr_status s)
+{
+    if (s != DR_OK) throw Error(s, std::string(dr_status_string(s)) + ": " + dr_last_error());
+}
+
+Material Material::diffuse(Vec3 reflectance, Vec3 emission)
+{
+    Material r;
+    r.m.kind = DR_MAT_LAMBERT;
+    r.m.albedo[0] = reflectance.x;
+    r.m.albedo[1] = reflectance.y;
+    r.m.albedo[2] = reflectance.z;
+    r.m.emission[0] = emission.x;
+    r.m.emission[1] = emission.y;
+    r.m.emission[2] = emission.z;
+    r.m.ior = 1.0f;
+    return r;
+}
+
+Material Material::transparent(float ior)
+{
+    Material r;
+    r.m.kind = DR_MAT_DIELECTRIC;
+    r.m.albedo[0] = r.m.albedo[1] = r.m.albedo[2] = 1.0f;
+    r.m.ior = ior;
+    return r;
+}
+
+Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size)
+{
+    // up = right x normal; corners (-,-), (+,-), (+,+), (-,+); triangles (0,2,1), (0,3,2)
+    const Vec3 up{right.y * normal.z - right.z * normal.y, right.z * normal.x - right.x * normal.z,
+                  right.x * normal.y - right.y * normal.x};
+    const float h = size / 2.0f;
+    auto corner = [&](float a, float b) {
+        return Vec3{center.x + right.x * a + up.x * b, center.y + right.y * a + up.y * b,
+                    center.z + right.z * a + up.z * b};
+    };
+    Mesh m;
+    m.points = {corner(-h, -h), corner(h, -h), corner(h, h), corner(-h, h)};
+    m.triangles = {{0, 2, 1}, {0, 3, 2}};
+    return m;
+}
+
+Context::Context(int device) { check(dr_context_create(device, &ctx_)); }
+Context::~Context()
+{
+    if (ctx_) dr_context_destroy(ctx_);
+}
+
+Scene::Scene(Scene &&o) noexcept
+    : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_)
+{
+    o.scene_ = nullptr;
+}
+
+Scene &Scene::operator=(Scene &&o) noexcept
+{
+    if (this != &o) {
+        if (scene_) dr_scene_destroy(scene_);
+        materials = std::move(o.materials);
+        objects = std::move(o.objects);
+        scene_ = o.scene_;
+        n_tris_ = o.n_tris_;
+        o.scene_ = nullptr;
+    }
+    return *this;
+}
+
+Scene::~Scene()
+{
+    if (scene_) dr_scene_destroy(scene_);
+}
+
+void Scene::build(Context &ctx, uint32_t leaf_max)
+{
+    std::vector<float> verts;
+    std::vector<uint32_t> idx, tri_mat;
+    for (const Object &o : objects) {
+        if (o.material >= materials.size()) throw Error(DR_E_INVALID, "object material index out of range");
+        const uint32_t base = (uint32_t)(verts.size() / 3);
+        for (const Vec3 &p : o.mesh.points) {
+            verts.push_back(p.x);
+            verts.push_back(p.y);
+            verts.push_back(p.z);
+        }
+        for (const auto &t : o.mesh.triangles) {
+            for (uint32_t k : t) {
+                if (k >= o.mesh.points.size()) throw Error(DR_E_INVALID, "triangle vertex index out of range");
+                idx.push_back(base + k);
+            }
+            tri_mat.push_back(o.material);
+        }
+    }
+    std::vector<dr_material> mats;
+    for (const Material &m : materials) mats.push_back(m.m);
+    dr_scene_desc d{};
+    d.vertices = verts.data();
+    d.n_vertices = (uint32_t)(verts.size() / 3);
+    d.indices = idx.data();
+    d.n_triangles = (uint32_t)tri_mat.size();
+    d.tri_material = tri_mat.data();
+    d.materials = mats.data();
+    d.n_materials = (uint32_t)mats.size();
+    if (scene_) {
+        dr_scene_destroy(scene_);
+        scene_ = nullptr;
+    }
+    check(dr_scene_create(ctx.handle(), &d, &scene_));
+    dr_bvh_params bp{leaf_max, 0};
+    check(dr_scene_build_bvh(scene_, &bp, nullptr));
+    n_tris_ = d.n_triangles;
+}
+
+namespace {
+struct DeviceBuffer {
+    Context &ctx;
+    void *ptr = nullptr;
+    DeviceBuffer(Context &c, size_t bytes) : ctx(c) { check(dr_device_alloc(c.handle(), bytes ? bytes : 16, &ptr)); }
+    ~DeviceBuffer() { dr_device_free(ctx.handle(), ptr); }
+};
+}  // namespace
+
+std::vector<std::optional<Intersection>> cast(Context &ctx, const Scene &scene, const std::vector<Ray> &rays)
+{
+    if (!scene.handle()) throw Error(DR_E_INVALID, "scene not built");
+    const size_t n = rays.size();
+    std::vector<dr_ray> in(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Ray &r = rays[i];
+        in[i] = dr_ray{r.p.x, r.p.y, r.p.z, r.tmin, r.v.x, r.v.y, r.v.z, r.tmax};
+    }
+    DeviceBuffer dr(ctx, sizeof(dr_ray) * n), dh(ctx, sizeof(dr_hit) * n);
+    check(dr_memcpy_htod(ctx.handle(), dr.ptr, in.data(), sizeof(dr_ray) * n));
+    check(dr_trace_closest(scene.handle(), (const dr_ray *)dr.ptr, (uint32_t)n, (dr_hit *)dh.ptr, nullptr));
+    std::vector<dr_hit> hits(n);
+    check(dr_memcpy_dtoh(ctx.handle(), hits.data(), dh.ptr, sizeof(dr_hit) * n));
+    std::vector<std::optional<Intersection>> out(n);
+    for (size_t i = 0; i < n; ++i)
+        if (hits[i].id >= 0)
+            out[i] = Intersection{hits[i].t, Vec3{1.0f - hits[i].u - hits[i].v, hits[i].u, hits[i].v}, hits[i].id};
+    return out;
+}
+
+Renderer::Renderer(Context &ctx, Size2i resolution) : ctx_(ctx), res_(resolution)
+{
+    if (resolution.width <= 0 || resolution.height <= 0) throw Error(DR_E_INVALID, "empty resolution");
+    const size_t npix = (size_t)resolution.width * resolution.height;
+    check(dr_device_alloc(ctx.handle(), npix * 4 * sizeof(double), &accum_));
+    pixels_.assign(npix, 0u);
+    display_.assign(npix, 0u);
+}
+
+Renderer::~Renderer()
+{
+    if (accum_) dr_device_free(ctx_.handle(), accum_);
+}
+
+void Renderer::setScene(Scene scene)
+{
+    std::scoped_lock guard{render_mutex_};
+    scene_ = std::move(scene);
+    if (!scene_.handle()) scene_.build(ctx_);
+    check(dr_scene_build_lights(scene_.handle(), nullptr));
+    needs_clear_ = true;
+}
+
+void Renderer::setCamera(Camera cam)
+{
+    std::scoped_lock guard{render_mutex_};
+    cam_ = cam;
+    needs_clear_ = true;
+}
+
+void Renderer::setPixelSampling(PixelSampling sampling)
+{
+    std::scoped_lock guard{render_mutex_};
+    sampling_ = sampling;
+    needs_clear_ = true;
+}
+
+void Renderer::setOutputOptions(OutputOptions options)
+{
+    std::scoped_lock guard{render_mutex_};
+    output_ = options;
+}
+
+void Renderer::setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fresnel, uint64_t seed)
+{
+    std::scoped_lock guard{render_mutex_};
+    max_depth_ = max_depth;
+    rr_depth_ = rr_depth;
+    fresnel_ = fresnel;
+    seed_ = seed;
+    needs_clear_ = true;
+}
+
+void Renderer::clear()
+{
+    std::scoped_lock guard{render_mutex_};
+    needs_clear_ = true;
+}
+
+void Renderer::clearIfNeeded()
+{
+    if (!needs_clear_) return;
+    const size_t n = (size_t)res_.width * res_.height * 4;
+    std::vector<double> zeros(n, 0.0);
+    check(dr_memcpy_htod(ctx_.handle(), accum_, zeros.data(), n * sizeof(double)));
+    samples_ = 0;
+    needs_clear_ = false;
+}
+
+void Renderer::render(uint32_t spp)
+{
+    std::scoped_lock guard{render_mutex_};
+    if (!scene_.handle()) throw Error(DR_E_INVALID, "no scene set");
+    clearIfNeeded();
+    dr_camera c{};
+    std::memcpy(c.transform, cam_.transform.data(), sizeof c.transform);
+    c.focal_length = cam_.intrinsics.focal_length;
+    c.center[0] = cam_.intrinsics.center.x;
+    c.center[1] = cam_.intrinsics.center.y;
+    c.pixel_size[0] = cam_.physical_pixel_size.x;
+    c.pixel_size[1] = cam_.physical_pixel_size.y;
+    c.width = (uint32_t)res_.width;
+    c.height = (uint32_t)res_.height;
+    c.pixel_sampling = (uint32_t)sampling_;
+    dr_render_params p{};
+    p.spp = spp;
+    p.sample_offset = samples_;
+    p.max_depth = max_depth_;
+    p.rr_depth = rr_depth_;
+    p.seed = seed_;
+    p.fresnel = fresnel_;
+    check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
+    check(dr_synchronize(ctx_.handle()));
+    samples_ += spp;
+}
+
+const double *Renderer::getRawPixelsFP64()
+{
+    const size_t n = (size_t)res_.width * res_.height * 4;
+    host_fp64_.resize(n);
+    check(dr_memcpy_dtoh(ctx_.handle(), host_fp64_.data(), accum_, n * sizeof(double)));
+    return host_fp64_.data();
+}
+
+const Vec4 *Renderer::getRawPixels()
+{
+    const double *a = getRawPixelsFP64();
+    const size_t npix = (size_t)res_.width * res_.height;
+    host_vec4_.resize(npix);
+    for (size_t i = 0; i < npix; ++i)
+        host_vec4_[i] = Vec4{(float)a[4 * i], (float)a[4 * i + 1], (float)a[4 * i + 2], (float)a[4 * i + 3]};
+    return host_vec4_.data();
+}
+
+// sRGB transfer (clamped) when gamma-corrected, x255, round, pack
+// r | g<<8 | b<<16 | a<<24, rows flipped (src/Renderer.cpp:51-90).
+void Renderer::createPixels()
+{
+    const double *a = getRawPixelsFP64();
+    const bool gamma = output_.linearity == OutputLinearity::GammaCorrected;
+    auto tf = [gamma](float v) -> float {
+        if (!gamma) return v;
+        const float r = std::clamp(v, 0.0f, 1.0f);
+        if (r < 0.0031308f) return r * 12.92f;
+        return std::pow(r, 1.0f / 2.4f) * 1.055f - 0.055f;
+    };
+    auto u8 = [](float f) -> uint32_t { return (uint32_t)std::lround(std::max(0.0f, f)); };
+    for (int y = 0; y < res_.height; ++y)
+        for (int x = 0; x < res_.width; ++x) {
+            const double *p = a + 4 * ((size_t)y * res_.width + x);
+            float r = 0, g = 0, b = 0;
+            if (p[3] != 0.0) {
+                r = (float)(p[0] / p[3]);
+                g = (float)(p[1] / p[3]);
+                b = (float)(p[2] / p[3]);
+            }
+            const int fy = res_.height - y - 1;
+            pixels_[(size_t)fy * res_.width + x] =
+                u8(tf(r) * 255.0f) | (u8(tf(g) * 255.0f) << 8) | (u8(tf(b) * 255.0f) << 16) | (255u << 24);
+        }
+    std::scoped_lock guard{display_mutex_};
+    display_ = pixels_;
+}
+
+const uint32_t *Renderer::getPixels()
+{
+    std::scoped_lock guard{display_mutex_};
+    return display_.data();
+}
+
+}  // namespace devilray

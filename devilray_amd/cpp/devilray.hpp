@@ -1,0 +1,174 @@
+// C++ host facade over the C ABI (include/dr_api.h), keeping the reference's
+// names and contracts for the render path so that its callers switch over
+// unchanged:
+//   Renderer      include/Renderer.hpp:23-74 (setScene / setCamera /
+//                 setPixelSampling / setOutputOptions / render / clear /
+//                 createPixels / getPixels / getRawPixels)
+//   Scene         include/tracing/Scene.hpp:15-26 (materials + objects),
+//                 built with createQuadMesh (include/tracing/GpuTris.hpp:22-24)
+//   cast          include/tracing/IntersectionTestsImpl.hpp:234-251, here
+//                 over a batch of rays (the reference casts one ray per call)
+// Errors: a non-OK dr_status becomes devilray::Error (the reference throws
+// std::runtime_error on misuse, DeviceMemory.cu:51); no-hit is std::nullopt.
+// Threading: as the reference, setters and render() take one mutex and the
+// display copy another (include/Renderer.hpp:43-44).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dr_api.h"
+
+namespace devilray {
+
+struct Error : std::runtime_error {
+    dr_status status;
+    Error(dr_status s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+void check(dr_status s);
+
+struct Vec2 { float x, y; };
+struct Vec3 { float x, y, z; };
+struct Vec4 { float x, y, z, w; };
+struct Size2i { int width, height; };
+
+struct Ray {
+    Vec3 p, v;  // origin, direction
+    float tmin = 0.0f, tmax = 3.4e38f;
+};
+
+// Closest hit: t and the barycentric weights (w_A, w_B, w_C) of the
+// triangle's vertices, as the reference's TriangleIntersection
+// (include/tracing/Intersection.hpp:12-20), plus the triangle id.
+struct Intersection {
+    float t;
+    Vec3 bari;
+    int32_t triangle;
+};
+
+// DiffuseMaterial / TransparentMaterial (include/tracing/Material.hpp:9-25).
+struct Material {
+    static Material diffuse(Vec3 reflectance, Vec3 emission = {0, 0, 0});
+    static Material transparent(float ior);
+    dr_material m{};
+};
+
+struct Mesh {
+    std::vector<Vec3> points;
+    std::vector<std::array<uint32_t, 3>> triangles;
+};
+// Two-triangle square of side `size` (createQuadMesh, src/GpuTris.cpp:87-107).
+Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size);
+
+enum class PixelSampling : int { Center = 0, UniformRandom = 1 };
+enum class OutputLinearity { Linear, GammaCorrected };
+struct OutputOptions { OutputLinearity linearity = OutputLinearity::GammaCorrected; };
+
+struct Intrinsics {
+    float focal_length;
+    Vec2 center;
+};
+// Camera (include/tracing/Camera.hpp:6-19): rows 0..2 of the row-major 4x4
+// camera-to-world transform, pinhole intrinsics, physical pixel size.
+struct Camera {
+    std::array<float, 12> transform{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    Intrinsics intrinsics{1.0f, {0.0f, 0.0f}};
+    Size2i resolution{0, 0};
+    Vec2 physical_pixel_size{1.0f, 1.0f};
+};
+
+class Context {
+public:
+    explicit Context(int device = 0);
+    ~Context();
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    dr_context *handle() const { return ctx_; }
+
+private:
+    dr_context *ctx_ = nullptr;
+};
+
+// Scene: materials plus objects (a mesh and a material index each), copied
+// into one device scene and BVH at build() (setScene builds if needed).
+class Scene {
+public:
+    std::vector<Material> materials;
+    struct Object {
+        Mesh mesh;
+        uint32_t material;
+    };
+    std::vector<Object> objects;
+
+    Scene() = default;
+    Scene(Scene &&) noexcept;
+    Scene &operator=(Scene &&) noexcept;
+    ~Scene();
+
+    void build(Context &ctx, uint32_t leaf_max = 4);
+    dr_scene *handle() const { return scene_; }
+    uint32_t triangleCount() const { return n_tris_; }
+
+private:
+    dr_scene *scene_ = nullptr;
+    uint32_t n_tris_ = 0;
+};
+
+// Closest hit for each ray (std::nullopt on a miss); triangle ids index the
+// scene's triangles in object order.
+std::vector<std::optional<Intersection>> cast(Context &ctx, const Scene &scene, const std::vector<Ray> &rays);
+
+class Renderer {
+public:
+    Renderer(Context &ctx, Size2i resolution);
+    ~Renderer();
+    Renderer(const Renderer &) = delete;
+    Renderer &operator=(const Renderer &) = delete;
+
+    void setScene(Scene scene);
+    void setCamera(Camera cam);
+    void setPixelSampling(PixelSampling sampling);
+    void setOutputOptions(OutputOptions options);
+    // Path tracer parameters beyond the reference (max_depth counts path
+    // vertices; the reference's fixed behaviour is max_depth 10, no Russian
+    // roulette, Schlick Fresnel).
+    void setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fresnel = DR_FRESNEL_EXACT,
+                        uint64_t seed = 0);
+
+    // Adds exactly one sample per pixel and waits for the device (as the
+    // reference's render()); render(n) adds n in one launch sequence.
+    void render(uint32_t spp = 1);
+    void clear();
+
+    void createPixels();
+    const uint32_t *getPixels();
+    const Vec4 *getRawPixels();      // FP32 view {sum r, g, b, count}
+    const double *getRawPixelsFP64();  // the FP64 accumulation itself
+    uint32_t getSampleCount() const { return samples_; }
+
+private:
+    void clearIfNeeded();
+    Context &ctx_;
+    Size2i res_;
+    Scene scene_;
+    Camera cam_;
+    PixelSampling sampling_ = PixelSampling::UniformRandom;
+    OutputOptions output_;
+    uint32_t max_depth_ = 10, rr_depth_ = 10, fresnel_ = DR_FRESNEL_SCHLICK;
+    uint64_t seed_ = 0;
+    void *accum_ = nullptr;  // device FP64 RGBW
+    uint32_t samples_ = 0;
+    bool needs_clear_ = true;
+    std::vector<double> host_fp64_;
+    std::vector<Vec4> host_vec4_;
+    std::vector<uint32_t> pixels_, display_;
+    std::mutex render_mutex_, display_mutex_;
+};
+
+}  // namespace devilray

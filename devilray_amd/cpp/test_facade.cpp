@@ -1,0 +1,202 @@
+// The reference's own GoogleTest cases for the render path, restated against
+// the C++ facade (needs a GPU): triangle intersection cases
+// (test/src/test_intersection.cpp), nearest-of-stacked and cast
+// (test/src/test_tracing.cpp:221-298) and the analytic diffuse reflection
+// render (test/src/test_render.cpp:50-165). Prints one line per case and
+// exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "devilray.hpp"
+
+using namespace devilray;
+
+namespace {
+int g_fail = 0;
+#define EXPECT(cond)                                                                     \
+    do {                                                                                 \
+        if (!(cond)) {                                                                   \
+            std::printf("    FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);            \
+            ++g_fail;                                                                    \
+        }                                                                                \
+    } while (0)
+#define EXPECT_NEAR(a, b, tol) EXPECT(std::fabs((double)(a) - (double)(b)) <= (double)(tol))
+
+void run(const char *name, const std::function<void()> &fn)
+{
+    const int before = g_fail;
+    try {
+        fn();
+    } catch (const std::exception &e) {
+        std::printf("    exception: %s\n", e.what());
+        ++g_fail;
+    }
+    std::printf("%s %s\n", g_fail == before ? "PASS" : "FAIL", name);
+    std::fflush(stdout);
+}
+
+// A(0,0,0) B(1,0,0) C(0,1,0), the reference's unitTri
+Scene unit_triangle(Context &ctx)
+{
+    Scene s;
+    s.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
+    Mesh m;
+    m.points = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+    m.triangles = {{0, 1, 2}};
+    s.objects.push_back({m, 0});
+    s.build(ctx);
+    return s;
+}
+
+std::optional<Intersection> cast1(Context &ctx, const Scene &s, Vec3 p, Vec3 v)
+{
+    return cast(ctx, s, {Ray{p, v}})[0];
+}
+}  // namespace
+
+int main()
+{
+    Context ctx(0);
+    Scene tri = unit_triangle(ctx);
+
+    run("IntersectionTest.HitsCenterOfTriangle", [&] {
+        auto r = cast1(ctx, tri, {0.2f, 0.2f, 5.0f}, {0, 0, -1});
+        EXPECT(r.has_value());
+        if (!r) return;
+        EXPECT_NEAR(r->t, 5.0f, 1e-5f);
+        EXPECT(r->bari.x >= 0 && r->bari.y >= 0 && r->bari.z >= 0);
+        EXPECT_NEAR(r->bari.x + r->bari.y + r->bari.z, 1.0f, 1e-5f);
+        EXPECT_NEAR(r->bari.y, 0.2f, 1e-6f);
+        EXPECT_NEAR(r->bari.z, 0.2f, 1e-6f);
+    });
+    run("IntersectionTest.HitsExactlyOnVertex", [&] {
+        auto r = cast1(ctx, tri, {0, 0, 10}, {0, 0, -1});
+        EXPECT(r.has_value() && std::fabs(r->t - 10.0f) <= 1e-5f);
+    });
+    run("IntersectionTest.MissesJustOutsideEdge", [&] {
+        EXPECT(!cast1(ctx, tri, {-0.01f, 0.5f, 1.0f}, {0, 0, -1}).has_value());
+    });
+    run("IntersectionTest.MissesBehindRayOrigin", [&] {
+        EXPECT(!cast1(ctx, tri, {0.2f, 0.2f, 1.0f}, {0, 0, 1}).has_value());
+    });
+    run("IntersectionTest.ParallelRayDoesNotIntersect", [&] {
+        EXPECT(!cast1(ctx, tri, {-1.0f, 0.2f, 1.0f}, {1, 0, 0}).has_value());
+    });
+    auto single = [&](Vec3 a, Vec3 b, Vec3 c) {
+        Scene s;
+        s.materials.push_back(Material::diffuse({1, 1, 1}));
+        Mesh m;
+        m.points = {a, b, c};
+        m.triangles = {{0, 1, 2}};
+        s.objects.push_back({m, 0});
+        s.build(ctx);
+        return s;
+    };
+    run("IntersectionTest.DegenerateTriangleFails", [&] {
+        Scene s = single({0, 0, 0}, {0, 0, 0}, {0, 0, 0});
+        EXPECT(!cast1(ctx, s, {0, 0, 1}, {0, 0, -1}).has_value());
+    });
+    run("IntersectionTest.NoBackfaceCulling", [&] {
+        auto r = cast1(ctx, tri, {0.2f, 0.2f, -1.0f}, {0, 0, 1});
+        EXPECT(r.has_value() && std::isfinite(r->t) && std::isfinite(r->bari.x));
+    });
+    run("IntersectionTest.VeryDistantIntersection", [&] {
+        auto r = cast1(ctx, tri, {0.2f, 0.2f, 10000.0f}, {0, 0, -1});
+        EXPECT(r.has_value() && std::fabs(r->t - 10000.0f) <= 1e-2f);
+    });
+    run("IntersectionTest.DegenerateTriangleCollinear", [&] {
+        Scene s = single({0, 0, 0}, {1, 0, 0}, {2, 0, 0});
+        EXPECT(!cast1(ctx, s, {0.5f, 0, 1}, {0, 0, -1}).has_value());
+    });
+    run("IntersectionTest.DegenerateRayZeroDirection", [&] {
+        EXPECT(!cast1(ctx, tri, {0.2f, 0.2f, 1.0f}, {0, 0, 0}).has_value());
+    });
+    run("IntersectionTest.DegenerateRayNaNInDirection", [&] {
+        EXPECT(!cast1(ctx, tri, {0, 0, 0}, {NAN, NAN, NAN}).has_value());
+    });
+    run("IntersectionTest.ExtremelyThinTriangle", [&] {
+        Scene s = single({0, 0, 0}, {100.0f, 0, 0}, {100.0f, 2e-8f, 0});
+        auto r = cast1(ctx, s, {50.0f, 1e-9f, 1.0f}, {0, 0, -1});
+        EXPECT(r.has_value() && std::isfinite(r->t));
+    });
+    run("IntersectionTest.MultipleAngleSanityCheck", [&] {
+        const Vec3 inside{0.2f, 0.2f, 0.0f}, outside{5.0f, 5.0f, 0.0f};
+        const Vec3 origins[4] = {{0.2f, 0.2f, 10.0f}, {10.0f, 0.2f, 10.0f}, {-5.0f, -5.0f, 2.0f}, {0.0f, 0.0f, 1.0f}};
+        for (const Vec3 &o : origins) {
+            auto r = cast1(ctx, tri, o, {inside.x - o.x, inside.y - o.y, inside.z - o.z});
+            EXPECT(r.has_value());
+            if (r) {
+                EXPECT_NEAR(r->t, 1.0f, 1e-4f);
+                EXPECT_NEAR(r->bari.x + r->bari.y + r->bari.z, 1.0f, 1e-5f);
+                EXPECT_NEAR(r->bari.x, 0.6f, 1e-4f);
+                EXPECT_NEAR(r->bari.y, 0.2f, 1e-4f);
+                EXPECT_NEAR(r->bari.z, 0.2f, 1e-4f);
+            }
+            EXPECT(!cast1(ctx, tri, o, {outside.x - o.x, outside.y - o.y, outside.z - o.z}).has_value());
+        }
+    });
+    run("TracingTest.NearestOfStackedTriangles", [&] {
+        Scene s;
+        s.materials.push_back(Material::diffuse({1, 1, 1}));
+        for (float z : {-2.0f, 0.0f, 3.0f}) {
+            Mesh m;
+            m.points = {{-1, -1, z}, {1, -1, z}, {0, 1, z}};
+            m.triangles = {{0, 1, 2}};
+            s.objects.push_back({m, 0});
+        }
+        s.build(ctx);
+        auto r = cast1(ctx, s, {0, 0, 10}, {0, 0, -1});
+        EXPECT(r.has_value() && r->triangle == 2 && std::fabs(r->t - 7.0f) <= 1e-5f);
+        auto r2 = cast1(ctx, s, {0, 0, -10}, {0, 0, 1});
+        EXPECT(r2.has_value() && r2->triangle == 0 && std::fabs(r2->t - 8.0f) <= 1e-5f);
+        EXPECT(!cast1(ctx, s, {5, 5, 10}, {0, 0, -1}).has_value());
+    });
+    run("RendererTest.AnalyticalDiffuseReflection", [&] {
+        const Size2i res{256, 256};
+        Renderer renderer(ctx, res);
+        Camera cam;
+        cam.intrinsics = {50.0f, {res.width / 2.0f * 0.1f, res.height / 2.0f * 0.1f}};
+        cam.resolution = res;
+        cam.physical_pixel_size = {0.1f, 0.1f};
+        renderer.setCamera(cam);
+        const float emittance = 0.55f;
+        Scene scene;
+        scene.materials.push_back(Material::diffuse({1, 1, 1}));
+        scene.materials.push_back(Material::diffuse({0, 0, 0}, {emittance, emittance, emittance}));
+        scene.objects.push_back({createQuadMesh({0, 0, 1}, {0, 0, -1}, {1, 0, 0}, 1000.0f), 0});
+        scene.objects.push_back({createQuadMesh({0, 0, -1}, {0, 0, 1}, {1, 0, 0}, 1000.0f), 1});
+        renderer.setScene(std::move(scene));
+        renderer.setOutputOptions({OutputLinearity::Linear});
+        for (int i = 0; i < 100; ++i) renderer.render();
+        renderer.createPixels();
+        const Vec4 *px = renderer.getRawPixels();
+        const float count0 = px[0].w;
+        double s1 = 0, s2 = 0, s3 = 0;
+        bool counts = true, grey = true;
+        for (int i = 0; i < res.width * res.height; ++i) {
+            counts &= px[i].w == count0;
+            const float r = px[i].x / px[i].w, g = px[i].y / px[i].w, b = px[i].z / px[i].w;
+            grey &= (r == g && g == b);
+            s1 += r;
+            s2 += (double)r * r;
+            s3 += (double)r * r * r;
+        }
+        const double n = res.width * res.height, mean = s1 / n, sd = std::sqrt(s2 / n - mean * mean);
+        const double skew = (s3 / n - 3 * mean * sd * sd - mean * mean * mean) / (sd * sd * sd);
+        std::printf("    mean %.5f stddev %.5f skew %.3g samples %.0f\n", mean, sd, skew, count0);
+        EXPECT(counts);
+        EXPECT(grey);
+        EXPECT(count0 == 100.0f);
+        EXPECT_NEAR(mean, emittance, 0.01);
+        EXPECT(sd <= 0.04);
+        EXPECT(skew <= 4e-5);
+        const uint32_t *p = renderer.getPixels();
+        EXPECT(((p[0] >> 24) & 0xFF) == 255u);
+    });
+    std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
+    return g_fail ? 1 : 0;
+}
