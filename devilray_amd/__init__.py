@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    ALIAS_DTYPE, DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, GUIDE_HEAD_FLOATS,
+    ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, GUIDE_HEAD_FLOATS,
     GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
 )
@@ -368,3 +368,41 @@ class Guide:
     def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64):
         check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices))
         self.collect_fraction, self.guided_vertices = collect_fraction, guided_vertices
+
+
+class Batch:
+    """Scenes merged for batched rendering (dr_batch): views of all scenes
+    render in one sequence of wavefront launches."""
+
+    def __init__(self, ctx: Context, scenes):
+        self.ctx = ctx
+        arr = (C.c_void_p * len(scenes))(*[s.handle.value if hasattr(s.handle, "value") else s.handle
+                                           for s in scenes])
+        h = C.c_void_p()
+        check(lib().dr_batch_create(ctx.handle, arr, len(scenes), C.byref(h)))
+        self.handle = h
+        self.n_scenes = len(scenes)
+
+    def close(self):
+        if self.handle:
+            check(lib().dr_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_dev(self, views, max_depth: int = 12, rr_depth: int = 3, seed: int = 0, fresnel: int = 0,
+                   chunk_paths: int = 1 << 21, trace_sched: int = 1, refill_min: int = 16, use_graph: bool = True,
+                   stream=None):
+        """views: [(scene index, DrCamera, spp, sample_offset, accum device pointer)]."""
+        vs = (DrBatchView * len(views))()
+        for i, (sc, cam, spp, off, acc) in enumerate(views):
+            vs[i].scene, vs[i].spp, vs[i].sample_offset = sc, spp, off
+            vs[i].camera = cam
+            vs[i].accum = acc
+        p = DrRenderParams(0, 0, max_depth, rr_depth, seed, fresnel, 0)
+        o = DrBatchOptions(chunk_paths, trace_sched, refill_min, int(use_graph))
+        check(lib().dr_batch_render(self.handle, vs, len(views), C.byref(p), C.byref(o), stream))

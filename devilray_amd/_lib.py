@@ -67,6 +67,16 @@ class DrRenderParams(C.Structure):
                 ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("fresnel", C.c_uint32), ("flags", C.c_uint32)]
 
 
+class DrBatchView(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32), ("pad", C.c_uint32),
+                ("camera", DrCamera), ("accum", C.c_void_p)]
+
+
+class DrBatchOptions(C.Structure):
+    _fields_ = [("chunk_paths", C.c_uint32), ("trace_sched", C.c_uint32), ("refill_min", C.c_uint32),
+                ("use_graph", C.c_uint32)]
+
+
 class DrGuideParams(C.Structure):
     _fields_ = [("bounds", C.c_float * 6), ("capacity", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
@@ -116,6 +126,9 @@ PROTOTYPES = {
     "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
     "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),
     "dr_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
+    "dr_batch_create": (_st, [_vp, _vp, _u32, C.POINTER(_vp)]),
+    "dr_batch_destroy": (_st, [_vp]),
+    "dr_batch_render": (_st, [_vp, _vp, _u32, C.POINTER(DrRenderParams), C.POINTER(DrBatchOptions), _vp]),
     "dr_render_guided": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _u32, _vp, _vp]),
     "dr_guide_create": (_st, [_vp, C.POINTER(DrGuideParams), C.POINTER(_vp)]),
     "dr_guide_destroy": (_st, [_vp]),

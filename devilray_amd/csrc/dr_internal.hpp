@@ -102,6 +102,7 @@ struct Scene {
     // wavefront render state (render.hip)
     DevBuf rs_rays, rs_hits, rs_beta, rs_radiance, rs_list_a, rs_list_b, rs_counters;
     DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib;
+    DevBuf rs_view, rs_root, rs_sroot, rs_chunk, rs_views;  // per-path view / BVH root, chunk cursor, view table
     DevBuf rs_guide_head;   // DR_GUIDE_HEAD_FLOATS floats per path (guided renders)
     DevBuf rs_pending;      // u32 per path: training sample awaiting its radiance snapshot
     uint32_t rs_capacity = 0;
@@ -152,6 +153,50 @@ dr_status guide_finalize(Guide *g, const float4 *radiance, hipStream_t stream);
 dr_status guide_compact(Guide *g, hipStream_t stream);
 dr_status guide_reset_counters(Guide *g, hipStream_t stream);
 
+// ---- rendering (render.hip) ------------------------------------------------
+struct CamParams {
+    float m[12];
+    float f, cx, cy, psx, psy;
+    uint32_t w, h, sampling;
+};
+
+// One view of a render call: its items are item_off + sample * npix + pixel
+// for sample in [0, spp); its samples carry indices sample_offset + sample.
+struct ViewInfo {
+    CamParams cam;
+    uint64_t item_off;
+    uint64_t seed;
+    double *accum;
+    uint32_t npix, spp, sample_offset, scene;
+};
+
+// Per scene of a batch: BVH root and light-table range.
+struct SceneRange {
+    uint32_t root, alias_off, n_emit, pad;
+};
+
+constexpr uint32_t kPathCap = 1u << 21;  // default paths per chunk
+
+// A render call: views (device table) over one scene, or over a batch's
+// merged scene with a per-scene range table.
+struct RenderJob {
+    Scene *s = nullptr;
+    const SceneRange *scenes = nullptr;
+    const ViewInfo *views = nullptr;
+    uint32_t n_views = 0;
+    uint64_t total = 0;  // items over all views
+    const dr_render_params *prm = nullptr;
+    Guide *guide = nullptr;
+    uint32_t guide_flags = 0;
+    uint32_t cap = kPathCap;  // paths per chunk
+    uint32_t sched = DR_SCHED_PER_RAY_SPECULATIVE;
+    bool graph = false;  // record the bounce loop once, replay it per chunk
+};
+
+CamParams cam_params(const dr_camera &c);
+dr_status check_render_args(const dr_camera *cam, const dr_render_params *prm, const double *accum);
+dr_status render_job(const RenderJob &j, hipStream_t stream);
+
 // light.hip
 dr_status build_lights(Scene *s, hipStream_t stream);
 
@@ -165,9 +210,12 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);
 // trace.hip
 // ray_list (optional): indices of the rays to trace (results go to those
 // indices); n_dev (optional): device-resident count, at most n_max.
+// ray_root (optional): per ray, the BVH node its traversal starts at (a
+// batch's merged forest); NULL starts every ray at node 0.
 dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
                           const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
-                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream);
+                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream,
+                          const uint32_t *ray_root = nullptr);
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
 

@@ -26,48 +26,95 @@ namespace dr {
 namespace {
 
 constexpr int kT = 256;
-constexpr uint32_t kPathCap = 1u << 21;
 
 struct AliasEntry {
     float p_a, pdf_a, pdf_b;
     int32_t a, b;
 };
 
-struct CamParams {
-    float m[12];
-    float f, cx, cy, psx, psy;
-    uint32_t w, h, sampling;
+
+
+// Device-resident chunk cursor: every kernel reads the chunk's first item and
+// size from here, so one recorded bounce loop serves every chunk.
+struct ChunkState {
+    uint64_t item0, total;
+    uint32_t n, next, cap, pad;
 };
 
 struct PathParams {
-    uint64_t seed;
-    uint32_t npix, sample_offset;
-    uint64_t item0;  // first work item of the chunk: item = sample * npix + pixel
-    uint32_t n;      // paths in the chunk
+    const ViewInfo *views;
+    const SceneRange *scenes;  // NULL: one scene, root 0, the whole light table
+    const ChunkState *chunk;
+    uint32_t n_views;
     uint32_t max_depth, rr_depth, schlick;
 };
 
-__device__ __forceinline__ void item_of(const PathParams &pp, uint32_t path, uint32_t &pixel, uint32_t &sample)
+// the last view whose first item is <= item (empty views are skipped)
+__device__ __forceinline__ uint32_t find_view(const ViewInfo *__restrict__ v, uint32_t nv, uint64_t item)
 {
-    const uint64_t item = pp.item0 + path;
-    pixel = (uint32_t)(item % pp.npix);
-    sample = pp.sample_offset + (uint32_t)(item / pp.npix);
+    uint32_t lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) / 2u;
+        if (v[mid].item_off <= item) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
 }
 
-__global__ __launch_bounds__(kT) void k_camera(CamParams cam, PathParams pp, dr_ray *__restrict__ rays,
+__device__ __forceinline__ void item_of(const ViewInfo &V, uint64_t item, uint32_t &pixel, uint32_t &sample)
+{
+    const uint64_t local = item - V.item_off;
+    pixel = (uint32_t)(local % V.npix);
+    sample = V.sample_offset + (uint32_t)(local / V.npix);
+}
+
+__global__ void k_chunk_init(ChunkState *cs, uint64_t total, uint32_t cap)
+{
+    if (threadIdx.x != 0) return;
+    cs->item0 = 0;
+    cs->total = total;
+    cs->n = 0;
+    cs->next = 0;
+    cs->cap = cap;
+}
+
+__global__ void k_next_chunk(ChunkState *cs)
+{
+    if (threadIdx.x != 0) return;
+    const uint64_t item0 = (uint64_t)cs->next * cs->cap;
+    cs->item0 = item0;
+    const uint64_t left = item0 < cs->total ? cs->total - item0 : 0ull;
+    cs->n = left < cs->cap ? (uint32_t)left : cs->cap;
+    cs->next = cs->next + 1u;
+}
+
+__global__ void k_set_view(ViewInfo *dst, ViewInfo v)
+{
+    if (threadIdx.x == 0) *dst = v;
+}
+
+__global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict__ rays,
                                                float4 *__restrict__ beta, float4 *__restrict__ rad,
                                                uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
-                                               uint32_t *__restrict__ pending)
+                                               uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
+                                               uint32_t *__restrict__ proot)
 {
     const uint32_t p = blockIdx.x * kT + threadIdx.x;
-    if (p == 0) { counters[0] = pp.n; counters[1] = 0u; counters[2] = 0u; }
-    if (p >= pp.n) return;
+    const uint32_t n = pp.chunk->n;
+    if (p == 0) { counters[0] = n; counters[1] = 0u; counters[2] = 0u; }
+    if (p >= n) return;
+    const uint64_t item = pp.chunk->item0 + p;
+    const uint32_t view = find_view(pp.views, pp.n_views, item);
+    const ViewInfo &V = pp.views[view];
+    const CamParams &cam = V.cam;
     uint32_t pixel, sample;
-    item_of(pp, p, pixel, sample);
+    item_of(V, item, pixel, sample);
+    pview[p] = view;
+    if (proot) proot[p] = pp.scenes[V.scene].root;
     const uint32_t x = pixel % cam.w, y = pixel / cam.w;
     float sx = 0.5f, sy = 0.5f;
     if (cam.sampling == DR_PIXEL_UNIFORM_RANDOM) {
-        const U4 r = path_rng(pixel, sample, 0xFFFFFFFFu, 0u, pp.seed);
+        const U4 r = path_rng(pixel, sample, 0xFFFFFFFFu, 0u, V.seed);
         sx = u01(r.x);
         sy = u01(r.y);
     }
@@ -130,7 +177,8 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                                               const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
                                               float4 *__restrict__ rad, dr_ray *__restrict__ srays,
                                               float4 *__restrict__ scontrib, uint32_t *__restrict__ scount,
-                                              GuideView gv)
+                                              GuideView gv, const uint32_t *__restrict__ pview,
+                                              uint32_t *__restrict__ sroot)
 {
     const uint32_t slot = blockIdx.x * kT + threadIdx.x;
     if (slot >= *in_count) return;
@@ -149,8 +197,12 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
         }
     }
     if (h.id < 0) return;  // escaped: no environment light
+    const ViewInfo &V = pp.views[pview[path]];
+    const uint64_t seed = V.seed;
     uint32_t pixel, sample;
-    item_of(pp, path, pixel, sample);
+    item_of(V, pp.chunk->item0 + path, pixel, sample);
+    // the path's scene: light-table range and BVH root for shadow rays
+    const SceneRange S = pp.scenes ? pp.scenes[V.scene] : SceneRange{0u, 0u, sv.n_emit, 0u};
     const dr_ray ray = rays[path];
     float4 b4 = beta[path];
     V3 beta3 = V3{b4.x, b4.y, b4.z};
@@ -168,8 +220,8 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
     const bool back = dot(ng, d) > 0.0f;
     const V3 n = back ? mul(ng, -1.0f) : ng;
     const float eps = origin_eps(A, B, C);
-    const U4 r0 = path_rng(pixel, sample, vertex, 0u, pp.seed);
-    const U4 r1 = path_rng(pixel, sample, vertex, 1u, pp.seed);
+    const U4 r0 = path_rng(pixel, sample, vertex, 0u, seed);
+    const U4 r1 = path_rng(pixel, sample, vertex, 1u, seed);
 
     V3 L = V3{l4.x, l4.y, l4.z};
     V3 nd;
@@ -205,9 +257,9 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
         }
         // next-event estimation
         const V3 alb = V3{mat.albedo[0], mat.albedo[1], mat.albedo[2]};
-        if (sv.n_emit > 0) {
-            const uint32_t s = (uint32_t)(((uint64_t)r1.x * sv.n_emit) >> 32);
-            const AliasEntry en = sv.alias[s];
+        if (S.n_emit > 0) {
+            const uint32_t s = (uint32_t)(((uint64_t)r1.x * S.n_emit) >> 32);
+            const AliasEntry en = sv.alias[S.alias_off + s];
             const bool take_a = u01(r1.y) < en.p_a;
             const int32_t lt = take_a ? en.a : en.b;
             const float psel = take_a ? en.pdf_a : en.pdf_b;
@@ -241,6 +293,7 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                         const uint32_t k = atomicAdd(scount, 1u);
                         const V3 so = add(p, mul(om, eps));
                         srays[k] = dr_ray{so.x, so.y, so.z, 0.0f, om.x, om.y, om.z, tmax};
+                        if (sroot) sroot[k] = S.root;
                         scontrib[k] = make_float4(c.x, c.y, c.z, __int_as_float((int32_t)path));
                     }
                 }
@@ -248,7 +301,7 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
         }
         if (guided) {
             // one-sample MIS: a guide lobe with probability alpha, else the BSDF
-            const U4 r2 = path_rng(pixel, sample, vertex, 2u, pp.seed);
+            const U4 r2 = path_rng(pixel, sample, vertex, 2u, seed);
             if (u01(r2.x) < alpha) {
                 const float u = u01(r2.y);
                 int j = kGuideLobes - 1;
@@ -354,20 +407,27 @@ __global__ __launch_bounds__(kT) void k_shadow_apply(const uint32_t *__restrict_
     rad[path] = l;
 }
 
-// Lane p owns the chunk's first item of pixel (item0 + p) % npix and adds
-// that pixel's samples in increasing sample order.
-__global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__restrict__ rad, double *__restrict__ accum)
+// Lane p resolves path p if it is the chunk's first sample of its pixel and
+// adds that pixel's samples of the chunk in increasing sample order (the
+// next sample of a pixel is npix items later, within the same view).
+__global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__restrict__ rad,
+                                                const uint32_t *__restrict__ pview)
 {
     const uint32_t p = blockIdx.x * kT + threadIdx.x;
-    if (p >= pp.n || p >= pp.npix) return;
-    const uint32_t pixel = (uint32_t)((pp.item0 + p) % pp.npix);
-    double *a = accum + 4 * (size_t)pixel;
+    const uint32_t n = pp.chunk->n;
+    if (p >= n) return;
+    const ViewInfo &V = pp.views[pview[p]];
+    const uint64_t local = pp.chunk->item0 + p - V.item_off;
+    if (local >= V.npix && p >= V.npix) return;  // an earlier sample of this pixel is in the chunk
+    const uint64_t view_end = (uint64_t)V.npix * V.spp;
+    double *a = V.accum + 4 * (size_t)(local % V.npix);
     double r = a[0], g = a[1], b = a[2], w = a[3];
-    for (uint32_t path = p; path < pp.n; path += pp.npix) {
-        const float4 l = rad[path];
-        r = r + (double)l.x;
-        g = g + (double)l.y;
-        b = b + (double)l.z;
+    uint64_t l = local;
+    for (uint32_t path = p; path < n && l < view_end; path += V.npix, l += V.npix) {
+        const float4 c = rad[path];
+        r = r + (double)c.x;
+        g = g + (double)c.y;
+        b = b + (double)c.z;
         w = w + 1.0;
     }
     a[0] = r; a[1] = g; a[2] = b; a[3] = w;
@@ -375,25 +435,101 @@ __global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__r
 
 inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
 
+// One chunk of the bounce loop. Every argument is fixed for the whole call
+// (chunk position and queue counts live in device memory), so the same
+// launches can be recorded once and replayed for every chunk.
+dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &sv, const GuideView &gv_in,
+                    hipStream_t stream)
+{
+    Scene *s = j.s;
+    const uint32_t cap = j.cap;
+    const bool use_guide = (j.guide_flags & DR_GUIDE_USE) != 0;
+    const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
+    GuideView gv = gv_in;
+    uint32_t *cnt = s->rs_counters.as<uint32_t>();
+    auto *rays = s->rs_rays.as<dr_ray>();
+    auto *hits = s->rs_hits.as<dr_hit>();
+    auto *beta = s->rs_beta.as<float4>();
+    auto *rad = s->rs_radiance.as<float4>();
+    auto *srays = s->rs_shadow_rays.as<dr_ray>();
+    auto *socc = s->rs_shadow_occ.as<uint32_t>();
+    auto *scon = s->rs_shadow_contrib.as<float4>();
+    auto *pview = s->rs_view.as<uint32_t>();
+    uint32_t *proot = j.scenes ? s->rs_root.as<uint32_t>() : nullptr;
+    uint32_t *sroot = j.scenes ? s->rs_sroot.as<uint32_t>() : nullptr;
+    k_next_chunk<<<1, 64, 0, stream>>>(s->rs_chunk.as<ChunkState>());
+    DR_HIP_TRY(hipGetLastError());
+    k_camera<<<blocks(cap), kT, 0, stream>>>(pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt,
+                                             collect ? gv.pending : nullptr, pview, proot);
+    DR_HIP_TRY(hipGetLastError());
+    for (uint32_t v = 0; v < j.prm->max_depth; ++v) {
+        uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
+        uint32_t *lout = (v & 1u) ? s->rs_list_a.as<uint32_t>() : s->rs_list_b.as<uint32_t>();
+        uint32_t *cin = cnt + (v & 1u);
+        uint32_t *cout = cnt + ((v + 1u) & 1u);
+        if (dr_status st = launch_trace_ex(s, rays, lin, cap, cin, hits, nullptr, nullptr, nullptr, false, j.sched,
+                                           stream, proot))
+            return st;
+        const bool guided_v = use_guide && v < j.guide->guided_vertices;
+        if (guided_v)
+            if (dr_status st = guide_infer_paths(j.guide, s, rays, hits, lin, cin, cap, s->rs_guide_head.as<float>(),
+                                                 stream))
+                return st;
+        gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
+        k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2);
+        k_shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
+                                                cnt + 2, gv, pview, sroot);
+        DR_HIP_TRY(hipGetLastError());
+        if (s->n_emit) {
+            if (dr_status st = launch_trace_ex(s, srays, nullptr, cap, cnt + 2, nullptr, socc, nullptr, nullptr, true,
+                                               j.sched, stream, sroot))
+                return st;
+            k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
+            DR_HIP_TRY(hipGetLastError());
+        }
+    }
+    if (collect)
+        if (dr_status st = guide_finalize(j.guide, rad, stream)) return st;
+    k_resolve<<<blocks(cap), kT, 0, stream>>>(pp, rad, pview);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
 }  // namespace
 
-dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
-                 double *accum, hipStream_t stream)
+CamParams cam_params(const dr_camera &c)
 {
-    if (guide_flags & ~(uint32_t)(DR_GUIDE_USE | DR_GUIDE_COLLECT)) return fail(DR_E_INVALID, "unknown guide flags");
-    if (guide_flags && !guide) return fail(DR_E_INVALID, "guide flags without a guide");
-    const bool use_guide = (guide_flags & DR_GUIDE_USE) != 0;
-    const bool collect = (guide_flags & DR_GUIDE_COLLECT) != 0;
-    if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
+    CamParams cp;
+    for (int i = 0; i < 12; ++i) cp.m[i] = c.transform[i];
+    cp.f = c.focal_length;
+    cp.cx = c.center[0];
+    cp.cy = c.center[1];
+    cp.psx = c.pixel_size[0];
+    cp.psy = c.pixel_size[1];
+    cp.w = c.width;
+    cp.h = c.height;
+    cp.sampling = c.pixel_sampling;
+    return cp;
+}
+
+dr_status check_render_args(const dr_camera *cam, const dr_render_params *prm, const double *accum)
+{
     if (!cam || !prm || !accum) return fail(DR_E_INVALID, "null argument");
     if (cam->width == 0 || cam->height == 0) return fail(DR_E_INVALID, "empty image");
+    if ((uint64_t)cam->width * cam->height > 0xFFFFFFFFull) return fail(DR_E_INVALID, "image too large");
     if (prm->max_depth == 0 || prm->max_depth > 64) return fail(DR_E_INVALID, "max_depth must be 1..64");
-    if (!s->lights_built)
-        if (dr_status st = build_lights(s, stream)) return st;
-    const uint32_t npix = cam->width * cam->height;
-    const uint64_t items = (uint64_t)npix * prm->spp;
-    if (items == 0) return DR_OK;
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(items, kPathCap);
+    return DR_OK;
+}
+
+dr_status render_job(const RenderJob &j, hipStream_t stream)
+{
+    Scene *s = j.s;
+    if (j.guide_flags & ~(uint32_t)(DR_GUIDE_USE | DR_GUIDE_COLLECT)) return fail(DR_E_INVALID, "unknown guide flags");
+    if (j.guide_flags && !j.guide) return fail(DR_E_INVALID, "guide flags without a guide");
+    const bool use_guide = (j.guide_flags & DR_GUIDE_USE) != 0;
+    const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
+    if (j.total == 0) return DR_OK;
+    const uint32_t cap = j.cap;
     if (s->rs_capacity < cap) {
         dr_status st = DR_OK;
         auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
@@ -403,39 +539,34 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
         need(s->rs_radiance, sizeof(float4) * (size_t)cap);
         need(s->rs_list_a, 4ull * cap);
         need(s->rs_list_b, 4ull * cap);
+        need(s->rs_view, 4ull * cap);
         need(s->rs_counters, 64);
+        need(s->rs_chunk, sizeof(ChunkState));
         need(s->rs_shadow_rays, sizeof(dr_ray) * (size_t)cap);
         need(s->rs_shadow_occ, 4ull * cap);
         need(s->rs_shadow_contrib, sizeof(float4) * (size_t)cap);
         if (st) return st;
         s->rs_capacity = cap;
     }
+    if (j.scenes) {
+        if (dr_status st = s->rs_root.ensure(4ull * s->rs_capacity)) return st;
+        if (dr_status st = s->rs_sroot.ensure(4ull * s->rs_capacity)) return st;
+    }
     if (use_guide)
         if (dr_status st = s->rs_guide_head.ensure(sizeof(float) * kHeadFloats * (size_t)s->rs_capacity)) return st;
     if (collect) {
         if (dr_status st = s->rs_pending.ensure(4ull * s->rs_capacity)) return st;
-        if (dr_status st = guide_reset_counters(guide, stream)) return st;
+        if (dr_status st = guide_reset_counters(j.guide, stream)) return st;
     }
     GuideView gv{};
-    if (use_guide) gv.head = s->rs_guide_head.as<float>();
     if (collect) {
-        gv.smp = guide->samples.as<dr_guide_sample>();
-        gv.aux = guide->aux.as<GuideAux>();
-        gv.counter = guide->counters.as<uint32_t>();
+        gv.smp = j.guide->samples.as<dr_guide_sample>();
+        gv.aux = j.guide->aux.as<GuideAux>();
+        gv.counter = j.guide->counters.as<uint32_t>();
         gv.pending = s->rs_pending.as<uint32_t>();
-        gv.cap = guide->stage_cap;
-        gv.frac = guide->collect_fraction;
+        gv.cap = j.guide->stage_cap;
+        gv.frac = j.guide->collect_fraction;
     }
-    CamParams cp;
-    for (int i = 0; i < 12; ++i) cp.m[i] = cam->transform[i];
-    cp.f = cam->focal_length;
-    cp.cx = cam->center[0];
-    cp.cy = cam->center[1];
-    cp.psx = cam->pixel_size[0];
-    cp.psy = cam->pixel_size[1];
-    cp.w = cam->width;
-    cp.h = cam->height;
-    cp.sampling = cam->pixel_sampling;
     SceneView sv;
     sv.verts = s->verts.as<float>();
     sv.idx = s->indices.as<uint32_t>();
@@ -444,59 +575,82 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
     sv.alias = s->alias.as<AliasEntry>();
     sv.pdf_area = s->light_pdf_area.as<float>();
     sv.n_emit = s->n_emit;
-    uint32_t *cnt = s->rs_counters.as<uint32_t>();
-    auto *rays = s->rs_rays.as<dr_ray>();
-    auto *hits = s->rs_hits.as<dr_hit>();
-    auto *beta = s->rs_beta.as<float4>();
-    auto *rad = s->rs_radiance.as<float4>();
-    auto *srays = s->rs_shadow_rays.as<dr_ray>();
-    auto *socc = s->rs_shadow_occ.as<uint32_t>();
-    auto *scon = s->rs_shadow_contrib.as<float4>();
-    const uint32_t sched = DR_SCHED_PER_RAY_SPECULATIVE;
-    for (uint64_t item0 = 0; item0 < items; item0 += cap) {
-        PathParams pp;
-        pp.seed = prm->seed;
-        pp.npix = npix;
-        pp.sample_offset = prm->sample_offset;
-        pp.item0 = item0;
-        pp.n = (uint32_t)std::min<uint64_t>(cap, items - item0);
-        pp.max_depth = prm->max_depth;
-        pp.rr_depth = prm->rr_depth;
-        pp.schlick = prm->fresnel == DR_FRESNEL_SCHLICK ? 1u : 0u;
-        k_camera<<<blocks(pp.n), kT, 0, stream>>>(cp, pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt,
-                                                  collect ? gv.pending : nullptr);
-        DR_HIP_TRY(hipGetLastError());
-        for (uint32_t v = 0; v < prm->max_depth; ++v) {
-            uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
-            uint32_t *lout = (v & 1u) ? s->rs_list_a.as<uint32_t>() : s->rs_list_b.as<uint32_t>();
-            uint32_t *cin = cnt + (v & 1u);
-            uint32_t *cout = cnt + ((v + 1u) & 1u);
-            if (dr_status st = launch_trace_ex(s, rays, lin, pp.n, cin, hits, nullptr, nullptr, nullptr, false, sched, stream))
-                return st;
-            const bool guided_v = use_guide && v < guide->guided_vertices;
-            if (guided_v)
-                if (dr_status st = guide_infer_paths(guide, s, rays, hits, lin, cin, pp.n, s->rs_guide_head.as<float>(), stream)) return st;
-            gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
-            k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2);
-            k_shade<<<blocks(pp.n), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
-                                                     cnt + 2, gv);
-            DR_HIP_TRY(hipGetLastError());
-            if (s->n_emit) {
-                if (dr_status st = launch_trace_ex(s, srays, nullptr, pp.n, cnt + 2, nullptr, socc, nullptr, nullptr, true,
-                                                   sched, stream))
-                    return st;
-                k_shadow_apply<<<blocks(pp.n), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
-                DR_HIP_TRY(hipGetLastError());
-            }
+    PathParams pp;
+    pp.views = j.views;
+    pp.scenes = j.scenes;
+    pp.chunk = s->rs_chunk.as<ChunkState>();
+    pp.n_views = j.n_views;
+    pp.max_depth = j.prm->max_depth;
+    pp.rr_depth = j.prm->rr_depth;
+    pp.schlick = j.prm->fresnel == DR_FRESNEL_SCHLICK ? 1u : 0u;
+    k_chunk_init<<<1, 64, 0, stream>>>(s->rs_chunk.as<ChunkState>(), j.total, cap);
+    DR_HIP_TRY(hipGetLastError());
+    const uint64_t n_chunks = (j.total + cap - 1) / cap;
+    if (!j.graph || n_chunks == 1) {
+        for (uint64_t c = 0; c < n_chunks; ++c)
+            if (dr_status st = run_chunk(j, pp, sv, gv, stream)) return st;
+    } else {
+        // the first chunk runs eagerly (it sizes every buffer and caches the
+        // launch shapes); the loop is then recorded once and replayed
+        if (dr_status st = run_chunk(j, pp, sv, gv, stream)) return st;
+        DR_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        dr_status cst = run_chunk(j, pp, sv, gv, stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        if (cst) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return cst;
         }
-        if (collect)
-            if (dr_status st = guide_finalize(guide, rad, stream)) return st;
-        k_resolve<<<blocks(std::min(pp.n, npix)), kT, 0, stream>>>(pp, rad, accum);
-        DR_HIP_TRY(hipGetLastError());
+        DR_HIP_TRY(ce);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        DR_HIP_TRY(ie);
+        hipError_t le = hipSuccess;
+        for (uint64_t c = 1; c < n_chunks && le == hipSuccess; ++c) le = hipGraphLaunch(exec, stream);
+        (void)hipGraphExecDestroy(exec);
+        DR_HIP_TRY(le);
     }
     if (collect)
-        if (dr_status st = guide_compact(guide, stream)) return st;
+        if (dr_status st = guide_compact(j.guide, stream)) return st;
     return DR_OK;
+}
+
+dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
+                 double *accum, hipStream_t stream)
+{
+    if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
+    if (dr_status st = check_render_args(cam, prm, accum)) return st;
+    if (!s->lights_built)
+        if (dr_status st = build_lights(s, stream)) return st;
+    const uint32_t npix = cam->width * cam->height;
+    const uint64_t items = (uint64_t)npix * prm->spp;
+    if (items == 0) return DR_OK;
+    if (dr_status st = s->rs_views.ensure(sizeof(ViewInfo))) return st;
+    ViewInfo v;
+    v.cam = cam_params(*cam);
+    v.item_off = 0;
+    v.seed = prm->seed;
+    v.accum = accum;
+    v.npix = npix;
+    v.spp = prm->spp;
+    v.sample_offset = prm->sample_offset;
+    v.scene = 0;
+    k_set_view<<<1, 64, 0, stream>>>(s->rs_views.as<ViewInfo>(), v);
+    DR_HIP_TRY(hipGetLastError());
+    RenderJob j;
+    j.s = s;
+    j.scenes = nullptr;
+    j.views = s->rs_views.as<ViewInfo>();
+    j.n_views = 1;
+    j.total = items;
+    j.prm = prm;
+    j.guide = guide;
+    j.guide_flags = guide_flags;
+    j.cap = (uint32_t)std::min<uint64_t>(items, kPathCap);
+    j.sched = DR_SCHED_PER_RAY_SPECULATIVE;
+    j.graph = false;
+    return render_job(j, stream);
 }
 
 }  // namespace dr

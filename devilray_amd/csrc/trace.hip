@@ -91,7 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                                                   uint32_t *__restrict__ box_tests,
                                                   uint32_t *__restrict__ tri_tests,
                                                   int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
-                                                  uint32_t *__restrict__ ray_counter, uint32_t refill_min)
+                                                  uint32_t *__restrict__ ray_counter, uint32_t refill_min,
+                                                  const uint32_t *__restrict__ roots)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
     // ray count: fixed at launch, or read from device memory (on-device queues)
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     // rejects it too), and NaN slabs would admit empty slots
                     const bool nan_ray = (r0.x != r0.x) || (r0.y != r0.y) || (r0.z != r0.z) || (r0.w != r0.w) ||
                                          (r1.x != r1.x) || (r1.y != r1.y) || (r1.z != r1.z) || (r1.w != r1.w);
-                    node = nan_ray ? kNone : 0;  // the root is always an interior node
+                    // the root is always an interior node
+                    node = nan_ray ? kNone : (roots ? (int32_t)roots[r] : 0);
                     // byte offsets of the near slab planes for this ray's octant
                     nx = signbit(p.ix) ? 16u : 0u;
                     ny = signbit(p.iy) ? 48u : 32u;
@@ -365,7 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
 
 using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
                             const uint32_t *, dr_hit *,
-                            uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t);
+                            uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t,
+                            const uint32_t *);
 
 template <bool ANY, bool COUNT>
 TraceKernel pick(uint32_t sched)
@@ -389,7 +392,8 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
 
 dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
                           const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
-                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream)
+                          uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream,
+                          const uint32_t *ray_root)
 {
     const uint32_t n = n_max;
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
@@ -414,7 +418,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * nlanes)) return st;
     const auto *r4 = reinterpret_cast<const float4 *>(rays);
     const uint32_t *perm = ray_list;
-    if (!ray_list && !n_dev && (sched & DR_SCHED_REORDER) && n >= 4096) {
+    if (!ray_list && !n_dev && !ray_root && (sched & DR_SCHED_REORDER) && n >= 4096) {
         if (dr_status st = s->ray_keys.ensure(sizeof(uint32_t) * (size_t)n)) return st;
         if (dr_status st = s->ray_perm.ensure(sizeof(uint32_t) * (size_t)n)) return st;
         const float *lo = s->bounds, *hi = s->bounds + 3;
@@ -436,7 +440,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
         for (int k = 2; k <= 5; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
-                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min);
+                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

@@ -259,3 +259,30 @@ def triangle_umbrella() -> SceneData:
                 b.mesh([p00, p01, p10], [[0, 1, 2]], mat)
                 b.mesh([p01, p11, p10], [[0, 1, 2]], mat)
     return b.build()
+
+
+def batch_c3(n_scenes: int = 64, seed: int = 0, tris_range=(1_000, 2_000_000), pixels_range=(128 ** 2, 1024 ** 2),
+             spp_choices=(4, 16, 64)):
+    """The batched workload: n_scenes scenes built like many_lights (displaced
+    icospheres, 30% dielectric, 5% of the triangles emissive with LogNormal(0,
+    2) radiance) with triangle counts log-uniform in tris_range, 16:9 images
+    with pixel counts log-uniform in pixels_range and spp uniform over
+    spp_choices. Returns [(SceneData, width, height, spp)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_scenes):
+        t = float(np.exp(rng.uniform(np.log(tris_range[0]), np.log(tris_range[1]))))
+        subdiv = 4
+        while subdiv > 1 and 20 * 4 ** subdiv * 8 > t:
+            subdiv -= 1
+        n_spheres = max(1, int(round(t / (20 * 4 ** subdiv))))
+        n_tris = n_spheres * 20 * 4 ** subdiv
+        extent = 20.0 * (n_spheres / 400.0) ** (1.0 / 3.0) + 2.0
+        sd = many_lights(n_spheres=n_spheres, subdiv=subdiv, n_emissive=max(1, int(round(0.05 * n_tris))),
+                         seed=seed * 1000 + k, extent=extent)
+        px = float(np.exp(rng.uniform(np.log(pixels_range[0]), np.log(pixels_range[1]))))
+        w = max(16, int(round(np.sqrt(px * 16.0 / 9.0))))
+        h = max(9, int(round(w * 9.0 / 16.0)))
+        spp = int(rng.choice(spp_choices))
+        out.append((sd, w, h, spp))
+    return out

@@ -216,6 +216,45 @@ dr_status dr_scene_copy_lights(dr_scene *scene, void *entries, float *pdf_area);
 dr_status dr_render(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, double *accum,
                     dr_stream stream);
 
+/* ---- batched rendering (no reference counterpart: a reference Renderer
+ * renders one scene, src/device/RendererImpl.hpp:49-102) ----------------
+ *
+ * A batch merges built scenes into one device forest (each scene keeps its
+ * own BVH root, materials and light table), so that views of many scenes of
+ * uneven size render in one sequence of wavefront launches whose bounce loop
+ * can be recorded once as a HIP graph and replayed per chunk of paths. */
+typedef struct dr_batch dr_batch;
+
+typedef struct {
+    uint32_t scene;          /* index into the batch's scenes */
+    uint32_t spp;            /* samples per pixel added by this call */
+    uint32_t sample_offset;  /* first sample index (progressive rendering) */
+    uint32_t pad;
+    dr_camera camera;
+    double *accum;           /* device FP64 RGBW, width*height*32 bytes */
+} dr_batch_view;
+
+/* Launch shape (the autotuner's knobs). NULL = defaults {1 << 21,
+ * DR_SCHED_PER_RAY_SPECULATIVE, 16, 1}. */
+typedef struct {
+    uint32_t chunk_paths;  /* paths in flight per chunk (4096 .. 1 << 23) */
+    uint32_t trace_sched;  /* DR_SCHED_* for the extension and shadow traces */
+    uint32_t refill_min;   /* persistent traversal refill threshold (1..64) */
+    uint32_t use_graph;    /* record the bounce loop once, replay it per chunk */
+} dr_batch_options;
+
+/* Scenes must have their BVH built (their light tables are built here if
+ * needed). The batch copies what it needs; the scenes may be destroyed. */
+dr_status dr_batch_create(dr_context *ctx, dr_scene *const *scenes, uint32_t n_scenes, dr_batch **out);
+dr_status dr_batch_destroy(dr_batch *batch);
+/* Render every view; view v uses the random stream of a dr_render call
+ * with seed params->seed ^ ((uint64_t)v << 32), so each view's result
+ * equals the corresponding single-scene render. params->spp and
+ * sample_offset are ignored (per view). Waits for earlier work on `stream`
+ * at entry (the view table is rewritten), then runs asynchronously. */
+dr_status dr_batch_render(dr_batch *batch, const dr_batch_view *views, uint32_t n_views,
+                          const dr_render_params *params, const dr_batch_options *options, dr_stream stream);
+
 /* ---- learned path guide (no reference counterpart: the reference samples
  * Lambert bounces from the BSDF only, PathGeneration.hpp:406-420) ----------
  *
