@@ -224,6 +224,76 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
             "mse_guided_curve": [[round(t * 1e3, 1), m] for t, m in guided]}
 
 
+def batched_vs_separate(ctx, n_scenes=64, max_depth=12, rr_depth=3):
+    """The batched workload (64 scenes, triangles log-uniform in [1e3, 2e6],
+    pixels log-uniform in [128^2, 1024^2], spp in {4, 16, 64}): every view
+    rendered by one dr_render call per scene, against one dr_batch_render
+    over all views with autotuned launch options. Both are timed on their
+    second run (the first sizes the buffers) with HIP events."""
+    from devilray_amd import autotune
+    work = scenes.batch_c3(n_scenes)
+    scs, cams, npix = [], [], []
+    for sd, w, h, spp in work:
+        s = dr.Scene(ctx, sd.vertices, sd.indices, sd.tri_material, sd.dr_materials())
+        s.build_bvh(leaf_max=4)
+        s.build_lights()
+        scs.append(s)
+        c = sd.camera
+        cams.append(dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], w, h))
+        npix.append(w * h)
+    tris = int(sum(sd.indices.shape[0] for sd, _, _, _ in work))
+    paths = int(sum(n * spp for n, (_, _, _, spp) in zip(npix, work)))
+    acc = ctx.alloc(32 * sum(npix))
+    offs = np.concatenate([[0], np.cumsum(npix)[:-1]]) * 32
+
+    def clear():
+        acc.copy_from(np.zeros(4 * sum(npix)))
+
+    def separate():
+        e0, e1 = Event(), Event()
+        e0.record()
+        for k, (s, cam) in enumerate(zip(scs, cams)):
+            p = dr.DrRenderParams(work[k][3], 0, max_depth, rr_depth, k << 32, 0, 0)
+            s.render_dev(cam, p, acc.ptr + int(offs[k]))
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_ms(e1)
+
+    clear()
+    separate()
+    clear()
+    sep_ms = separate()
+    sep_img = acc.to_numpy(np.float64, 4 * sum(npix))
+    b = dr.Batch(ctx, scs)
+    views = [(k, cams[k], work[k][3], 0, acc.ptr + int(offs[k])) for k in range(len(scs))]
+    tuned = autotune.tune(ctx, b, views, tris, max_depth=max_depth, rr_depth=rr_depth, cache_path=None, log=log)
+    opts = {k: v for k, v in tuned.items() if not k.startswith("_")}
+
+    def batched():
+        e0, e1 = Event(), Event()
+        e0.record()
+        b.render_dev(views, max_depth=max_depth, rr_depth=rr_depth, seed=0, **opts)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_ms(e1)
+
+    clear()
+    batched()
+    clear()
+    bat_ms = batched()
+    same = bool(np.array_equal(acc.to_numpy(np.float64, 4 * sum(npix)), sep_img))
+    b.close()
+    for s in scs:
+        s.close()
+    acc.free()
+    return {"scenes": len(work), "triangles": tris, "pixels": int(sum(npix)), "paths": paths,
+            "separate_ms": round(sep_ms, 2), "batched_ms": round(bat_ms, 2),
+            "speedup": round(sep_ms / bat_ms, 3),
+            "separate_samples_per_s": round(paths / sep_ms * 1e3, 1),
+            "batched_samples_per_s": round(paths / bat_ms * 1e3, 1),
+            "images_identical": same, "tuned": opts, "sweep": tuned.get("_sweep")}
+
+
 def probe() -> int:
     """Traversal launches + a calibration copy, for rocprofv3 counter runs."""
     v, f = dr.gen_icosphere(SUBDIV, SIGMA, SEED)
@@ -252,6 +322,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-guide", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--probe", action="store_true")
     args = ap.parse_args()
@@ -325,8 +396,13 @@ def main() -> int:
         guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
                                                    8, 64, 8, 4, 12)
         log("[bench] guide: hidden-light reflector scene, guided vs plain")
-        guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 256, 256, 65536,
+        guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 512, 512, 16384,
                                                       16, 64, 8, 16, 8, capacity=1 << 16, train_passes=8)
+
+    batch = None
+    if not args.no_batch:
+        log("[bench] batched rendering: 64 uneven scenes, separate vs one batched launch sequence")
+        batch = batched_vs_separate(ctx)
 
     base = None
     if not args.no_cpu_baseline:
@@ -368,6 +444,7 @@ def main() -> int:
             "stack_overflow": int(info2.overflow_flag),
             "render": renders,
             "guide": guide,
+            "batch": batch,
         },
     }
     print(json.dumps(out), flush=True)
