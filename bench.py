@@ -183,7 +183,7 @@ def time_for(curve, target):
 
 
 def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, checkpoint, train_steps, max_depth,
-                    capacity=1 << 18, train_passes=None, seeds=3):
+                    capacity=1 << 18, train_passes=None, seeds=3, train_batch=0):
     """Equal-quality time: the plain sampler renders passes x spp; the target
     is its final MSE against a ref_spp reference (the same plain path, FP64
     accumulated); the guided sampler (trained online while it renders)
@@ -207,6 +207,7 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     runs = []
     for k in range(seeds):
         g = dr.Guide(ctx, s.bounds(), capacity=capacity, seed=3 + k)
+        g.set_options(1.0, 64, train_batch)
         runs.append(time_to_mse(ctx, s, cam, ref, g, spp_per_pass, 2 * passes, checkpoint, 950 + k, train_steps,
                                 max_depth, train_passes))
         g.close()
@@ -216,7 +217,7 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     tp, tg = plain[-1][0], time_for(guided, target)
     return {"resolution": [width, height], "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
             "spp_per_pass": spp_per_pass, "plain_spp": spp_per_pass * passes, "train_steps_per_pass": train_steps,
-            "train_passes": train_passes, "seeds": seeds,
+            "train_passes": train_passes, "train_batch": train_batch, "seeds": seeds,
             "guided_over_plain_mse_equal_spp": round(guided[passes // checkpoint - 1][1] / target, 3),
             "target_mse": target, "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1),
             "guided_over_plain_time": round(tg / tp, 3),
@@ -394,7 +395,7 @@ def main() -> int:
         guide["train_step"] = guide_train_rate(ctx)
         log("[bench] guide: indirect-lit partition scene (2,048,000 sphere tris), guided vs plain")
         guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
-                                                   8, 64, 8, 4, 12)
+                                                   8, 64, 8, 2, 12, train_batch=1 << 16)
         log("[bench] guide: hidden-light reflector scene, guided vs plain")
         guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 512, 512, 16384,
                                                       16, 64, 8, 16, 8, capacity=1 << 16, train_passes=8)

@@ -298,7 +298,7 @@ class Guide:
         h = C.c_void_p()
         check(lib().dr_guide_create(ctx.handle, C.byref(p), C.byref(h)))
         self.handle = h
-        self.collect_fraction, self.guided_vertices = 1.0, 64
+        self.collect_fraction, self.guided_vertices, self.train_batch = 1.0, 64, 0
 
     def close(self):
         if self.handle:
@@ -365,9 +365,9 @@ class Guide:
             check(lib().dr_guide_copy_samples(self.handle, ptr(out), n, C.byref(kept), C.byref(total)))
         return out, total.value
 
-    def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64):
-        check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices))
-        self.collect_fraction, self.guided_vertices = collect_fraction, guided_vertices
+    def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64, train_batch: int = 0):
+        check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices, train_batch))
+        self.collect_fraction, self.guided_vertices, self.train_batch = collect_fraction, guided_vertices, train_batch
 
 
 class Batch:

@@ -248,7 +248,7 @@ dr_status dr_scene_build_bvh(dr_scene *sc, const dr_bvh_params *p, dr_stream str
 dr_status dr_scene_set_trace_schedule(dr_scene *sc, uint32_t closest_sched, uint32_t any_sched)
 {
     if (!sc) return fail(DR_E_INVALID, "scene is null");
-    if (closest_sched > 7 || any_sched > 7) return fail(DR_E_INVALID, "schedule must be 0..7");
+    if (closest_sched > 15 || any_sched > 15) return fail(DR_E_INVALID, "schedule must be 0..15");
     auto *s = reinterpret_cast<Scene *>(sc);
     s->sched_closest = closest_sched;
     s->sched_any = any_sched;

@@ -231,7 +231,7 @@ dr_status dr_batch_render(dr_batch *bt, const dr_batch_view *views, uint32_t n_v
     dr_batch_options opt{kPathCap, DR_SCHED_PER_RAY_SPECULATIVE, 16u, 1u};
     if (options) opt = *options;
     if (opt.chunk_paths < 4096 || opt.chunk_paths > (1u << 23)) return fail(DR_E_INVALID, "chunk_paths out of range");
-    if (opt.trace_sched > 7u) return fail(DR_E_INVALID, "unknown trace schedule");
+    if (opt.trace_sched > 15u) return fail(DR_E_INVALID, "unknown trace schedule");
     if (opt.refill_min < 1u || opt.refill_min > 64u) return fail(DR_E_INVALID, "refill_min must be 1..64");
     std::vector<ViewInfo> vt(n_views);
     uint64_t total = 0;

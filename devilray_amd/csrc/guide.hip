@@ -376,6 +376,7 @@ struct TrainIn {
     const dr_guide_sample *smp;
     const uint32_t *count;  // device count (NULL: n)
     uint32_t n, cap;
+    uint32_t batch, window;  // batch > 0: samples (window * batch + i) mod count, i < batch
     float *partials;        // gridDim.x x (kNP + 1)
 };
 
@@ -461,7 +462,9 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net net, Box box, TrainIn in
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 15, g = lane >> 4;
     const int col = wave * 16 + s;
-    const uint32_t n = in.count ? min(*in.count, in.cap) : in.n;
+    const uint32_t avail = in.count ? min(*in.count, in.cap) : in.n;
+    const uint32_t n = in.batch ? min(avail, in.batch) : avail;
+    const uint32_t start = (in.batch && avail) ? (uint32_t)(((uint64_t)in.window * in.batch) % avail) : 0u;
     const uint32_t ntiles = (n + 63u) / 64u;
     f32x4 acc[kAcc];
 #pragma unroll
@@ -472,7 +475,8 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net net, Box box, TrainIn in
         V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f}, wi{0.0f, 0.0f, 1.0f};
         float w = 0.0f;
         if (i < n) {
-            const dr_guide_sample q = in.smp[i];
+            const uint32_t src = start + i < avail ? start + i : start + i - avail;
+            const dr_guide_sample q = in.smp[src];
             p = V3{q.pos[0], q.pos[1], q.pos[2]};
             nn = V3{q.normal[0], q.normal[1], q.normal[2]};
             wo = V3{q.wo[0], q.wo[1], q.wo[2]};
@@ -732,11 +736,11 @@ dr_status upload_params(Guide *g, const float *host)
 }
 
 dr_status run_train(Guide *g, const dr_guide_sample *smp, const uint32_t *count, uint32_t n, uint32_t cap,
-                    float *grad, hipStream_t stream)
+                    float *grad, hipStream_t stream, uint32_t batch = 0)
 {
-    const uint32_t n_max = count ? cap : n;
+    const uint32_t n_max = batch ? std::min(batch, count ? cap : n) : (count ? cap : n);
     const int nblk = (int)std::max<uint32_t>(1u, std::min<uint32_t>(kGuideTrainBlocks, cdiv(n_max, 64)));
-    TrainIn in{smp, count, n, cap, g->partials.as<float>()};
+    TrainIn in{smp, count, n, cap, batch, g->window++, g->partials.as<float>()};
     k_guide_train<<<nblk, kT, 0, stream>>>(net_of(g), box_of(g), in);
     DR_HIP_TRY(hipGetLastError());
     AdamIn ad{};
@@ -920,7 +924,7 @@ dr_status dr_guide_train_step(dr_guide *gd, const dr_guide_sample *samples, uint
     const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (samples) return run_train(g, samples, nullptr, n, n, nullptr, st);
     return run_train(g, g->ring.as<dr_guide_sample>(), g->counters.as<uint32_t>() + 2, 0, g->prm.capacity, nullptr,
-                     st);
+                     st, g->train_batch);
 }
 
 dr_status dr_guide_copy_samples(dr_guide *gd, dr_guide_sample *host, uint32_t max_n, uint32_t *n_kept,
@@ -941,13 +945,14 @@ dr_status dr_guide_copy_samples(dr_guide *gd, dr_guide_sample *host, uint32_t ma
     return DR_OK;
 }
 
-dr_status dr_guide_set_options(dr_guide *gd, float collect_fraction, uint32_t guided_vertices)
+dr_status dr_guide_set_options(dr_guide *gd, float collect_fraction, uint32_t guided_vertices, uint32_t train_batch)
 {
     if (!gd) return fail(DR_E_INVALID, "guide is null");
     if (!(collect_fraction >= 0.0f && collect_fraction <= 1.0f)) return fail(DR_E_INVALID, "fraction must be in [0, 1]");
     auto *g = reinterpret_cast<Guide *>(gd);
     g->collect_fraction = collect_fraction;
     g->guided_vertices = guided_vertices;
+    g->train_batch = train_batch;
     return DR_OK;
 }
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                                                   uint32_t *__restrict__ tri_tests,
                                                   int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
                                                   uint32_t *__restrict__ ray_counter, uint32_t refill_min,
-                                                  const uint32_t *__restrict__ roots)
+                                                  const uint32_t *__restrict__ roots, uint32_t slices)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
     // ray count: fixed at launch, or read from device memory (on-device queues)
@@ -102,6 +102,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
     const size_t gl = (size_t)blockIdx.x * kBlock + tid;
     const size_t nlanes = (size_t)gridDim.x * kBlock;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    // persistent: the (sorted) ray range splits into `slices` contiguous
+    // parts with a counter each; workgroups are dispatched to the XCDs in
+    // turn, so with 8 slices each XCD's L2 serves one region of the batch
+    const uint32_t part = blockIdx.x % slices;
+    const uint32_t part_lo = (uint32_t)(((uint64_t)n * part) / slices);
+    const uint32_t part_hi = (uint32_t)(((uint64_t)n * (part + 1u)) / slices);
 
     int32_t ray = -1;
     bool exhausted = false;
@@ -140,13 +146,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
             uint32_t base = 0;
             if (PERSISTENT) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
-                if ((int)lane == leader) base = atomicAdd(ray_counter, (uint32_t)__popcll(m));
+                if ((int)lane == leader) base = atomicAdd(ray_counter + part, (uint32_t)__popcll(m));
                 base = __shfl(base, leader, 64);
             }
             if (need) {
-                const uint32_t slot = PERSISTENT ? base + (uint32_t)__popcll(m & lt) : (uint32_t)gl;
+                const uint32_t slot = PERSISTENT ? part_lo + base + (uint32_t)__popcll(m & lt) : (uint32_t)gl;
                 if (!PERSISTENT) exhausted = true;
-                if (slot < n) {
+                if (slot < (PERSISTENT ? part_hi : n)) {
                     const uint32_t r = perm ? perm[slot] : slot;
                     ray = (int32_t)r;
                     const float4 r0 = rays[2 * (size_t)r];
@@ -307,6 +313,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
 }
 
 __global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
+__global__ void k_zero_counters8(uint32_t *c)
+{
+    if (threadIdx.x < 8) c[threadIdx.x] = 0u;
+}
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v)
 {
@@ -368,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
 using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
                             const uint32_t *, dr_hit *,
                             uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t,
-                            const uint32_t *);
+                            const uint32_t *, uint32_t);
 
 template <bool ANY, bool COUNT>
 TraceKernel pick(uint32_t sched)
@@ -434,13 +444,15 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     }
     // flags[0]: overflow; flags[16 ..]: per-launch ray counters (ring)
     uint32_t *flags = s->flags.as<uint32_t>();
-    uint32_t *counter = flags + 16 + (s->launch_seq++ % kCounterSlots);
-    if (persistent) k_zero_counter<<<1, 1, 0, stream>>>(counter);
+    uint32_t *counter = flags + 16 + 8 * (s->launch_seq++ % (kCounterSlots / 8));
+    if (persistent) k_zero_counters8<<<1, 64, 0, stream>>>(counter);
+    // every slice needs at least one workgroup to drain it
+    const uint32_t slices = (persistent && (sched & DR_SCHED_XCD_SLICES) && grid >= 8u) ? 8u : 1u;
     if (count)
         for (int k = 2; k <= 5; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
-                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root);
+                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

@@ -53,7 +53,8 @@ class ProgressiveRenderer:
                 if not self._fraction_set:
                     _, total = self.guide.copy_samples(0)
                     if total > 0:
-                        self.guide.set_options(min(1.0, 4 * self.guide.capacity / total), self.guide.guided_vertices)
+                        self.guide.set_options(min(1.0, 4 * self.guide.capacity / total), self.guide.guided_vertices,
+                                               self.guide.train_batch)
                     self._fraction_set = True
                 for _ in range(self.train_steps):
                     self.guide.train_step_dev(stream=self.stream)

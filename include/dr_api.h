@@ -143,7 +143,10 @@ enum {
     DR_SCHED_PERSISTENT = 2,
     DR_SCHED_PERSISTENT_SPECULATIVE = 3,
     /* flag: sort the batch by (dominant axis, origin Morton code) first */
-    DR_SCHED_REORDER = 4
+    DR_SCHED_REORDER = 4,
+    /* flag (persistent schedules): split the ray range into 8 contiguous
+     * slices with one counter each, one per XCD (its own L2) */
+    DR_SCHED_XCD_SLICES = 8
 };
 dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
 /* Tuning knobs (chosen by the autotuner): persistent waves refill their
@@ -324,10 +327,13 @@ dr_status dr_guide_train_step(dr_guide *guide, const dr_guide_sample *samples, u
  * offered). host may be NULL to query the counts. Synchronises. */
 dr_status dr_guide_copy_samples(dr_guide *guide, dr_guide_sample *host, uint32_t max_n, uint32_t *n_kept,
                                 uint32_t *n_total);
-/* Fraction of eligible vertices a collecting render records (default 1),
- * and the number of leading path vertices that sample from the guide
- * (default 64 = all; later vertices sample the BSDF alone). */
-dr_status dr_guide_set_options(dr_guide *guide, float collect_fraction, uint32_t guided_vertices);
+/* Fraction of eligible vertices a collecting render records (default 1);
+ * the number of leading path vertices that sample from the guide (default
+ * 64 = all; later vertices sample the BSDF alone); samples per training step
+ * from the ring (0 = all of it; otherwise consecutive steps take successive
+ * windows of that size, wrapping around the ring). */
+dr_status dr_guide_set_options(dr_guide *guide, float collect_fraction, uint32_t guided_vertices,
+                               uint32_t train_batch);
 
 /* dr_render with the guide: DR_GUIDE_USE samples Lambert bounces from the
  * one-sample MIS of the guide mixture and the cosine BSDF (with probability
