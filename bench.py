@@ -304,7 +304,7 @@ def probe() -> int:
     scene.build_bvh(leaf_max=4)
     d_rays = ctx.upload(rays)
     d_hits = ctx.alloc(NRAYS * 16)
-    for _ in range(3):
+    for _ in range(10):
         scene.trace_closest_dev(d_rays.ptr, NRAYS, d_hits.ptr)
     ctx.synchronize()
     a, b = ctx.alloc(256 << 20), ctx.alloc(256 << 20)

@@ -83,8 +83,8 @@ struct Scene {
     DevBuf ovf_stack;
     DevBuf flags;     // uint32[512]: [0] overflow flag, [16..271] per-launch ray counters
     uint64_t launch_seq = 0;
-    uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER | DR_SCHED_XCD_SLICES;  // traversal schedules
-    uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER | DR_SCHED_XCD_SLICES;
+    uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
+    uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;
     uint32_t refill_min = 16;  // persistent waves refill when this many lanes are idle
     uint32_t blocks_per_cu = 0;  // persistent grid: resident blocks per CU (0 = occupancy limit)
     uint32_t sort_key = 0;       // ray reorder key: 0 origin, 1 direction + origin

@@ -104,10 +104,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
     // persistent: the (sorted) ray range splits into `slices` contiguous
     // parts with a counter each; workgroups are dispatched to the XCDs in
-    // turn, so with 8 slices each XCD's L2 serves one region of the batch
-    const uint32_t part = blockIdx.x % slices;
-    const uint32_t part_lo = (uint32_t)(((uint64_t)n * part) / slices);
-    const uint32_t part_hi = (uint32_t)(((uint64_t)n * (part + 1u)) / slices);
+    // turn, so with 8 slices each XCD's L2 serves one region of the batch. A
+    // wave whose part is drained moves on to the next one (wave-uniform
+    // state), so uneven parts do not leave an XCD idle.
+    uint32_t part = blockIdx.x % slices, parts_seen = 1u;
 
     int32_t ray = -1;
     bool exhausted = false;
@@ -144,15 +144,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
         const bool refill = PERSISTENT ? ((uint32_t)__popcll(m) >= refill_min || __ballot(ray >= 0) == 0ull) : true;
         if (m && refill) {
             uint32_t base = 0;
+            const uint32_t part_lo = PERSISTENT ? (uint32_t)(((uint64_t)n * part) / slices) : 0u;
+            const uint32_t part_hi = PERSISTENT ? (uint32_t)(((uint64_t)n * (part + 1u)) / slices) : n;
             if (PERSISTENT) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
                 if ((int)lane == leader) base = atomicAdd(ray_counter + part, (uint32_t)__popcll(m));
                 base = __shfl(base, leader, 64);
             }
+            const bool drained = PERSISTENT && part_lo + base + (uint32_t)__popcll(m) >= part_hi;
             if (need) {
                 const uint32_t slot = PERSISTENT ? part_lo + base + (uint32_t)__popcll(m & lt) : (uint32_t)gl;
                 if (!PERSISTENT) exhausted = true;
-                if (slot < (PERSISTENT ? part_hi : n)) {
+                if (slot < part_hi) {
                     const uint32_t r = perm ? perm[slot] : slot;
                     ray = (int32_t)r;
                     const float4 r0 = rays[2 * (size_t)r];
@@ -174,12 +177,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     nx = signbit(p.ix) ? 16u : 0u;
                     ny = signbit(p.iy) ? 48u : 32u;
                     nz = signbit(p.iz) ? 80u : 64u;
-                } else {
+                } else if (!PERSISTENT || parts_seen >= slices) {
                     exhausted = true;
                 }
             }
+            if (drained && parts_seen < slices) {
+                part = part + 1u == slices ? 0u : part + 1u;
+                ++parts_seen;
+            }
         }
-        if (__ballot(ray >= 0) == 0ull) break;
+        // done when no lane holds a ray and none can fetch one (every part
+        // visited and drained); each pass either fetches or visits a part
+        if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
 
         if (ray >= 0) {
             // ---- interior nodes, leaves postponed

@@ -145,7 +145,10 @@ enum {
     /* flag: sort the batch by (dominant axis, origin Morton code) first */
     DR_SCHED_REORDER = 4,
     /* flag (persistent schedules): split the ray range into 8 contiguous
-     * slices with one counter each, one per XCD (its own L2) */
+     * slices with one counter each, one per XCD (its own L2); waves move on
+     * to the next slice when theirs is drained. Measured on the traversal
+     * benchmark: within noise for incoherent rays, about 10% slower for
+     * coherent camera rays, so not in the default. */
     DR_SCHED_XCD_SLICES = 8
 };
 dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
