@@ -12,7 +12,8 @@ at 1920x1080, timed over a few samples per pixel), and the learned guide:
 one training step on 2^18 samples (TFLOP/s against the BF16 MFMA peak) and
 the time to a target MSE, guided vs plain, on the indirect-lit partition
 scene (240x135, reference 65,536 spp of the plain path in FP64) and on the
-hidden-light reflector scene.
+hidden-light reflector scene; and batched rendering of 64 uneven scenes
+(one dr_render per scene against one autotuned dr_batch_render).
 
 Prints one JSON line. `roofline.achieved` uses the algorithmic bytes per ray
 48 + 32*N_node + 40*N_tri with N from the canonical binary SAH BVH counts
