@@ -185,8 +185,12 @@ class Scene:
             check(lib().dr_scene_destroy(self.handle))
             self.handle = None
 
-    def build_bvh(self, leaf_max: int = 4, stream=None):
-        p = DrBvhParams(leaf_max, 0)
+    def build_bvh(self, leaf_max: int = 4, builder: str = "ploc", radius: int = 0, stream=None):
+        """builder: "ploc" (clustering, default) or "lbvh" (radix tree);
+        radius: PLOC search radius (0 = the library default)."""
+        if builder not in ("ploc", "lbvh"):
+            raise ValueError("builder must be 'ploc' or 'lbvh'")
+        p = DrBvhParams(leaf_max, (1 if builder == "lbvh" else 0) | ((radius & 0xFF) << 8))
         check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
 
     def set_schedule(self, closest: int, any_hit: int | None = None):

@@ -2,9 +2,11 @@
 //
 //  1. per-triangle boxes and the centroid bounds (one atomic per block)
 //  2. 30-bit Morton codes of the centroids, radix-sorted with their ids
-//  3. binary radix tree over the sorted codes (Karras, "Maximizing Parallelism
-//     in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012); equal
-//     codes are ordered by their position
+//  3. binary tree: PLOC clustering of the Morton-ordered leaves (default,
+//     k_ploc_* below, leaves renumbered depth-first), or the binary radix tree
+//     over the sorted codes (Karras, "Maximizing Parallelism in the
+//     Construction of BVHs, Octrees, and k-d Trees", HPG 2012; equal codes are
+//     ordered by their position)
 //  4. bottom-up pass (atomic arrival flags): boxes, triangle counts and the
 //     SAH cost C(n) = min(Ci*A(n)*N(n), Ct*A(n) + C(l) + C(r)); a subtree is
 //     collapsed into one leaf when the leaf term wins and N(n) <= leaf_max
@@ -12,7 +14,7 @@
 //     High-Quality BVHs", HPG 2013)
 //  5. top-down level-synchronous collapse into 4-wide nodes: each wide node
 //     opens its largest-area interior child until it has four children
-//  6. triangles are written leaf-ordered (sorted order) as 3 float4 with the
+//  6. triangles are written leaf-ordered as 3 float4 with the
 //     caller's triangle id in the first w.
 // Child boxes are rounded outward by one ulp so every vertex lies strictly
 // inside its leaf box.
@@ -336,6 +338,228 @@ __global__ __launch_bounds__(kT) void k_fill_u32(uint32_t *p, uint32_t n, uint32
 
 inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
 
+// ---- PLOC binary tree (Meister & Bittner, "Parallel Locally-Ordered
+// Clustering for Bounding Volume Hierarchy Construction", IEEE TVCG 2018).
+// Clusters stay in Morton order. Each round every cluster finds the cluster
+// within +-r positions whose merged box has the least surface area; mutual
+// nearest neighbours merge, the rest carry over, and a scan compacts the
+// list. Pairs are strictly ordered (area first, ties broken by the indices),
+// so the smallest pair of a round is always mutual and every round merges at
+// least one pair. Internal node ids are handed out from n-2 downwards, so the last
+// merge is node 0 (the root) and a child always has a larger id than its
+// parent. Leaves are then renumbered in depth-first order so each subtree
+// covers a contiguous range, which the SAH collapse needs.
+constexpr int kPlocRMax = 64;
+
+__global__ __launch_bounds__(kT) void k_ploc_init(uint32_t n, int32_t *__restrict__ ref, uint32_t *__restrict__ cnt,
+                                                  int32_t *__restrict__ pi)
+{
+    const uint32_t k = blockIdx.x * kT + threadIdx.x;
+    if (k >= n) return;
+    ref[k] = ~(int32_t)k;
+    cnt[k] = 1u;
+    if (k == 0) pi[0] = -1;
+}
+
+__global__ __launch_bounds__(kT) void k_ploc_nn(const float4 *__restrict__ lo, const float4 *__restrict__ hi,
+                                                uint32_t m, int r, int32_t *__restrict__ nn)
+{
+    __shared__ float4 slo[kT + 2 * kPlocRMax], shi[kT + 2 * kPlocRMax];
+    const int base = (int)(blockIdx.x * kT) - r;
+    for (int t = threadIdx.x; t < kT + 2 * r; t += kT) {
+        const int g = base + t;
+        if (g >= 0 && g < (int)m) { slo[t] = lo[g]; shi[t] = hi[g]; }
+    }
+    __syncthreads();
+    const int i = (int)(blockIdx.x * kT + threadIdx.x);
+    if (i >= (int)m) return;
+    const float4 a = slo[i - base], b = shi[i - base];
+    const int j0 = i - r > 0 ? i - r : 0;
+    const int j1 = i + r < (int)m - 1 ? i + r : (int)m - 1;
+    // Pair order: (area, index distance, parity of the lower index, lower
+    // index). On exact ties this pairs (0,1), (2,3), ... instead of chaining
+    // every cluster to its left neighbour, which would merge one pair a round.
+    float best = 0.0f;
+    int bj = -1, bdist = 0, blow = 0;
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float4 c = slo[j - base], d = shi[j - base];
+        const float4 l = make_float4(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z), 0.0f);
+        const float4 h = make_float4(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z), 0.0f);
+        const float area = half_area(l, h);
+        const int dist = j > i ? j - i : i - j;
+        const int low = j < i ? j : i;
+        const bool better =
+            bj < 0 || area < best ||
+            (area == best && (dist < bdist || (dist == bdist && ((low & 1) < (blow & 1) ||
+                                                                 ((low & 1) == (blow & 1) && low < blow)))));
+        if (better) { best = area; bj = j; bdist = dist; blow = low; }
+    }
+    nn[i] = bj;
+}
+
+// f[i] = (merges here << 32) | (cluster i survives); f[m] = 0 for the totals
+__global__ __launch_bounds__(kT) void k_ploc_flags(const int32_t *__restrict__ nn, uint32_t m,
+                                                   uint64_t *__restrict__ f)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i > m) return;
+    if (i == m) { f[m] = 0; return; }
+    const int32_t j = nn[i];
+    const bool mutual = nn[j] == (int32_t)i;
+    const bool merge = mutual && (int32_t)i < j;
+    const bool keep = !mutual || (int32_t)i < j;
+    f[i] = ((uint64_t)(merge ? 1u : 0u) << 32) | (keep ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(kT) void k_ploc_emit(
+    uint32_t m, const int32_t *__restrict__ nn, const uint64_t *__restrict__ f, const int32_t *__restrict__ ref_in,
+    const float4 *__restrict__ lo_in, const float4 *__restrict__ hi_in, const uint32_t *__restrict__ cnt_in,
+    int32_t *__restrict__ ref_out, float4 *__restrict__ lo_out, float4 *__restrict__ hi_out,
+    uint32_t *__restrict__ cnt_out, int32_t *__restrict__ cl, int32_t *__restrict__ cr, int32_t *__restrict__ pi,
+    int32_t *__restrict__ pl, uint32_t *__restrict__ ncnt, uint32_t id_base)
+{
+    const int32_t i = (int32_t)(blockIdx.x * kT + threadIdx.x);
+    if (i >= (int32_t)m) return;
+    const int32_t j = nn[i];
+    const bool mutual = nn[j] == i;
+    if (mutual && i > j) return;
+    const uint64_t fi = f[i];
+    const uint32_t p = (uint32_t)fi;
+    if (!mutual) {
+        ref_out[p] = ref_in[i];
+        lo_out[p] = lo_in[i];
+        hi_out[p] = hi_in[i];
+        cnt_out[p] = cnt_in[i];
+        return;
+    }
+    const int32_t id = (int32_t)(id_base - (uint32_t)(fi >> 32));
+    const int32_t a = ref_in[i], b = ref_in[j];
+    cl[id] = a;
+    cr[id] = b;
+    if (a < 0) pl[~a] = id; else pi[a] = id;
+    if (b < 0) pl[~b] = id; else pi[b] = id;
+    const float4 al = lo_in[i], ah = hi_in[i], bl = lo_in[j], bh = hi_in[j];
+    const uint32_t c = cnt_in[i] + cnt_in[j];
+    ncnt[id] = c;
+    ref_out[p] = id;
+    lo_out[p] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+    hi_out[p] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+    cnt_out[p] = c;
+}
+
+// Depth-first first-leaf position of every node: the counts of the left
+// siblings on the way up to the root. Internal nodes x < n-1 get rfirst[x],
+// leaf k (x = n-1+k, Morton position) gets pos[k].
+__global__ __launch_bounds__(kT) void k_ploc_first(uint32_t n, const int32_t *__restrict__ cl,
+                                                   const int32_t *__restrict__ cr, const int32_t *__restrict__ pi,
+                                                   const int32_t *__restrict__ pl, const uint32_t *__restrict__ cnt,
+                                                   uint32_t *__restrict__ rfirst, uint32_t *__restrict__ pos)
+{
+    const uint32_t x = blockIdx.x * kT + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    const bool leaf = x >= n - 1;
+    int32_t me = leaf ? ~(int32_t)(x - (n - 1)) : (int32_t)x;
+    int32_t p = leaf ? pl[x - (n - 1)] : pi[x];
+    uint32_t first = 0;
+    while (p >= 0) {
+        if (cr[p] == me) {
+            const int32_t l = cl[p];
+            first += l < 0 ? 1u : cnt[l];
+        }
+        me = p;
+        p = pi[p];
+    }
+    if (leaf) pos[x - (n - 1)] = first; else rfirst[x] = first;
+}
+
+__global__ __launch_bounds__(kT) void k_ploc_relabel_nodes(uint32_t ni, int32_t *__restrict__ cl,
+                                                           int32_t *__restrict__ cr, const uint32_t *__restrict__ pos)
+{
+    const uint32_t x = blockIdx.x * kT + threadIdx.x;
+    if (x >= ni) return;
+    const int32_t a = cl[x], b = cr[x];
+    if (a < 0) cl[x] = ~(int32_t)pos[~a];
+    if (b < 0) cr[x] = ~(int32_t)pos[~b];
+}
+
+__global__ __launch_bounds__(kT) void k_ploc_relabel_leaves(uint32_t n, const uint32_t *__restrict__ vals,
+                                                            const int32_t *__restrict__ pl_morton,
+                                                            const uint32_t *__restrict__ pos,
+                                                            uint32_t *__restrict__ vals_dfs,
+                                                            int32_t *__restrict__ pl_dfs)
+{
+    const uint32_t k = blockIdx.x * kT + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = pos[k];
+    vals_dfs[p] = vals[k];
+    pl_dfs[p] = pl_morton[k];
+}
+
+// Builds the binary tree over the n >= 2 leaves whose boxes t.leaf_lo/hi hold
+// in Morton order, into the arrays the SAH fit and the collapse read
+// (child_l/r, parent_i/l, range_first, count), and re-gathers the leaf boxes
+// and triangles in depth-first order.
+dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, hipStream_t stream)
+{
+    Context *ctx = s->ctx;
+    DevBuf ref[2], lo1, hi1, cnt[2], nn, f, pl_morton, vals_dfs;
+    dr_status st = DR_OK;
+    auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
+    for (int b = 0; b < 2; ++b) {
+        need(ref[b], sizeof(int32_t) * n);
+        need(cnt[b], sizeof(uint32_t) * n);
+    }
+    need(lo1, sizeof(float4) * n);
+    need(hi1, sizeof(float4) * n);
+    need(nn, sizeof(int32_t) * n);
+    need(f, sizeof(uint64_t) * (n + 1));
+    need(pl_morton, sizeof(int32_t) * n);
+    need(vals_dfs, sizeof(uint32_t) * n);
+    if (st) return st;
+    float4 *lo[2] = {t.leaf_lo.as<float4>(), lo1.as<float4>()};
+    float4 *hi[2] = {t.leaf_hi.as<float4>(), hi1.as<float4>()};
+    int32_t *cl = t.child_l.as<int32_t>(), *cr = t.child_r.as<int32_t>(), *pi = t.parent_i.as<int32_t>();
+    uint32_t *ncnt = t.count.as<uint32_t>();
+    k_ploc_init<<<blocks(n), kT, 0, stream>>>(n, ref[0].as<int32_t>(), cnt[0].as<uint32_t>(), pi);
+    DR_HIP_TRY(hipGetLastError());
+    uint32_t m = n, made = 0;
+    int cur = 0;
+    while (m > 1) {
+        k_ploc_nn<<<blocks(m), kT, 0, stream>>>(lo[cur], hi[cur], m, radius, nn.as<int32_t>());
+        k_ploc_flags<<<blocks(m + 1), kT, 0, stream>>>(nn.as<int32_t>(), m, f.as<uint64_t>());
+        DR_HIP_TRY(hipGetLastError());
+        if ((st = exclusive_scan_u64(ctx, f.as<uint64_t>(), m + 1, stream))) return st;
+        k_ploc_emit<<<blocks(m), kT, 0, stream>>>(m, nn.as<int32_t>(), f.as<uint64_t>(), ref[cur].as<int32_t>(),
+                                                  lo[cur], hi[cur], cnt[cur].as<uint32_t>(), ref[1 - cur].as<int32_t>(),
+                                                  lo[1 - cur], hi[1 - cur], cnt[1 - cur].as<uint32_t>(), cl, cr, pi,
+                                                  pl_morton.as<int32_t>(), ncnt, n - 2 - made);
+        DR_HIP_TRY(hipGetLastError());
+        uint64_t tot = 0;
+        DR_HIP_TRY(hipMemcpyAsync(&tot, f.as<uint64_t>() + m, 8, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipStreamSynchronize(stream));
+        const uint32_t merges = (uint32_t)(tot >> 32), kept = (uint32_t)tot;
+        // A round without a merge means an unordered (NaN) area: the bounds are not finite.
+        if (merges == 0 || kept != m - merges || made + merges > n - 1)
+            return fail(DR_E_INVALID, "BVH build: triangle bounds are not finite");
+        made += merges;
+        m = kept;
+        cur = 1 - cur;
+    }
+    k_ploc_first<<<blocks(2 * n - 1), kT, 0, stream>>>(n, cl, cr, pi, pl_morton.as<int32_t>(), ncnt,
+                                                       t.range_first.as<uint32_t>(), t.keys.as<uint32_t>());
+    k_ploc_relabel_nodes<<<blocks(n - 1), kT, 0, stream>>>(n - 1, cl, cr, t.keys.as<uint32_t>());
+    k_ploc_relabel_leaves<<<blocks(n), kT, 0, stream>>>(n, t.vals.as<uint32_t>(), pl_morton.as<int32_t>(),
+                                                        t.keys.as<uint32_t>(), vals_dfs.as<uint32_t>(),
+                                                        t.parent_l.as<int32_t>());
+    k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), vals_dfs.as<uint32_t>(), n,
+                                           t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.leaf_lo.as<float4>(),
+                                           t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
+    DR_HIP_TRY(hipGetLastError());
+    DR_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
+    return DR_OK;
+}
+
 }  // namespace
 
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
@@ -345,6 +569,11 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     if (leaf_max < 1 || leaf_max > (uint32_t)kLeafMaxCap) return fail(DR_E_INVALID, "leaf_max must be 1..8");
     if (n == 0) return fail(DR_E_INVALID, "scene has no triangles");
     if (n >= (1u << 27)) return fail(DR_E_UNSUPPORTED, "more than 2^27 triangles per scene");
+    const uint32_t flags = p ? p->flags : 0u;
+    if (flags & ~(DR_BVH_LBVH | DR_BVH_RADIUS_MASK)) return fail(DR_E_INVALID, "unknown BVH flags");
+    const bool ploc = !(flags & DR_BVH_LBVH);
+    const int radius = (flags & DR_BVH_RADIUS_MASK) ? (int)((flags & DR_BVH_RADIUS_MASK) >> 8) : 16;
+    if (radius > kPlocRMax) return fail(DR_E_INVALID, "PLOC search radius must be 1..64");
     Context *ctx = s->ctx;
 
     hipEvent_t e0, e1;
@@ -410,10 +639,14 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         const float b[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
         for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
     } else {
-        k_karras<<<blocks(n - 1), kT, 0, stream>>>(t.keys.as<uint32_t>(), (int)n, t.child_l.as<int32_t>(),
-                                                   t.child_r.as<int32_t>(), t.parent_i.as<int32_t>(),
-                                                   t.parent_l.as<int32_t>(), t.range_first.as<uint32_t>(),
-                                                   t.count.as<uint32_t>());
+        if (ploc) {
+            if ((st = build_ploc(s, t, n, radius, stream))) return st;
+        } else {
+            k_karras<<<blocks(n - 1), kT, 0, stream>>>(t.keys.as<uint32_t>(), (int)n, t.child_l.as<int32_t>(),
+                                                       t.child_r.as<int32_t>(), t.parent_i.as<int32_t>(),
+                                                       t.parent_l.as<int32_t>(), t.range_first.as<uint32_t>(),
+                                                       t.count.as<uint32_t>());
+        }
         k_fill_u32<<<blocks(n - 1), kT, 0, stream>>>(t.visit.as<uint32_t>(), n - 1, 0u);
         k_fit<<<blocks(n), kT, 0, stream>>>((int)n, t.child_l.as<int32_t>(), t.child_r.as<int32_t>(),
                                             t.parent_i.as<int32_t>(), t.parent_l.as<int32_t>(), t.count.as<uint32_t>(),

@@ -73,11 +73,17 @@ typedef struct {
     uint32_t n_materials;
 } dr_scene_desc;
 
-/* BVH build parameters: max triangles per leaf (1..8). flags reserved (0). */
+/* BVH build parameters: max triangles per leaf (1..8) and DR_BVH_* flags.
+ * The binary tree is built by PLOC clustering over the Morton order (default)
+ * or as a Karras radix tree over the Morton codes (DR_BVH_LBVH, faster to
+ * build, lower quality); both then go through the same SAH leaf collapse and
+ * 4-wide collapse. Bits 8-15 set the PLOC search radius (0: 16, max 64). */
 typedef struct {
     uint32_t leaf_max;
     uint32_t flags;
 } dr_bvh_params;
+#define DR_BVH_LBVH 1u
+#define DR_BVH_RADIUS_MASK 0xFF00u
 
 typedef struct {
     uint32_t n_triangles;

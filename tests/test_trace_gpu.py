@@ -42,6 +42,43 @@ def test_small_sphere_vs_brute_force(gpu_ctx, leaf_max):
     scene.close()
 
 
+@pytest.mark.parametrize("builder,radius", [("lbvh", 0), ("ploc", 1), ("ploc", 3), ("ploc", 64)])
+def test_builders_vs_brute_force(gpu_ctx, builder, radius):
+    print(f"[case] builder {builder} radius {radius}", flush=True)
+    v, f = dr.gen_icosphere(3, 0.05, 11)
+    rays = np.concatenate([dr.gen_rays_surface(v, f, 4096, seed=5, inward=True),
+                           dr.gen_rays_surface(v, f, 2048, seed=6, inward=False)])
+    scene = dr.Scene(gpu_ctx, v, f)
+    for leaf_max in (1, 4):
+        scene.build_bvh(leaf_max=leaf_max, builder=builder, radius=radius)
+        assert_hits_equal(scene.trace_closest(rays), oracle.brute_closest(v, f, rays), f"{builder} r{radius}")
+        assert np.array_equal(scene.trace_any(rays), oracle.brute_any(v, f, rays))
+    scene.close()
+
+
+@pytest.mark.parametrize("n_tris", [2, 3, 5, 4000])
+def test_ploc_identical_boxes(gpu_ctx, n_tris):
+    """Every triangle the same (exact ties in every clustering round) plus a
+    far-away one: the build must finish and the lowest id must win each tie."""
+    print(f"[case] identical boxes n={n_tris}", flush=True)
+    tri = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], np.float32)
+    v = np.concatenate([np.tile(tri, (n_tris - 1, 1)), tri + np.float32([5.0, 5.0, 5.0])]).astype(np.float32)
+    f = np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)
+    rng = np.random.default_rng(n_tris)
+    rays = np.zeros(2048, dr.RAY_DTYPE)
+    o = np.stack([rng.uniform(-0.2, 1.2, 2048), rng.uniform(-0.2, 1.2, 2048), np.full(2048, 1.0)], 1)
+    o[1024:] += 5.0
+    rays["ox"], rays["oy"], rays["oz"] = o[:, 0], o[:, 1], o[:, 2]
+    rays["dz"] = -1.0
+    rays["tmax"] = np.inf
+    scene = dr.Scene(gpu_ctx, v, f)
+    scene.build_bvh(leaf_max=2)
+    got = scene.trace_closest(rays)
+    assert_hits_equal(got, oracle.brute_closest(v, f, rays), f"identical n={n_tris}")
+    assert np.all(got["id"][:1024][got["id"][:1024] >= 0] == 0)
+    scene.close()
+
+
 def test_c1_full_size_incoherent_and_coherent(gpu_ctx):
     """Full traversal benchmark size: 1,310,720 triangles, 2^20 rays per batch."""
     print("[case] C1 full size", flush=True)
