@@ -185,12 +185,21 @@ class Scene:
             check(lib().dr_scene_destroy(self.handle))
             self.handle = None
 
-    def build_bvh(self, leaf_max: int = 4, builder: str = "ploc", radius: int = 0, stream=None):
+    def build_bvh(self, leaf_max: int = 4, builder: str = "ploc", radius: int = 0, split: float | None = None,
+                  stream=None):
         """builder: "ploc" (clustering, default) or "lbvh" (radix tree);
-        radius: PLOC search radius (0 = the library default)."""
+        radius: PLOC search radius (0 = the library default); split: extra
+        triangle references as a fraction of the triangle count, in steps of
+        1/4 up to 63.75 (None = the library default, 0 = no splitting)."""
         if builder not in ("ploc", "lbvh"):
             raise ValueError("builder must be 'ploc' or 'lbvh'")
-        p = DrBvhParams(leaf_max, (1 if builder == "lbvh" else 0) | ((radius & 0xFF) << 8))
+        flags = (1 if builder == "lbvh" else 0) | ((radius & 0xFF) << 8)
+        if split is not None:
+            q = int(round(split * 4))
+            if not 0 <= q <= 255:
+                raise ValueError("split must be in [0, 63.75]")
+            flags |= (q << 16) if q else 2
+        p = DrBvhParams(leaf_max, flags)
         check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
 
     def set_schedule(self, closest: int, any_hit: int | None = None):

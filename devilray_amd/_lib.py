@@ -52,7 +52,8 @@ class DrSceneInfo(C.Structure):
     _fields_ = [
         ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
         ("depth", C.c_uint32), ("overflow_flag", C.c_uint32), ("simd_stats", C.c_uint32 * 4),
-        ("build_ms", C.c_double), ("sah_cost", C.c_double),
+        ("build_ms", C.c_double), ("sah_cost", C.c_double), ("n_references", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -228,6 +228,8 @@ dr_status dr_scene_get_info(dr_scene *sc, dr_scene_info *info)
     info->depth = s->depth;
     info->build_ms = s->build_ms;
     info->sah_cost = s->sah_cost;
+    info->n_references = s->n_refs;
+    info->reserved = 0;
     if (s->flags.ptr) {
         uint32_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         DR_HIP_TRY(hipMemcpy(f, s->flags.ptr, 32, hipMemcpyDeviceToHost));

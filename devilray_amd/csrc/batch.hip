@@ -43,9 +43,9 @@ __global__ __launch_bounds__(kT) void k_rebase_triv(const float4 *__restrict__ s
     dst[3 * (size_t)i + 2] = src[3 * (size_t)i + 2];
 }
 
-// interior references move by node_off, leaf references by tri_off
+// interior references move by node_off, leaf references by ref_off
 __global__ __launch_bounds__(kT) void k_rebase_nodes(const WideNode *__restrict__ src, uint32_t n, uint32_t node_off,
-                                                     uint32_t tri_off, WideNode *__restrict__ dst)
+                                                     uint32_t ref_off, WideNode *__restrict__ dst)
 {
     const uint32_t i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kT) void k_rebase_nodes(const WideNode *__restrict_
     for (int c = 0; c < 4; ++c) {
         const int32_t r = w.child[c];
         if (r >= 0) w.child[c] = r + (int32_t)node_off;
-        else if (r != DR_EMPTY_REF) w.child[c] = make_leaf_ref(leaf_first(r) + tri_off, leaf_count(r));
+        else if (r != DR_EMPTY_REF) w.child[c] = make_leaf_ref(leaf_first(r) + ref_off, leaf_count(r));
     }
     dst[i] = w;
 }
@@ -93,18 +93,20 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
 {
     Scene &m = b->merged;
     m.ctx = b->ctx;
-    uint64_t nv = 0, nt = 0, nm = 0, nn = 0, ne = 0;
+    uint64_t nv = 0, nt = 0, nm = 0, nn = 0, ne = 0, nr = 0;
     for (uint32_t k = 0; k < n; ++k) {
         nv += sc[k]->n_verts;
         nt += sc[k]->n_tris;
+        nr += sc[k]->n_refs;
         nm += sc[k]->n_mats;
         nn += sc[k]->n_nodes;
         ne += sc[k]->n_emit;
     }
-    if (nt >= (1ull << 27) || nn >= (1ull << 31) || nv >= (1ull << 32))
+    if (nr >= (1ull << 27) || nt >= (1ull << 31) || nn >= (1ull << 31) || nv >= (1ull << 32))
         return fail(DR_E_INVALID, "batch too large for 27-bit leaf triangle indices");
     m.n_verts = (uint32_t)nv;
     m.n_tris = (uint32_t)nt;
+    m.n_refs = (uint32_t)nr;
     m.n_mats = (uint32_t)nm;
     m.n_nodes = (uint32_t)nn;
     m.n_emit = (uint32_t)ne;
@@ -114,7 +116,7 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
     need(m.indices, 12ull * nt);
     need(m.tri_mat, 4ull * nt);
     need(m.materials, sizeof(dr_material) * nm);
-    need(m.tri_v, 48ull * nt);
+    need(m.tri_v, 48ull * nr);
     need(m.nodes, sizeof(WideNode) * nn);
     need(m.light_pdf_area, 4ull * nt);
     need(m.alias, sizeof(AliasEntryB) * std::max<uint64_t>(ne, 1));
@@ -124,7 +126,7 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
     DR_HIP_TRY(hipMemset(m.flags.ptr, 0, 2048));
     std::vector<SceneRange> ranges(n);
     m.host_materials.clear();
-    uint64_t vo = 0, to = 0, mo = 0, no = 0, eo = 0;
+    uint64_t vo = 0, to = 0, mo = 0, no = 0, eo = 0, ro = 0;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     uint32_t depth = 0, leaves = 0;
     const hipStream_t s0 = nullptr;
@@ -138,13 +140,13 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
                                                                (uint32_t)vo, m.indices.as<uint32_t>() + 3 * to);
             k_add_u32<<<blocks(s->n_tris), kT, 0, s0>>>(s->tri_mat.as<uint32_t>(), s->n_tris, (uint32_t)mo,
                                                         m.tri_mat.as<uint32_t>() + to);
-            k_rebase_triv<<<blocks(s->n_tris), kT, 0, s0>>>(s->tri_v.as<float4>(), s->n_tris, (uint32_t)to,
-                                                            m.tri_v.as<float4>() + 3 * to);
+            k_rebase_triv<<<blocks(s->n_refs), kT, 0, s0>>>(s->tri_v.as<float4>(), s->n_refs, (uint32_t)to,
+                                                            m.tri_v.as<float4>() + 3 * ro);
             DR_HIP_TRY(hipMemcpyAsync(m.light_pdf_area.as<float>() + to, s->light_pdf_area.ptr, 4ull * s->n_tris,
                                       hipMemcpyDeviceToDevice, s0));
         }
         k_rebase_nodes<<<blocks(s->n_nodes), kT, 0, s0>>>(s->nodes.as<WideNode>(), s->n_nodes, (uint32_t)no,
-                                                          (uint32_t)to, m.nodes.as<WideNode>() + no);
+                                                          (uint32_t)ro, m.nodes.as<WideNode>() + no);
         if (s->n_emit)
             k_rebase_alias<<<blocks(s->n_emit), kT, 0, s0>>>(s->alias.as<AliasEntryB>(), s->n_emit, (uint32_t)to,
                                                              m.alias.as<AliasEntryB>() + eo);
@@ -159,6 +161,7 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
         leaves += s->n_leaves;
         vo += s->n_verts;
         to += s->n_tris;
+        ro += s->n_refs;
         mo += s->n_mats;
         no += s->n_nodes;
         eo += s->n_emit;

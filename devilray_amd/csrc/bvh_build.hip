@@ -1,7 +1,9 @@
 // On-device BVH build: Morton-code LBVH, SAH leaf collapse, 4-wide nodes.
 //
-//  1. per-triangle boxes and the centroid bounds (one atomic per block)
-//  2. 30-bit Morton codes of the centroids, radix-sorted with their ids
+//  1. per-triangle boxes and the centroid bounds (one atomic per block);
+//     optionally, triangles with large empty boxes are split into several
+//     references with clipped boxes (k_split below)
+//  2. 30-bit Morton codes of the reference centroids, radix-sorted with their ids
 //  3. binary tree: PLOC clustering of the Morton-ordered leaves (default,
 //     k_ploc_* below, leaves renumbered depth-first), or the binary radix tree
 //     over the sorted codes (Karras, "Maximizing Parallelism in the
@@ -31,6 +33,10 @@ constexpr int kT = 256;
 
 struct BuildTmp {
     DevBuf tri_lo, tri_hi;          // float4 per triangle (input order)
+    DevBuf ref_lo, ref_hi, ref_tri; // split references: box and triangle
+    DevBuf prio, offs, partial;     // split priorities, reference offsets, partial sums
+    const float4 *rlo = nullptr, *rhi = nullptr;  // reference boxes (the triangle boxes without splitting)
+    const uint32_t *rtri = nullptr;               // reference -> triangle (null: identity)
     DevBuf keys, vals;              // sorted codes / ids
     DevBuf leaf_lo, leaf_hi;        // float4 per leaf (sorted order)
     DevBuf child_l, child_r;        // int per internal node (<0: ~leaf)
@@ -41,7 +47,7 @@ struct BuildTmp {
     DevBuf visit;                   // u32 arrival flags
     DevBuf collapse;                // u8-in-u32 per internal node
     DevBuf queue_a, queue_b;        // int2 (binary node, wide node)
-    DevBuf counters;                // u32[8] + ordered-int bounds[6]
+    DevBuf counters;                // u32[8] + ordered-int centroid bounds[6] at 8, of references at 16
 };
 
 __device__ __forceinline__ int32_t f2ord(float f)
@@ -119,18 +125,21 @@ __global__ __launch_bounds__(kT) void k_morton(const float4 *__restrict__ lo, co
     vals[i] = i;
 }
 
-// leaf boxes and leaf-ordered triangle vertices
+// leaf boxes and leaf-ordered triangle vertices; vals[k] is a reference,
+// whose triangle is ref_tri[vals[k]] (or vals[k] itself without a map)
 __global__ __launch_bounds__(kT) void k_gather(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                                const uint32_t *__restrict__ vals, uint32_t n,
                                                const float4 *__restrict__ tlo, const float4 *__restrict__ thi,
+                                               const uint32_t *__restrict__ ref_tri,
                                                float4 *__restrict__ llo, float4 *__restrict__ lhi,
                                                float4 *__restrict__ tri_v)
 {
     const uint32_t k = blockIdx.x * kT + threadIdx.x;
     if (k >= n) return;
-    const uint32_t id = vals[k];
-    llo[k] = tlo[id];
-    lhi[k] = thi[id];
+    const uint32_t r = vals[k];
+    const uint32_t id = ref_tri ? ref_tri[r] : r;
+    llo[k] = tlo[r];
+    lhi[k] = thi[r];
     for (int j = 0; j < 3; ++j) {
         const uint32_t vi = idx[3 * (size_t)id + j];
         const float w = j == 0 ? __int_as_float((int32_t)id) : 0.0f;
@@ -314,11 +323,14 @@ __global__ void k_single_leaf_root(const float4 *__restrict__ lo, const float4 *
     *wide = wn;
 }
 
-__global__ void k_init_counters(uint32_t *c, int32_t *cb)
+__global__ void k_init_counters(uint32_t *c, int32_t *cb, int32_t *rb)
 {
     const int t = threadIdx.x;
     if (t < 8) c[t] = 0;
-    if (t < 3) { cb[t] = 0x7FFFFFFF; cb[3 + t] = (int32_t)0x80000000; }
+    if (t < 3) {
+        cb[t] = 0x7FFFFFFF; cb[3 + t] = (int32_t)0x80000000;
+        rb[t] = 0x7FFFFFFF; rb[3 + t] = (int32_t)0x80000000;
+    }
 }
 
 // counters: [0] wide nodes allocated (the root), [1] queue out, [2] leaves, [3] error
@@ -337,6 +349,254 @@ __global__ __launch_bounds__(kT) void k_fill_u32(uint32_t *p, uint32_t n, uint32
 }
 
 inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
+
+// ---- Triangle reference splitting: a long thin triangle at an angle to the
+// axes has a box that is mostly empty, so rays that pass near it test it. Such
+// triangles are cut into pieces before the build, each piece bounded by the box
+// of the triangle clipped to its cell ("early split clipping": Ernst &
+// Greiner, RT 2007). The extra references are shared out from a budget by the
+// cube root of the box area a triangle could lose, as in Karras & Aila, "Fast
+// Parallel Construction of High-Quality BVHs", HPG 2013, section 5. Every
+// piece refers to the whole triangle, so hits are those of the unsplit scene.
+constexpr int kSplitMax = 16;  // pieces per triangle
+// Default budget, in quarters of the triangle count: 4 extra references per
+// triangle. On the displaced 1.3M-triangle icosphere (slivers 17x longer than
+// wide) it cuts triangle tests per incoherent ray from 163 to 24.
+constexpr uint32_t kSplitDefaultQuarters = 16;
+constexpr uint32_t kMaxRefs = (1u << 27) - 1;  // leaf indices are 27 bits
+
+__global__ __launch_bounds__(kT) void k_split_priority(const float *__restrict__ verts,
+                                                       const uint32_t *__restrict__ idx, uint32_t n,
+                                                       const float4 *__restrict__ tlo, const float4 *__restrict__ thi,
+                                                       float *__restrict__ prio, double *__restrict__ partial)
+{
+    __shared__ double red[kT];
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    float pr = 0.0f;
+    if (i < n) {
+        float v[3][3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = idx[3 * (size_t)i + k];
+            for (int a = 0; a < 3; ++a) v[k][a] = verts[3 * (size_t)vi + a];
+        }
+        const float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+        const float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+        const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+        const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+        const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+        // Pieces cut ever finer bound the triangle with boxes of total half
+        // area A (|nx| + |ny| + |nz|) = (|cx| + |cy| + |cz|) / 2 at the limit.
+        const float ideal = 0.5f * (fabsf(cx) + fabsf(cy) + fabsf(cz));
+        const float excess = half_area(tlo[i], thi[i]) - ideal;
+        pr = (excess > 0.0f && excess <= FLT_MAX) ? cbrtf(excess) : 0.0f;
+        prio[i] = pr;
+    }
+    red[threadIdx.x] = pr;
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// fixed-order sum of the block partials (the build is deterministic)
+__global__ __launch_bounds__(kT) void k_sum_partials(const double *__restrict__ partial, uint32_t nb,
+                                                     double *__restrict__ total)
+{
+    __shared__ double red[kT];
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kT) acc += partial[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = red[0];
+}
+
+// pieces per triangle: 1 + budget * p / sum(p), at most kSplitMax; cnt[n] = 0
+__global__ __launch_bounds__(kT) void k_split_counts(const float *__restrict__ prio, uint32_t n,
+                                                     const double *__restrict__ total, double budget,
+                                                     uint32_t *__restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { cnt[n] = 0u; return; }
+    const double tot = *total;
+    uint32_t k = 1u;
+    if (tot > 0.0 && tot <= DBL_MAX) {
+        const double extra = floor(budget * (double)prio[i] / tot);
+        k += (uint32_t)fmin(fmax(extra, 0.0), (double)(kSplitMax - 1));
+    }
+    cnt[i] = k;
+}
+
+__device__ __forceinline__ float round_down(double d)
+{
+    float f = (float)d;
+    if ((double)f > d) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float round_up(double d)
+{
+    float f = (float)d;
+    if ((double)f < d) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// Box of the triangle clipped to [lo, hi] (Sutherland-Hodgman in double, so
+// the clipped corners carry no float rounding); false if nothing is left.
+// A convex polygon gains at most one vertex per plane, so 3 + 6 fit in 9; the
+// arrays hold 12 and a pathological overflow falls back to [lo, hi] itself.
+__device__ bool clip_box(const double (&tri)[3][3], const float lo[3], const float hi[3], float out_lo[3],
+                         float out_hi[3])
+{
+    constexpr int kMaxV = 12;
+    double P[kMaxV][3], Q[kMaxV][3];
+    int np = 3;
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) P[k][a] = tri[k][a];
+    bool overflow = false;
+    for (int pl = 0; pl < 6 && np > 0 && !overflow; ++pl) {
+        const int a = pl >> 1;
+        const bool low = (pl & 1) == 0;
+        const double b = low ? (double)lo[a] : (double)hi[a];
+        int nq = 0;
+        for (int k = 0; k < np && !overflow; ++k) {
+            const int k1 = k + 1 == np ? 0 : k + 1;
+            const double dc = low ? P[k][a] - b : b - P[k][a];
+            const double dd = low ? P[k1][a] - b : b - P[k1][a];
+            if (dc >= 0.0) {
+                if (nq == kMaxV) { overflow = true; break; }
+                for (int j = 0; j < 3; ++j) Q[nq][j] = P[k][j];
+                ++nq;
+            }
+            if ((dc >= 0.0) != (dd >= 0.0)) {
+                if (nq == kMaxV) { overflow = true; break; }
+                const double f = dc / (dc - dd);
+                for (int j = 0; j < 3; ++j) Q[nq][j] = P[k][j] + f * (P[k1][j] - P[k][j]);
+                Q[nq][a] = b;
+                ++nq;
+            }
+        }
+        for (int k = 0; k < nq; ++k)
+            for (int j = 0; j < 3; ++j) P[k][j] = Q[k][j];
+        np = nq;
+    }
+    if (overflow) {
+        for (int a = 0; a < 3; ++a) { out_lo[a] = lo[a]; out_hi[a] = hi[a]; }
+        return true;
+    }
+    if (np == 0) return false;
+    double l[3] = {P[0][0], P[0][1], P[0][2]}, h[3] = {P[0][0], P[0][1], P[0][2]};
+    for (int k = 1; k < np; ++k)
+        for (int a = 0; a < 3; ++a) { l[a] = fmin(l[a], P[k][a]); h[a] = fmax(h[a], P[k][a]); }
+    for (int a = 0; a < 3; ++a) { out_lo[a] = round_down(l[a]); out_hi[a] = round_up(h[a]); }
+    return true;
+}
+
+// Cuts triangle i into offs[i+1] - offs[i] pieces: the piece with the longest
+// box edge is halved across that edge until the count is reached. A piece
+// box is the clipped box widened by a margin of 2^-17 of the scene's largest
+// coordinate (so rays that meet the triangle on a cut are inside the boxes on
+// both sides whatever the rounding of the triangle test) and cut back to the
+// triangle's own box. Triangles left whole keep their exact box.
+__global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                                              uint32_t n, const float4 *__restrict__ tlo,
+                                              const float4 *__restrict__ thi, const uint32_t *__restrict__ offs,
+                                              const int32_t *__restrict__ cbounds, float4 *__restrict__ rlo,
+                                              float4 *__restrict__ rhi, uint32_t *__restrict__ rtri,
+                                              int32_t *__restrict__ rbounds)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    float c[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, C[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (i < n) {
+        const uint32_t base = offs[i], k = offs[i + 1] - base;
+        const float4 L = tlo[i], H = thi[i];
+        float flo[kSplitMax][3], fhi[kSplitMax][3];
+        flo[0][0] = L.x; flo[0][1] = L.y; flo[0][2] = L.z;
+        fhi[0][0] = H.x; fhi[0][1] = H.y; fhi[0][2] = H.z;
+        uint32_t nf = 1;
+        if (k > 1) {
+            double tri[3][3];
+            for (int q = 0; q < 3; ++q) {
+                const uint32_t vi = idx[3 * (size_t)i + q];
+                for (int a = 0; a < 3; ++a) tri[q][a] = verts[3 * (size_t)vi + a];
+            }
+            float mag = 0.0f;
+            for (int a = 0; a < 6; ++a) mag = fmaxf(mag, fabsf(ord2f(cbounds[a])));
+            mag = fmaxf(mag, fmaxf(fmaxf(fabsf(L.x), fabsf(L.y)), fmaxf(fabsf(L.z), fabsf(H.x))));
+            mag = fmaxf(mag, fmaxf(fabsf(H.y), fabsf(H.z)));
+            const float margin = mag * 0x1p-17f;
+            const float tl[3] = {L.x, L.y, L.z}, th[3] = {H.x, H.y, H.z};
+            for (int guard = 0; nf < k && guard < 4 * kSplitMax; ++guard) {
+                uint32_t bf = 0;
+                int ba = 0;
+                float bext = -1.0f;
+                for (uint32_t f = 0; f < nf; ++f)
+                    for (int a = 0; a < 3; ++a) {
+                        const float e = fhi[f][a] - flo[f][a];
+                        if (e > bext) { bext = e; bf = f; ba = a; }
+                    }
+                const float cut = 0.5f * (flo[bf][ba] + fhi[bf][ba]);
+                if (!(cut > flo[bf][ba] && cut < fhi[bf][ba])) break;  // too small to halve
+                float alo[3], ahi[3], blo[3], bhi[3];
+                for (int a = 0; a < 3; ++a) {
+                    alo[a] = blo[a] = flo[bf][a];
+                    ahi[a] = bhi[a] = fhi[bf][a];
+                }
+                ahi[ba] = cut;
+                blo[ba] = cut;
+                float qa_lo[3], qa_hi[3], qb_lo[3], qb_hi[3];
+                const bool ha = clip_box(tri, alo, ahi, qa_lo, qa_hi);
+                const bool hb = clip_box(tri, blo, bhi, qb_lo, qb_hi);
+                for (int a = 0; a < 3; ++a) {
+                    qa_lo[a] = fmaxf(qa_lo[a] - margin, tl[a]); qa_hi[a] = fminf(qa_hi[a] + margin, th[a]);
+                    qb_lo[a] = fmaxf(qb_lo[a] - margin, tl[a]); qb_hi[a] = fminf(qb_hi[a] + margin, th[a]);
+                }
+                if (ha && hb) {
+                    for (int a = 0; a < 3; ++a) {
+                        flo[bf][a] = qa_lo[a]; fhi[bf][a] = qa_hi[a];
+                        flo[nf][a] = qb_lo[a]; fhi[nf][a] = qb_hi[a];
+                    }
+                    ++nf;
+                } else if (ha || hb) {  // the piece lies on one side: keep its tighter box
+                    for (int a = 0; a < 3; ++a) {
+                        flo[bf][a] = ha ? qa_lo[a] : qb_lo[a];
+                        fhi[bf][a] = ha ? qa_hi[a] : qb_hi[a];
+                    }
+                } else {
+                    break;  // cannot happen for a box that holds part of the triangle
+                }
+            }
+        }
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t f = j < nf ? j : j % nf;  // pieces short of the count repeat
+            rlo[base + j] = make_float4(flo[f][0], flo[f][1], flo[f][2], 0.0f);
+            rhi[base + j] = make_float4(fhi[f][0], fhi[f][1], fhi[f][2], 0.0f);
+            rtri[base + j] = i;
+            for (int a = 0; a < 3; ++a) {
+                const float m = 0.5f * (flo[f][a] + fhi[f][a]);
+                c[a] = fminf(c[a], m);
+                C[a] = fmaxf(C[a], m);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            c[a] = fminf(c[a], __shfl_xor(c[a], off, 64));
+            C[a] = fmaxf(C[a], __shfl_xor(C[a], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&rbounds[a], f2ord(c[a]));
+            atomicMax(&rbounds[3 + a], f2ord(C[a]));
+        }
+    }
+}
 
 // ---- PLOC binary tree (Meister & Bittner, "Parallel Locally-Ordered
 // Clustering for Bounding Volume Hierarchy Construction", IEEE TVCG 2018).
@@ -553,8 +813,8 @@ dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, hipStream_t 
                                                         t.keys.as<uint32_t>(), vals_dfs.as<uint32_t>(),
                                                         t.parent_l.as<int32_t>());
     k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), vals_dfs.as<uint32_t>(), n,
-                                           t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.leaf_lo.as<float4>(),
-                                           t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
+                                           t.rlo, t.rhi, t.rtri, t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(),
+                                           s->tri_v.as<float4>());
     DR_HIP_TRY(hipGetLastError());
     DR_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
     return DR_OK;
@@ -564,16 +824,20 @@ dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, hipStream_t 
 
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
 {
-    const uint32_t n = s->n_tris;
+    const uint32_t nt = s->n_tris;
     uint32_t leaf_max = p ? p->leaf_max : 4u;
     if (leaf_max < 1 || leaf_max > (uint32_t)kLeafMaxCap) return fail(DR_E_INVALID, "leaf_max must be 1..8");
-    if (n == 0) return fail(DR_E_INVALID, "scene has no triangles");
-    if (n >= (1u << 27)) return fail(DR_E_UNSUPPORTED, "more than 2^27 triangles per scene");
+    if (nt == 0) return fail(DR_E_INVALID, "scene has no triangles");
+    if (nt >= (1u << 27)) return fail(DR_E_UNSUPPORTED, "more than 2^27 triangles per scene");
     const uint32_t flags = p ? p->flags : 0u;
-    if (flags & ~(DR_BVH_LBVH | DR_BVH_RADIUS_MASK)) return fail(DR_E_INVALID, "unknown BVH flags");
+    if (flags & ~(DR_BVH_LBVH | DR_BVH_NO_SPLIT | DR_BVH_RADIUS_MASK | DR_BVH_SPLIT_MASK))
+        return fail(DR_E_INVALID, "unknown BVH flags");
     const bool ploc = !(flags & DR_BVH_LBVH);
     const int radius = (flags & DR_BVH_RADIUS_MASK) ? (int)((flags & DR_BVH_RADIUS_MASK) >> 8) : 16;
     if (radius > kPlocRMax) return fail(DR_E_INVALID, "PLOC search radius must be 1..64");
+    uint32_t split_q = (flags & DR_BVH_SPLIT_MASK) >> DR_BVH_SPLIT_SHIFT;
+    if (split_q == 0) split_q = kSplitDefaultQuarters;
+    if (flags & DR_BVH_NO_SPLIT) split_q = 0;
     Context *ctx = s->ctx;
 
     hipEvent_t e0, e1;
@@ -582,11 +846,63 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     DR_HIP_TRY(hipEventRecord(e0, stream));
 
     BuildTmp t;
-    const uint32_t ni = n > 1 ? n - 1 : 1;
     dr_status st = DR_OK;
     auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
-    need(t.tri_lo, sizeof(float4) * n);
-    need(t.tri_hi, sizeof(float4) * n);
+    need(t.tri_lo, sizeof(float4) * nt);
+    need(t.tri_hi, sizeof(float4) * nt);
+    need(t.counters, 128);
+    if (st) return st;
+    uint32_t *counters = t.counters.as<uint32_t>();
+    int32_t *cbounds = reinterpret_cast<int32_t *>(counters + 8);
+    int32_t *rbounds = reinterpret_cast<int32_t *>(counters + 16);
+    k_init_counters<<<1, 64, 0, stream>>>(counters, cbounds, rbounds);
+    k_tri_bounds<<<blocks(nt), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
+                                                t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), cbounds);
+    DR_HIP_TRY(hipGetLastError());
+
+    // leaf references: the triangles, or their split pieces
+    uint32_t n = nt;
+    t.rlo = t.tri_lo.as<float4>();
+    t.rhi = t.tri_hi.as<float4>();
+    t.rtri = nullptr;
+    const int32_t *ref_bounds = cbounds;
+    if (split_q > 0 && nt > 1) {
+        const uint32_t nb = blocks(nt);
+        need(t.prio, sizeof(float) * nt);
+        need(t.offs, sizeof(uint32_t) * (nt + 1));
+        need(t.partial, sizeof(double) * (nb + 1));
+        if (st) return st;
+        double *total = t.partial.as<double>() + nb;
+        k_split_priority<<<nb, kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
+                                                t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.prio.as<float>(),
+                                                t.partial.as<double>());
+        k_sum_partials<<<1, kT, 0, stream>>>(t.partial.as<double>(), nb, total);
+        // the pieces beyond one per triangle sum to at most the budget
+        const double budget = fmin(0.25 * split_q * (double)nt, (double)(kMaxRefs - nt));
+        k_split_counts<<<blocks(nt + 1), kT, 0, stream>>>(t.prio.as<float>(), nt, total, budget,
+                                                          t.offs.as<uint32_t>());
+        DR_HIP_TRY(hipGetLastError());
+        if ((st = exclusive_scan_u32(ctx, t.offs.as<uint32_t>(), nt + 1, stream))) return st;
+        uint32_t total_refs = 0;
+        DR_HIP_TRY(hipMemcpyAsync(&total_refs, t.offs.as<uint32_t>() + nt, 4, hipMemcpyDeviceToHost, stream));
+        DR_HIP_TRY(hipStreamSynchronize(stream));
+        if (total_refs < nt || total_refs > kMaxRefs) return fail(DR_E_DEVICE, "split reference count out of range");
+        n = total_refs;
+        need(t.ref_lo, sizeof(float4) * n);
+        need(t.ref_hi, sizeof(float4) * n);
+        need(t.ref_tri, sizeof(uint32_t) * n);
+        if (st) return st;
+        k_split<<<nb, kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt, t.tri_lo.as<float4>(),
+                                       t.tri_hi.as<float4>(), t.offs.as<uint32_t>(), cbounds, t.ref_lo.as<float4>(),
+                                       t.ref_hi.as<float4>(), t.ref_tri.as<uint32_t>(), rbounds);
+        DR_HIP_TRY(hipGetLastError());
+        t.rlo = t.ref_lo.as<float4>();
+        t.rhi = t.ref_hi.as<float4>();
+        t.rtri = t.ref_tri.as<uint32_t>();
+        ref_bounds = rbounds;
+    }
+
+    const uint32_t ni = n > 1 ? n - 1 : 1;
     need(t.keys, sizeof(uint32_t) * n);
     need(t.vals, sizeof(uint32_t) * n);
     need(t.leaf_lo, sizeof(float4) * n);
@@ -604,7 +920,6 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     need(t.collapse, sizeof(uint32_t) * ni);
     need(t.queue_a, sizeof(int2) * ni);
     need(t.queue_b, sizeof(int2) * ni);
-    need(t.counters, 64);
     need(s->tri_v, sizeof(float4) * 3 * (size_t)n);
     // Wide nodes with a wide child have 4 children, the others only leaf
     // children (>= 2), so W <= 2L/3 + 1 <= 2n/3 + 1.
@@ -612,18 +927,12 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     need(s->nodes, sizeof(WideNode) * (size_t)wide_cap);
     if (st) return st;
 
-    uint32_t *counters = t.counters.as<uint32_t>();
-    int32_t *cbounds = reinterpret_cast<int32_t *>(counters + 8);
-    k_init_counters<<<1, 64, 0, stream>>>(counters, cbounds);
-    k_tri_bounds<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), n,
-                                               t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), cbounds);
-    k_morton<<<blocks(n), kT, 0, stream>>>(t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), n, cbounds,
-                                           t.keys.as<uint32_t>(), t.vals.as<uint32_t>());
+    k_morton<<<blocks(n), kT, 0, stream>>>(t.rlo, t.rhi, n, ref_bounds, t.keys.as<uint32_t>(), t.vals.as<uint32_t>());
     DR_HIP_TRY(hipGetLastError());
     if ((st = radix_sort_pairs(ctx, t.keys.as<uint32_t>(), t.vals.as<uint32_t>(), n, 30, stream))) return st;
     k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), t.vals.as<uint32_t>(), n,
-                                           t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.leaf_lo.as<float4>(),
-                                           t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
+                                           t.rlo, t.rhi, t.rtri, t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(),
+                                           s->tri_v.as<float4>());
     DR_HIP_TRY(hipGetLastError());
 
     const float ct = 1.0f, ci = 1.0f;
@@ -705,6 +1014,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     DR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    s->n_refs = n;
     s->n_nodes = n_nodes;
     s->n_leaves = n_leaves;
     s->depth = depth;

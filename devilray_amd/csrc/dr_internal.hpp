@@ -74,8 +74,9 @@ struct Scene {
     DevBuf materials; // dr_material * n_mats
     std::vector<dr_material> host_materials;
     // acceleration structure
-    DevBuf tri_v;     // 3 float4 per triangle, leaf order; w of the first = id bits
+    DevBuf tri_v;     // 3 float4 per triangle reference, leaf order; w of the first = id bits
     DevBuf nodes;     // WideNode
+    uint32_t n_refs = 0;  // tri_v entries: triangles plus split pieces
     uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
     double build_ms = 0.0, sah_cost = 0.0;
     bool built = false;

@@ -77,13 +77,21 @@ typedef struct {
  * The binary tree is built by PLOC clustering over the Morton order (default)
  * or as a Karras radix tree over the Morton codes (DR_BVH_LBVH, faster to
  * build, lower quality); both then go through the same SAH leaf collapse and
- * 4-wide collapse. Bits 8-15 set the PLOC search radius (0: 16, max 64). */
+ * 4-wide collapse. Bits 8-15 set the PLOC search radius (0: 16, max 64).
+ * Bits 16-23 set the triangle-splitting budget in quarters: b > 0 adds up to
+ * b/4 * n_triangles extra references (and never more than 2^27 - 1 in all),
+ * each a clipped piece of a triangle whose box is large and mostly empty;
+ * 0 asks for the default (16: four extra references per triangle), and
+ * DR_BVH_NO_SPLIT turns splitting off. Hits do not depend on these choices. */
 typedef struct {
     uint32_t leaf_max;
     uint32_t flags;
 } dr_bvh_params;
 #define DR_BVH_LBVH 1u
+#define DR_BVH_NO_SPLIT 2u
 #define DR_BVH_RADIUS_MASK 0xFF00u
+#define DR_BVH_SPLIT_SHIFT 16
+#define DR_BVH_SPLIT_MASK 0xFF0000u
 
 typedef struct {
     uint32_t n_triangles;
@@ -97,6 +105,8 @@ typedef struct {
     uint32_t simd_stats[4];
     double build_ms;         /* device time of the last build */
     double sah_cost;         /* SAH cost of the binary tree before collapse */
+    uint32_t n_references;   /* leaf entries: triangles plus split pieces */
+    uint32_t reserved;
 } dr_scene_info;
 
 const char *dr_status_string(dr_status s);

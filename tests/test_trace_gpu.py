@@ -42,16 +42,21 @@ def test_small_sphere_vs_brute_force(gpu_ctx, leaf_max):
     scene.close()
 
 
-@pytest.mark.parametrize("builder,radius", [("lbvh", 0), ("ploc", 1), ("ploc", 3), ("ploc", 64)])
-def test_builders_vs_brute_force(gpu_ctx, builder, radius):
-    print(f"[case] builder {builder} radius {radius}", flush=True)
+@pytest.mark.parametrize("builder,radius,split", [("lbvh", 0, 0.0), ("ploc", 1, 0.0), ("ploc", 3, 0.0),
+                                                   ("ploc", 64, 0.0), ("ploc", 16, 1.0), ("ploc", 16, 8.0),
+                                                   ("lbvh", 0, 2.0)])
+def test_builders_vs_brute_force(gpu_ctx, builder, radius, split):
+    print(f"[case] builder {builder} radius {radius} split {split}", flush=True)
     v, f = dr.gen_icosphere(3, 0.05, 11)
     rays = np.concatenate([dr.gen_rays_surface(v, f, 4096, seed=5, inward=True),
                            dr.gen_rays_surface(v, f, 2048, seed=6, inward=False)])
     scene = dr.Scene(gpu_ctx, v, f)
     for leaf_max in (1, 4):
-        scene.build_bvh(leaf_max=leaf_max, builder=builder, radius=radius)
-        assert_hits_equal(scene.trace_closest(rays), oracle.brute_closest(v, f, rays), f"{builder} r{radius}")
+        scene.build_bvh(leaf_max=leaf_max, builder=builder, radius=radius, split=split)
+        refs = scene.info().n_references
+        assert (refs == f.shape[0]) if split == 0 else (f.shape[0] < refs <= (1 + split) * f.shape[0] + 1)
+        label = f"{builder} r{radius} split {split} leaf {leaf_max}"
+        assert_hits_equal(scene.trace_closest(rays), oracle.brute_closest(v, f, rays), label)
         assert np.array_equal(scene.trace_any(rays), oracle.brute_any(v, f, rays))
     scene.close()
 
@@ -79,15 +84,21 @@ def test_ploc_identical_boxes(gpu_ctx, n_tris):
     scene.close()
 
 
-def test_c1_full_size_incoherent_and_coherent(gpu_ctx):
-    """Full traversal benchmark size: 1,310,720 triangles, 2^20 rays per batch."""
-    print("[case] C1 full size", flush=True)
+@pytest.fixture(scope="module")
+def c1_oracle():
     v, f = dr.gen_icosphere(8, 0.05, 0)
+    return v, f, oracle.BVH(v, f)
+
+
+@pytest.mark.parametrize("split", [None, 0.0, 2.0])
+def test_c1_full_size_incoherent_and_coherent(gpu_ctx, c1_oracle, split):
+    """Full traversal benchmark size: 1,310,720 triangles, 2^20 rays per batch."""
+    print(f"[case] C1 full size split {split}", flush=True)
+    v, f, bvh = c1_oracle
     rays = dr.gen_rays_surface(v, f, 1 << 20, seed=0, inward=True, offset=1e-5)
     cam = dr.gen_rays_camera((0.0, 0.0, 3.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, 1024, 1024)
     scene = dr.Scene(gpu_ctx, v, f)
-    scene.build_bvh(leaf_max=4)
-    bvh = oracle.BVH(v, f)
+    scene.build_bvh(leaf_max=4, split=split)
     ref = bvh.closest(rays)
     got = scene.trace_closest(rays)
     assert_hits_equal(got, ref, "incoherent")
