@@ -186,11 +186,14 @@ class Scene:
             self.handle = None
 
     def build_bvh(self, leaf_max: int = 4, builder: str = "ploc", radius: int = 0, split: float | None = None,
-                  stream=None):
+                  split_ratio: float | None = None, stream=None):
         """builder: "ploc" (clustering, default) or "lbvh" (radix tree);
-        radius: PLOC search radius (0 = the library default); split: extra
-        triangle references as a fraction of the triangle count, in steps of
-        1/4 up to 63.75 (None = the library default, 0 = no splitting)."""
+        radius: PLOC search radius (0 = the library default); split: cap on
+        the extra triangle references as a multiple of the triangle count, in
+        steps of 1/4 up to 63.75 (None = the library default, 0 = no
+        splitting); split_ratio: r0, a triangle whose box is `ratio` times its
+        ideal half area gets floor(ratio / r0) pieces (steps of 1/4, None =
+        the library default)."""
         if builder not in ("ploc", "lbvh"):
             raise ValueError("builder must be 'ploc' or 'lbvh'")
         flags = (1 if builder == "lbvh" else 0) | ((radius & 0xFF) << 8)
@@ -199,6 +202,11 @@ class Scene:
             if not 0 <= q <= 255:
                 raise ValueError("split must be in [0, 63.75]")
             flags |= (q << 16) if q else 2
+        if split_ratio is not None:
+            q = int(round(split_ratio * 4))
+            if not 1 <= q <= 255:
+                raise ValueError("split_ratio must be in [0.25, 63.75]")
+            flags |= q << 24
         p = DrBvhParams(leaf_max, flags)
         check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
 

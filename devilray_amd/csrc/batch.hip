@@ -102,8 +102,8 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
         nn += sc[k]->n_nodes;
         ne += sc[k]->n_emit;
     }
-    if (nr >= (1ull << 27) || nt >= (1ull << 31) || nn >= (1ull << 31) || nv >= (1ull << 32))
-        return fail(DR_E_INVALID, "batch too large for 27-bit leaf triangle indices");
+    if (nr > kMaxLeafRefs || nt >= (1ull << 31) || nn >= (1ull << 31) || nv >= (1ull << 32))
+        return fail(DR_E_INVALID, "batch too large: more than 2^28 - 2 triangle references in all");
     m.n_verts = (uint32_t)nv;
     m.n_tris = (uint32_t)nt;
     m.n_refs = (uint32_t)nr;

@@ -9,8 +9,9 @@
 //     over the sorted codes (Karras, "Maximizing Parallelism in the
 //     Construction of BVHs, Octrees, and k-d Trees", HPG 2012; equal codes are
 //     ordered by their position)
-//  4. bottom-up pass (atomic arrival flags): boxes, triangle counts and the
-//     SAH cost C(n) = min(Ci*A(n)*N(n), Ct*A(n) + C(l) + C(r)); a subtree is
+//  4. node boxes, reference counts and the SAH cost
+//     C(n) = min(Ci*A(n)*N(n), Ct*A(n) + C(l) + C(r)), at each PLOC merge or,
+//     for the radix tree, in a bottom-up pass with atomic arrival flags; a subtree is
 //     collapsed into one leaf when the leaf term wins and N(n) <= leaf_max
 //     (the collapse rule of Karras & Aila, "Fast Parallel Construction of
 //     High-Quality BVHs", HPG 2013)
@@ -34,7 +35,7 @@ constexpr int kT = 256;
 struct BuildTmp {
     DevBuf tri_lo, tri_hi;          // float4 per triangle (input order)
     DevBuf ref_lo, ref_hi, ref_tri; // split references: box and triangle
-    DevBuf prio, offs, partial;     // split priorities, reference offsets, partial sums
+    DevBuf offs;                    // split: reference offsets per triangle
     const float4 *rlo = nullptr, *rhi = nullptr;  // reference boxes (the triangle boxes without splitting)
     const uint32_t *rtri = nullptr;               // reference -> triangle (null: identity)
     DevBuf keys, vals;              // sorted codes / ids
@@ -354,83 +355,54 @@ inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
 // axes has a box that is mostly empty, so rays that pass near it test it. Such
 // triangles are cut into pieces before the build, each piece bounded by the box
 // of the triangle clipped to its cell ("early split clipping": Ernst &
-// Greiner, RT 2007). The extra references are shared out from a budget by the
-// cube root of the box area a triangle could lose, as in Karras & Aila, "Fast
-// Parallel Construction of High-Quality BVHs", HPG 2013, section 5. Every
-// piece refers to the whole triangle, so hits are those of the unsplit scene.
+// Greiner, RT 2007). The piece count follows the triangle's shape: its box's
+// half area over the half area that boxes of ever finer pieces tend to,
+// A (|nx| + |ny| + |nz|). That ratio is 2 to 3 for well-shaped triangles and 5
+// to 50 for the slivers of a displaced fine mesh; a triangle gets
+// floor(ratio / r0) pieces, 1 to kSplitMax, and a cap on the extra references
+// scales the counts down if needed. Every piece refers to the whole triangle,
+// so hits are those of the unsplit scene.
 constexpr int kSplitMax = 16;  // pieces per triangle
-// Default budget, in quarters of the triangle count: 4 extra references per
-// triangle. On the displaced 1.3M-triangle icosphere (slivers 17x longer than
-// wide) it cuts triangle tests per incoherent ray from 163 to 24.
-constexpr uint32_t kSplitDefaultQuarters = 16;
-constexpr uint32_t kMaxRefs = (1u << 27) - 1;  // leaf indices are 27 bits
+// Defaults: r0 = 2 (8 quarters) and a cap of 8 extra references per triangle
+// (32 quarters). On the displaced 1.3M-triangle icosphere this makes about 7.7
+// references per triangle and cuts triangle tests per incoherent ray from 163
+// to about 16; scenes of well-shaped triangles are left unsplit.
+constexpr uint32_t kSplitRatioDefaultQuarters = 8;
+constexpr uint32_t kSplitCapDefaultQuarters = 32;
+constexpr uint32_t kMaxRefs = kMaxLeafRefs;
 
-__global__ __launch_bounds__(kT) void k_split_priority(const float *__restrict__ verts,
-                                                       const uint32_t *__restrict__ idx, uint32_t n,
-                                                       const float4 *__restrict__ tlo, const float4 *__restrict__ thi,
-                                                       float *__restrict__ prio, double *__restrict__ partial)
-{
-    __shared__ double red[kT];
-    const uint32_t i = blockIdx.x * kT + threadIdx.x;
-    float pr = 0.0f;
-    if (i < n) {
-        float v[3][3];
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t vi = idx[3 * (size_t)i + k];
-            for (int a = 0; a < 3; ++a) v[k][a] = verts[3 * (size_t)vi + a];
-        }
-        const float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
-        const float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
-        const float cx = e1[1] * e2[2] - e1[2] * e2[1];
-        const float cy = e1[2] * e2[0] - e1[0] * e2[2];
-        const float cz = e1[0] * e2[1] - e1[1] * e2[0];
-        // Pieces cut ever finer bound the triangle with boxes of total half
-        // area A (|nx| + |ny| + |nz|) = (|cx| + |cy| + |cz|) / 2 at the limit.
-        const float ideal = 0.5f * (fabsf(cx) + fabsf(cy) + fabsf(cz));
-        const float excess = half_area(tlo[i], thi[i]) - ideal;
-        pr = (excess > 0.0f && excess <= FLT_MAX) ? cbrtf(excess) : 0.0f;
-        prio[i] = pr;
-    }
-    red[threadIdx.x] = pr;
-    __syncthreads();
-    for (int h = kT / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-// fixed-order sum of the block partials (the build is deterministic)
-__global__ __launch_bounds__(kT) void k_sum_partials(const double *__restrict__ partial, uint32_t nb,
-                                                     double *__restrict__ total)
-{
-    __shared__ double red[kT];
-    double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < nb; b += kT) acc += partial[b];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = kT / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = red[0];
-}
-
-// pieces per triangle: 1 + budget * p / sum(p), at most kSplitMax; cnt[n] = 0
-__global__ __launch_bounds__(kT) void k_split_counts(const float *__restrict__ prio, uint32_t n,
-                                                     const double *__restrict__ total, double budget,
+// pieces per triangle; cnt[n] = 0 (the scan then gives offsets and the total)
+__global__ __launch_bounds__(kT) void k_split_counts(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                                                     uint32_t n, const float4 *__restrict__ tlo,
+                                                     const float4 *__restrict__ thi, float r0,
                                                      uint32_t *__restrict__ cnt)
 {
     const uint32_t i = blockIdx.x * kT + threadIdx.x;
     if (i > n) return;
     if (i == n) { cnt[n] = 0u; return; }
-    const double tot = *total;
-    uint32_t k = 1u;
-    if (tot > 0.0 && tot <= DBL_MAX) {
-        const double extra = floor(budget * (double)prio[i] / tot);
-        k += (uint32_t)fmin(fmax(extra, 0.0), (double)(kSplitMax - 1));
+    float v[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = idx[3 * (size_t)i + k];
+        for (int a = 0; a < 3; ++a) v[k][a] = verts[3 * (size_t)vi + a];
     }
+    const float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+    const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+    const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+    const float ideal = 0.5f * (fabsf(cx) + fabsf(cy) + fabsf(cz));
+    const float ratio = half_area(tlo[i], thi[i]) / ideal;  // inf or NaN for degenerate triangles
+    uint32_t k = 1u;
+    if (ratio >= 2.0f * r0 && ratio <= FLT_MAX) k = (uint32_t)fminf(floorf(ratio / r0), (float)kSplitMax);
     cnt[i] = k;
+}
+
+// cap: k -> 1 + floor((k - 1) * cap / extra) when the extra pieces exceed the cap
+__global__ __launch_bounds__(kT) void k_split_rescale(uint32_t *__restrict__ cnt, uint32_t n, double scale)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    cnt[i] = 1u + (uint32_t)floor((double)(cnt[i] - 1u) * scale);
 }
 
 __device__ __forceinline__ float round_down(double d)
@@ -677,7 +649,9 @@ __global__ __launch_bounds__(kT) void k_ploc_emit(
     const float4 *__restrict__ lo_in, const float4 *__restrict__ hi_in, const uint32_t *__restrict__ cnt_in,
     int32_t *__restrict__ ref_out, float4 *__restrict__ lo_out, float4 *__restrict__ hi_out,
     uint32_t *__restrict__ cnt_out, int32_t *__restrict__ cl, int32_t *__restrict__ cr, int32_t *__restrict__ pi,
-    int32_t *__restrict__ pl, uint32_t *__restrict__ ncnt, uint32_t id_base)
+    int32_t *__restrict__ pl, uint32_t *__restrict__ ncnt, uint32_t id_base, float4 *__restrict__ node_lo,
+    float4 *__restrict__ node_hi, float *__restrict__ cost, uint32_t *__restrict__ collapse, uint32_t leaf_max,
+    float ct, float ci)
 {
     const int32_t i = (int32_t)(blockIdx.x * kT + threadIdx.x);
     if (i >= (int32_t)m) return;
@@ -701,10 +675,23 @@ __global__ __launch_bounds__(kT) void k_ploc_emit(
     if (b < 0) pl[~b] = id; else pi[b] = id;
     const float4 al = lo_in[i], ah = hi_in[i], bl = lo_in[j], bh = hi_in[j];
     const uint32_t c = cnt_in[i] + cnt_in[j];
+    const float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+    const float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+    // The SAH fit of k_fit, done here: both children are finished subtrees.
+    const float ac = a < 0 ? ci * half_area(al, ah) : cost[a];
+    const float bc = b < 0 ? ci * half_area(bl, bh) : cost[b];
+    const float area = half_area(lo, hi);
+    const float c_split = ct * area + ac + bc;
+    const float c_leaf = (c <= leaf_max) ? ci * area * (float)c : INFINITY;
+    const bool col = c_leaf <= c_split;
+    node_lo[id] = lo;
+    node_hi[id] = hi;
+    cost[id] = col ? c_leaf : c_split;
+    collapse[id] = col ? 1u : 0u;
     ncnt[id] = c;
     ref_out[p] = id;
-    lo_out[p] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
-    hi_out[p] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+    lo_out[p] = lo;
+    hi_out[p] = hi;
     cnt_out[p] = c;
 }
 
@@ -757,10 +744,12 @@ __global__ __launch_bounds__(kT) void k_ploc_relabel_leaves(uint32_t n, const ui
 }
 
 // Builds the binary tree over the n >= 2 leaves whose boxes t.leaf_lo/hi hold
-// in Morton order, into the arrays the SAH fit and the collapse read
-// (child_l/r, parent_i/l, range_first, count), and re-gathers the leaf boxes
-// and triangles in depth-first order.
-dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, hipStream_t stream)
+// in Morton order, into the arrays the collapse reads (child_l/r, parent_i/l,
+// range_first, count, and the boxes, SAH costs and collapse flags that k_fit
+// computes for the radix tree), and re-gathers the leaf boxes and triangles
+// in depth-first order.
+dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, uint32_t leaf_max, float ct, float ci,
+                     hipStream_t stream)
 {
     Context *ctx = s->ctx;
     DevBuf ref[2], lo1, hi1, cnt[2], nn, f, pl_morton, vals_dfs;
@@ -793,7 +782,9 @@ dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, hipStream_t 
         k_ploc_emit<<<blocks(m), kT, 0, stream>>>(m, nn.as<int32_t>(), f.as<uint64_t>(), ref[cur].as<int32_t>(),
                                                   lo[cur], hi[cur], cnt[cur].as<uint32_t>(), ref[1 - cur].as<int32_t>(),
                                                   lo[1 - cur], hi[1 - cur], cnt[1 - cur].as<uint32_t>(), cl, cr, pi,
-                                                  pl_morton.as<int32_t>(), ncnt, n - 2 - made);
+                                                  pl_morton.as<int32_t>(), ncnt, n - 2 - made, t.node_lo.as<float4>(),
+                                                  t.node_hi.as<float4>(), t.cost.as<float>(), t.collapse.as<uint32_t>(),
+                                                  leaf_max, ct, ci);
         DR_HIP_TRY(hipGetLastError());
         uint64_t tot = 0;
         DR_HIP_TRY(hipMemcpyAsync(&tot, f.as<uint64_t>() + m, 8, hipMemcpyDeviceToHost, stream));
@@ -828,16 +819,18 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     uint32_t leaf_max = p ? p->leaf_max : 4u;
     if (leaf_max < 1 || leaf_max > (uint32_t)kLeafMaxCap) return fail(DR_E_INVALID, "leaf_max must be 1..8");
     if (nt == 0) return fail(DR_E_INVALID, "scene has no triangles");
-    if (nt >= (1u << 27)) return fail(DR_E_UNSUPPORTED, "more than 2^27 triangles per scene");
+    if (nt > kMaxLeafRefs) return fail(DR_E_UNSUPPORTED, "more than 2^28 - 2 triangles per scene");
     const uint32_t flags = p ? p->flags : 0u;
-    if (flags & ~(DR_BVH_LBVH | DR_BVH_NO_SPLIT | DR_BVH_RADIUS_MASK | DR_BVH_SPLIT_MASK))
+    if (flags & ~(DR_BVH_LBVH | DR_BVH_NO_SPLIT | DR_BVH_RADIUS_MASK | DR_BVH_SPLIT_MASK | DR_BVH_SPLIT_RATIO_MASK))
         return fail(DR_E_INVALID, "unknown BVH flags");
     const bool ploc = !(flags & DR_BVH_LBVH);
     const int radius = (flags & DR_BVH_RADIUS_MASK) ? (int)((flags & DR_BVH_RADIUS_MASK) >> 8) : 16;
     if (radius > kPlocRMax) return fail(DR_E_INVALID, "PLOC search radius must be 1..64");
-    uint32_t split_q = (flags & DR_BVH_SPLIT_MASK) >> DR_BVH_SPLIT_SHIFT;
-    if (split_q == 0) split_q = kSplitDefaultQuarters;
-    if (flags & DR_BVH_NO_SPLIT) split_q = 0;
+    uint32_t split_cap_q = (flags & DR_BVH_SPLIT_MASK) >> DR_BVH_SPLIT_SHIFT;
+    uint32_t split_ratio_q = (flags & DR_BVH_SPLIT_RATIO_MASK) >> DR_BVH_SPLIT_RATIO_SHIFT;
+    if (split_cap_q == 0) split_cap_q = kSplitCapDefaultQuarters;
+    if (split_ratio_q == 0) split_ratio_q = kSplitRatioDefaultQuarters;
+    if (flags & DR_BVH_NO_SPLIT) split_cap_q = 0;
     Context *ctx = s->ctx;
 
     hipEvent_t e0, e1;
@@ -866,33 +859,33 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     t.rhi = t.tri_hi.as<float4>();
     t.rtri = nullptr;
     const int32_t *ref_bounds = cbounds;
-    if (split_q > 0 && nt > 1) {
-        const uint32_t nb = blocks(nt);
-        need(t.prio, sizeof(float) * nt);
+    if (split_cap_q > 0 && nt > 1) {
         need(t.offs, sizeof(uint32_t) * (nt + 1));
-        need(t.partial, sizeof(double) * (nb + 1));
         if (st) return st;
-        double *total = t.partial.as<double>() + nb;
-        k_split_priority<<<nb, kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
-                                                t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), t.prio.as<float>(),
-                                                t.partial.as<double>());
-        k_sum_partials<<<1, kT, 0, stream>>>(t.partial.as<double>(), nb, total);
-        // the pieces beyond one per triangle sum to at most the budget
-        const double budget = fmin(0.25 * split_q * (double)nt, (double)(kMaxRefs - nt));
-        k_split_counts<<<blocks(nt + 1), kT, 0, stream>>>(t.prio.as<float>(), nt, total, budget,
-                                                          t.offs.as<uint32_t>());
-        DR_HIP_TRY(hipGetLastError());
-        if ((st = exclusive_scan_u32(ctx, t.offs.as<uint32_t>(), nt + 1, stream))) return st;
+        uint32_t *offs = t.offs.as<uint32_t>();
+        const float r0 = 0.25f * (float)split_ratio_q;
+        const double cap = fmin(0.25 * split_cap_q * (double)nt, (double)(kMaxRefs - nt));
         uint32_t total_refs = 0;
-        DR_HIP_TRY(hipMemcpyAsync(&total_refs, t.offs.as<uint32_t>() + nt, 4, hipMemcpyDeviceToHost, stream));
-        DR_HIP_TRY(hipStreamSynchronize(stream));
+        for (int pass = 0; pass < 2; ++pass) {
+            k_split_counts<<<blocks(nt + 1), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
+                                                              t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), r0, offs);
+            if (pass == 1) {
+                const double extra = (double)total_refs - (double)nt;
+                k_split_rescale<<<blocks(nt), kT, 0, stream>>>(offs, nt, cap / extra);
+            }
+            DR_HIP_TRY(hipGetLastError());
+            if ((st = exclusive_scan_u32(ctx, offs, nt + 1, stream))) return st;
+            DR_HIP_TRY(hipMemcpyAsync(&total_refs, offs + nt, 4, hipMemcpyDeviceToHost, stream));
+            DR_HIP_TRY(hipStreamSynchronize(stream));
+            if ((double)total_refs - (double)nt <= cap) break;
+        }
         if (total_refs < nt || total_refs > kMaxRefs) return fail(DR_E_DEVICE, "split reference count out of range");
         n = total_refs;
         need(t.ref_lo, sizeof(float4) * n);
         need(t.ref_hi, sizeof(float4) * n);
         need(t.ref_tri, sizeof(uint32_t) * n);
         if (st) return st;
-        k_split<<<nb, kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt, t.tri_lo.as<float4>(),
+        k_split<<<blocks(nt), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt, t.tri_lo.as<float4>(),
                                        t.tri_hi.as<float4>(), t.offs.as<uint32_t>(), cbounds, t.ref_lo.as<float4>(),
                                        t.ref_hi.as<float4>(), t.ref_tri.as<uint32_t>(), rbounds);
         DR_HIP_TRY(hipGetLastError());
@@ -949,19 +942,19 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
     } else {
         if (ploc) {
-            if ((st = build_ploc(s, t, n, radius, stream))) return st;
+            if ((st = build_ploc(s, t, n, radius, leaf_max, ct, ci, stream))) return st;
         } else {
             k_karras<<<blocks(n - 1), kT, 0, stream>>>(t.keys.as<uint32_t>(), (int)n, t.child_l.as<int32_t>(),
                                                        t.child_r.as<int32_t>(), t.parent_i.as<int32_t>(),
                                                        t.parent_l.as<int32_t>(), t.range_first.as<uint32_t>(),
                                                        t.count.as<uint32_t>());
+            k_fill_u32<<<blocks(n - 1), kT, 0, stream>>>(t.visit.as<uint32_t>(), n - 1, 0u);
+            k_fit<<<blocks(n), kT, 0, stream>>>((int)n, t.child_l.as<int32_t>(), t.child_r.as<int32_t>(),
+                                                t.parent_i.as<int32_t>(), t.parent_l.as<int32_t>(),
+                                                t.count.as<uint32_t>(), t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(),
+                                                t.node_lo.as<float4>(), t.node_hi.as<float4>(), t.cost.as<float>(),
+                                                t.visit.as<uint32_t>(), t.collapse.as<uint32_t>(), leaf_max, ct, ci);
         }
-        k_fill_u32<<<blocks(n - 1), kT, 0, stream>>>(t.visit.as<uint32_t>(), n - 1, 0u);
-        k_fit<<<blocks(n), kT, 0, stream>>>((int)n, t.child_l.as<int32_t>(), t.child_r.as<int32_t>(),
-                                            t.parent_i.as<int32_t>(), t.parent_l.as<int32_t>(), t.count.as<uint32_t>(),
-                                            t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), t.node_lo.as<float4>(),
-                                            t.node_hi.as<float4>(), t.cost.as<float>(), t.visit.as<uint32_t>(),
-                                            t.collapse.as<uint32_t>(), leaf_max, ct, ci);
         DR_HIP_TRY(hipGetLastError());
         uint32_t root_collapse = 0;
         float root_cost = 0.0f;

@@ -38,7 +38,8 @@ struct DevBuf {
 
 // ---- wide BVH node (128 B): SoA child boxes, child references --------------
 // child ref >= 0: index of an interior wide node
-// child ref <  0: leaf, bit 31 set, bits 27..30 = count-1, bits 0..26 = first
+// child ref <  0: leaf, bit 31 set, bits 28..30 = count-1, bits 0..27 = first
+// (first + count - 1 <= kMaxLeafRefs keeps every leaf distinct from DR_EMPTY_REF)
 // empty slot: DR_EMPTY_REF with a box at +inf in all axes (never hit)
 struct alignas(16) WideNode {
     float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
@@ -49,13 +50,14 @@ static_assert(sizeof(WideNode) == 128, "wide node is 128 B");
 
 constexpr int32_t DR_EMPTY_REF = -1;  // 0xFFFFFFFF
 constexpr int kLeafMaxCap = 8;
+constexpr uint32_t kMaxLeafRefs = (1u << 28) - 2;  // triangle references per scene or batch
 
 __host__ __device__ inline int32_t make_leaf_ref(uint32_t first, uint32_t count)
 {
-    return (int32_t)(0x80000000u | ((count - 1u) << 27) | first);
+    return (int32_t)(0x80000000u | ((count - 1u) << 28) | first);
 }
-__host__ __device__ inline uint32_t leaf_first(int32_t ref) { return (uint32_t)ref & 0x07FFFFFFu; }
-__host__ __device__ inline uint32_t leaf_count(int32_t ref) { return (((uint32_t)ref >> 27) & 0xFu) + 1u; }
+__host__ __device__ inline uint32_t leaf_first(int32_t ref) { return (uint32_t)ref & 0x0FFFFFFFu; }
+__host__ __device__ inline uint32_t leaf_count(int32_t ref) { return (((uint32_t)ref >> 28) & 0x7u) + 1u; }
 
 struct Context {
     int device = 0;
@@ -84,8 +86,8 @@ struct Scene {
     DevBuf ovf_stack;
     DevBuf flags;     // uint32[512]: [0] overflow flag, [16..271] per-launch ray counters
     uint64_t launch_seq = 0;
-    uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;  // traversal schedules
-    uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_REORDER;
+    uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE;  // traversal schedules
+    uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE;
     uint32_t refill_min = 16;  // persistent waves refill when this many lanes are idle
     uint32_t blocks_per_cu = 0;  // persistent grid: resident blocks per CU (0 = occupancy limit)
     uint32_t sort_key = 0;       // ray reorder key: 0 origin, 1 direction + origin

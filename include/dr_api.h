@@ -78,11 +78,14 @@ typedef struct {
  * or as a Karras radix tree over the Morton codes (DR_BVH_LBVH, faster to
  * build, lower quality); both then go through the same SAH leaf collapse and
  * 4-wide collapse. Bits 8-15 set the PLOC search radius (0: 16, max 64).
- * Bits 16-23 set the triangle-splitting budget in quarters: b > 0 adds up to
- * b/4 * n_triangles extra references (and never more than 2^27 - 1 in all),
- * each a clipped piece of a triangle whose box is large and mostly empty;
- * 0 asks for the default (16: four extra references per triangle), and
- * DR_BVH_NO_SPLIT turns splitting off. Hits do not depend on these choices. */
+ * Triangle splitting: a triangle whose box half area is `ratio` times
+ * A (|nx| + |ny| + |nz|) (2 to 3 for well-shaped triangles, far more for long
+ * thin ones at an angle to the axes) is cut into floor(ratio / r0) pieces,
+ * 1 to 16, each bounded by the box of its clipped part. Bits 24-31 set r0 in
+ * quarters (0: default 8, r0 = 2); bits 16-23 cap the extra references in
+ * quarters of the triangle count (0: default 32, i.e. 8 per triangle; never
+ * more than 2^28 - 2 references in all); DR_BVH_NO_SPLIT turns it off. Hits
+ * do not depend on any of these choices. */
 typedef struct {
     uint32_t leaf_max;
     uint32_t flags;
@@ -92,6 +95,8 @@ typedef struct {
 #define DR_BVH_RADIUS_MASK 0xFF00u
 #define DR_BVH_SPLIT_SHIFT 16
 #define DR_BVH_SPLIT_MASK 0xFF0000u
+#define DR_BVH_SPLIT_RATIO_SHIFT 24
+#define DR_BVH_SPLIT_RATIO_MASK 0xFF000000u
 
 typedef struct {
     uint32_t n_triangles;
@@ -151,8 +156,10 @@ dr_status dr_trace_any(dr_scene *scene, const dr_ray *rays, uint32_t n, uint32_t
 /* Traversal schedules (how lanes take rays and order interior/leaf work):
  * PER_RAY: one lane per ray for the launch; SPECULATIVE: leaves are parked
  * until every lane of the wave holds one (Aila & Laine 2009); PERSISTENT:
- * resident waves refill finished lanes from a global counter. The default is
- * chosen by measurement; dr_scene_set_trace_schedule overrides it. */
+ * resident waves refill finished lanes from a global counter. The default,
+ * chosen by measurement, is PERSISTENT_SPECULATIVE without REORDER (over split
+ * triangle references the reorder's sort costs more than it saves);
+ * dr_scene_set_trace_schedule overrides it. */
 enum {
     DR_SCHED_PER_RAY = 0,
     DR_SCHED_PER_RAY_SPECULATIVE = 1,

@@ -42,6 +42,9 @@ def test_small_sphere_vs_brute_force(gpu_ctx, leaf_max):
     scene.close()
 
 
+# split: cap on extra references per triangle; ratio 0.5 splits every
+# triangle of this scene (box/ideal area ratios are 2 and more), so the cap of
+# 1.0 exercises the rescaling and 8.0 the full counts.
 @pytest.mark.parametrize("builder,radius,split", [("lbvh", 0, 0.0), ("ploc", 1, 0.0), ("ploc", 3, 0.0),
                                                    ("ploc", 64, 0.0), ("ploc", 16, 1.0), ("ploc", 16, 8.0),
                                                    ("lbvh", 0, 2.0)])
@@ -52,12 +55,32 @@ def test_builders_vs_brute_force(gpu_ctx, builder, radius, split):
                            dr.gen_rays_surface(v, f, 2048, seed=6, inward=False)])
     scene = dr.Scene(gpu_ctx, v, f)
     for leaf_max in (1, 4):
-        scene.build_bvh(leaf_max=leaf_max, builder=builder, radius=radius, split=split)
+        scene.build_bvh(leaf_max=leaf_max, builder=builder, radius=radius, split=split,
+                        split_ratio=0.5 if split else None)
         refs = scene.info().n_references
         assert (refs == f.shape[0]) if split == 0 else (f.shape[0] < refs <= (1 + split) * f.shape[0] + 1)
         label = f"{builder} r{radius} split {split} leaf {leaf_max}"
         assert_hits_equal(scene.trace_closest(rays), oracle.brute_closest(v, f, rays), label)
         assert np.array_equal(scene.trace_any(rays), oracle.brute_any(v, f, rays))
+    scene.close()
+
+
+def test_split_follows_triangle_shape(gpu_ctx):
+    """Default build: a smooth sphere's well-shaped triangles stay whole, the
+    slivers of a strongly displaced fine sphere are cut into pieces."""
+    print("[case] split follows shape", flush=True)
+    v, f = dr.gen_icosphere(4, 0.0, 0)
+    scene = dr.Scene(gpu_ctx, v, f)
+    scene.build_bvh()
+    assert scene.info().n_references == f.shape[0]
+    scene.close()
+    v, f = dr.gen_icosphere(6, 0.05, 3)
+    scene = dr.Scene(gpu_ctx, v, f)
+    scene.build_bvh()
+    refs = scene.info().n_references
+    assert 1.5 * f.shape[0] < refs <= 9 * f.shape[0]
+    rays = dr.gen_rays_surface(v, f, 8192, seed=9, inward=True)
+    assert_hits_equal(scene.trace_closest(rays), oracle.brute_closest(v, f, rays), "displaced subdiv 6")
     scene.close()
 
 
