@@ -220,10 +220,19 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     s.close()
     target = plain[-1][1]
     tp, tg = plain[-1][0], time_for(guided, target)
+    n_ck = passes // checkpoint
+    # seed-averaged MSE ratio at each checkpoint of equal spp; their mean over
+    # the checkpoints after the first (the guide is untrained for pass 0) is a
+    # steadier summary than the single time-to-target crossing
+    eq = [guided[i][1] / plain[i][1] for i in range(n_ck)]
+    cost = (guided[n_ck - 1][0] / plain[n_ck - 1][0])
     return {"resolution": [width, height], "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
             "spp_per_pass": spp_per_pass, "plain_spp": spp_per_pass * passes, "train_steps_per_pass": train_steps,
             "train_passes": train_passes, "train_batch": train_batch, "seeds": seeds,
-            "guided_over_plain_mse_equal_spp": round(guided[passes // checkpoint - 1][1] / target, 3),
+            "guided_over_plain_mse_equal_spp": round(guided[n_ck - 1][1] / target, 3),
+            "equal_spp_mse_ratio_per_checkpoint": [round(x, 3) for x in eq],
+            "guided_time_per_sample_over_plain": round(cost, 3),
+            "mean_equal_spp_ratio_x_cost": round(float(np.mean(eq[1:])) * cost, 3) if n_ck > 1 else None,
             "target_mse": target, "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1),
             "guided_over_plain_time": round(tg / tp, 3),
             "mse_plain_curve": [[round(t * 1e3, 1), m] for t, m in plain],
@@ -403,7 +412,7 @@ def main() -> int:
         guide["train_step"] = guide_train_rate(ctx)
         log("[bench] guide: indirect-lit partition scene (2,048,000 sphere tris), guided vs plain")
         guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
-                                                   8, 64, 8, 2, 12, train_batch=1 << 16)
+                                                   8, 64, 8, 2, 12, train_batch=1 << 16, seeds=16)
         log("[bench] guide: hidden-light reflector scene, guided vs plain")
         guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 512, 512, 16384,
                                                       16, 64, 8, 16, 8, capacity=1 << 16, train_passes=8)

@@ -597,24 +597,41 @@ struct AdamIn {
     float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1 - beta1^t), c2 = 1/(1 - beta2^t)
 };
 
-// Sum of the block partials in block order (deterministic); with grad set,
-// only the sums are written; otherwise one Adam step on the loss normalised
-// by the total sample weight, and the BF16 fragments are refreshed.
-__global__ __launch_bounds__(kT) void k_guide_adam(const float *__restrict__ partials, int nblk, float *master,
-                                                   float *m1, float *m2, __bf16 *ff, __bf16 *fb, AdamIn ad,
-                                                   float *grad)
+// Sum of the block partials in a fixed order (deterministic): a block owns 64
+// parameters; each of its 16 waves sums every 16th partial of them (one
+// coalesced 256-byte row per load) and wave 0 adds the 16 slices in order.
+// With grad set, only the sums are written; otherwise one Adam step on the
+// loss normalised by the total sample weight, and the BF16 fragments are
+// refreshed.
+constexpr int kAdamSlices = 16;
+__global__ __launch_bounds__(64 * kAdamSlices) void k_guide_adam(const float *__restrict__ partials, int nblk,
+                                                                 float *master, float *m1, float *m2, __bf16 *ff,
+                                                                 __bf16 *fb, AdamIn ad, float *grad)
 {
-    const int p = blockIdx.x * kT + threadIdx.x;
-    if (p > kNP) return;
+    __shared__ float part[kAdamSlices][64];
+    __shared__ float s_sw;
+    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    float acc = 0.0f;
+    if (p <= kNP)
+        for (int b = slice; b < nblk; b += kAdamSlices) acc += partials[(size_t)b * (kNP + 1) + p];
+    part[slice][lane] = acc;
+    if (slice == 1) {  // the total sample weight (column kNP), the same in every block
+        float w = 0.0f;
+        for (int b = lane; b < nblk; b += 64) w += partials[(size_t)b * (kNP + 1) + kNP];
+        for (int off = 32; off >= 1; off >>= 1) w += __shfl_xor(w, off, 64);
+        if (lane == 0) s_sw = w;
+    }
+    __syncthreads();
+    if (slice != 0 || p > kNP) return;
     float sum = 0.0f;
-    for (int b = 0; b < nblk; ++b) sum += partials[(size_t)b * (kNP + 1) + p];
+    for (int k = 0; k < kAdamSlices; ++k) sum += part[k][lane];
     if (grad) {
         grad[p] = sum;
         return;
     }
     if (p == kNP) return;
-    float sw = 0.0f;
-    for (int b = 0; b < nblk; ++b) sw += partials[(size_t)b * (kNP + 1) + kNP];
+    const float sw = s_sw;
     if (!(sw > 0.0f) || !isfinite(sw)) return;
     const float gr = sum / sw;
     if (!isfinite(gr)) return;
@@ -754,7 +771,7 @@ dr_status run_train(Guide *g, const dr_guide_sample *smp, const uint32_t *count,
         ad.c1 = (float)(1.0 / (1.0 - std::pow((double)g->prm.beta1, t)));
         ad.c2 = (float)(1.0 / (1.0 - std::pow((double)g->prm.beta2, t)));
     }
-    k_guide_adam<<<cdiv(kNP + 1, kT), kT, 0, stream>>>(g->partials.as<float>(), nblk, g->master.as<float>(),
+    k_guide_adam<<<cdiv(kNP + 1, 64), 64 * kAdamSlices, 0, stream>>>(g->partials.as<float>(), nblk, g->master.as<float>(),
                                                        g->m1.as<float>(), g->m2.as<float>(), g->frag_f.as<__bf16>(),
                                                        g->frag_b.as<__bf16>(), ad, grad);
     DR_HIP_TRY(hipGetLastError());
