@@ -386,6 +386,9 @@ class Guide:
             check(lib().dr_guide_copy_samples(self.handle, ptr(out), n, C.byref(kept), C.byref(total)))
         return out, total.value
 
+    def set_learning_rate(self, lr: float):
+        check(lib().dr_guide_set_learning_rate(self.handle, lr))
+
     def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64, train_batch: int = 0):
         check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices, train_batch))
         self.collect_fraction, self.guided_vertices, self.train_batch = collect_fraction, guided_vertices, train_batch

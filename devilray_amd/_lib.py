@@ -140,6 +140,7 @@ PROTOTYPES = {
     "dr_guide_train_step": (_st, [_vp, _vp, _u32, _vp]),
     "dr_guide_copy_samples": (_st, [_vp, _vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
     "dr_guide_set_options": (_st, [_vp, C.c_float, _u32, _u32]),
+    "dr_guide_set_learning_rate": (_st, [_vp, C.c_float]),
     "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
     "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),

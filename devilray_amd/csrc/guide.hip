@@ -973,4 +973,12 @@ dr_status dr_guide_set_options(dr_guide *gd, float collect_fraction, uint32_t gu
     return DR_OK;
 }
 
+dr_status dr_guide_set_learning_rate(dr_guide *gd, float lr)
+{
+    if (!gd) return fail(DR_E_INVALID, "guide is null");
+    if (!(lr > 0.0f && lr <= 1.0f)) return fail(DR_E_INVALID, "learning rate must be in (0, 1]");
+    reinterpret_cast<Guide *>(gd)->prm.lr = lr;
+    return DR_OK;
+}
+
 }  // extern "C"

@@ -18,12 +18,13 @@ from . import DR_GUIDE_COLLECT, DR_GUIDE_USE, DrCamera, DrRenderParams, Guide, S
 class ProgressiveRenderer:
     def __init__(self, scene: Scene, camera: DrCamera, guide: Guide | None = None, spp_per_pass: int = 4,
                  max_depth: int = 12, rr_depth: int = 3, seed: int = 0, train_passes: int | None = None,
-                 train_steps: int = 1, stream=None):
+                 train_steps: int = 1, lr_schedule=None, stream=None):
         self.scene, self.camera, self.guide = scene, camera, guide
         self.spp_per_pass = spp_per_pass
         self.max_depth, self.rr_depth, self.seed = max_depth, rr_depth, seed
         self.train_passes = train_passes  # None: train on every pass
         self.train_steps = train_steps
+        self.lr_schedule = lr_schedule  # None, or pass index -> Adam step size for that pass's steps
         self.stream = stream
         self.npix = camera.width * camera.height
         self.accum = scene.ctx.alloc(self.npix * 32)
@@ -56,6 +57,8 @@ class ProgressiveRenderer:
                         self.guide.set_options(min(1.0, 4 * self.guide.capacity / total), self.guide.guided_vertices,
                                                self.guide.train_batch)
                     self._fraction_set = True
+                if self.lr_schedule is not None:
+                    self.guide.set_learning_rate(float(self.lr_schedule(self.passes)))
                 for _ in range(self.train_steps):
                     self.guide.train_step_dev(stream=self.stream)
                 self.trained += 1

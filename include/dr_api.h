@@ -361,6 +361,9 @@ dr_status dr_guide_copy_samples(dr_guide *guide, dr_guide_sample *host, uint32_t
  * windows of that size, wrapping around the ring). */
 dr_status dr_guide_set_options(dr_guide *guide, float collect_fraction, uint32_t guided_vertices,
                                uint32_t train_batch);
+/* Adam step size for the following training steps (> 0), for schedules that
+ * decay it as the guide converges. */
+dr_status dr_guide_set_learning_rate(dr_guide *guide, float lr);
 
 /* dr_render with the guide: DR_GUIDE_USE samples Lambert bounces from the
  * one-sample MIS of the guide mixture and the cosine BSDF (with probability

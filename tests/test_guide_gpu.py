@@ -129,6 +129,35 @@ def test_adam_step(guide):
         w = got  # continue from the device state (FP32 rounding)
 
 
+def test_learning_rate_schedule(gpu_ctx):
+    """dr_guide_set_learning_rate: later steps use the new step size (Adam
+    closed form with a per-step lr), and invalid rates are refused."""
+    g = dr.Guide(gpu_ctx, BOUNDS, capacity=1 << 16, seed=6)
+    rng = np.random.default_rng(29)
+    p = random_params(rng)
+    g.set_params(p)
+    s = random_samples(rng, 1000)
+    b1, b2, eps = 0.9, 0.99, 1e-8
+    m = np.zeros(NP, np.float64)
+    v = np.zeros(NP, np.float64)
+    w = p.astype(np.float64)
+    for t, lr in ((1, 1e-2), (2, 2.5e-3), (3, 5e-4)):
+        g.set_learning_rate(lr)
+        grad, sw = g.gradient(s)
+        gn = grad.astype(np.float64) / sw
+        m = b1 * m + (1 - b1) * gn
+        v = b2 * v + (1 - b2) * gn * gn
+        w = w - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+        g.train_step(s)
+        got = g.get_params().astype(np.float64)
+        assert np.max(np.abs(got - w)) < 1e-5, (t, lr)
+        w = got
+    for bad in (0.0, -1e-3, float("nan"), 2.0):
+        with pytest.raises(dr.DevilRayError):
+            g.set_learning_rate(bad)
+    g.close()
+
+
 def test_learns_a_known_target(gpu_ctx):
     """Samples weighted by a vMF target (kappa 20) per position: the trained
     mixture's cross-entropy against the target drops by more than 1 nat."""
