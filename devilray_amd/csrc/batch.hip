@@ -81,6 +81,7 @@ struct Batch {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     ~Batch()
     {
+        merged.release_graph();  // before the stream its replays ran on goes away
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (stream) (void)hipStreamDestroy(stream);

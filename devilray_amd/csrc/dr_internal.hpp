@@ -113,6 +113,16 @@ struct Scene {
     DevBuf rs_guide_head;   // DR_GUIDE_HEAD_FLOATS floats per path (guided renders)
     DevBuf rs_pending;      // u32 per path: training sample awaiting its radiance snapshot
     uint32_t rs_capacity = 0;
+    // The recorded bounce loop of the last graph render and an event after its
+    // last replay: an executable graph is freed only once its launches are
+    // done (at the next graph render, or with the scene).
+    hipGraphExec_t rs_exec = nullptr;
+    hipEvent_t rs_exec_done = nullptr;
+    void release_graph();
+    Scene() = default;
+    Scene(const Scene &) = delete;
+    Scene &operator=(const Scene &) = delete;
+    ~Scene() { release_graph(); }
 };
 
 // ---- learned path guide (guide.hip) ----------------------------------------

@@ -608,12 +608,33 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         DR_HIP_TRY(ie);
         hipError_t le = hipSuccess;
         for (uint64_t c = 1; c < n_chunks && le == hipSuccess; ++c) le = hipGraphLaunch(exec, stream);
-        (void)hipGraphExecDestroy(exec);
+        // the replays may still be queued: keep the executable graph until
+        // an event after them has completed
+        j.s->release_graph();
+        j.s->rs_exec = exec;
+        hipError_t ee = hipEventCreateWithFlags(&j.s->rs_exec_done, hipEventDisableTiming);
+        if (ee == hipSuccess) ee = hipEventRecord(j.s->rs_exec_done, stream);
         DR_HIP_TRY(le);
+        DR_HIP_TRY(ee);
     }
     if (collect)
         if (dr_status st = guide_compact(j.guide, stream)) return st;
     return DR_OK;
+}
+
+void Scene::release_graph()
+{
+    if (rs_exec) {
+        // with no completion event (its creation failed) wait for the device
+        if (rs_exec_done) (void)hipEventSynchronize(rs_exec_done);
+        else (void)hipDeviceSynchronize();
+        (void)hipGraphExecDestroy(rs_exec);
+        rs_exec = nullptr;
+    }
+    if (rs_exec_done) {
+        (void)hipEventDestroy(rs_exec_done);
+        rs_exec_done = nullptr;
+    }
 }
 
 dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,

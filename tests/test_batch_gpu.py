@@ -69,6 +69,28 @@ def test_batch_views_equal_single_renders(gpu_ctx, three_scenes, graph, sched):
     b.close()
 
 
+def test_graph_renders_back_to_back_then_destroy(gpu_ctx, three_scenes):
+    """Two graph renders queued without a synchronisation in between (the
+    second records a new graph while the first one's replays may still be
+    queued), then the batch destroyed at once: the executable graphs must
+    outlive their launches, and both images equal the eager render."""
+    sds, scs = three_scenes
+    cam = _cam(sds[1], 64, 36)
+    ref = scs[1].render(cam, spp=6, max_depth=8, seed=4)
+    b = dr.Batch(gpu_ctx, [scs[1]])
+    accs = []
+    for chunk in (4096, 5000):
+        a = gpu_ctx.alloc(64 * 36 * 32)
+        a.copy_from(np.zeros(64 * 36 * 4))
+        accs.append(a)
+        b.render_dev([(0, cam, 6, 0, a.ptr)], max_depth=8, seed=4, chunk_paths=chunk, use_graph=True)
+    b.close()
+    gpu_ctx.synchronize()
+    for a in accs:
+        assert np.array_equal(a.to_numpy(np.float64, 64 * 36 * 4).reshape(36, 64, 4), ref)
+        a.free()
+
+
 def test_batch_rejects_bad_views(gpu_ctx, three_scenes):
     sds, scs = three_scenes
     b = dr.Batch(gpu_ctx, scs[:2])
