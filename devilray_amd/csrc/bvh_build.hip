@@ -418,63 +418,22 @@ __device__ __forceinline__ float round_up(double d)
     return f;
 }
 
-// Box of the triangle clipped to [lo, hi] (Sutherland-Hodgman in double, so
-// the clipped corners carry no float rounding); false if nothing is left.
-// A convex polygon gains at most one vertex per plane, so 3 + 6 fit in 9; the
-// arrays hold 12 and a pathological overflow falls back to [lo, hi] itself.
-__device__ bool clip_box(const double (&tri)[3][3], const float lo[3], const float hi[3], float out_lo[3],
-                         float out_hi[3])
+// Cuts triangle i into k = offs[i+1] - offs[i] pieces: k slabs of equal
+// width across the longest axis of its box (for the slivers that get split,
+// the cut that shrinks the pieces' boxes most). The part of a triangle inside
+// a closed slab is a convex polygon whose corners are the triangle's vertices
+// inside the slab and the crossings of its edges with the two planes, so the
+// box of those points is the piece's box; crossings are computed in double and
+// rounded outward to float. The box is then widened by a margin of 2^-17 of the
+// scene's largest coordinate (so rays that meet the triangle on a cut are
+// inside the boxes on both sides whatever the rounding of the triangle test)
+// and cut back to the triangle's own box. Triangles left whole keep their
+// exact box.
+__device__ __forceinline__ float slab_cut(float lo, float hi, uint32_t j, uint32_t k)
 {
-    constexpr int kMaxV = 12;
-    double P[kMaxV][3], Q[kMaxV][3];
-    int np = 3;
-    for (int k = 0; k < 3; ++k)
-        for (int a = 0; a < 3; ++a) P[k][a] = tri[k][a];
-    bool overflow = false;
-    for (int pl = 0; pl < 6 && np > 0 && !overflow; ++pl) {
-        const int a = pl >> 1;
-        const bool low = (pl & 1) == 0;
-        const double b = low ? (double)lo[a] : (double)hi[a];
-        int nq = 0;
-        for (int k = 0; k < np && !overflow; ++k) {
-            const int k1 = k + 1 == np ? 0 : k + 1;
-            const double dc = low ? P[k][a] - b : b - P[k][a];
-            const double dd = low ? P[k1][a] - b : b - P[k1][a];
-            if (dc >= 0.0) {
-                if (nq == kMaxV) { overflow = true; break; }
-                for (int j = 0; j < 3; ++j) Q[nq][j] = P[k][j];
-                ++nq;
-            }
-            if ((dc >= 0.0) != (dd >= 0.0)) {
-                if (nq == kMaxV) { overflow = true; break; }
-                const double f = dc / (dc - dd);
-                for (int j = 0; j < 3; ++j) Q[nq][j] = P[k][j] + f * (P[k1][j] - P[k][j]);
-                Q[nq][a] = b;
-                ++nq;
-            }
-        }
-        for (int k = 0; k < nq; ++k)
-            for (int j = 0; j < 3; ++j) P[k][j] = Q[k][j];
-        np = nq;
-    }
-    if (overflow) {
-        for (int a = 0; a < 3; ++a) { out_lo[a] = lo[a]; out_hi[a] = hi[a]; }
-        return true;
-    }
-    if (np == 0) return false;
-    double l[3] = {P[0][0], P[0][1], P[0][2]}, h[3] = {P[0][0], P[0][1], P[0][2]};
-    for (int k = 1; k < np; ++k)
-        for (int a = 0; a < 3; ++a) { l[a] = fmin(l[a], P[k][a]); h[a] = fmax(h[a], P[k][a]); }
-    for (int a = 0; a < 3; ++a) { out_lo[a] = round_down(l[a]); out_hi[a] = round_up(h[a]); }
-    return true;
+    return j == 0 ? lo : (j == k ? hi : lo + (hi - lo) * ((float)j / (float)k));
 }
 
-// Cuts triangle i into offs[i+1] - offs[i] pieces: the piece with the longest
-// box edge is halved across that edge until the count is reached. A piece
-// box is the clipped box widened by a margin of 2^-17 of the scene's largest
-// coordinate (so rays that meet the triangle on a cut are inside the boxes on
-// both sides whatever the rounding of the triangle test) and cut back to the
-// triangle's own box. Triangles left whole keep their exact box.
 __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                               uint32_t n, const float4 *__restrict__ tlo,
                                               const float4 *__restrict__ thi, const uint32_t *__restrict__ offs,
@@ -487,72 +446,69 @@ __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, c
     if (i < n) {
         const uint32_t base = offs[i], k = offs[i + 1] - base;
         const float4 L = tlo[i], H = thi[i];
-        float flo[kSplitMax][3], fhi[kSplitMax][3];
-        flo[0][0] = L.x; flo[0][1] = L.y; flo[0][2] = L.z;
-        fhi[0][0] = H.x; fhi[0][1] = H.y; fhi[0][2] = H.z;
-        uint32_t nf = 1;
-        if (k > 1) {
-            double tri[3][3];
+        const float tl[3] = {L.x, L.y, L.z}, th[3] = {H.x, H.y, H.z};
+        if (k == 1) {
+            rlo[base] = L;
+            rhi[base] = H;
+            rtri[base] = i;
+            for (int a = 0; a < 3; ++a) {
+                const float m = 0.5f * (tl[a] + th[a]);
+                c[a] = fminf(c[a], m);
+                C[a] = fmaxf(C[a], m);
+            }
+        } else {
+            double v[3][3];
             for (int q = 0; q < 3; ++q) {
                 const uint32_t vi = idx[3 * (size_t)i + q];
-                for (int a = 0; a < 3; ++a) tri[q][a] = verts[3 * (size_t)vi + a];
+                for (int a = 0; a < 3; ++a) v[q][a] = verts[3 * (size_t)vi + a];
             }
             float mag = 0.0f;
             for (int a = 0; a < 6; ++a) mag = fmaxf(mag, fabsf(ord2f(cbounds[a])));
-            mag = fmaxf(mag, fmaxf(fmaxf(fabsf(L.x), fabsf(L.y)), fmaxf(fabsf(L.z), fabsf(H.x))));
-            mag = fmaxf(mag, fmaxf(fabsf(H.y), fabsf(H.z)));
+            for (int a = 0; a < 3; ++a) mag = fmaxf(mag, fmaxf(fabsf(tl[a]), fabsf(th[a])));
             const float margin = mag * 0x1p-17f;
-            const float tl[3] = {L.x, L.y, L.z}, th[3] = {H.x, H.y, H.z};
-            for (int guard = 0; nf < k && guard < 4 * kSplitMax; ++guard) {
-                uint32_t bf = 0;
-                int ba = 0;
-                float bext = -1.0f;
-                for (uint32_t f = 0; f < nf; ++f)
-                    for (int a = 0; a < 3; ++a) {
-                        const float e = fhi[f][a] - flo[f][a];
-                        if (e > bext) { bext = e; bf = f; ba = a; }
+            int ax = 0;
+            for (int a = 1; a < 3; ++a)
+                if (th[a] - tl[a] > th[ax] - tl[ax]) ax = a;
+            for (uint32_t j = 0; j < k; ++j) {
+                const double s0 = slab_cut(tl[ax], th[ax], j, k), s1 = slab_cut(tl[ax], th[ax], j + 1, k);
+                double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+                auto add = [&](const double *p) {
+                    for (int a = 0; a < 3; ++a) { l[a] = fmin(l[a], p[a]); h[a] = fmax(h[a], p[a]); }
+                };
+                for (int q = 0; q < 3; ++q)
+                    if (v[q][ax] >= s0 && v[q][ax] <= s1) add(v[q]);
+                for (int q = 0; q < 3; ++q) {
+                    const double *a0 = v[q], *b0 = v[q == 2 ? 0 : q + 1];
+                    for (int e = 0; e < 2; ++e) {
+                        const double sp = e ? s1 : s0;
+                        const double da = a0[ax] - sp, db = b0[ax] - sp;
+                        if ((da < 0.0 && db > 0.0) || (da > 0.0 && db < 0.0)) {
+                            const double f = da / (da - db);
+                            double p[3];
+                            for (int a = 0; a < 3; ++a) p[a] = a0[a] + f * (b0[a] - a0[a]);
+                            p[ax] = sp;
+                            add(p);
+                        }
                     }
-                const float cut = 0.5f * (flo[bf][ba] + fhi[bf][ba]);
-                if (!(cut > flo[bf][ba] && cut < fhi[bf][ba])) break;  // too small to halve
-                float alo[3], ahi[3], blo[3], bhi[3];
+                }
+                float plo[3], phi[3];
                 for (int a = 0; a < 3; ++a) {
-                    alo[a] = blo[a] = flo[bf][a];
-                    ahi[a] = bhi[a] = fhi[bf][a];
+                    if (l[a] <= h[a]) {
+                        plo[a] = fmaxf(round_down(l[a]) - margin, tl[a]);
+                        phi[a] = fminf(round_up(h[a]) + margin, th[a]);
+                    } else {  // no point found (cannot happen for a slab inside the box): the whole box
+                        plo[a] = tl[a];
+                        phi[a] = th[a];
+                    }
                 }
-                ahi[ba] = cut;
-                blo[ba] = cut;
-                float qa_lo[3], qa_hi[3], qb_lo[3], qb_hi[3];
-                const bool ha = clip_box(tri, alo, ahi, qa_lo, qa_hi);
-                const bool hb = clip_box(tri, blo, bhi, qb_lo, qb_hi);
+                rlo[base + j] = make_float4(plo[0], plo[1], plo[2], 0.0f);
+                rhi[base + j] = make_float4(phi[0], phi[1], phi[2], 0.0f);
+                rtri[base + j] = i;
                 for (int a = 0; a < 3; ++a) {
-                    qa_lo[a] = fmaxf(qa_lo[a] - margin, tl[a]); qa_hi[a] = fminf(qa_hi[a] + margin, th[a]);
-                    qb_lo[a] = fmaxf(qb_lo[a] - margin, tl[a]); qb_hi[a] = fminf(qb_hi[a] + margin, th[a]);
+                    const float m = 0.5f * (plo[a] + phi[a]);
+                    c[a] = fminf(c[a], m);
+                    C[a] = fmaxf(C[a], m);
                 }
-                if (ha && hb) {
-                    for (int a = 0; a < 3; ++a) {
-                        flo[bf][a] = qa_lo[a]; fhi[bf][a] = qa_hi[a];
-                        flo[nf][a] = qb_lo[a]; fhi[nf][a] = qb_hi[a];
-                    }
-                    ++nf;
-                } else if (ha || hb) {  // the piece lies on one side: keep its tighter box
-                    for (int a = 0; a < 3; ++a) {
-                        flo[bf][a] = ha ? qa_lo[a] : qb_lo[a];
-                        fhi[bf][a] = ha ? qa_hi[a] : qb_hi[a];
-                    }
-                } else {
-                    break;  // cannot happen for a box that holds part of the triangle
-                }
-            }
-        }
-        for (uint32_t j = 0; j < k; ++j) {
-            const uint32_t f = j < nf ? j : j % nf;  // pieces short of the count repeat
-            rlo[base + j] = make_float4(flo[f][0], flo[f][1], flo[f][2], 0.0f);
-            rhi[base + j] = make_float4(fhi[f][0], fhi[f][1], fhi[f][2], 0.0f);
-            rtri[base + j] = i;
-            for (int a = 0; a < 3; ++a) {
-                const float m = 0.5f * (flo[f][a] + fhi[f][a]);
-                c[a] = fminf(c[a], m);
-                C[a] = fmaxf(C[a], m);
             }
         }
     }
