@@ -75,12 +75,20 @@ __global__ __launch_bounds__(kT) void k_tri_bounds(const float *__restrict__ ver
                 h[a] = fmaxf(h[a], x);
             }
         }
+        // A triangle with a non-finite coordinate can never be hit (the
+        // triangle test rejects it): it gets an empty box, which no ray
+        // enters and which merges away, and stays out of the centroid bounds.
+        bool finite = true;
+        for (int a = 0; a < 3; ++a) finite = finite && isfinite(l[a]) && isfinite(h[a]);
+        if (!finite)
+            for (int a = 0; a < 3; ++a) { l[a] = INFINITY; h[a] = -INFINITY; }
         lo[i] = make_float4(l[0], l[1], l[2], 0.0f);
         hi[i] = make_float4(h[0], h[1], h[2], 0.0f);
-        for (int a = 0; a < 3; ++a) {
-            const float m = 0.5f * (l[a] + h[a]);
-            c[a] = m; C[a] = m;
-        }
+        if (finite)
+            for (int a = 0; a < 3; ++a) {
+                const float m = 0.5f * (l[a] + h[a]);
+                c[a] = m; C[a] = m;
+            }
     }
     // wave reduction then one atomic per wave
     for (int a = 0; a < 3; ++a) {
@@ -224,7 +232,8 @@ __global__ __launch_bounds__(kT) void k_fit(int n, const int32_t *__restrict__ c
         const uint32_t N = cnt[node];
         const float c_split = ct * area + ac + bc;
         const float c_leaf = (N <= leaf_max) ? ci * area * (float)N : INFINITY;
-        const bool col = c_leaf <= c_split;
+        // N <= leaf_max as well: with overflowing areas both costs are infinite
+        const bool col = N <= leaf_max && c_leaf <= c_split;
         node_lo[node] = lo;
         node_hi[node] = hi;
         cost[node] = col ? c_leaf : c_split;
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(kT) void k_ploc_emit(
     const float area = half_area(lo, hi);
     const float c_split = ct * area + ac + bc;
     const float c_leaf = (c <= leaf_max) ? ci * area * (float)c : INFINITY;
-    const bool col = c_leaf <= c_split;
+    const bool col = c <= leaf_max && c_leaf <= c_split;
     node_lo[id] = lo;
     node_hi[id] = hi;
     cost[id] = col ? c_leaf : c_split;
@@ -746,9 +755,9 @@ dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, uint32_t lea
         DR_HIP_TRY(hipMemcpyAsync(&tot, f.as<uint64_t>() + m, 8, hipMemcpyDeviceToHost, stream));
         DR_HIP_TRY(hipStreamSynchronize(stream));
         const uint32_t merges = (uint32_t)(tot >> 32), kept = (uint32_t)tot;
-        // A round without a merge means an unordered (NaN) area: the bounds are not finite.
+        // every round merges at least one pair (the smallest pair is mutual)
         if (merges == 0 || kept != m - merges || made + merges > n - 1)
-            return fail(DR_E_INVALID, "BVH build: triangle bounds are not finite");
+            return fail(DR_E_DEVICE, "BVH build: clustering made no progress");
         made += merges;
         m = kept;
         cur = 1 - cur;

@@ -84,6 +84,42 @@ def test_split_follows_triangle_shape(gpu_ctx):
     scene.close()
 
 
+@pytest.mark.parametrize("builder", ["ploc", "lbvh"])
+def test_nonfinite_and_huge_coordinates(gpu_ctx, builder):
+    """Triangles with a NaN or infinite vertex can never be hit; triangles
+    with coordinates near 1e25 overflow the box areas to infinity. The build
+    must still give leaves of at most leaf_max references and the same hits as
+    brute force."""
+    print(f"[case] non-finite and huge coordinates, {builder}", flush=True)
+    v, f = dr.gen_icosphere(3, 0.05, 13)
+    rng = np.random.default_rng(31)
+    extra_v, extra_f = [], []
+    base = v.shape[0]
+    for k in range(40):
+        tri = rng.normal(size=(3, 3)).astype(np.float32)
+        if k % 4 == 0:
+            tri[rng.integers(3), rng.integers(3)] = np.nan
+        elif k % 4 == 1:
+            tri[rng.integers(3), rng.integers(3)] = np.inf if k % 8 == 1 else -np.inf
+        else:
+            tri = (tri * np.float32(1e24) + np.float32(3e25)).astype(np.float32)
+        extra_v.append(tri)
+        extra_f.append(base + 3 * k + np.arange(3))
+    v2 = np.concatenate([v] + extra_v).astype(np.float32)
+    f2 = np.concatenate([f, np.array(extra_f, np.uint32)]).astype(np.uint32)
+    rays = np.concatenate([dr.gen_rays_surface(v, f, 4096, seed=8, inward=True),
+                           dr.gen_rays_surface(v, f, 2048, seed=9, inward=False)])
+    ref = oracle.brute_closest(v2, f2, rays)
+    ref_any = oracle.brute_any(v2, f2, rays)
+    scene = dr.Scene(gpu_ctx, v2, f2)
+    for leaf_max, split in ((1, None), (4, None), (8, 0.0), (4, 2.0)):
+        scene.build_bvh(leaf_max=leaf_max, builder=builder, split=split)
+        assert_hits_equal(scene.trace_closest(rays), ref, f"{builder} leaf {leaf_max} split {split}")
+        assert np.array_equal(scene.trace_any(rays), ref_any)
+        assert scene.info().overflow_flag == 0
+    scene.close()
+
+
 @pytest.mark.parametrize("n_tris", [2, 3, 5, 4000])
 def test_ploc_identical_boxes(gpu_ctx, n_tris):
     """Every triangle the same (exact ties in every clustering round) plus a
