@@ -343,7 +343,7 @@ def main() -> int:
     ap.add_argument("--probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
+        print(json.dumps({"error": "the library runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus": args.gpus}))
         return 2
     if args.probe:

@@ -33,7 +33,7 @@ typedef enum {
     DR_E_INVALID = 1,     /* bad argument */
     DR_E_OOM = 2,         /* device allocation failed */
     DR_E_DEVICE = 3,      /* HIP runtime error (no device, launch failure) */
-    DR_E_UNSUPPORTED = 4, /* valid request this build does not implement */
+    DR_E_UNSUPPORTED = 4, /* valid request the library does not implement */
     DR_E_OVERFLOW = 5     /* a bounded structure (traversal stack) overflowed */
 } dr_status;
 

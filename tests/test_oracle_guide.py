@@ -1,4 +1,4 @@
-"""Pins the FP64 guide oracle (oracle/or_guide.c) before it judges the device:
+"""Pins the FP64 guide oracle (oracle/or_guide.c) before it checks the device:
 vMF normalisation by quadrature, Jakob's sampler against the closed-form CDF
 and mean cosine, the hand-derived gradient against central differences of
 the loss, and the input encoding at points where its values are known."""
