@@ -33,6 +33,7 @@ known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -122,6 +123,57 @@ def render_rate(ctx, sc, spp, max_depth, rr_depth, reps, width=None, height=None
     s.close()
     return {"ms_per_pass": round(ms, 3), "spp_per_pass": spp, "samples_per_s": round(w * h * spp / ms * 1e3, 1),
             "image_mean": round(mean, 6), "resolution": [w, h], "max_depth": max_depth, "rr_depth": rr_depth}
+
+
+def display_rate(ctx, sc):
+    """dr_resolve_rgba8 at 1920x1080 (HIP events over 50 launches): 36
+    algorithmic bytes per pixel (FP64 RGBW in, RGBA8 out) against the HBM
+    roofline; and dr_render_debug (barycentric view, 1 spp) on a scene."""
+    w, h = 1920, 1080
+    rng = np.random.default_rng(0)
+    a = rng.random((h, w, 4))
+    a[..., 3] = 16.0
+    d = ctx.upload(a)
+    o = ctx.alloc(w * h * 4)
+    call = lambda: dr.check(dr.lib().dr_resolve_rgba8(ctx.handle, C.c_void_p(d.ptr), w, h, dr.DR_OUTPUT_SRGB,
+                                                     C.c_void_p(o.ptr), None))
+    for _ in range(5):
+        call()
+    ctx.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(50):
+        call()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / 50
+    gbs = 36.0 * w * h / (ms * 1e-3) / 1e9
+    d.free()
+    o.free()
+    s = dr.Scene(ctx, sc.vertices, sc.indices, sc.tri_material, sc.dr_materials())
+    s.build_bvh(leaf_max=4)
+    c = sc.camera
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], w, h)
+    acc = ctx.alloc(w * h * 32)
+    acc.copy_from(np.zeros(w * h * 4))
+    p = dr.DrRenderParams(1, 0, 1, 1, 0, 0, 0)
+    dcall = lambda: dr.check(dr.lib().dr_render_debug(s.handle, C.byref(cam), C.byref(p), dr.DR_DEBUG_BARYCENTRIC,
+                                                      C.c_void_p(acc.ptr), None))
+    dcall()
+    ctx.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(10):
+        dcall()
+    e1.record()
+    e1.synchronize()
+    dms = e0.elapsed_ms(e1) / 10
+    acc.free()
+    s.close()
+    return {"resolve_1080p": {"ms": round(ms, 4), "bytes_per_pixel": 36, "achieved_gbs": round(gbs, 1),
+                              "peak_gbs": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            "debug_view_1080p": {"ms_per_spp": round(dms, 3), "mrays": round(w * h / (dms * 1e-3) / 1e6, 1),
+                                 "mode": "barycentric", "scene": "many-lights, 2,048,000 triangles"}}
 
 
 GUIDE_FLOP_PER_SAMPLE = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 48) * 2 + 2 * (3 * 64 * 64 + 48 * 64)
@@ -406,6 +458,8 @@ def main() -> int:
         renders["cornell_256_16spp"] = render_rate(ctx, scenes.cornell_c0(), 16, 8, 3, 5)
         log("[bench] render: many-lights scene 1920x1080")
         renders["many_lights_1080p"] = render_rate(ctx, scenes.many_lights(), 2, 12, 3, 2)
+        log("[bench] display resolve and debug view at 1920x1080")
+        renders["display"] = display_rate(ctx, scenes.many_lights())
 
     guide = {}
     if not args.no_guide:

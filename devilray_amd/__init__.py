@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, GUIDE_HEAD_FLOATS,
+    ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
+    DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
     GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
 )
@@ -112,6 +113,21 @@ class Context:
 
     def synchronize(self):
         check(lib().dr_synchronize(self.handle))
+
+    def resolve_rgba8(self, accum: np.ndarray, srgb: bool = True) -> np.ndarray:
+        """Display pixels of an FP64 RGBW image (height, width, 4) through
+        dr_resolve_rgba8: packed RGBA8 u32 (height, width), rows flipped."""
+        a = np.ascontiguousarray(accum, np.float64)
+        h, w = a.shape[0], a.shape[1]
+        d = self.upload(a)
+        o = self.alloc(h * w * 4)
+        check(lib().dr_resolve_rgba8(self.handle, C.c_void_p(d.ptr), w, h, DR_OUTPUT_SRGB if srgb else DR_OUTPUT_LINEAR,
+                                     C.c_void_p(o.ptr), None))
+        self.synchronize()
+        out = o.to_numpy(np.uint32, h * w).reshape(h, w)
+        d.free()
+        o.free()
+        return out
 
     def alloc(self, nbytes: int) -> "DeviceArray":
         return DeviceArray(self, nbytes)
@@ -230,6 +246,24 @@ class Scene:
 
     def render_dev(self, camera: DrCamera, params: DrRenderParams, accum_ptr: int, stream=None):
         check(lib().dr_render(self.handle, C.byref(camera), C.byref(params), C.c_void_p(accum_ptr), stream))
+
+    def set_debug_colors(self, rgba):
+        c = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+        check(lib().dr_scene_set_debug_colors(self.handle, ptr(c), c.shape[0]))
+
+    def render_debug(self, camera: DrCamera, mode: int, spp: int, seed: int = 0, sample_offset: int = 0,
+                     accum: np.ndarray | None = None) -> np.ndarray:
+        """Debug view (DR_DEBUG_*) into an FP64 RGBW accumulation image (height, width, 4)."""
+        npix = camera.width * camera.height
+        d = self.ctx.alloc(npix * 32)
+        init = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64)
+        d.copy_from(init)
+        p = DrRenderParams(spp, sample_offset, 1, 1, seed, 0, 0)
+        check(lib().dr_render_debug(self.handle, C.byref(camera), C.byref(p), mode, C.c_void_p(d.ptr), None))
+        self.ctx.synchronize()
+        out = d.to_numpy(np.float64, npix * 4).reshape(camera.height, camera.width, 4)
+        d.free()
+        return out
 
     def render(self, camera: DrCamera, spp: int, max_depth: int = 8, rr_depth: int = 3, seed: int = 0,
                fresnel: int = 0, sample_offset: int = 0, accum: np.ndarray | None = None) -> np.ndarray:

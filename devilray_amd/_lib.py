@@ -86,6 +86,8 @@ class DrGuideParams(C.Structure):
 GUIDE_N_PARAMS = 17712
 GUIDE_HEAD_FLOATS = 44
 DR_GUIDE_USE, DR_GUIDE_COLLECT = 1, 2
+DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB = 0, 1
+DR_DEBUG_UV_CHECKER, DR_DEBUG_BARYCENTRIC, DR_DEBUG_WINDING = 1, 2, 3
 GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
                                ("pad0", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
                                ("wi", np.float32, 3), ("pad2", np.float32)])
@@ -127,6 +129,9 @@ PROTOTYPES = {
     "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
     "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),
     "dr_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
+    "dr_resolve_rgba8": (_st, [_vp, _vp, _u32, _u32, _u32, _vp, _vp]),
+    "dr_render_debug": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _u32, _vp, _vp]),
+    "dr_scene_set_debug_colors": (_st, [_vp, _vp, _u32]),
     "dr_batch_create": (_st, [_vp, _vp, _u32, C.POINTER(_vp)]),
     "dr_batch_destroy": (_st, [_vp]),
     "dr_batch_render": (_st, [_vp, _vp, _u32, C.POINTER(DrRenderParams), C.POINTER(DrBatchOptions), _vp]),

@@ -163,6 +163,7 @@ Renderer::Renderer(Context &ctx, Size2i resolution) : ctx_(ctx), res_(resolution
 Renderer::~Renderer()
 {
     if (accum_) dr_device_free(ctx_.handle(), accum_);
+    if (rgba_) dr_device_free(ctx_.handle(), rgba_);
 }
 
 void Renderer::setScene(Scene scene)
@@ -192,6 +193,13 @@ void Renderer::setOutputOptions(OutputOptions options)
 {
     std::scoped_lock guard{render_mutex_};
     output_ = options;
+}
+
+void Renderer::setDebug(DebugOptions dbg)
+{
+    std::scoped_lock guard{render_mutex_};
+    debug_ = dbg;
+    needs_clear_ = true;
 }
 
 void Renderer::setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fresnel, uint64_t seed)
@@ -242,7 +250,8 @@ void Renderer::render(uint32_t spp)
     p.rr_depth = rr_depth_;
     p.seed = seed_;
     p.fresnel = fresnel_;
-    check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
+    if (debug_ == DebugOptions::Off) check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
+    else check(dr_render_debug(scene_.handle(), &c, &p, (uint32_t)debug_, (double *)accum_, nullptr));
     check(dr_synchronize(ctx_.handle()));
     samples_ += spp;
 }
@@ -269,28 +278,15 @@ const Vec4 *Renderer::getRawPixels()
 // r | g<<8 | b<<16 | a<<24, rows flipped (src/Renderer.cpp:51-90).
 void Renderer::createPixels()
 {
-    const double *a = getRawPixelsFP64();
-    const bool gamma = output_.linearity == OutputLinearity::GammaCorrected;
-    auto tf = [gamma](float v) -> float {
-        if (!gamma) return v;
-        const float r = std::clamp(v, 0.0f, 1.0f);
-        if (r < 0.0031308f) return r * 12.92f;
-        return std::pow(r, 1.0f / 2.4f) * 1.055f - 0.055f;
-    };
-    auto u8 = [](float f) -> uint32_t { return (uint32_t)std::lround(std::max(0.0f, f)); };
-    for (int y = 0; y < res_.height; ++y)
-        for (int x = 0; x < res_.width; ++x) {
-            const double *p = a + 4 * ((size_t)y * res_.width + x);
-            float r = 0, g = 0, b = 0;
-            if (p[3] != 0.0) {
-                r = (float)(p[0] / p[3]);
-                g = (float)(p[1] / p[3]);
-                b = (float)(p[2] / p[3]);
-            }
-            const int fy = res_.height - y - 1;
-            pixels_[(size_t)fy * res_.width + x] =
-                u8(tf(r) * 255.0f) | (u8(tf(g) * 255.0f) << 8) | (u8(tf(b) * 255.0f) << 16) | (255u << 24);
-        }
+    // the reference's host loop (src/Renderer.cpp:51-90) as one device pass;
+    // only the 8-bit pixels cross to the host
+    const size_t n = (size_t)res_.width * res_.height;
+    if (!rgba_) check(dr_device_alloc(ctx_.handle(), n * sizeof(uint32_t), &rgba_));
+    const uint32_t enc = output_.linearity == OutputLinearity::GammaCorrected ? DR_OUTPUT_SRGB : DR_OUTPUT_LINEAR;
+    check(dr_resolve_rgba8(ctx_.handle(), (const double *)accum_, (uint32_t)res_.width, (uint32_t)res_.height, enc,
+                           (uint32_t *)rgba_, nullptr));
+    pixels_.resize(n);
+    check(dr_memcpy_dtoh(ctx_.handle(), pixels_.data(), rgba_, n * sizeof(uint32_t)));
     std::scoped_lock guard{display_mutex_};
     display_ = pixels_;
 }

@@ -68,6 +68,8 @@ Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size);
 
 enum class PixelSampling : int { Center = 0, UniformRandom = 1 };
 enum class OutputLinearity { Linear, GammaCorrected };
+// include/DebugOptions.hpp:3-8
+enum class DebugOptions : int { Off = 0, UVChecker = 1, BariCoords = 2, WindingOrder = 3 };
 struct OutputOptions { OutputLinearity linearity = OutputLinearity::GammaCorrected; };
 
 struct Intrinsics {
@@ -135,6 +137,9 @@ public:
     void setCamera(Camera cam);
     void setPixelSampling(PixelSampling sampling);
     void setOutputOptions(OutputOptions options);
+    // Debug views of the first hit instead of the path tracer; clears the
+    // image, as the reference's setDebug (src/Renderer.cpp:155-161).
+    void setDebug(DebugOptions dbg);
     // Path tracer parameters beyond the reference (max_depth counts path
     // vertices; the reference's fixed behaviour is max_depth 10, no Russian
     // roulette, Schlick Fresnel).
@@ -160,6 +165,8 @@ private:
     Camera cam_;
     PixelSampling sampling_ = PixelSampling::UniformRandom;
     OutputOptions output_;
+    DebugOptions debug_ = DebugOptions::Off;
+    void *rgba_ = nullptr;  // device RGBA8 display pixels
     uint32_t max_depth_ = 10, rr_depth_ = 10, fresnel_ = DR_FRESNEL_SCHLICK;
     uint64_t seed_ = 0;
     void *accum_ = nullptr;  // device FP64 RGBW

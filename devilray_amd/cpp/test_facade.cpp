@@ -197,6 +197,42 @@ int main()
         const uint32_t *p = renderer.getPixels();
         EXPECT(((p[0] >> 24) & 0xFF) == 255u);
     });
+    run("RendererTest.DebugViews", [&] {
+        // a quad filling the view: winding colour everywhere, barycentric
+        // weights summing to one, rows flipped and packed by createPixels
+        const Size2i res{64, 64};
+        Renderer renderer(ctx, res);
+        Camera cam;
+        cam.intrinsics = {50.0f, {res.width / 2.0f * 0.1f, res.height / 2.0f * 0.1f}};
+        cam.resolution = res;
+        cam.physical_pixel_size = {0.1f, 0.1f};
+        renderer.setCamera(cam);
+        Scene scene;
+        scene.materials.push_back(Material::diffuse({0.5f, 0.25f, 1.0f}));
+        scene.objects.push_back({createQuadMesh({0, 0, 1}, {0, 0, -1}, {1, 0, 0}, 1000.0f), 0});
+        renderer.setScene(std::move(scene));
+        renderer.setOutputOptions({OutputLinearity::Linear});
+        renderer.setDebug(DebugOptions::WindingOrder);
+        renderer.render(2);
+        renderer.createPixels();
+        const uint32_t *p = renderer.getPixels();
+        const uint32_t ccw = 255u | (186u << 8) | (120u << 16) | (255u << 24);
+        const uint32_t cw = 135u | (209u << 8) | (255u << 16) | (255u << 24);
+        bool same = true;
+        for (int i = 0; i < res.width * res.height; ++i) same &= p[i] == p[0];
+        EXPECT(same && (p[0] == ccw || p[0] == cw));
+        renderer.setDebug(DebugOptions::BariCoords);
+        renderer.render(3);
+        const Vec4 *px = renderer.getRawPixels();
+        bool sums = true;
+        for (int i = 0; i < res.width * res.height; ++i)
+            sums &= px[i].w == 3.0f && std::fabs(px[i].x + px[i].y + px[i].z - 3.0f) <= 1e-5f;
+        EXPECT(sums);
+        renderer.setDebug(DebugOptions::UVChecker);
+        renderer.render(1);
+        px = renderer.getRawPixels();
+        EXPECT(px[0].w == 1.0f && px[0].x == 0.25f && px[0].y == 0.125f && px[0].z == 0.5f);
+    });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

@@ -64,6 +64,7 @@ struct Context {
     int cu_count = 0;
     hipStream_t stream = nullptr;  // owned helper stream (unused by the async API)
     DevBuf scratch_sort[6];         // radix-sort / scan temporaries
+    DevBuf display_thr;             // 2 x 255 FP64 decision thresholds of the 8-bit display encodings
 };
 
 struct Scene {
@@ -106,6 +107,9 @@ struct Scene {
     DevBuf emit_tri;        // u32 per emitter
     DevBuf alias;           // 20-byte alias entries, one per emitter
     DevBuf light_scalars;
+    DevBuf debug_colors;    // float4 per material for the UV-checker view (default: the albedo)
+    bool debug_colors_set = false;
+    DevBuf dbg_rays, dbg_hits;  // debug views: one sample of rays and hits
     // wavefront render state (render.hip)
     DevBuf rs_rays, rs_hits, rs_beta, rs_radiance, rs_list_a, rs_list_b, rs_counters;
     DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib;

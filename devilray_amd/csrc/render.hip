@@ -111,25 +111,7 @@ __global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict
     item_of(V, item, pixel, sample);
     pview[p] = view;
     if (proot) proot[p] = pp.scenes[V.scene].root;
-    const uint32_t x = pixel % cam.w, y = pixel / cam.w;
-    float sx = 0.5f, sy = 0.5f;
-    if (cam.sampling == DR_PIXEL_UNIFORM_RANDOM) {
-        const U4 r = path_rng(pixel, sample, 0xFFFFFFFFu, 0u, V.seed);
-        sx = u01(r.x);
-        sy = u01(r.y);
-    }
-    const float pcx = ((float)x + sx) * cam.psx - cam.cx;
-    const float pcy = ((float)y + sy) * cam.psy - cam.cy;
-    const V3 v = normalize(V3{pcx / cam.f, pcy / cam.f, 1.0f});
-    const float *m = cam.m;
-    dr_ray r;
-    r.ox = m[3]; r.oy = m[7]; r.oz = m[11];
-    r.dx = m[0] * v.x + m[1] * v.y + m[2] * v.z;
-    r.dy = m[4] * v.x + m[5] * v.y + m[6] * v.z;
-    r.dz = m[8] * v.x + m[9] * v.y + m[10] * v.z;
-    r.tmin = 0.0f;
-    r.tmax = INFINITY;
-    rays[p] = r;
+    rays[p] = camera_ray(cam, pixel, sample, V.seed);
     beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);       // w: pdf of the last BSDF sample
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);        // w: 1 = last bounce specular / camera
     list[p] = p;

@@ -133,4 +133,31 @@ __device__ __forceinline__ U4 path_rng(uint32_t pixel, uint32_t sample, uint32_t
     return philox4x32_10(U4{sample, vertex, stream, (uint32_t)(seed >> 32)}, pixel, (uint32_t)seed);
 }
 
+// Pinhole camera ray of a pixel sample (the reference's cameraRay,
+// include/tracing/CameraRay.hpp:6-34): the pixel centre, or a Philox jitter
+// drawn from the (pixel, sample) stream, through the sensor at focal length f,
+// rotated by the camera transform; origin at its translation.
+__device__ __forceinline__ dr_ray camera_ray(const CamParams &cam, uint32_t pixel, uint32_t sample, uint64_t seed)
+{
+    const uint32_t x = pixel % cam.w, y = pixel / cam.w;
+    float sx = 0.5f, sy = 0.5f;
+    if (cam.sampling == DR_PIXEL_UNIFORM_RANDOM) {
+        const U4 r = path_rng(pixel, sample, 0xFFFFFFFFu, 0u, seed);
+        sx = u01(r.x);
+        sy = u01(r.y);
+    }
+    const float pcx = ((float)x + sx) * cam.psx - cam.cx;
+    const float pcy = ((float)y + sy) * cam.psy - cam.cy;
+    const V3 v = normalize(V3{pcx / cam.f, pcy / cam.f, 1.0f});
+    const float *m = cam.m;
+    dr_ray r;
+    r.ox = m[3]; r.oy = m[7]; r.oz = m[11];
+    r.dx = m[0] * v.x + m[1] * v.y + m[2] * v.z;
+    r.dy = m[4] * v.x + m[5] * v.y + m[6] * v.z;
+    r.dz = m[8] * v.x + m[9] * v.y + m[10] * v.z;
+    r.tmin = 0.0f;
+    r.tmax = INFINITY;
+    return r;
+}
+
 }  // namespace dr

@@ -246,6 +246,35 @@ dr_status dr_scene_copy_lights(dr_scene *scene, void *entries, float *pdf_area);
 dr_status dr_render(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, double *accum,
                     dr_stream stream);
 
+/* ---- display and debug views --------------------------------------------- */
+
+/* 8-bit display pixels from an FP64 accumulation image (width x height,
+ * device memory): rgb x (1 / w) (black where w = 0), clamped to [0, 1], encoded,
+ * rounded to 8 bits and packed R | G << 8 | B << 16 | 255 << 24 into rgba
+ * (device memory, width*height u32) with the rows flipped. Replaces
+ * Renderer::createPixels (src/Renderer.cpp:51-90); the rounding is exact
+ * (decision thresholds computed once in FP64), so every device and the CPU
+ * oracle give the same bytes. Asynchronous on `stream`. */
+enum { DR_OUTPUT_LINEAR = 0, DR_OUTPUT_SRGB = 1 };
+dr_status dr_resolve_rgba8(dr_context *ctx, const double *accum, uint32_t width, uint32_t height, uint32_t output,
+                           uint32_t *rgba, dr_stream stream);
+
+/* Debug views of the camera ray's first hit (DebugOptions,
+ * include/DebugOptions.hpp:3-8; sampleColorDebug and the debug branch of
+ * sampleColor, include/tracing/PathGeneration.hpp:274-299,377-429):
+ * UV_CHECKER adds 0.5 x the material's debug colour (the reference's uv is
+ * always (0, 0)) and skips misses entirely; BARYCENTRIC adds (1-u-v, u, v);
+ * WINDING adds (1, 0.73, 0.47) where ((A-B) x (A-C)) . d > 0, else
+ * (0.53, 0.82, 1), and black for a miss. params: spp, sample_offset, seed and
+ * the camera's pixel sampling, as dr_render; accumulation as dr_render. */
+enum { DR_DEBUG_UV_CHECKER = 1, DR_DEBUG_BARYCENTRIC = 2, DR_DEBUG_WINDING = 3 };
+dr_status dr_render_debug(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, uint32_t mode,
+                          double *accum, dr_stream stream);
+/* The materials' debug colours (Material::debug_color,
+ * include/tracing/Material.hpp:9-12), 4 floats per material (rgba); the
+ * default is each material's albedo. */
+dr_status dr_scene_set_debug_colors(dr_scene *scene, const float *rgba, uint32_t n_materials);
+
 /* ---- batched rendering (no reference counterpart: a reference Renderer
  * renders one scene, src/device/RendererImpl.hpp:49-102) ----------------
  *

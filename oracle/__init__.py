@@ -47,6 +47,9 @@ PROTOTYPES = {
     "or_guide_grad": (C.c_double, [_vp, _vp, _vp, _i32, _i32, _vp]),
     "or_vmf_pdf": (C.c_double, [_vp, C.c_double, _vp]),
     "or_vmf_sample": (None, [_vp, C.c_double, C.c_double, C.c_double, _vp]),
+    "or_display_thresholds": (None, [_i32, _vp]),
+    "or_resolve_rgba8": (None, [_vp, C.c_uint32, C.c_uint32, _i32, _vp]),
+    "or_debug_aov": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
 }
 
 MATERIAL_DTYPE = np.dtype([("kind", np.int32), ("albedo", np.float32, 3), ("emission", np.float32, 3),
@@ -106,6 +109,36 @@ def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int =
     lib().or_render(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(m), _p(ent) if len(ent) else None, _p(pdf), len(ent),
                     C.byref(camera), C.byref(prm), lo, hi, _p(acc), _p(hist), nthreads or threads())
     return acc.reshape(camera.height, camera.width, 4), hist
+
+def display_thresholds(srgb: bool) -> np.ndarray:
+    """The 255 decision thresholds of the 8-bit linear or sRGB encoding."""
+    t = np.zeros(255, np.float64)
+    lib().or_display_thresholds(1 if srgb else 0, _p(t))
+    return t
+
+
+def resolve_rgba8(accum, srgb: bool = True) -> np.ndarray:
+    """Renderer::createPixels restated (src/Renderer.cpp:51-90): FP64 RGBW
+    [h, w, 4] -> packed RGBA8 u32 [h, w] with the rows flipped."""
+    a = np.ascontiguousarray(accum, np.float64)
+    h, w = a.shape[0], a.shape[1]
+    out = np.zeros((h, w), np.uint32)
+    lib().or_resolve_rgba8(_p(a), w, h, 1 if srgb else 0, _p(out))
+    return out
+
+
+def debug_aov(bvh: "BVH", tri_mat, debug_rgba, camera: Camera, mode: int, spp: int, seed: int = 0,
+              sample_offset: int = 0, accum=None):
+    """Debug views of the first hit (mode 1 UV checker, 2 barycentric, 3
+    winding), FP64 RGBW accumulation [h, w, 4] in sample order."""
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    dc = np.ascontiguousarray(debug_rgba, np.float32)
+    npix = camera.width * camera.height
+    acc = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64).reshape(npix, 4)
+    prm = RenderParams(spp, sample_offset, 1, 1, seed, 0, 0)
+    lib().or_debug_aov(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(dc), C.byref(camera), C.byref(prm), mode, _p(acc))
+    return acc.reshape(camera.height, camera.width, 4)
+
 
 _lib = None
 

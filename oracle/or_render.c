@@ -227,16 +227,15 @@ typedef struct {
     int32_t *vertex_hist;  /* optional: paths reaching each vertex count */
 } render_ctx;
 
-/* One path: returns its radiance. */
-static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
+/* Pinhole camera ray of a pixel sample (cameraRay,
+ * include/tracing/CameraRay.hpp:6-34, with the build's Philox jitter). */
+static or_ray camera_ray(const or_camera *cam, uint64_t seed, uint32_t pixel, uint32_t sample)
 {
-    const or_camera *cam = rc->cam;
-    const or_render_params *prm = rc->prm;
     const uint32_t x = pixel % cam->width, y = pixel / cam->width;
     float sx = 0.5f, sy = 0.5f;
     if (cam->pixel_sampling == 1) {
         uint32_t r[4];
-        path_rng(pixel, sample, 0xFFFFFFFFu, 0u, prm->seed, r);
+        path_rng(pixel, sample, 0xFFFFFFFFu, 0u, seed, r);
         sx = or_u01(r[0]);
         sy = or_u01(r[1]);
     }
@@ -251,6 +250,14 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
     ray.dz = m[8] * cv.x + m[9] * cv.y + m[10] * cv.z;
     ray.tmin = 0.0f;
     ray.tmax = INFINITY;
+    return ray;
+}
+
+/* One path: returns its radiance. */
+static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
+{
+    const or_render_params *prm = rc->prm;
+    or_ray ray = camera_ray(rc->cam, prm->seed, pixel, sample);
     v3 beta = mk(1.0f, 1.0f, 1.0f);
     v3 L = mk(0.0f, 0.0f, 0.0f);
     int spec_prev = 1;
@@ -423,4 +430,85 @@ void or_render(const void *bvh, const float *verts, const int32_t *tris, const i
         ++started;
     }
     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+}
+
+/* ---- display and debug views --------------------------------------------- */
+
+/* The reference's display encoding (Renderer::createPixels,
+ * src/Renderer.cpp:51-90): toSRGB(c) = 12.92 c below c = 0.0031308, else
+ * 1.055 c^(1/2.4) - 0.055, then round(255 x). The 8-bit value of c is the
+ * number of thresholds f^-1((b - 1/2) / 255), b = 1..255, that c reaches. */
+void or_display_thresholds(int32_t srgb, double thr[255])
+{
+    const double knee = 0.0031308 * 12.92;
+    for (int b = 1; b <= 255; ++b) {
+        const double y = (b - 0.5) / 255.0;
+        thr[b - 1] = !srgb ? y : (y < knee ? y / 12.92 : pow((y + 0.055) / 1.055, 2.4));
+    }
+}
+
+static uint32_t quantize8(double c, const double thr[255])
+{
+    if (!(c > 0.0)) return 0u;
+    if (c >= 1.0) return 255u;
+    uint32_t n = 0;
+    while (n < 255u && c >= thr[n]) ++n;
+    return n;
+}
+
+void or_resolve_rgba8(const double *accum, uint32_t width, uint32_t height, int32_t srgb, uint32_t *out)
+{
+    double thr[255];
+    or_display_thresholds(srgb, thr);
+    for (uint32_t y = 0; y < height; ++y)
+        for (uint32_t x = 0; x < width; ++x) {
+            const double *a = accum + 4 * ((size_t)y * width + x);
+            uint32_t r = 0, g = 0, b = 0;
+            if (a[3] != 0.0) {
+                const double inv = 1.0 / a[3];
+                r = quantize8(a[0] * inv, thr);
+                g = quantize8(a[1] * inv, thr);
+                b = quantize8(a[2] * inv, thr);
+            }
+            out[(size_t)(height - 1 - y) * width + x] = r | (g << 8) | (b << 16) | (255u << 24);
+        }
+}
+
+/* Debug views of the first hit (sampleColorDebug and the debug branch of
+ * sampleColor, include/tracing/PathGeneration.hpp:274-299,377-429). mode 1:
+ * UV checker (uv is (0, 0) in the reference: 0.5 x debug colour; misses are
+ * not counted), 2: barycentric (1-u-v, u, v), 3: winding ((A-B) x (A-C)) . d
+ * > 0 -> (1, 0.73, 0.47) else (0.53, 0.82, 1). Samples added in order. */
+void or_debug_aov(const void *bvh, const float *verts, const int32_t *tris, const int32_t *tri_mat,
+                  const float *debug_rgba, const or_camera *cam, const or_render_params *prm, int32_t mode,
+                  double *accum)
+{
+    const uint32_t npix = cam->width * cam->height;
+    for (uint32_t k = 0; k < prm->spp; ++k)
+        for (uint32_t p = 0; p < npix; ++p) {
+            or_ray ray = camera_ray(cam, prm->seed, p, prm->sample_offset + k);
+            or_hit h;
+            or_trace_one(bvh, verts, tris, &ray, 0, &h, NULL);
+            float c[3] = {0.0f, 0.0f, 0.0f};
+            if (h.id >= 0) {
+                const int32_t t = h.id;
+                if (mode == 1) {
+                    const float *dc = debug_rgba + 4 * (size_t)(tri_mat ? tri_mat[t] : 0);
+                    c[0] = 0.5f * dc[0]; c[1] = 0.5f * dc[1]; c[2] = 0.5f * dc[2];
+                } else if (mode == 2) {
+                    c[0] = 1.0f - h.u - h.v; c[1] = h.u; c[2] = h.v;
+                } else {
+                    const v3 A = vtx(verts, tris[3 * (size_t)t]);
+                    const v3 B = vtx(verts, tris[3 * (size_t)t + 1]);
+                    const v3 C = vtx(verts, tris[3 * (size_t)t + 2]);
+                    const v3 n = vcross(vsub(A, B), vsub(A, C));
+                    const int ccw = n.x * ray.dx + n.y * ray.dy + n.z * ray.dz > 0.0f;
+                    c[0] = ccw ? 1.0f : 0.53f; c[1] = ccw ? 0.73f : 0.82f; c[2] = ccw ? 0.47f : 1.0f;
+                }
+            } else if (mode == 1) {
+                continue;
+            }
+            double *a = accum + 4 * (size_t)p;
+            a[0] += (double)c[0]; a[1] += (double)c[1]; a[2] += (double)c[2]; a[3] += 1.0;
+        }
 }

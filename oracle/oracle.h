@@ -132,6 +132,16 @@ void or_render(const void *bvh, const float *verts, const int32_t *tris, const i
                const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
                double *accum, int32_t *vertex_hist, int32_t threads);
 
+/* ---- display and debug views ------------------------------------------------ */
+/* 8-bit decision thresholds of the linear (srgb = 0) or sRGB encoding. */
+void or_display_thresholds(int32_t srgb, double thr[255]);
+/* Renderer::createPixels (src/Renderer.cpp:51-90) from FP64 RGBW: rows flipped, RGBA8 packed. */
+void or_resolve_rgba8(const double *accum, uint32_t width, uint32_t height, int32_t srgb, uint32_t *out);
+/* Debug views of the first hit; mode 1 UV checker, 2 barycentric, 3 winding. */
+void or_debug_aov(const void *bvh, const float *verts, const int32_t *tris, const int32_t *tri_mat,
+                  const float *debug_rgba, const or_camera *cam, const or_render_params *prm, int32_t mode,
+                  double *accum);
+
 /* ---- learned path guide: FP64 network, head, loss and gradient ----------- */
 #define OR_GUIDE_NP 17712
 typedef struct {
