@@ -1,6 +1,8 @@
 // C++ host facade over the C ABI: see devilray.hpp.
 #include "devilray.hpp"
 
+#include <cstdio>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -289,6 +291,85 @@ void Renderer::createPixels()
     check(dr_memcpy_dtoh(ctx_.handle(), pixels_.data(), rgba_, n * sizeof(uint32_t)));
     std::scoped_lock guard{display_mutex_};
     display_ = pixels_;
+}
+
+namespace {
+
+uint32_t crc32_png(const uint8_t *d, size_t n, uint32_t c = 0xFFFFFFFFu)
+{
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1u) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        ready = true;
+    }
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ d[i]) & 0xFFu] ^ (c >> 8);
+    return c;
+}
+
+void put_be32(std::vector<uint8_t> &o, uint32_t v)
+{
+    for (int s = 24; s >= 0; s -= 8) o.push_back((uint8_t)(v >> s));
+}
+
+void png_chunk(std::vector<uint8_t> &o, const char *type, const std::vector<uint8_t> &body)
+{
+    put_be32(o, (uint32_t)body.size());
+    const size_t start = o.size();
+    o.insert(o.end(), type, type + 4);
+    o.insert(o.end(), body.begin(), body.end());
+    put_be32(o, crc32_png(o.data() + start, o.size() - start) ^ 0xFFFFFFFFu);
+}
+
+}  // namespace
+
+bool Renderer::saveImage(const std::string &path)
+{
+    // PNG with the image data in stored (uncompressed) deflate blocks: a
+    // valid zlib stream that needs no compressor
+    std::vector<uint32_t> px;
+    {
+        std::scoped_lock guard{display_mutex_};
+        px = display_;
+    }
+    const uint32_t w = (uint32_t)res_.width, h = (uint32_t)res_.height;
+    if (px.size() != (size_t)w * h) px.assign((size_t)w * h, 0xFF000000u);
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)h * (4 * w + 1));
+    for (uint32_t y = 0; y < h; ++y) {
+        raw.push_back(0);  // filter: none
+        const uint8_t *row = reinterpret_cast<const uint8_t *>(px.data() + (size_t)y * w);
+        raw.insert(raw.end(), row, row + 4 * (size_t)w);  // R, G, B, A (little-endian u32)
+    }
+    std::vector<uint8_t> z{0x78, 0x01};
+    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        z.push_back(off + len >= raw.size() ? 1 : 0);
+        z.push_back((uint8_t)len); z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)~len); z.push_back((uint8_t)(~len >> 8));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        if (raw.empty()) break;
+    }
+    uint32_t a = 1, b = 0;  // Adler-32
+    for (uint8_t v : raw) { a = (a + v) % 65521u; b = (b + a) % 65521u; }
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, w);
+    put_be32(ihdr, h);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});  // 8 bits, RGBA, deflate, adaptive filters, no interlace
+    std::vector<uint8_t> out{0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    png_chunk(out, "IHDR", ihdr);
+    png_chunk(out, "IDAT", z);
+    png_chunk(out, "IEND", {});
+    FILE *f = std::fopen(path.c_str(), "wb");
+    const bool ok = f && std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if (f) std::fclose(f);
+    if (!ok) std::fprintf(stderr, "WARN: Failed to save %s\n", path.c_str());
+    return ok;
 }
 
 const uint32_t *Renderer::getPixels()

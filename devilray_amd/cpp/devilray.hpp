@@ -152,6 +152,10 @@ public:
     void clear();
 
     void createPixels();
+    // The display pixels as an 8-bit RGBA PNG (Renderer::saveImage,
+    // src/Renderer.cpp:92-97); returns false, with a warning, if the file
+    // cannot be written.
+    bool saveImage(const std::string &path);
     const uint32_t *getPixels();
     const Vec4 *getRawPixels();      // FP32 view {sum r, g, b, count}
     const double *getRawPixelsFP64();  // the FP64 accumulation itself

@@ -58,8 +58,10 @@ std::optional<Intersection> cast1(Context &ctx, const Scene &s, Vec3 p, Vec3 v)
 }
 }  // namespace
 
-int main()
+int main(int argc, char **argv)
 {
+    // argv[1] (optional): a directory for the saved image and its raw pixels
+    const std::string out_dir = argc > 1 ? argv[1] : "";
     Context ctx(0);
     Scene tri = unit_triangle(ctx);
 
@@ -221,6 +223,12 @@ int main()
         bool same = true;
         for (int i = 0; i < res.width * res.height; ++i) same &= p[i] == p[0];
         EXPECT(same && (p[0] == ccw || p[0] == cw));
+        if (!out_dir.empty()) {
+            EXPECT(renderer.saveImage(out_dir + "/winding.png"));
+            FILE *f = std::fopen((out_dir + "/winding.rgba").c_str(), "wb");
+            EXPECT(f && std::fwrite(p, 4, (size_t)res.width * res.height, f) == (size_t)res.width * res.height);
+            if (f) std::fclose(f);
+        }
         renderer.setDebug(DebugOptions::BariCoords);
         renderer.render(3);
         const Vec4 *px = renderer.getRawPixels();
