@@ -67,6 +67,7 @@ def build_library(verbose: bool = True) -> str:
 CPP = os.path.join(PKG, "cpp")
 HOST_LIB = os.path.join(PKG, "libdevilray_cpp.so")
 FACADE_TEST = os.path.join(CPP, "test_facade")
+BENCH_CLI = os.path.join(CPP, "devil_ray_benchmark")  # the reference's benchmark command line
 
 
 def build_host(verbose: bool = True) -> None:
@@ -78,13 +79,15 @@ def build_host(verbose: bool = True) -> None:
         common + ["-shared", "-o", HOST_LIB, os.path.join(CPP, "devilray.cpp")] + link,
         common + ["-o", FACADE_TEST, os.path.join(CPP, "test_facade.cpp"), f"-L{PKG}", "-ldevilray_cpp",
                   "-ldevilray", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"],
+        common + ["-O3", "-o", BENCH_CLI, os.path.join(CPP, "devil_ray_benchmark.cpp"), f"-L{PKG}",
+                  "-ldevilray_cpp", "-ldevilray", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"],
     ]
     for cmd in steps:
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"g++ failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     if verbose:
-        print(f"built {HOST_LIB} and {FACADE_TEST}")
+        print(f"built {HOST_LIB}, {FACADE_TEST} and {BENCH_CLI}")
 
 
 if __name__ == "__main__":

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
+#include <fstream>
 
 namespace devilray {
 
@@ -53,6 +55,140 @@ Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size)
     return m;
 }
 
+namespace {
+
+void trim_right(std::string &line)
+{
+    size_t i = line.size();
+    while (i > 0 && (line[i - 1] == ' ' || line[i - 1] == '\t' || line[i - 1] == '\n' || line[i - 1] == '\r')) --i;
+    line.resize(i);
+}
+
+// "pi", "pi/ti", "pi//ni" or "pi/ti/ni"; missing indices stay 1 (the first)
+std::array<int, 2> parse_face_vertex(const std::string &str)
+{
+    for (const char c : str)
+        if (std::string("0123456789/").find(c) == std::string::npos)
+            throw std::runtime_error(std::string("Found unknown character in face descriptor: '") + c + "'");
+    std::array<int, 2> v{1, 1};  // position, normal
+    int kind = 0;
+    size_t start = 0;
+    for (;;) {
+        const size_t slash = str.find('/', start);
+        const std::string part = str.substr(start, slash == std::string::npos ? std::string::npos : slash - start);
+        if (!part.empty() && kind < 3) {
+            const int index = std::stoi(part);  // throws std::out_of_range beyond int
+            if (kind == 0) v[0] = index;
+            if (kind == 2) v[1] = index;
+        }
+        ++kind;
+        if (slash == std::string::npos) break;
+        start = slash + 1;
+    }
+    if (kind > 3) throw std::runtime_error("Unexpected number of indices in face vertex: " + std::to_string(kind));
+    return v;
+}
+
+Vec3 read_vec3(std::istringstream &ss)
+{
+    Vec3 p{};
+    ss >> p.x >> p.y >> p.z;
+    if (ss.fail()) throw std::runtime_error("Failed to parse: " + ss.str());
+    return p;
+}
+
+}  // namespace
+
+Mesh loadMesh(const std::string &path)
+{
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("File " + path + " does not exist");
+    Mesh mesh;
+    mesh.name = "Mesh";
+    std::vector<std::array<std::array<int, 2>, 3>> faces;
+    std::string line;
+    while (std::getline(in, line)) {
+        trim_right(line);
+        std::istringstream ss(line);
+        std::string tag;
+        ss >> tag;
+        if (tag.empty() || tag[0] == '#' || tag == "mtllib" || tag == "s" || tag == "vt") continue;
+        if (tag == "v") { mesh.points.push_back(read_vec3(ss)); continue; }
+        if (tag == "vn") { mesh.normals.push_back(read_vec3(ss)); continue; }
+        if (tag == "o") { ss >> mesh.name; continue; }
+        if (tag == "f") {
+            std::vector<std::array<int, 2>> vs;
+            std::string token;
+            while (ss >> token) vs.push_back(parse_face_vertex(token));
+            if (vs.size() < 3) continue;
+            if (vs.size() > 3)
+                throw std::runtime_error("Triangulation not supported. Found face with " + std::to_string(vs.size()) +
+                                         " vertices");
+            faces.push_back({vs[0], vs[1], vs[2]});
+            continue;
+        }
+        throw std::runtime_error("Unrecognized tag: " + tag);
+    }
+    if (mesh.normals.empty()) mesh.normals.push_back(Vec3{0.0f, 0.0f, 0.0f});
+    const int np = (int)mesh.points.size(), nn = (int)mesh.normals.size();
+    for (size_t i = 0; i < faces.size(); ++i) {
+        std::array<uint32_t, 3> t{}, tn{};
+        for (int k = 0; k < 3; ++k) {
+            int pi = faces[i][k][0], ni = faces[i][k][1];
+            pi = pi < 0 ? np + pi : pi - 1;  // 1-based, or relative to the end
+            ni = ni < 0 ? nn + ni : ni - 1;
+            if (pi < 0 || pi >= np)
+                throw std::runtime_error("Invalid position index on triangle " + std::to_string(i) + ": " +
+                                         std::to_string(pi) + ". Number of points is " + std::to_string(np) + ".");
+            if (ni < 0 || ni >= nn)
+                throw std::runtime_error("Invalid normal index on triangle " + std::to_string(i) + ": " +
+                                         std::to_string(ni) + ". Number of normals is " + std::to_string(nn) + ".");
+            t[k] = (uint32_t)pi;
+            tn[k] = (uint32_t)ni;
+        }
+        mesh.triangles.push_back(t);
+        mesh.normal_indices.push_back(tn);
+    }
+    return mesh;
+}
+
+MeshBounds calculateMeshBounds(const Mesh &mesh)
+{
+    Vec3 sum{0.0f, 0.0f, 0.0f};
+    for (const Vec3 &p : mesh.points) { sum.x += p.x; sum.y += p.y; sum.z += p.z; }
+    const float n = (float)mesh.points.size();
+    const Vec3 c{sum.x / n, sum.y / n, sum.z / n};
+    float extent = 0.0f;
+    for (const Vec3 &p : mesh.points) {
+        const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        extent = std::max(extent, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    return MeshBounds{c, extent};
+}
+
+void normalizeMeshSize(Mesh &mesh)
+{
+    const MeshBounds b = calculateMeshBounds(mesh);
+    for (Vec3 &p : mesh.points)
+        p = Vec3{(p.x - b.center.x) / b.extent, (p.y - b.center.y) / b.extent, (p.z - b.center.z) / b.extent};
+}
+
+void generateCoarseNormals(Mesh &mesh)
+{
+    mesh.normals.clear();
+    mesh.normal_indices.clear();
+    for (const auto &t : mesh.triangles) {
+        const Vec3 A = mesh.points[t[0]], B = mesh.points[t[1]], C = mesh.points[t[2]];
+        const Vec3 e1{A.x - B.x, A.y - B.y, A.z - B.z}, e2{A.x - C.x, A.y - C.y, A.z - C.z};
+        Vec3 n{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+        const float l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+        n = Vec3{n.x / l, n.y / l, n.z / l};
+        const uint32_t k = (uint32_t)mesh.normals.size();
+        mesh.normals.push_back(n);
+        mesh.normal_indices.push_back({k, k, k});
+    }
+}
+
 Context::Context(int device) { check(dr_context_create(device, &ctx_)); }
 Context::~Context()
 {
@@ -90,10 +226,11 @@ void Scene::build(Context &ctx, uint32_t leaf_max)
     for (const Object &o : objects) {
         if (o.material >= materials.size()) throw Error(DR_E_INVALID, "object material index out of range");
         const uint32_t base = (uint32_t)(verts.size() / 3);
+        const Transform &tf = o.transform;
         for (const Vec3 &p : o.mesh.points) {
-            verts.push_back(p.x);
-            verts.push_back(p.y);
-            verts.push_back(p.z);
+            verts.push_back(p.x * tf.s.x + tf.p.x);
+            verts.push_back(p.y * tf.s.y + tf.p.y);
+            verts.push_back(p.z * tf.s.z + tf.p.z);
         }
         for (const auto &t : o.mesh.triangles) {
             for (uint32_t k : t) {

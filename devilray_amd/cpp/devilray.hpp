@@ -59,12 +59,40 @@ struct Material {
     dr_material m{};
 };
 
+// Mesh (include/models/Mesh.hpp:7-27): positions, normals, and per triangle
+// the position indices and (when loaded or generated) the normal indices.
 struct Mesh {
+    std::string name;
     std::vector<Vec3> points;
     std::vector<std::array<uint32_t, 3>> triangles;
+    std::vector<Vec3> normals;
+    std::vector<std::array<uint32_t, 3>> normal_indices;
 };
 // Two-triangle square of side `size` (createQuadMesh, src/GpuTris.cpp:87-107).
 Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size);
+
+// OBJ reading (loadMesh, src/models/Mesh.cpp:117-186), with the reference's
+// rules: v / vn / o / f are read; #, mtllib, s and vt lines are skipped; a
+// face is v, v/vt, v//vn or v/vt/vn with 1-based (or negative, relative)
+// indices; faces of fewer than three vertices are dropped; polygons of more
+// than three, unknown tags, bad numbers or out-of-range indices throw
+// std::runtime_error (std::out_of_range for indices beyond int). Without vn
+// lines every triangle gets a (0, 0, 0) normal at index 0.
+Mesh loadMesh(const std::string &path);
+struct MeshBounds { Vec3 center; float extent; };
+// Mean of the points and the largest distance from it (src/models/Mesh.cpp:220-245).
+MeshBounds calculateMeshBounds(const Mesh &mesh);
+// Points moved to (p - center) / extent (src/models/Mesh.cpp:188-195).
+void normalizeMeshSize(Mesh &mesh);
+// One normal per triangle, normalize((A - B) x (A - C)) (src/models/Mesh.cpp:197-218).
+void generateCoarseNormals(Mesh &mesh);
+
+// Scale-and-translate placement of an object (include/Transform.hpp:5-26):
+// world point = p * s + position, applied when the scene is built.
+struct Transform {
+    Vec3 s{1.0f, 1.0f, 1.0f};
+    Vec3 p{0.0f, 0.0f, 0.0f};
+};
 
 enum class PixelSampling : int { Center = 0, UniformRandom = 1 };
 enum class OutputLinearity { Linear, GammaCorrected };
@@ -105,6 +133,11 @@ public:
     struct Object {
         Mesh mesh;
         uint32_t material;
+        Transform transform{};
+        // TriangleMesh::setPosition / setScale (src/tracing/TriangleMesh.cpp:3-15);
+        // the flattened BVH is rebuilt at the next build()
+        void setPosition(Vec3 p) { transform.p = p; }
+        void setScale(Vec3 s) { transform.s = s; }
     };
     std::vector<Object> objects;
 

@@ -241,6 +241,129 @@ int main(int argc, char **argv)
         px = renderer.getRawPixels();
         EXPECT(px[0].w == 1.0f && px[0].x == 0.25f && px[0].y == 0.125f && px[0].z == 0.5f);
     });
+    // ---- the reference's mesh tests (test/src/test_mesh.cpp), restated ----
+    const std::string tmp = out_dir.empty() ? std::string("/tmp") : out_dir;
+    auto write_obj = [&](const std::string &name, const std::string &text) {
+        const std::string path = tmp + "/" + name;
+        FILE *f = std::fopen(path.c_str(), "wb");
+        if (f) { std::fwrite(text.data(), 1, text.size(), f); std::fclose(f); }
+        return path;
+    };
+    auto throws = [](const std::string &path) {
+        try { loadMesh(path); } catch (const std::exception &) { return true; }
+        return false;
+    };
+    run("MeshLoaderTest.HandlesFullVertexDefinition", [&] {
+        Mesh m = loadMesh(write_obj("full_vertex.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.5 0.5\nvn 0 0 1\nf 1/1/1 2/1/1 3/1/1\n"));
+        EXPECT(m.triangles.size() == 1);
+        if (m.triangles.size() != 1) return;
+        EXPECT(m.triangles[0][0] == 0 && m.triangles[0][1] == 1 && m.triangles[0][2] == 2);
+        EXPECT(m.normal_indices[0][0] == 0 && m.normal_indices[0][1] == 0 && m.normal_indices[0][2] == 0);
+    });
+    run("MeshLoaderTest.QuadsThrowError", [&] {
+        EXPECT(throws(write_obj("quad.obj", "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")));
+    });
+    run("MeshLoaderTest.NGonsThrow", [&] {
+        EXPECT(throws(write_obj("ngon.obj", "v 0 0 0\nv 1 0 0\nv 2 1 0\nv 1 2 0\nv 0 1 0\nf 1 2 3 4 5\n")));
+    });
+    run("MeshLoaderTest.DiscardsDegenerateFaces", [&] {
+        Mesh m = loadMesh(write_obj("degenerate.obj", "v 0 0 0\nv 1 1 1\nf 1//1 2//1\nf 1//1\n"));
+        EXPECT(m.triangles.empty());
+    });
+    run("MeshLoaderTest.HandlesAllFaceIndexFormats", [&] {
+        Mesh m = loadMesh(write_obj("index_formats.obj",
+                                    "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.5 0.5\nvn 0 0 1\nf 1 2 3\nf 1/1 2/1 3/1\n"
+                                    "f 1//1 2//1 3//1\nf 1/1/1 2/1/1 3/1/1\n"));
+        EXPECT(m.triangles.size() == 4);
+        if (m.triangles.size() != 4) return;
+        EXPECT(m.triangles[0][0] == 0);
+        EXPECT(m.triangles[1][1] == 1);
+        EXPECT(m.triangles[2][2] == 2 && m.normal_indices[2][2] == 0);
+        EXPECT(m.triangles[3][0] == 0 && m.normal_indices[3][0] == 0);
+    });
+    run("MeshLoaderTest.ThrowsOnInvalidContent", [&] {
+        const std::vector<std::pair<std::string, std::string>> bad = {
+            {"oob_pos.obj", "v 0 0 0\nf 1 2 3\n"},
+            {"oob_norm.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//5 2//5 3//5\n"},
+            {"bad_coords.obj", "v 0.0 apple 1.0\n"},
+            {"bad_face_syntax.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1///1 2///1 3///1\n"},
+            {"zero_index.obj", "v 0 0 0\nf 0 0 0\n"},
+            {"negative_oob.obj", "v 0 0 0\nf -2 -2 -2\n"},
+            {"incomplete_vertex.obj", "v 0.1 0.2\nf 1 1 1\n"},
+            {"overflow_index.obj", "v 0 0 0\nf 9999999999999999999999 1 1\n"},
+        };
+        for (const auto &[name, text] : bad) EXPECT(throws(write_obj(name, text)));
+        EXPECT(throws(tmp + "/no_such_file.obj"));
+    });
+    auto tri_mesh = [](std::vector<Vec3> pts) {
+        Mesh m;
+        m.points = std::move(pts);
+        m.triangles = {{0, 1, 2}};
+        return m;
+    };
+    run("MeshGeometryTest.CalculateBoundsFindsCentroidAndExtent", [&] {
+        const MeshBounds b = calculateMeshBounds(tri_mesh({{0, 0, 0}, {2, 0, 0}, {0, 2, 0}}));
+        EXPECT_NEAR(b.center.x, 2.0f / 3.0f, 1e-5f);
+        EXPECT_NEAR(b.center.y, 2.0f / 3.0f, 1e-5f);
+        EXPECT_NEAR(b.center.z, 0.0f, 1e-5f);
+        const float dx = 2.0f - b.center.x, dy = -b.center.y;
+        EXPECT_NEAR(b.extent, std::sqrt(dx * dx + dy * dy), 1e-5f);
+    });
+    run("MeshGeometryTest.NormalizeRecentersAndRescales", [&] {
+        Mesh m = tri_mesh({{0, 0, 0}, {2, 0, 0}, {0, 2, 0}});
+        normalizeMeshSize(m);
+        const MeshBounds b = calculateMeshBounds(m);
+        EXPECT_NEAR(b.center.x, 0.0f, 1e-5f);
+        EXPECT_NEAR(b.center.y, 0.0f, 1e-5f);
+        EXPECT_NEAR(b.extent, 1.0f, 1e-5f);
+    });
+    run("MeshGeometryTest.CoarseNormalsAreFaceNormals", [&] {
+        Mesh m = tri_mesh({{0, 0, 0}, {1, 0, 0}, {0, 1, 0}});
+        generateCoarseNormals(m);
+        EXPECT(m.normals.size() == 1);
+        if (m.normals.size() != 1) return;
+        EXPECT_NEAR(m.normals[0].x, 0.0f, 1e-6f);
+        EXPECT_NEAR(m.normals[0].y, 0.0f, 1e-6f);
+        EXPECT_NEAR(m.normals[0].z, 1.0f, 1e-6f);
+        EXPECT(m.normal_indices[0][0] == 0 && m.normal_indices[0][1] == 0 && m.normal_indices[0][2] == 0);
+    });
+    run("MeshGeometryTest.CoarseNormalsAreGeneratedPerTriangle", [&] {
+        Mesh m;
+        m.points = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        m.triangles = {{0, 1, 2}, {0, 1, 3}};
+        generateCoarseNormals(m);
+        EXPECT(m.normals.size() == 2);
+        EXPECT(m.normal_indices.size() == 2 && m.normal_indices[0][0] == 0 && m.normal_indices[1][0] == 1);
+    });
+    // ---- the reference's models, and placed objects ----
+    const std::string models = argc > 2 ? argv[2] : "";
+    run("Models.LoadReferenceModels", [&] {
+        if (models.empty()) return;
+        struct Expect { const char *file; size_t points, normals, tris; };
+        for (const Expect &e : {Expect{"cube.obj", 8, 6, 12}, Expect{"ico.obj", 12, 20, 20},
+                                Expect{"suzanne.obj", 507, 942, 967}}) {
+            Mesh m = loadMesh(models + "/" + e.file);
+            EXPECT(m.points.size() == e.points && m.normals.size() == e.normals && m.triangles.size() == e.tris);
+        }
+    });
+    run("Models.TransformedObjectsAreHitWherePlaced", [&] {
+        if (models.empty()) return;
+        Mesh cube = loadMesh(models + "/cube.obj");
+        normalizeMeshSize(cube);  // centred, radius 1
+        Scene s;
+        s.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
+        s.objects.push_back({cube, 0});
+        s.objects.push_back({cube, 0});
+        s.objects[1].setScale({2.0f, 2.0f, 2.0f});
+        s.objects[1].setPosition({10.0f, 0.0f, 0.0f});
+        s.build(ctx);
+        auto a = cast1(ctx, s, {0, 0, 5}, {0, 0, -1});
+        auto b = cast1(ctx, s, {10, 0, 5}, {0, 0, -1});
+        const float half = 1.0f / std::sqrt(3.0f);  // the normalised cube's half side
+        EXPECT(a.has_value() && a->triangle < 12 && std::fabs(a->t - (5.0f - half)) <= 1e-5f);
+        EXPECT(b.has_value() && b->triangle >= 12 && std::fabs(b->t - (5.0f - 2.0f * half)) <= 1e-5f);
+        EXPECT(!cast1(ctx, s, {5, 0, 5}, {0, 0, -1}).has_value());
+    });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }
