@@ -129,6 +129,11 @@ class Context:
         o.free()
         return out
 
+    def fill_zero(self, d: "DeviceArray", nbytes: int | None = None, stream=None):
+        """Zero the first nbytes (a multiple of 16; default all) of a device array with a kernel."""
+        n = d.nbytes if nbytes is None else int(nbytes)
+        check(lib().dr_fill_zero(self.handle, C.c_void_p(d.ptr), n, stream))
+
     def alloc(self, nbytes: int) -> "DeviceArray":
         return DeviceArray(self, nbytes)
 
@@ -246,6 +251,13 @@ class Scene:
 
     def render_dev(self, camera: DrCamera, params: DrRenderParams, accum_ptr: int, stream=None):
         check(lib().dr_render(self.handle, C.byref(camera), C.byref(params), C.c_void_p(accum_ptr), stream))
+
+    def ray_casts(self, reset: bool = False) -> int:
+        """Camera and bounce rays traced by this scene's renders since creation or the last reset
+        (shadow rays excluded); synchronises the null stream."""
+        n = C.c_uint64()
+        check(lib().dr_scene_ray_casts(self.handle, C.byref(n), 1 if reset else 0, None))
+        return n.value
 
     def set_debug_colors(self, rgba):
         c = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)

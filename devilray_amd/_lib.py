@@ -116,6 +116,7 @@ PROTOTYPES = {
     "dr_memcpy_dtoh": (_st, [_vp, _vp, _vp, C.c_size_t]),
     "dr_memcpy_dtod": (_st, [_vp, _vp, _vp, C.c_size_t, _vp]),
     "dr_synchronize": (_st, [_vp]),
+    "dr_fill_zero": (_st, [_vp, _vp, C.c_size_t, _vp]),
     "dr_scene_create": (_st, [_vp, C.POINTER(DrSceneDesc), C.POINTER(_vp)]),
     "dr_scene_destroy": (_st, [_vp]),
     "dr_scene_get_info": (_st, [_vp, C.POINTER(DrSceneInfo)]),
@@ -131,6 +132,7 @@ PROTOTYPES = {
     "dr_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
     "dr_resolve_rgba8": (_st, [_vp, _vp, _u32, _u32, _u32, _vp, _vp]),
     "dr_render_debug": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _u32, _vp, _vp]),
+    "dr_scene_ray_casts": (_st, [_vp, C.POINTER(C.c_uint64), C.c_int32, _vp]),
     "dr_scene_set_debug_colors": (_st, [_vp, _vp, _u32]),
     "dr_batch_create": (_st, [_vp, _vp, _u32, C.POINTER(_vp)]),
     "dr_batch_destroy": (_st, [_vp]),

@@ -4,8 +4,10 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <sstream>
 #include <fstream>
 
@@ -305,8 +307,21 @@ Renderer::~Renderer()
     if (rgba_) dr_device_free(ctx_.handle(), rgba_);
 }
 
+void RunningAverage::add(double value)
+{
+    values_.push_back(value);
+    while (values_.size() > window_) values_.pop_front();
+}
+
+double RunningAverage::mean() const
+{
+    if (values_.empty()) return 0.0;
+    return std::accumulate(values_.begin(), values_.end(), 0.0) / (double)values_.size();
+}
+
 void Renderer::setScene(Scene scene)
 {
+    std::scoped_lock frame{frame_mutex_};
     std::scoped_lock guard{render_mutex_};
     scene_ = std::move(scene);
     if (!scene_.handle()) scene_.build(ctx_);
@@ -357,46 +372,68 @@ void Renderer::clear()
     needs_clear_ = true;
 }
 
-void Renderer::clearIfNeeded()
+float Renderer::getMeanRenderTimes() const
 {
-    if (!needs_clear_) return;
-    const size_t n = (size_t)res_.width * res_.height * 4;
-    std::vector<double> zeros(n, 0.0);
-    check(dr_memcpy_htod(ctx_.handle(), accum_, zeros.data(), n * sizeof(double)));
-    samples_ = 0;
-    needs_clear_ = false;
+    std::scoped_lock guard{stats_mutex_};
+    return (float)render_times_.mean();
 }
 
 void Renderer::render(uint32_t spp)
 {
-    std::scoped_lock guard{render_mutex_};
-    if (!scene_.handle()) throw Error(DR_E_INVALID, "no scene set");
-    clearIfNeeded();
+    std::scoped_lock frame{frame_mutex_};
     dr_camera c{};
-    std::memcpy(c.transform, cam_.transform.data(), sizeof c.transform);
-    c.focal_length = cam_.intrinsics.focal_length;
-    c.center[0] = cam_.intrinsics.center.x;
-    c.center[1] = cam_.intrinsics.center.y;
-    c.pixel_size[0] = cam_.physical_pixel_size.x;
-    c.pixel_size[1] = cam_.physical_pixel_size.y;
-    c.width = (uint32_t)res_.width;
-    c.height = (uint32_t)res_.height;
-    c.pixel_sampling = (uint32_t)sampling_;
     dr_render_params p{};
-    p.spp = spp;
-    p.sample_offset = samples_;
-    p.max_depth = max_depth_;
-    p.rr_depth = rr_depth_;
-    p.seed = seed_;
-    p.fresnel = fresnel_;
-    if (debug_ == DebugOptions::Off) check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
-    else check(dr_render_debug(scene_.handle(), &c, &p, (uint32_t)debug_, (double *)accum_, nullptr));
-    check(dr_synchronize(ctx_.handle()));
+    DebugOptions debug;
+    uint32_t encoding;
+    {
+        std::scoped_lock guard{render_mutex_};
+        if (!scene_.handle()) throw Error(DR_E_INVALID, "no scene set");
+        if (needs_clear_) {
+            check(dr_fill_zero(ctx_.handle(), accum_, (size_t)res_.width * res_.height * 4 * sizeof(double), nullptr));
+            uint64_t unused;
+            check(dr_scene_ray_casts(scene_.handle(), &unused, 1, nullptr));
+            samples_ = 0;
+            total_casts_ = 0;
+            std::scoped_lock stats{stats_mutex_};
+            render_times_.reset();
+            needs_clear_ = false;
+        }
+        std::memcpy(c.transform, cam_.transform.data(), sizeof c.transform);
+        c.focal_length = cam_.intrinsics.focal_length;
+        c.center[0] = cam_.intrinsics.center.x;
+        c.center[1] = cam_.intrinsics.center.y;
+        c.pixel_size[0] = cam_.physical_pixel_size.x;
+        c.pixel_size[1] = cam_.physical_pixel_size.y;
+        c.width = (uint32_t)res_.width;
+        c.height = (uint32_t)res_.height;
+        c.pixel_sampling = (uint32_t)sampling_;
+        p.spp = spp;
+        p.sample_offset = samples_;
+        p.max_depth = max_depth_;
+        p.rr_depth = rr_depth_;
+        p.seed = seed_;
+        p.fresnel = fresnel_;
+        debug = debug_;
+        encoding = output_.linearity == OutputLinearity::GammaCorrected ? DR_OUTPUT_SRGB : DR_OUTPUT_LINEAR;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    if (debug == DebugOptions::Off) check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
+    else check(dr_render_debug(scene_.handle(), &c, &p, (uint32_t)debug, (double *)accum_, nullptr));
+    createPixelsLocked(encoding);  // synchronises
     samples_ += spp;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    {
+        std::scoped_lock stats{stats_mutex_};
+        render_times_.add(ms);
+    }
+    uint64_t casts = 0;
+    check(dr_scene_ray_casts(scene_.handle(), &casts, 0, nullptr));
+    total_casts_ = casts;
 }
 
 const double *Renderer::getRawPixelsFP64()
 {
+    std::scoped_lock frame{frame_mutex_};
     const size_t n = (size_t)res_.width * res_.height * 4;
     host_fp64_.resize(n);
     check(dr_memcpy_dtoh(ctx_.handle(), host_fp64_.data(), accum_, n * sizeof(double)));
@@ -417,11 +454,21 @@ const Vec4 *Renderer::getRawPixels()
 // r | g<<8 | b<<16 | a<<24, rows flipped (src/Renderer.cpp:51-90).
 void Renderer::createPixels()
 {
+    std::scoped_lock frame{frame_mutex_};
+    uint32_t enc;
+    {
+        std::scoped_lock guard{render_mutex_};
+        enc = output_.linearity == OutputLinearity::GammaCorrected ? DR_OUTPUT_SRGB : DR_OUTPUT_LINEAR;
+    }
+    createPixelsLocked(enc);
+}
+
+void Renderer::createPixelsLocked(uint32_t enc)
+{
     // the reference's host loop (src/Renderer.cpp:51-90) as one device pass;
     // only the 8-bit pixels cross to the host
     const size_t n = (size_t)res_.width * res_.height;
     if (!rgba_) check(dr_device_alloc(ctx_.handle(), n * sizeof(uint32_t), &rgba_));
-    const uint32_t enc = output_.linearity == OutputLinearity::GammaCorrected ? DR_OUTPUT_SRGB : DR_OUTPUT_LINEAR;
     check(dr_resolve_rgba8(ctx_.handle(), (const double *)accum_, (uint32_t)res_.width, (uint32_t)res_.height, enc,
                            (uint32_t *)rgba_, nullptr));
     pixels_.resize(n);
@@ -513,6 +560,37 @@ const uint32_t *Renderer::getPixels()
 {
     std::scoped_lock guard{display_mutex_};
     return display_.data();
+}
+
+RenderLoop::RenderLoop(Renderer &renderer) : renderer_(renderer)
+{
+    thread_ = std::thread([this] {
+        try {
+            while (!stop_.load(std::memory_order_relaxed)) {
+                renderer_.render();
+                average_render_time_.store(renderer_.getMeanRenderTimes(), std::memory_order_relaxed);
+                total_ray_casts_.store(renderer_.getTotalCasts(), std::memory_order_relaxed);
+                passes_.fetch_add(1);
+            }
+        } catch (...) {
+            std::scoped_lock guard{error_mutex_};
+            error_ = std::current_exception();
+        }
+    });
+}
+
+RenderLoop::~RenderLoop() { stop(); }
+
+void RenderLoop::stop()
+{
+    stop_.store(true);
+    if (thread_.joinable()) thread_.join();
+}
+
+std::exception_ptr RenderLoop::error() const
+{
+    std::scoped_lock guard{error_mutex_};
+    return error_;
 }
 
 }  // namespace devilray

@@ -10,17 +10,25 @@
 //                 over a batch of rays (the reference casts one ray per call)
 // Errors: a non-OK dr_status becomes devilray::Error (the reference throws
 // std::runtime_error on misuse, DeviceMemory.cu:51); no-hit is std::nullopt.
-// Threading: as the reference, setters and render() take one mutex and the
-// display copy another (include/Renderer.hpp:43-44).
+// Threading: as the reference, setters take one mutex and the display copy
+// another (include/Renderer.hpp:43-44); a third serialises the frame work of
+// render(), createPixels() and the raw-pixel reads, so a setter called from
+// another thread waits at most for a snapshot of the settings, not for a
+// render (setScene alone waits for the render in flight, whose scene it
+// replaces).
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <cstdint>
+#include <deque>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dr_api.h"
@@ -113,6 +121,20 @@ struct Camera {
     Vec2 physical_pixel_size{1.0f, 1.0f};
 };
 
+// Mean of the last `window` values added, 0 when there are none
+// (include/RunningAverage.hpp:5-16).
+class RunningAverage {
+public:
+    explicit RunningAverage(int window) : window_(window) {}
+    void add(double value);
+    double mean() const;
+    void reset() { values_.clear(); }
+
+private:
+    std::deque<double> values_;
+    size_t window_;
+};
+
 class Context {
 public:
     explicit Context(int device = 0);
@@ -179,10 +201,17 @@ public:
     void setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fresnel = DR_FRESNEL_EXACT,
                         uint64_t seed = 0);
 
-    // Adds exactly one sample per pixel and waits for the device (as the
-    // reference's render()); render(n) adds n in one launch sequence.
+    // Adds exactly one sample per pixel, updates the display pixels and waits
+    // for the device (as the reference's render(), src/Renderer.cpp:198-233);
+    // render(n) adds n in one launch sequence.
     void render(uint32_t spp = 1);
     void clear();
+
+    // Mean wall time in ms of the last 20 render() calls since the last clear
+    // (include/Renderer.hpp:50), and the camera and bounce rays cast since
+    // the last clear, shadow rays excluded (include/Renderer.hpp:51).
+    float getMeanRenderTimes() const;
+    uint64_t getTotalCasts() const { return total_casts_.load(); }
 
     void createPixels();
     // The display pixels as an 8-bit RGBA PNG (Renderer::saveImage,
@@ -192,10 +221,10 @@ public:
     const uint32_t *getPixels();
     const Vec4 *getRawPixels();      // FP32 view {sum r, g, b, count}
     const double *getRawPixelsFP64();  // the FP64 accumulation itself
-    uint32_t getSampleCount() const { return samples_; }
+    uint32_t getSampleCount() const { return samples_.load(); }
 
 private:
-    void clearIfNeeded();
+    void createPixelsLocked(uint32_t encoding);
     Context &ctx_;
     Size2i res_;
     Scene scene_;
@@ -207,12 +236,45 @@ private:
     uint32_t max_depth_ = 10, rr_depth_ = 10, fresnel_ = DR_FRESNEL_SCHLICK;
     uint64_t seed_ = 0;
     void *accum_ = nullptr;  // device FP64 RGBW
-    uint32_t samples_ = 0;
+    std::atomic<uint32_t> samples_{0};
+    std::atomic<uint64_t> total_casts_{0};
+    RunningAverage render_times_{20};
     bool needs_clear_ = true;
     std::vector<double> host_fp64_;
     std::vector<Vec4> host_vec4_;
     std::vector<uint32_t> pixels_, display_;
-    std::mutex render_mutex_, display_mutex_;
+    std::mutex frame_mutex_, render_mutex_, display_mutex_;
+    mutable std::mutex stats_mutex_;
+};
+
+// Application::renderWorker (renderer/src/RenderThread.cpp:3-12): a
+// background thread that calls render() until stopped and publishes the mean
+// render time and the ray casts after every pass, while other threads change
+// the camera or settings (each change clears the image at the start of the
+// next pass) and read getPixels(). An exception from render() stops the loop
+// and is kept for error().
+class RenderLoop {
+public:
+    explicit RenderLoop(Renderer &renderer);
+    ~RenderLoop();  // stop()
+    RenderLoop(const RenderLoop &) = delete;
+    RenderLoop &operator=(const RenderLoop &) = delete;
+
+    void stop();  // asks the loop to end after the pass in flight and joins it
+    float averageRenderTime() const { return average_render_time_.load(std::memory_order_relaxed); }
+    uint64_t totalRayCasts() const { return total_ray_casts_.load(std::memory_order_relaxed); }
+    uint64_t passes() const { return passes_.load(); }
+    std::exception_ptr error() const;
+
+private:
+    Renderer &renderer_;
+    std::atomic<bool> stop_{false};
+    std::atomic<float> average_render_time_{0.0f};
+    std::atomic<uint64_t> total_ray_casts_{0};
+    std::atomic<uint64_t> passes_{0};
+    mutable std::mutex error_mutex_;
+    std::exception_ptr error_;
+    std::thread thread_;
 };
 
 }  // namespace devilray

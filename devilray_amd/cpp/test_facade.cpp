@@ -2,11 +2,17 @@
 // the C++ facade (needs a GPU): triangle intersection cases
 // (test/src/test_intersection.cpp), nearest-of-stacked and cast
 // (test/src/test_tracing.cpp:221-298) and the analytic diffuse reflection
-// render (test/src/test_render.cpp:50-165). Prints one line per case and
-// exits non-zero on the first failure.
+// render (test/src/test_render.cpp:50-165), the mesh loader
+// (test/src/test_mesh.cpp) and the running average
+// (test/src/test_running_average.cpp); then the façade's own cases for the
+// render statistics and the background render loop of the interactive
+// application (renderer/src/RenderThread.cpp:3-12). Prints one line per case
+// and exits non-zero if any failed.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <string>
 #include <vector>
@@ -55,6 +61,38 @@ Scene unit_triangle(Context &ctx)
 std::optional<Intersection> cast1(Context &ctx, const Scene &s, Vec3 p, Vec3 v)
 {
     return cast(ctx, s, {Ray{p, v}})[0];
+}
+
+// the analytic scene of test_render.cpp: a white quad in front of the camera
+// and an emitting one behind it, both 1000 wide
+Scene facing_quads(float emittance)
+{
+    Scene scene;
+    scene.materials.push_back(Material::diffuse({1, 1, 1}));
+    scene.materials.push_back(Material::diffuse({0, 0, 0}, {emittance, emittance, emittance}));
+    scene.objects.push_back({createQuadMesh({0, 0, 1}, {0, 0, -1}, {1, 0, 0}, 1000.0f), 0});
+    scene.objects.push_back({createQuadMesh({0, 0, -1}, {0, 0, 1}, {1, 0, 0}, 1000.0f), 1});
+    return scene;
+}
+
+Camera pinhole(Size2i res, float x_offset = 0.0f)
+{
+    Camera cam;
+    cam.intrinsics = {50.0f, {res.width / 2.0f * 0.1f, res.height / 2.0f * 0.1f}};
+    cam.resolution = res;
+    cam.physical_pixel_size = {0.1f, 0.1f};
+    cam.transform[3] = x_offset;
+    return cam;
+}
+
+template <class F> bool wait_for(F done, double seconds)
+{
+    const auto end = std::chrono::steady_clock::now() + std::chrono::duration<double>(seconds);
+    while (!done()) {
+        if (std::chrono::steady_clock::now() > end) return false;
+        std::this_thread::sleep_for(std::chrono::milliseconds(1));
+    }
+    return true;
 }
 }  // namespace
 
@@ -363,6 +401,109 @@ int main(int argc, char **argv)
         EXPECT(a.has_value() && a->triangle < 12 && std::fabs(a->t - (5.0f - half)) <= 1e-5f);
         EXPECT(b.has_value() && b->triangle >= 12 && std::fabs(b->t - (5.0f - 2.0f * half)) <= 1e-5f);
         EXPECT(!cast1(ctx, s, {5, 0, 5}, {0, 0, -1}).has_value());
+    });
+    // ---- test/src/test_running_average.cpp, restated ----
+    run("RunningAverageTest.EmptyAverageIsZero", [&] { EXPECT(RunningAverage(5).mean() == 0.0); });
+    run("RunningAverageTest.AveragesAddedValues", [&] {
+        RunningAverage a(5);
+        a.add(2.0);
+        a.add(4.0);
+        a.add(6.0);
+        EXPECT(a.mean() == 4.0);
+    });
+    run("RunningAverageTest.DropsOldestBeyondWindow", [&] {
+        RunningAverage a(3);
+        for (double v : {1.0, 2.0, 3.0, 4.0}) a.add(v);
+        EXPECT(a.mean() == 3.0);
+    });
+    run("RunningAverageTest.KeepsOnlyWindowSizeSamples", [&] {
+        RunningAverage a(2);
+        for (int i = 0; i < 100; ++i) a.add((double)i);
+        EXPECT(a.mean() == 98.5);
+    });
+    run("RunningAverageTest.ResetClearsHistory", [&] {
+        RunningAverage a(5);
+        a.add(10.0);
+        a.add(20.0);
+        a.reset();
+        EXPECT(a.mean() == 0.0);
+    });
+    // ---- render statistics and the render loop ----
+    run("RendererStats.CastsAndTimesSinceClear", [&] {
+        const Size2i res{48, 32};
+        const uint64_t npix = (uint64_t)res.width * res.height;
+        Renderer renderer(ctx, res);
+        renderer.setCamera(pinhole(res));
+        renderer.setScene(facing_quads(0.55f));
+        EXPECT(renderer.getTotalCasts() == 0 && renderer.getMeanRenderTimes() == 0.0f);
+        renderer.setPathOptions(1, 1);  // camera rays only
+        renderer.render(3);
+        EXPECT(renderer.getTotalCasts() == 3 * npix);
+        EXPECT(renderer.getMeanRenderTimes() > 0.0f);
+        renderer.setPathOptions(10, 10);  // clears: counts start again
+        renderer.render(2);
+        const uint64_t c2 = renderer.getTotalCasts();
+        std::printf("    casts %llu for 2 spp of %llu pixels\n", (unsigned long long)c2, (unsigned long long)npix);
+        EXPECT(c2 > 2 * npix && c2 <= 20 * npix);
+        renderer.render(1);
+        EXPECT(renderer.getTotalCasts() > c2);
+        renderer.setDebug(DebugOptions::BariCoords);  // one cast per debug sample
+        renderer.render(4);
+        EXPECT(renderer.getTotalCasts() == 4 * npix);
+        EXPECT(renderer.getSampleCount() == 4);
+    });
+    run("RenderThread.ProgressiveUnderConcurrentChanges", [&] {
+        // a render loop runs while this thread moves the camera and reads the
+        // display pixels; what it accumulates after the move is the image of
+        // the same number of samples rendered in one call
+        const Size2i res{96, 64};
+        const Camera cam_a = pinhole(res), cam_b = pinhole(res, 0.25f);
+        Renderer renderer(ctx, res);
+        renderer.setCamera(cam_a);
+        renderer.setScene(facing_quads(0.55f));
+        renderer.setPathOptions(6, 3, DR_FRESNEL_SCHLICK, 5);
+        uint64_t reads = 0;
+        {
+            RenderLoop loop(renderer);
+            EXPECT(wait_for([&] { return loop.passes() >= 3 || loop.error(); }, 60.0));
+            renderer.setCamera(cam_b);
+            const uint64_t p0 = loop.passes();
+            EXPECT(wait_for([&] {
+                reads += (renderer.getPixels()[0] >> 24) == 255u;
+                return loop.passes() >= p0 + 6 || loop.error();
+            }, 60.0));
+            loop.stop();
+            EXPECT(!loop.error());
+            EXPECT(loop.averageRenderTime() > 0.0f);
+            EXPECT(loop.totalRayCasts() == renderer.getTotalCasts());
+        }
+        const uint32_t n = renderer.getSampleCount();
+        std::printf("    %u samples after the camera moved, %llu pixel reads\n", n, (unsigned long long)reads);
+        EXPECT(n >= 4 && reads > 0);
+        const size_t count = (size_t)res.width * res.height * 4;
+        std::vector<double> progressive(renderer.getRawPixelsFP64(), renderer.getRawPixelsFP64() + count);
+        Renderer once(ctx, res);
+        once.setCamera(cam_b);
+        once.setScene(facing_quads(0.55f));
+        once.setPathOptions(6, 3, DR_FRESNEL_SCHLICK, 5);
+        once.render(n);
+        EXPECT(std::memcmp(progressive.data(), once.getRawPixelsFP64(), count * sizeof(double)) == 0);
+        EXPECT(once.getTotalCasts() == renderer.getTotalCasts());
+        EXPECT(std::memcmp(renderer.getPixels(), once.getPixels(), (size_t)res.width * res.height * 4) == 0);
+    });
+    run("RenderThread.StopsAndKeepsTheError", [&] {
+        Renderer renderer(ctx, {8, 8});  // no scene: render() throws
+        RenderLoop loop(renderer);
+        EXPECT(wait_for([&] { return (bool)loop.error(); }, 30.0));
+        loop.stop();
+        bool invalid = false;
+        try {
+            std::rethrow_exception(loop.error());
+        } catch (const Error &e) {
+            invalid = e.status == DR_E_INVALID;
+        } catch (...) {
+        }
+        EXPECT(invalid && loop.passes() == 0);
     });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;

@@ -2,6 +2,7 @@
 // dispatch to the traversal and build kernels.
 #include "dr_internal.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -9,6 +10,12 @@ namespace dr {
 
 namespace {
 thread_local std::string g_last_error;
+
+__global__ __launch_bounds__(256) void k_zero16(uint4 *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
 }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -139,6 +146,20 @@ dr_status dr_memcpy_dtod(dr_context *c, void *dst, const void *src, size_t bytes
     if (!c || (!dst && bytes) || (!src && bytes)) return fail(DR_E_INVALID, "null argument");
     if (bytes)
         DR_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream)));
+    return DR_OK;
+}
+
+dr_status dr_fill_zero(dr_context *c, void *dst, size_t bytes, dr_stream stream)
+{
+    if (!c || (!dst && bytes)) return fail(DR_E_INVALID, "null argument");
+    if ((reinterpret_cast<uintptr_t>(dst) & 15u) || (bytes & 15u))
+        return fail(DR_E_INVALID, "dst and bytes must be multiples of 16");
+    if (!bytes) return DR_OK;
+    DR_HIP_TRY(hipSetDevice(reinterpret_cast<Context *>(c)->device));
+    const size_t n = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
+    k_zero16<<<grid, 256, 0, reinterpret_cast<hipStream_t>(stream)>>>(static_cast<uint4 *>(dst), n);
+    DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }
 
@@ -329,6 +350,21 @@ dr_status dr_render(dr_scene *sc, const dr_camera *camera, const dr_render_param
     auto *s = reinterpret_cast<Scene *>(sc);
     DR_HIP_TRY(hipSetDevice(s->ctx->device));
     return render(s, camera, params, nullptr, 0u, accum, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_scene_ray_casts(dr_scene *sc, uint64_t *casts, int32_t reset, dr_stream stream)
+{
+    if (!sc || !casts) return fail(DR_E_INVALID, "null argument");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    DR_HIP_TRY(hipSetDevice(s->ctx->device));
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    DR_HIP_TRY(hipStreamSynchronize(st));
+    *casts = 0;
+    if (!s->ray_casts.ptr) return DR_OK;
+    DR_HIP_TRY(hipMemcpy(casts, s->ray_casts.ptr, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (reset)
+        if (dr_status e = count_ray_casts(s, 0, st, true)) return e;
+    return DR_OK;
 }
 
 dr_status dr_render_guided(dr_scene *sc, const dr_camera *camera, const dr_render_params *params, dr_guide *guide,

@@ -203,6 +203,7 @@ dr_status dr_render_debug(dr_scene *sc, const dr_camera *camera, const dr_render
     if (dr_status e = s->dbg_rays.ensure(sizeof(dr_ray) * (size_t)npix)) return e;
     if (dr_status e = s->dbg_hits.ensure(sizeof(dr_hit) * (size_t)npix)) return e;
     const CamParams cam = cam_params(*camera);
+    if (dr_status e = count_ray_casts(s, (uint64_t)npix * params->spp, st)) return e;
     for (uint32_t k = 0; k < params->spp; ++k) {
         k_debug_rays<<<blocks(npix), kT, 0, st>>>(cam, params->sample_offset + k, params->seed,
                                                   s->dbg_rays.as<dr_ray>());

@@ -116,6 +116,9 @@ struct Scene {
     DevBuf rs_view, rs_root, rs_sroot, rs_chunk, rs_views;  // per-path view / BVH root, chunk cursor, view table
     DevBuf rs_guide_head;   // DR_GUIDE_HEAD_FLOATS floats per path (guided renders)
     DevBuf rs_pending;      // u32 per path: training sample awaiting its radiance snapshot
+    // u64: extension rays (camera and bounce rays, not shadow rays) traced by
+    // the renders of this scene since it was created or the count was reset
+    DevBuf ray_casts;
     uint32_t rs_capacity = 0;
     // The recorded bounce loop of the last graph render and an event after its
     // last replay: an executable graph is freed only once its launches are
@@ -219,6 +222,9 @@ struct RenderJob {
 CamParams cam_params(const dr_camera &c);
 dr_status check_render_args(const dr_camera *cam, const dr_render_params *prm, const double *accum);
 dr_status render_job(const RenderJob &j, hipStream_t stream);
+// allocates and zeroes the scene's ray-cast counter on first use, then adds
+// n (a count known on the host), or sets the count to n when `set`
+dr_status count_ray_casts(Scene *s, uint64_t n, hipStream_t stream, bool set = false);
 
 // light.hip
 dr_status build_lights(Scene *s, hipStream_t stream);

@@ -118,10 +118,19 @@ __global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict
     if (pending) pending[p] = kGuideNoSample;
 }
 
-__global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count)
+// also adds this vertex's extension rays (the input queue) to the scene's
+// ray-cast count
+__global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
+                               unsigned long long *casts)
 {
     *out_count = 0u;
     *shadow_count = 0u;
+    *casts += *in_count;
+}
+
+__global__ void k_add_casts(unsigned long long *casts, unsigned long long n, int zero)
+{
+    *casts = (zero ? 0ull : *casts) + n;
 }
 
 struct SceneView {
@@ -458,7 +467,7 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
                                                  stream))
                 return st;
         gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
-        k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2);
+        k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2, cin, s->ray_casts.as<unsigned long long>());
         k_shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
                                                 cnt + 2, gv, pview, sroot);
         DR_HIP_TRY(hipGetLastError());
@@ -478,6 +487,18 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
 }
 
 }  // namespace
+
+dr_status count_ray_casts(Scene *s, uint64_t n, hipStream_t stream, bool set)
+{
+    const bool fresh = s->ray_casts.ptr == nullptr;
+    if (fresh)
+        if (dr_status st = s->ray_casts.ensure(sizeof(unsigned long long))) return st;
+    if (fresh || set || n) {
+        k_add_casts<<<1, 1, 0, stream>>>(s->ray_casts.as<unsigned long long>(), n, (fresh || set) ? 1 : 0);
+        DR_HIP_TRY(hipGetLastError());
+    }
+    return DR_OK;
+}
 
 CamParams cam_params(const dr_camera &c)
 {
@@ -530,6 +551,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         if (st) return st;
         s->rs_capacity = cap;
     }
+    if (dr_status st = count_ray_casts(s, 0, stream)) return st;
     if (j.scenes) {
         if (dr_status st = s->rs_root.ensure(4ull * s->rs_capacity)) return st;
         if (dr_status st = s->rs_sroot.ensure(4ull * s->rs_capacity)) return st;

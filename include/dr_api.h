@@ -130,6 +130,10 @@ dr_status dr_memcpy_dtoh(dr_context *ctx, void *dst, const void *src, size_t byt
  * kernel against which memory-traffic counters are calibrated). */
 dr_status dr_memcpy_dtod(dr_context *ctx, void *dst, const void *src, size_t bytes, dr_stream stream);
 dr_status dr_synchronize(dr_context *ctx);
+/* Zero bytes of device memory with a kernel (not a memset node, so callers
+ * may record it into a graph), asynchronous on `stream`; dst and bytes must
+ * be multiples of 16. */
+dr_status dr_fill_zero(dr_context *ctx, void *dst, size_t bytes, dr_stream stream);
 
 /* Scene upload. Replaces convertMeshToTris / viewGpuTris
  * (src/GpuTris.cpp:5-85) and Scene (include/tracing/Scene.hpp:15-26). */
@@ -245,6 +249,14 @@ dr_status dr_scene_copy_lights(dr_scene *scene, void *entries, float *pdf_area);
  * wavefront kernels (camera, extend, shade, shadow, resolve) on one stream. */
 dr_status dr_render(dr_scene *scene, const dr_camera *camera, const dr_render_params *params, double *accum,
                     dr_stream stream);
+/* Extension rays (camera and bounce rays; shadow rays are not counted, as in
+ * the reference's SampleStats::ray_casts, PathGeneration.hpp:88-111) traced
+ * by dr_render, dr_render_guided and dr_render_debug on this scene since it
+ * was created or last reset. Replaces the per-pixel casts buffer behind
+ * Renderer::getTotalCasts (src/Renderer.cpp:26-33, include/Renderer.hpp:51).
+ * Synchronises `stream` before reading; reset != 0 then zeroes the count,
+ * ordered on `stream` before later work. */
+dr_status dr_scene_ray_casts(dr_scene *scene, uint64_t *casts, int32_t reset, dr_stream stream);
 
 /* ---- display and debug views --------------------------------------------- */
 
