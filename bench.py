@@ -29,6 +29,7 @@ bounded sample of the same rays on the host cores.
 
 --probe: runs only the traversal launches and a 256 MiB device copy (the
 known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
+--probe-guide: ten guide training steps on 2^18 samples; for rocprofv3.
 """
 from __future__ import annotations
 
@@ -179,11 +180,9 @@ def display_rate(ctx, sc):
 GUIDE_FLOP_PER_SAMPLE = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 48) * 2 + 2 * (3 * 64 * 64 + 48 * 64)
 
 
-def guide_train_rate(ctx):
-    """One Adam step (fused forward + backward + reduction + update) on 2^18
-    samples: mean HIP-event time and algorithmic TFLOP/s against the BF16
-    MFMA peak."""
-    n = 1 << 18
+def guide_samples(n):
+    """Synthetic training samples: positions in [-1, 1]^3, unit normals,
+    wi in the normal's hemisphere, exponential weights (seed 0)."""
     rng = np.random.default_rng(0)
     smp = np.zeros(n, dr.GUIDE_SAMPLE_DTYPE)
     smp["pos"] = rng.uniform(-1, 1, (n, 3))
@@ -194,6 +193,15 @@ def guide_train_rate(ctx):
     wi *= np.sign(np.sum(wi * nrm, 1))[:, None]
     smp["normal"], smp["wo"], smp["wi"] = nrm, nrm, wi
     smp["weight"] = rng.exponential(1.0, n)
+    return smp
+
+
+def guide_train_rate(ctx):
+    """One Adam step (fused forward + backward + reduction + update) on 2^18
+    samples: mean HIP-event time and algorithmic TFLOP/s against the BF16
+    MFMA peak."""
+    n = 1 << 18
+    smp = guide_samples(n)
     g = dr.Guide(ctx, [-1, -1, -1, 1, 1, 1], capacity=n, seed=0)
     d = ctx.upload(smp)
     _, per = timed_launches(lambda: g.train_step_dev(d.ptr, n), 20, 3, ctx)
@@ -381,6 +389,21 @@ def probe() -> int:
     return 0
 
 
+def probe_guide() -> int:
+    """Ten guide training steps on 2^18 synthetic samples, for rocprofv3
+    counter runs on k_guide_train."""
+    n = 1 << 18
+    ctx = dr.Context(0)
+    g = dr.Guide(ctx, [-1, -1, -1, 1, 1, 1], capacity=n, seed=0)
+    d = ctx.upload(guide_samples(n))
+    for _ in range(10):
+        g.train_step_dev(d.ptr, n)
+    ctx.synchronize()
+    g.close()
+    print("probe done", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,6 +416,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--probe-guide", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU figures are not produced",
@@ -400,6 +424,8 @@ def main() -> int:
         return 2
     if args.probe:
         return probe()
+    if args.probe_guide:
+        return probe_guide()
 
     log("[bench] generating scene and rays")
     v, f = dr.gen_icosphere(SUBDIV, SIGMA, SEED)

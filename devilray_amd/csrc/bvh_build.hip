@@ -58,14 +58,47 @@ __device__ __forceinline__ int32_t f2ord(float f)
 }
 __device__ __forceinline__ float ord2f(int32_t i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
 
+// Block-wide min (c) and max (C) of a centroid box, then one atomic per
+// component per block: same-address atomics serialise at the L2, so one per
+// wave over a million triangles cost milliseconds. Every thread of the block
+// must call it.
+__device__ __forceinline__ void block_bounds_atomic(float (&c)[3], float (&C)[3], int32_t *__restrict__ bounds)
+{
+    __shared__ float red[6][kT / 64];
+    for (int a = 0; a < 3; ++a) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            c[a] = fminf(c[a], __shfl_xor(c[a], off, 64));
+            C[a] = fmaxf(C[a], __shfl_xor(C[a], off, 64));
+        }
+    }
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            red[a][w] = c[a];
+            red[3 + a][w] = C[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = (int)threadIdx.x;
+        float v = red[a][0];
+        for (int k = 1; k < kT / 64; ++k) v = a < 3 ? fminf(v, red[a][k]) : fmaxf(v, red[a][k]);
+        if (a < 3) atomicMin(&bounds[a], f2ord(v));
+        else atomicMax(&bounds[a], f2ord(v));
+    }
+}
+
+// grid of the bounds kernels: a few blocks per CU, each thread striding over
+// the triangles
+constexpr uint32_t kBoundsBlocks = 2048;
+
 __global__ __launch_bounds__(kT) void k_tri_bounds(const float *__restrict__ verts,
                                                    const uint32_t *__restrict__ idx, uint32_t n,
                                                    float4 *__restrict__ lo, float4 *__restrict__ hi,
                                                    int32_t *__restrict__ cbounds)
 {
-    const uint32_t i = blockIdx.x * kT + threadIdx.x;
     float c[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, C[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
         float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
         for (int k = 0; k < 3; ++k) {
             const uint32_t vi = idx[3 * (size_t)i + k];
@@ -87,22 +120,11 @@ __global__ __launch_bounds__(kT) void k_tri_bounds(const float *__restrict__ ver
         if (finite)
             for (int a = 0; a < 3; ++a) {
                 const float m = 0.5f * (l[a] + h[a]);
-                c[a] = m; C[a] = m;
+                c[a] = fminf(c[a], m);
+                C[a] = fmaxf(C[a], m);
             }
     }
-    // wave reduction then one atomic per wave
-    for (int a = 0; a < 3; ++a) {
-        for (int off = 32; off >= 1; off >>= 1) {
-            c[a] = fminf(c[a], __shfl_xor(c[a], off, 64));
-            C[a] = fmaxf(C[a], __shfl_xor(C[a], off, 64));
-        }
-    }
-    if ((threadIdx.x & 63u) == 0) {
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&cbounds[a], f2ord(c[a]));
-            atomicMax(&cbounds[3 + a], f2ord(C[a]));
-        }
-    }
+    block_bounds_atomic(c, C, cbounds);
 }
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v)
@@ -450,9 +472,8 @@ __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, c
                                               float4 *__restrict__ rhi, uint32_t *__restrict__ rtri,
                                               int32_t *__restrict__ rbounds)
 {
-    const uint32_t i = blockIdx.x * kT + threadIdx.x;
     float c[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, C[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
         const uint32_t base = offs[i], k = offs[i + 1] - base;
         const float4 L = tlo[i], H = thi[i];
         const float tl[3] = {L.x, L.y, L.z}, th[3] = {H.x, H.y, H.z};
@@ -521,18 +542,7 @@ __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, c
             }
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        for (int off = 32; off >= 1; off >>= 1) {
-            c[a] = fminf(c[a], __shfl_xor(c[a], off, 64));
-            C[a] = fmaxf(C[a], __shfl_xor(C[a], off, 64));
-        }
-    }
-    if ((threadIdx.x & 63u) == 0) {
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&rbounds[a], f2ord(c[a]));
-            atomicMax(&rbounds[3 + a], f2ord(C[a]));
-        }
-    }
+    block_bounds_atomic(c, C, rbounds);
 }
 
 // ---- PLOC binary tree (Meister & Bittner, "Parallel Locally-Ordered
@@ -814,7 +824,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     int32_t *cbounds = reinterpret_cast<int32_t *>(counters + 8);
     int32_t *rbounds = reinterpret_cast<int32_t *>(counters + 16);
     k_init_counters<<<1, 64, 0, stream>>>(counters, cbounds, rbounds);
-    k_tri_bounds<<<blocks(nt), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
+    k_tri_bounds<<<std::min(blocks(nt), kBoundsBlocks), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt,
                                                 t.tri_lo.as<float4>(), t.tri_hi.as<float4>(), cbounds);
     DR_HIP_TRY(hipGetLastError());
 
@@ -850,7 +860,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         need(t.ref_hi, sizeof(float4) * n);
         need(t.ref_tri, sizeof(uint32_t) * n);
         if (st) return st;
-        k_split<<<blocks(nt), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt, t.tri_lo.as<float4>(),
+        k_split<<<std::min(blocks(nt), kBoundsBlocks), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), nt, t.tri_lo.as<float4>(),
                                        t.tri_hi.as<float4>(), t.offs.as<uint32_t>(), cbounds, t.ref_lo.as<float4>(),
                                        t.ref_hi.as<float4>(), t.ref_tri.as<uint32_t>(), rbounds);
         DR_HIP_TRY(hipGetLastError());
