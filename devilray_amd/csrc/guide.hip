@@ -96,7 +96,7 @@ __host__ __device__ inline int param_layer(int p, int &o, int &c)
 struct Net {
     const bf16x8 *ff;      // forward fragments
     const bf16x8 *fb;      // transposed fragments
-    const float *master;   // FP32 parameters (biases are read from here)
+    const float *bias[5];  // FP32 biases of each layer
 };
 
 struct Box {
@@ -108,9 +108,9 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ f32x4 bias4(const float *__restrict__ master, int l, int ob, int g)
+__device__ __forceinline__ f32x4 bias4(const Net &net, int l, int ob, int g)
 {
-    const float *b = master + b_off(l) + ob * 16 + 4 * g;
+    const float *b = net.bias[l] + ob * 16 + 4 * g;
     return f32x4{b[0], b[1], b[2], b[3]};
 }
 
@@ -164,14 +164,14 @@ __device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0,
     const int g = lane >> 4;
     f32x4 acc[4];
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff[(f_base(0) + ob) * 64 + lane], x0, bias4(net.master, 0, ob, g));
+    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff[(f_base(0) + ob) * 64 + lane], x0, bias4(net, 0, ob, g));
     xh[0][0] = pack_relu(acc[0], acc[1]);
     xh[0][1] = pack_relu(acc[2], acc[3]);
 #pragma unroll
     for (int l = 1; l < 4; ++l) {
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) {
-            acc[ob] = bias4(net.master, l, ob, g);
+            acc[ob] = bias4(net, l, ob, g);
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks) acc[ob] = mfma(net.ff[(f_base(l) + ob * 2 + ks) * 64 + lane], xh[l - 1][ks], acc[ob]);
         }
@@ -180,7 +180,7 @@ __device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0,
     }
 #pragma unroll
     for (int ob = 0; ob < 3; ++ob) {
-        y[ob] = bias4(net.master, 4, ob, g);
+        y[ob] = bias4(net, 4, ob, g);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) y[ob] = mfma(net.ff[(f_base(4) + ob * 2 + ks) * 64 + lane], xh[3][ks], y[ob]);
     }
@@ -454,11 +454,24 @@ __device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&d
     dB[1] = nb[1];
 }
 
-__global__ __launch_bounds__(kT) void k_guide_train(Net net, Box box, TrainIn in)
+__global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn in)
 {
     __shared__ __attribute__((aligned(16))) __bf16 X[kXRows * kLds];
     __shared__ __attribute__((aligned(16))) __bf16 D[2][64 * kLds];
     __shared__ float red[kT / 64];
+    // the weights stay in LDS for the whole kernel: every tile reads all of
+    // them, and from L2 each of those reads was an exposed latency
+    __shared__ bf16x8 Wf[kFragF * 64];
+    __shared__ bf16x8 Wb[kFragB * 64];
+    __shared__ float Bs[5][64];
+    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff[k];
+    for (int k = threadIdx.x; k < kFragB * 64; k += kT) Wb[k] = gnet.fb[k];
+    for (int k = threadIdx.x; k < 5 * 64; k += kT) {
+        const int l = k / 64, o = k % 64;
+        Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
+    }
+    __syncthreads();
+    const Net net{Wf, Wb, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 15, g = lane >> 4;
     const int col = wave * 16 + s;
@@ -712,7 +725,9 @@ inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 Net net_of(const Guide *g)
 {
-    return Net{g->frag_f.as<const bf16x8>(), g->frag_b.as<const bf16x8>(), g->master.as<const float>()};
+    const float *m = g->master.as<const float>();
+    return Net{g->frag_f.as<const bf16x8>(), g->frag_b.as<const bf16x8>(),
+               {m + b_off(0), m + b_off(1), m + b_off(2), m + b_off(3), m + b_off(4)}};
 }
 
 Box box_of(const Guide *g)
