@@ -275,13 +275,17 @@ __global__ __launch_bounds__(kT) void k_collapse(
     WideNode *__restrict__ wide, uint32_t wide_cap)
 {
     const uint32_t qi = blockIdx.x * kT + threadIdx.x;
-    if (qi >= nin) return;
-    const int2 e = qin[qi];
-    int32_t kids[4];
-    int nk = 2;
-    kids[0] = cl[e.x];
-    kids[1] = cr[e.x];
-    while (nk < 4) {
+    const bool valid = qi < nin;  // every lane stays for the wave-wide allocation below
+    int2 e = make_int2(0, 0);
+    int32_t kids[4] = {0, 0, 0, 0};
+    int nk = 0;
+    if (valid) {
+        e = qin[qi];
+        kids[0] = cl[e.x];
+        kids[1] = cr[e.x];
+        nk = 2;
+    }
+    while (valid && nk < 4) {
         int best = -1;
         float best_a = -1.0f;
         for (int c = 0; c < nk; ++c) {
@@ -295,6 +299,36 @@ __global__ __launch_bounds__(kT) void k_collapse(
         kids[best] = cl[k];
         kids[nk++] = cr[k];
     }
+    // Wide-node slots and queue entries for the interior children, leaf
+    // counts: one atomic per wave and counter (a prefix sum over the lanes
+    // hands out the slots); per-child atomics on the same three counters
+    // serialised at the L2 and made the widest levels take milliseconds.
+    uint32_t n_int = 0, n_leaf = 0;
+    for (int c = 0; c < nk; ++c) {
+        const int32_t k = kids[c];
+        if (k < 0 || collapse[k]) ++n_leaf;
+        else ++n_int;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = n_int;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t leaves = n_leaf;
+    for (int off = 32; off >= 1; off >>= 1) leaves += __shfl_xor(leaves, off, 64);
+    uint32_t bw = 0, bq = 0;
+    if (lane == 0) {
+        if (total) {
+            bw = atomicAdd(&counters[0], total);
+            bq = atomicAdd(&counters[1], total);
+        }
+        if (leaves) atomicAdd(&counters[2], leaves);
+    }
+    uint32_t w = __shfl(bw, 0, 64) + incl - n_int;
+    uint32_t q = __shfl(bq, 0, 64) + incl - n_int;
+    if (!valid) return;
     WideNode wn;
     for (int c = 0; c < 4; ++c) {
         if (c >= nk) {
@@ -308,22 +342,20 @@ __global__ __launch_bounds__(kT) void k_collapse(
         if (k < 0) {
             lo = llo[~k]; hi = lhi[~k];
             ref = make_leaf_ref((uint32_t)~k, 1u);
-            atomicAdd(&counters[2], 1u);
         } else {
             lo = node_lo[k]; hi = node_hi[k];
             if (collapse[k]) {
                 ref = make_leaf_ref(rfirst[k], cnt[k]);
-                atomicAdd(&counters[2], 1u);
             } else {
-                const uint32_t w = atomicAdd(&counters[0], 1u);
                 if (w >= wide_cap) {  // cannot happen by the bound above; flag instead of writing
                     atomicOr(&counters[3], 1u);
                     ref = DR_EMPTY_REF;
                 } else {
-                    const uint32_t q = atomicAdd(&counters[1], 1u);
                     qout[q] = make_int2(k, (int)w);
                     ref = (int32_t)w;
                 }
+                ++w;
+                ++q;
             }
         }
         wn.lox[c] = down(lo.x); wn.loy[c] = down(lo.y); wn.loz[c] = down(lo.z);
