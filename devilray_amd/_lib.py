@@ -89,7 +89,7 @@ DR_GUIDE_USE, DR_GUIDE_COLLECT = 1, 2
 DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB = 0, 1
 DR_DEBUG_UV_CHECKER, DR_DEBUG_BARYCENTRIC, DR_DEBUG_WINDING = 1, 2, 3
 GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
-                               ("pad0", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
+                               ("pdf", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
                                ("wi", np.float32, 3), ("pad2", np.float32)])
 
 ALIAS_DTYPE = np.dtype([("p_a", np.float32), ("pdf_a", np.float32), ("pdf_b", np.float32),

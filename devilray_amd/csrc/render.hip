@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
         if (k < gv.cap) {
             dr_guide_sample q;
             q.pos[0] = p.x; q.pos[1] = p.y; q.pos[2] = p.z; q.weight = 0.0f;
-            q.normal[0] = n.x; q.normal[1] = n.y; q.normal[2] = n.z; q.pad0 = 0.0f;
+            q.normal[0] = n.x; q.normal[1] = n.y; q.normal[2] = n.z; q.pdf = pdf;
             q.wo[0] = -d.x; q.wo[1] = -d.y; q.wo[2] = -d.z; q.pad1 = 0.0f;
             q.wi[0] = nd.x; q.wi[1] = nd.y; q.wi[2] = nd.z; q.pad2 = 0.0f;
             gv.smp[k] = q;

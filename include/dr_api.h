@@ -362,10 +362,11 @@ typedef struct {
 /* One training sample (64 B): a Lambert vertex at pos with shading normal,
  * outgoing direction wo (towards the previous vertex) and sampled incident
  * direction wi; weight = luminance(L_i(wi)) * max(0, n . wi) / pdf(wi), the
- * one-sample estimate of the target product over the pdf it was drawn from. */
+ * one-sample estimate of the target product over the pdf it was drawn from,
+ * and pdf that density (the training loss does not read it). */
 typedef struct {
     float pos[3], weight;
-    float normal[3], pad0;
+    float normal[3], pdf;
     float wo[3], pad1;
     float wi[3], pad2;
 } dr_guide_sample;

@@ -295,7 +295,7 @@ class RefBBH:
 # ---- learned path guide (FP64) ---------------------------------------------
 GUIDE_N_PARAMS = 17712
 GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
-                               ("pad0", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
+                               ("pdf", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
                                ("wi", np.float32, 3), ("pad2", np.float32)])
 
 

@@ -146,7 +146,7 @@ void or_debug_aov(const void *bvh, const float *verts, const int32_t *tris, cons
 #define OR_GUIDE_NP 17712
 typedef struct {
     float pos[3], weight;
-    float normal[3], pad0;
+    float normal[3], pdf;
     float wo[3], pad1;
     float wi[3], pad2;
 } or_guide_sample; /* = dr_guide_sample */

@@ -220,6 +220,28 @@ def _cornell(gpu_ctx, res=48):
     return sc, cam
 
 
+def test_samples_record_their_pdf(gpu_ctx):
+    """Each staged sample carries the density its direction was drawn from:
+    cos/pi for BSDF sampling (no guide in use), and alpha q + (1 - alpha) cos/pi
+    >= 0.1 cos/pi once the guide samples (alpha <= 0.9)."""
+    print("[case] sample pdf", flush=True)
+    sc, cam = _cornell(gpu_ctx, res=32)
+    g = dr.Guide(gpu_ctx, sc.bounds(), capacity=1 << 14, seed=5)
+    sc.render_guided(cam, g, dr.DR_GUIDE_COLLECT, spp=4, seed=1)
+    s, _ = g.copy_samples()
+    assert len(s) > 100
+    b = np.maximum(np.sum(s["normal"].astype(np.float64) * s["wi"], 1), 0.0) / np.pi
+    np.testing.assert_allclose(s["pdf"], b, rtol=1e-5, atol=1e-6)
+    g.train_step()
+    sc.render_guided(cam, g, dr.DR_GUIDE_USE | dr.DR_GUIDE_COLLECT, spp=4, seed=2)
+    s, _ = g.copy_samples()
+    b = np.maximum(np.sum(s["normal"].astype(np.float64) * s["wi"], 1), 0.0) / np.pi
+    assert len(s) > 100 and np.all(s["pdf"] > 0.0)
+    assert np.all(s["pdf"] >= 0.1 * b * (1.0 - 1e-4) - 1e-7)
+    g.close()
+    sc.close()
+
+
 def test_collection(gpu_ctx):
     sc, cam = _cornell(gpu_ctx)
     g = dr.Guide(gpu_ctx, sc.bounds(), capacity=1 << 13, seed=1)
