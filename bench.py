@@ -30,6 +30,8 @@ bounded sample of the same rays on the host cores.
 --probe: runs only the traversal launches and a 256 MiB device copy (the
 known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
 --probe-guide: ten guide training steps on 2^18 samples; for rocprofv3.
+--probe-render: three passes of the many-lights render and the calibration
+copy; for rocprofv3.
 """
 from __future__ import annotations
 
@@ -389,6 +391,29 @@ def probe() -> int:
     return 0
 
 
+def probe_render() -> int:
+    """Three 2-spp passes of the 2,048,000-triangle many-lights scene at
+    1920x1080 (max depth 12, roulette from 3) and the 256 MiB calibration
+    copy, for rocprofv3 traffic runs on the render kernels."""
+    sc = scenes.many_lights()
+    ctx = dr.Context(0)
+    s = dr.Scene(ctx, sc.vertices, sc.indices, sc.tri_material, sc.dr_materials())
+    s.build_bvh(leaf_max=4)
+    c = sc.camera
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"])
+    acc = ctx.alloc(c["width"] * c["height"] * 32)
+    ctx.fill_zero(acc)
+    for k in range(3):
+        s.render_dev(cam, dr.DrRenderParams(2, 2 * k, 12, 3, 0, 0, 0), acc.ptr)
+    ctx.synchronize()
+    a, b = ctx.alloc(256 << 20), ctx.alloc(256 << 20)
+    dr.check(dr.lib().dr_memcpy_dtod(ctx.handle, dr.C.c_void_p(b.ptr), dr.C.c_void_p(a.ptr), 256 << 20, None))
+    ctx.synchronize()
+    s.close()
+    print("probe done", flush=True)
+    return 0
+
+
 def probe_guide() -> int:
     """Ten guide training steps on 2^18 synthetic samples, for rocprofv3
     counter runs on k_guide_train."""
@@ -417,6 +442,7 @@ def main() -> int:
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--probe-guide", action="store_true")
+    ap.add_argument("--probe-render", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU figures are not produced",
@@ -426,6 +452,8 @@ def main() -> int:
         return probe()
     if args.probe_guide:
         return probe_guide()
+    if args.probe_render:
+        return probe_render()
 
     log("[bench] generating scene and rays")
     v, f = dr.gen_icosphere(SUBDIV, SIGMA, SEED)
