@@ -182,6 +182,44 @@ def display_rate(ctx, sc):
 GUIDE_FLOP_PER_SAMPLE = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 48) * 2 + 2 * (3 * 64 * 64 + 48 * 64)
 
 
+def refit_rate(ctx, v, f, args):
+    """dr_scene_refit of the traversal scene's tree (wall time of the call,
+    vertex upload included) and incoherent closest-hit over the refitted tree
+    against a tree built for the new vertices, for two moves of the mesh: an
+    affine one (scale 1.25, 0.8, 1.1 and a shift), which keeps every box's
+    shape, and the same icosphere displaced with another seed, which moves
+    each vertex by about twelve triangle widths."""
+    out = {}
+    v_aff = (v * np.float32([1.25, 0.8, 1.1]) + np.float32([0.3, -0.2, 0.5])).astype(np.float32)
+    v_new, _ = dr.gen_icosphere(SUBDIV, SIGMA, SEED + 1)
+    d_hits = ctx.alloc(NRAYS * 16)
+    for name, v2 in (("affine", v_aff), ("redisplaced", v_new)):
+        d_rays = ctx.upload(dr.gen_rays_surface(v2, f, NRAYS, seed=SEED + 1, inward=True, offset=1e-5))
+        s = dr.Scene(ctx, v, f)
+        s.build_bvh(leaf_max=args.leaf_max)
+        times = []
+        for k in range(3):
+            t0 = time.perf_counter()
+            s.refit(v2 if k % 2 == 0 else v)
+            times.append((time.perf_counter() - t0) * 1e3)
+        s.refit(v2)
+        wall, _ = timed_launches(lambda: s.trace_closest_dev(d_rays.ptr, NRAYS, d_hits.ptr), args.steps,
+                                 args.warmup, ctx)
+        r = {"refit_ms": round(float(np.median(times)), 3), "refitted_mrays": round(NRAYS * args.steps / wall / 1e6, 2)}
+        s.close()
+        s = dr.Scene(ctx, v2, f)
+        s.build_bvh(leaf_max=args.leaf_max)
+        wall, _ = timed_launches(lambda: s.trace_closest_dev(d_rays.ptr, NRAYS, d_hits.ptr), args.steps,
+                                 args.warmup, ctx)
+        r["rebuilt_mrays"] = round(NRAYS * args.steps / wall / 1e6, 2)
+        r["rebuild_ms"] = round(s.info().build_ms, 3)
+        s.close()
+        d_rays.free()
+        out[name] = r
+    d_hits.free()
+    return out
+
+
 def guide_samples(n):
     """Synthetic training samples: positions in [-1, 1]^3, unit normals,
     wi in the normal's hemisphere, exponential weights (seed 0)."""
@@ -487,6 +525,9 @@ def main() -> int:
     hits = d_hits.to_numpy(dr.HIT_DTYPE, NRAYS)
     info2 = scene.info()
 
+    log("[bench] BVH refit to a new displacement of the same mesh")
+    refit = refit_rate(ctx, v, f, args)
+
     roofline = None
     counts = None
     if os.path.exists(COUNTS_FIXTURE):
@@ -571,6 +612,7 @@ def main() -> int:
             "bvh_nodes": info.n_nodes, "bvh_leaves": info.n_leaves, "bvh_depth": info.depth,
             "bvh_references": info.n_references,
             "bvh_sah_cost": round(info.sah_cost, 3),
+            "bvh_refit": refit,
             "hit_fraction": round(float(np.mean(hits["id"] >= 0)), 6),
             "occluded_fraction": round(float(np.mean(occ)), 6),
             "stack_overflow": int(info2.overflow_flag),

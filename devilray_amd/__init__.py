@@ -231,6 +231,14 @@ class Scene:
         p = DrBvhParams(leaf_max, flags)
         check(lib().dr_scene_build_bvh(self.handle, C.byref(p), stream))
 
+    def refit(self, vertices, stream=None):
+        """Move the vertices (same count and triangles) and refit the built BVH to them."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        if v.shape != self._v.shape:
+            raise ValueError(f"refit needs {self._v.shape[0]} vertices, got {v.shape[0]}")
+        check(lib().dr_scene_refit(self.handle, ptr(v), stream))
+        self._v = v
+
     def set_schedule(self, closest: int, any_hit: int | None = None):
         check(lib().dr_scene_set_trace_schedule(self.handle, closest, closest if any_hit is None else any_hit))
 

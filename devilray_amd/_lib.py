@@ -121,6 +121,7 @@ PROTOTYPES = {
     "dr_scene_destroy": (_st, [_vp]),
     "dr_scene_get_info": (_st, [_vp, C.POINTER(DrSceneInfo)]),
     "dr_scene_build_bvh": (_st, [_vp, C.POINTER(DrBvhParams), _vp]),
+    "dr_scene_refit": (_st, [_vp, _vp, _vp]),
     "dr_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),

@@ -198,7 +198,8 @@ Context::~Context()
 }
 
 Scene::Scene(Scene &&o) noexcept
-    : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_)
+    : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_),
+      n_points_(o.n_points_)
 {
     o.scene_ = nullptr;
 }
@@ -211,6 +212,7 @@ Scene &Scene::operator=(Scene &&o) noexcept
         objects = std::move(o.objects);
         scene_ = o.scene_;
         n_tris_ = o.n_tris_;
+        n_points_ = o.n_points_;
         o.scene_ = nullptr;
     }
     return *this;
@@ -221,19 +223,42 @@ Scene::~Scene()
     if (scene_) dr_scene_destroy(scene_);
 }
 
-void Scene::build(Context &ctx, uint32_t leaf_max)
+// every object's points placed by its transform, p * s + position, in object order
+std::vector<float> Scene::worldPoints() const
 {
     std::vector<float> verts;
-    std::vector<uint32_t> idx, tri_mat;
     for (const Object &o : objects) {
-        if (o.material >= materials.size()) throw Error(DR_E_INVALID, "object material index out of range");
-        const uint32_t base = (uint32_t)(verts.size() / 3);
         const Transform &tf = o.transform;
         for (const Vec3 &p : o.mesh.points) {
             verts.push_back(p.x * tf.s.x + tf.p.x);
             verts.push_back(p.y * tf.s.y + tf.p.y);
             verts.push_back(p.z * tf.s.z + tf.p.z);
         }
+    }
+    return verts;
+}
+
+void Scene::refit(Context &ctx)
+{
+    if (!scene_) {
+        build(ctx);
+        return;
+    }
+    const std::vector<float> verts = worldPoints();
+    uint32_t tris = 0;
+    for (const Object &o : objects) tris += (uint32_t)o.mesh.triangles.size();
+    if (verts.size() != 3ull * n_points_ || tris != n_tris_)
+        throw Error(DR_E_INVALID, "refit needs the objects and triangles of the last build");
+    check(dr_scene_refit(scene_, verts.data(), nullptr));
+}
+
+void Scene::build(Context &ctx, uint32_t leaf_max)
+{
+    const std::vector<float> verts = worldPoints();
+    std::vector<uint32_t> idx, tri_mat;
+    uint32_t base = 0;
+    for (const Object &o : objects) {
+        if (o.material >= materials.size()) throw Error(DR_E_INVALID, "object material index out of range");
         for (const auto &t : o.mesh.triangles) {
             for (uint32_t k : t) {
                 if (k >= o.mesh.points.size()) throw Error(DR_E_INVALID, "triangle vertex index out of range");
@@ -241,11 +266,12 @@ void Scene::build(Context &ctx, uint32_t leaf_max)
             }
             tri_mat.push_back(o.material);
         }
+        base += (uint32_t)o.mesh.points.size();
     }
     std::vector<dr_material> mats;
     for (const Material &m : materials) mats.push_back(m.m);
     dr_scene_desc d{};
-    d.vertices = verts.data();
+    d.vertices = verts.data();  // (the scene copies them in)
     d.n_vertices = (uint32_t)(verts.size() / 3);
     d.indices = idx.data();
     d.n_triangles = (uint32_t)tri_mat.size();
@@ -260,6 +286,7 @@ void Scene::build(Context &ctx, uint32_t leaf_max)
     dr_bvh_params bp{leaf_max, 0};
     check(dr_scene_build_bvh(scene_, &bp, nullptr));
     n_tris_ = d.n_triangles;
+    n_points_ = d.n_vertices;
 }
 
 namespace {
@@ -363,6 +390,17 @@ void Renderer::setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fr
     rr_depth_ = rr_depth;
     fresnel_ = fresnel;
     seed_ = seed;
+    needs_clear_ = true;
+}
+
+void Renderer::moveObject(size_t index, Vec3 position, Vec3 scale)
+{
+    std::scoped_lock frame{frame_mutex_};
+    std::scoped_lock guard{render_mutex_};
+    if (index >= scene_.objects.size()) throw Error(DR_E_INVALID, "object index out of range");
+    scene_.objects[index].setPosition(position);
+    scene_.objects[index].setScale(scale);
+    scene_.refit(ctx_);
     needs_clear_ = true;
 }
 

@@ -169,12 +169,17 @@ public:
     ~Scene();
 
     void build(Context &ctx, uint32_t leaf_max = 4);
+    // After objects moved (setPosition / setScale) or their points changed,
+    // with the same objects and triangles as at build(): refits the built BVH
+    // to the new world positions instead of rebuilding it (dr_scene_refit).
+    void refit(Context &ctx);
     dr_scene *handle() const { return scene_; }
     uint32_t triangleCount() const { return n_tris_; }
 
 private:
+    std::vector<float> worldPoints() const;
     dr_scene *scene_ = nullptr;
-    uint32_t n_tris_ = 0;
+    uint32_t n_tris_ = 0, n_points_ = 0;
 };
 
 // Closest hit for each ray (std::nullopt on a miss); triangle ids index the
@@ -200,6 +205,10 @@ public:
     // roulette, Schlick Fresnel).
     void setPathOptions(uint32_t max_depth, uint32_t rr_depth, uint32_t fresnel = DR_FRESNEL_EXACT,
                         uint64_t seed = 0);
+    // Moves object `index` of the scene set by setScene (TriangleMesh::
+    // setPosition / setScale on the renderer's scene), refits the BVH and
+    // clears the image.
+    void moveObject(size_t index, Vec3 position, Vec3 scale);
 
     // Adds exactly one sample per pixel, updates the display pixels and waits
     // for the device (as the reference's render(), src/Renderer.cpp:198-233);

@@ -402,6 +402,46 @@ int main(int argc, char **argv)
         EXPECT(b.has_value() && b->triangle >= 12 && std::fabs(b->t - (5.0f - 2.0f * half)) <= 1e-5f);
         EXPECT(!cast1(ctx, s, {5, 0, 5}, {0, 0, -1}).has_value());
     });
+    run("Models.RefitFollowsMovedObjects", [&] {
+        // two unit quads; moving the second by refit (no rebuild) is seen by
+        // cast, and by the renderer's moveObject
+        Scene s;
+        s.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
+        s.objects.push_back({createQuadMesh({0, 0, 0}, {0, 0, 1}, {1, 0, 0}, 1.0f), 0});
+        s.objects.push_back({createQuadMesh({0, 0, 0}, {0, 0, 1}, {1, 0, 0}, 1.0f), 0});
+        s.objects[1].setPosition({3, 0, 0});
+        s.build(ctx);
+        const auto before = cast1(ctx, s, {3, 0, 5}, {0, 0, -1});
+        EXPECT(before.has_value() && before->triangle >= 2);
+        s.objects[1].setPosition({0, 0, -2});  // now 4 wide, 2 behind the first quad
+        s.objects[1].setScale({4, 4, 4});
+        s.refit(ctx);
+        EXPECT(!cast1(ctx, s, {3, 0, 5}, {0, 0, -1}).has_value());
+        const auto behind = cast1(ctx, s, {1.5f, 0, 5}, {0, 0, -1});
+        EXPECT(behind.has_value() && behind->triangle >= 2 && std::fabs(behind->t - 7.0f) <= 1e-5f);
+        const auto front = cast1(ctx, s, {0.2f, 0.2f, 5}, {0, 0, -1});
+        EXPECT(front.has_value() && front->triangle < 2 && std::fabs(front->t - 5.0f) <= 1e-5f);
+        s.objects.pop_back();
+        bool threw = false;
+        try { s.refit(ctx); } catch (const Error &) { threw = true; }
+        EXPECT(threw);
+        // the renderer: move the emitter of the analytic scene out of view,
+        // the next frame is black
+        const Size2i res{32, 32};
+        Renderer renderer(ctx, res);
+        renderer.setCamera(pinhole(res));
+        renderer.setScene(facing_quads(0.55f));
+        renderer.setPathOptions(4, 4);
+        renderer.render(2);
+        const Vec4 *px = renderer.getRawPixels();
+        EXPECT(px[0].x > 0.0f);
+        renderer.moveObject(1, {0, 0, 5000}, {1, 1, 1});
+        renderer.render(2);
+        px = renderer.getRawPixels();
+        double sum = 0;
+        for (int i = 0; i < res.width * res.height; ++i) sum += px[i].x;
+        EXPECT(sum == 0.0 && px[0].w == 2.0f && renderer.getSampleCount() == 2);
+    });
     // ---- test/src/test_running_average.cpp, restated ----
     run("RunningAverageTest.EmptyAverageIsZero", [&] { EXPECT(RunningAverage(5).mean() == 0.0); });
     run("RunningAverageTest.AveragesAddedValues", [&] {

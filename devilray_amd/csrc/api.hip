@@ -268,6 +268,19 @@ dr_status dr_scene_build_bvh(dr_scene *sc, const dr_bvh_params *p, dr_stream str
     return build_bvh(s, p, reinterpret_cast<hipStream_t>(stream));
 }
 
+dr_status dr_scene_refit(dr_scene *sc, const float *vertices, dr_stream stream)
+{
+    if (!sc || !vertices) return fail(DR_E_INVALID, "null argument");
+    auto *s = reinterpret_cast<Scene *>(sc);
+    if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
+    DR_HIP_TRY(hipSetDevice(s->ctx->device));
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    DR_HIP_TRY(hipStreamSynchronize(st));
+    DR_HIP_TRY(hipMemcpy(s->verts.ptr, vertices, sizeof(float) * 3 * (size_t)s->n_verts, hipMemcpyHostToDevice));
+    s->lights_built = false;  // triangle areas changed
+    return refit_bvh(s, st);
+}
+
 dr_status dr_scene_set_trace_schedule(dr_scene *sc, uint32_t closest_sched, uint32_t any_sched)
 {
     if (!sc) return fail(DR_E_INVALID, "scene is null");

@@ -25,6 +25,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 namespace dr {
 
@@ -57,6 +58,14 @@ __device__ __forceinline__ int32_t f2ord(float f)
     return i >= 0 ? i : i ^ 0x7FFFFFFF;
 }
 __device__ __forceinline__ float ord2f(int32_t i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__device__ __forceinline__ int longest_axis(const float (&tl)[3], const float (&th)[3])
+{
+    int ax = 0;
+    for (int a = 1; a < 3; ++a)
+        if (th[a] - tl[a] > th[ax] - tl[ax]) ax = a;
+    return ax;
+}
 
 // Block-wide min (c) and max (C) of a centroid box, then one atomic per
 // component per block: same-address atomics serialise at the L2, so one per
@@ -157,11 +166,14 @@ __global__ __launch_bounds__(kT) void k_morton(const float4 *__restrict__ lo, co
 }
 
 // leaf boxes and leaf-ordered triangle vertices; vals[k] is a reference,
-// whose triangle is ref_tri[vals[k]] (or vals[k] itself without a map)
+// whose triangle is ref_tri[vals[k]] (or vals[k] itself without a map). The w
+// components carry the triangle id and, for a refit, which piece of how many
+// the reference is, with the axis of its slabs in bits 16..17 of the piece
+// (offs: the split's reference offsets per triangle).
 __global__ __launch_bounds__(kT) void k_gather(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                                const uint32_t *__restrict__ vals, uint32_t n,
                                                const float4 *__restrict__ tlo, const float4 *__restrict__ thi,
-                                               const uint32_t *__restrict__ ref_tri,
+                                               const uint32_t *__restrict__ ref_tri, const uint32_t *__restrict__ offs,
                                                float4 *__restrict__ llo, float4 *__restrict__ lhi,
                                                float4 *__restrict__ tri_v)
 {
@@ -169,11 +181,24 @@ __global__ __launch_bounds__(kT) void k_gather(const float *__restrict__ verts, 
     if (k >= n) return;
     const uint32_t r = vals[k];
     const uint32_t id = ref_tri ? ref_tri[r] : r;
+    const uint32_t pieces = (ref_tri && offs) ? offs[id + 1] - offs[id] : 1u;
+    uint32_t piece = 0;
+    if (pieces > 1) {  // piece index and the axis its slabs cut (the split's longest axis)
+        float tl[3] = {INFINITY, INFINITY, INFINITY}, th[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t vi = idx[3 * (size_t)id + j];
+            for (int a = 0; a < 3; ++a) {
+                tl[a] = fminf(tl[a], verts[3 * (size_t)vi + a]);
+                th[a] = fmaxf(th[a], verts[3 * (size_t)vi + a]);
+            }
+        }
+        piece = (r - offs[id]) | ((uint32_t)longest_axis(tl, th) << 16);
+    }
     llo[k] = tlo[r];
     lhi[k] = thi[r];
     for (int j = 0; j < 3; ++j) {
         const uint32_t vi = idx[3 * (size_t)id + j];
-        const float w = j == 0 ? __int_as_float((int32_t)id) : 0.0f;
+        const float w = __int_as_float((int32_t)(j == 0 ? id : (j == 1 ? piece : pieces)));
         tri_v[3 * (size_t)k + j] = make_float4(verts[3 * (size_t)vi], verts[3 * (size_t)vi + 1],
                                                verts[3 * (size_t)vi + 2], w);
     }
@@ -497,6 +522,43 @@ __device__ __forceinline__ float slab_cut(float lo, float hi, uint32_t j, uint32
     return j == 0 ? lo : (j == k ? hi : lo + (hi - lo) * ((float)j / (float)k));
 }
 
+// Box of piece j of k of a triangle (vertices v in FP64, box [tl, th]): slab
+// j across axis ax, as described above.
+__device__ void piece_box(const double (&v)[3][3], const float (&tl)[3], const float (&th)[3], int ax, uint32_t j,
+                          uint32_t k, float margin, float (&plo)[3], float (&phi)[3])
+{
+    const double s0 = slab_cut(tl[ax], th[ax], j, k), s1 = slab_cut(tl[ax], th[ax], j + 1, k);
+    double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto add = [&](const double *p) {
+        for (int a = 0; a < 3; ++a) { l[a] = fmin(l[a], p[a]); h[a] = fmax(h[a], p[a]); }
+    };
+    for (int q = 0; q < 3; ++q)
+        if (v[q][ax] >= s0 && v[q][ax] <= s1) add(v[q]);
+    for (int q = 0; q < 3; ++q) {
+        const double *a0 = v[q], *b0 = v[q == 2 ? 0 : q + 1];
+        for (int e = 0; e < 2; ++e) {
+            const double sp = e ? s1 : s0;
+            const double da = a0[ax] - sp, db = b0[ax] - sp;
+            if ((da < 0.0 && db > 0.0) || (da > 0.0 && db < 0.0)) {
+                const double f = da / (da - db);
+                double p[3];
+                for (int a = 0; a < 3; ++a) p[a] = a0[a] + f * (b0[a] - a0[a]);
+                p[ax] = sp;
+                add(p);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (l[a] <= h[a]) {
+            plo[a] = fmaxf(round_down(l[a]) - margin, tl[a]);
+            phi[a] = fminf(round_up(h[a]) + margin, th[a]);
+        } else {  // no point found (cannot happen for a slab inside the box): the whole box
+            plo[a] = tl[a];
+            phi[a] = th[a];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                               uint32_t n, const float4 *__restrict__ tlo,
                                               const float4 *__restrict__ thi, const uint32_t *__restrict__ offs,
@@ -528,41 +590,10 @@ __global__ __launch_bounds__(kT) void k_split(const float *__restrict__ verts, c
             for (int a = 0; a < 6; ++a) mag = fmaxf(mag, fabsf(ord2f(cbounds[a])));
             for (int a = 0; a < 3; ++a) mag = fmaxf(mag, fmaxf(fabsf(tl[a]), fabsf(th[a])));
             const float margin = mag * 0x1p-17f;
-            int ax = 0;
-            for (int a = 1; a < 3; ++a)
-                if (th[a] - tl[a] > th[ax] - tl[ax]) ax = a;
+            const int ax = longest_axis(tl, th);
             for (uint32_t j = 0; j < k; ++j) {
-                const double s0 = slab_cut(tl[ax], th[ax], j, k), s1 = slab_cut(tl[ax], th[ax], j + 1, k);
-                double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-                auto add = [&](const double *p) {
-                    for (int a = 0; a < 3; ++a) { l[a] = fmin(l[a], p[a]); h[a] = fmax(h[a], p[a]); }
-                };
-                for (int q = 0; q < 3; ++q)
-                    if (v[q][ax] >= s0 && v[q][ax] <= s1) add(v[q]);
-                for (int q = 0; q < 3; ++q) {
-                    const double *a0 = v[q], *b0 = v[q == 2 ? 0 : q + 1];
-                    for (int e = 0; e < 2; ++e) {
-                        const double sp = e ? s1 : s0;
-                        const double da = a0[ax] - sp, db = b0[ax] - sp;
-                        if ((da < 0.0 && db > 0.0) || (da > 0.0 && db < 0.0)) {
-                            const double f = da / (da - db);
-                            double p[3];
-                            for (int a = 0; a < 3; ++a) p[a] = a0[a] + f * (b0[a] - a0[a]);
-                            p[ax] = sp;
-                            add(p);
-                        }
-                    }
-                }
                 float plo[3], phi[3];
-                for (int a = 0; a < 3; ++a) {
-                    if (l[a] <= h[a]) {
-                        plo[a] = fmaxf(round_down(l[a]) - margin, tl[a]);
-                        phi[a] = fminf(round_up(h[a]) + margin, th[a]);
-                    } else {  // no point found (cannot happen for a slab inside the box): the whole box
-                        plo[a] = tl[a];
-                        phi[a] = th[a];
-                    }
-                }
+                piece_box(v, tl, th, ax, j, k, margin, plo, phi);
                 rlo[base + j] = make_float4(plo[0], plo[1], plo[2], 0.0f);
                 rhi[base + j] = make_float4(phi[0], phi[1], phi[2], 0.0f);
                 rtri[base + j] = i;
@@ -811,14 +842,144 @@ dr_status build_ploc(Scene *s, BuildTmp &t, uint32_t n, int radius, uint32_t lea
                                                         t.keys.as<uint32_t>(), vals_dfs.as<uint32_t>(),
                                                         t.parent_l.as<int32_t>());
     k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), vals_dfs.as<uint32_t>(), n,
-                                           t.rlo, t.rhi, t.rtri, t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(),
-                                           s->tri_v.as<float4>());
+                                           t.rlo, t.rhi, t.rtri, t.rtri ? t.offs.as<uint32_t>() : nullptr,
+                                           t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
     DR_HIP_TRY(hipGetLastError());
     DR_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
     return DR_OK;
 }
 
+// ---- refit: the built tree over moved vertices ------------------------------
+__global__ void k_init_bounds(int32_t *b)
+{
+    const int t = threadIdx.x;
+    if (t < 3) {
+        b[t] = 0x7FFFFFFF;
+        b[3 + t] = (int32_t)0x80000000;
+    }
+}
+
+// box of the finite vertex coordinates (ordered ints)
+__global__ __launch_bounds__(kT) void k_vertex_bounds(const float *__restrict__ verts, uint32_t nv,
+                                                      int32_t *__restrict__ vb)
+{
+    float c[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, C[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t i = blockIdx.x * kT + threadIdx.x; i < nv; i += gridDim.x * kT) {
+        for (int a = 0; a < 3; ++a) {
+            const float x = verts[3 * (size_t)i + a];
+            if (isfinite(x)) {
+                c[a] = fminf(c[a], x);
+                C[a] = fmaxf(C[a], x);
+            }
+        }
+    }
+    block_bounds_atomic(c, C, vb);
+}
+
+// One level of wide nodes [lo, hi), deepest level first. An interior child's
+// box becomes the union of its node's (already refitted) child boxes. A leaf's
+// references take their triangle's new vertices and, for a split triangle,
+// the box of their piece: slab j of k of the triangle's new box across the
+// axis the build cut (the k slabs still cover the triangle, and an axis-
+// aligned scale and shift maps each piece onto the same part as before). A
+// triangle with a non-finite coordinate gets no box, as in the build. Boxes
+// are widened by one ulp outward as in the collapse.
+__global__ __launch_bounds__(kT) void k_refit_level(WideNode *__restrict__ nodes, uint32_t lo, uint32_t hi,
+                                                    const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                                                    float4 *__restrict__ tri_v, const int32_t *__restrict__ vb)
+{
+    const uint32_t w = lo + blockIdx.x * kT + threadIdx.x;
+    if (w >= hi) return;
+    float mag = 0.0f;
+    for (int a = 0; a < 6; ++a) {
+        const float m = fabsf(ord2f(vb[a]));
+        if (m <= FLT_MAX) mag = fmaxf(mag, m);
+    }
+    WideNode wn = nodes[w];
+    for (int c = 0; c < 4; ++c) {
+        const int32_t ref = wn.child[c];
+        if (ref == DR_EMPTY_REF) continue;
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (ref >= 0) {
+            const WideNode &ch = nodes[ref];
+            for (int cc = 0; cc < 4; ++cc) {
+                if (ch.child[cc] == DR_EMPTY_REF) continue;
+                l[0] = fminf(l[0], ch.lox[cc]); h[0] = fmaxf(h[0], ch.hix[cc]);
+                l[1] = fminf(l[1], ch.loy[cc]); h[1] = fmaxf(h[1], ch.hiy[cc]);
+                l[2] = fminf(l[2], ch.loz[cc]); h[2] = fmaxf(h[2], ch.hiz[cc]);
+            }
+        } else {
+            const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
+            for (uint32_t r = first; r < first + cnt; ++r) {
+                float4 *tv = tri_v + 3 * (size_t)r;
+                const float4 w0 = tv[0], w1 = tv[1], w2 = tv[2];
+                const uint32_t id = (uint32_t)__float_as_int(w0.w);
+                const uint32_t pw = (uint32_t)__float_as_int(w1.w), pieces = (uint32_t)__float_as_int(w2.w);
+                double v[3][3];
+                float tl[3] = {INFINITY, INFINITY, INFINITY}, th[3] = {-INFINITY, -INFINITY, -INFINITY};
+                float f[3][3];
+                for (int q = 0; q < 3; ++q) {
+                    const uint32_t vi = idx[3 * (size_t)id + q];
+                    for (int a = 0; a < 3; ++a) {
+                        f[q][a] = verts[3 * (size_t)vi + a];
+                        v[q][a] = f[q][a];
+                        tl[a] = fminf(tl[a], f[q][a]);
+                        th[a] = fmaxf(th[a], f[q][a]);
+                    }
+                }
+                tv[0] = make_float4(f[0][0], f[0][1], f[0][2], w0.w);
+                tv[1] = make_float4(f[1][0], f[1][1], f[1][2], w1.w);
+                tv[2] = make_float4(f[2][0], f[2][1], f[2][2], w2.w);
+                bool finite = true;
+                for (int a = 0; a < 3; ++a) finite = finite && isfinite(tl[a]) && isfinite(th[a]);
+                if (!finite) continue;
+                float plo[3] = {tl[0], tl[1], tl[2]}, phi[3] = {th[0], th[1], th[2]};
+                if (pieces > 1) {
+                    float m = mag;
+                    for (int a = 0; a < 3; ++a) m = fmaxf(m, fmaxf(fabsf(tl[a]), fabsf(th[a])));
+                    piece_box(v, tl, th, (int)(pw >> 16), pw & 0xFFFFu, pieces, m * 0x1p-17f, plo, phi);
+                }
+                for (int a = 0; a < 3; ++a) {
+                    l[a] = fminf(l[a], plo[a]);
+                    h[a] = fmaxf(h[a], phi[a]);
+                }
+            }
+        }
+        wn.lox[c] = down(l[0]); wn.loy[c] = down(l[1]); wn.loz[c] = down(l[2]);
+        wn.hix[c] = up(h[0]); wn.hiy[c] = up(h[1]); wn.hiz[c] = up(h[2]);
+    }
+    nodes[w] = wn;
+}
+
 }  // namespace
+
+dr_status refit_bvh(Scene *s, hipStream_t stream)
+{
+    if (!s->built || s->level_ofs.size() < 2) return fail(DR_E_INVALID, "scene BVH not built");
+    if (dr_status st = s->refit_tmp.ensure(6 * sizeof(int32_t))) return st;
+    int32_t *vb = s->refit_tmp.as<int32_t>();
+    k_init_bounds<<<1, 64, 0, stream>>>(vb);
+    k_vertex_bounds<<<std::min(blocks(s->n_verts), kBoundsBlocks), kT, 0, stream>>>(s->verts.as<float>(), s->n_verts,
+                                                                                   vb);
+    DR_HIP_TRY(hipGetLastError());
+    for (size_t d = s->level_ofs.size() - 1; d-- > 0;) {
+        const uint32_t lo = s->level_ofs[d], hi = s->level_ofs[d + 1];
+        if (hi <= lo) continue;
+        k_refit_level<<<blocks(hi - lo), kT, 0, stream>>>(s->nodes.as<WideNode>(), lo, hi, s->verts.as<float>(),
+                                                          s->indices.as<uint32_t>(), s->tri_v.as<float4>(), vb);
+        DR_HIP_TRY(hipGetLastError());
+    }
+    int32_t b[6];
+    DR_HIP_TRY(hipMemcpyAsync(b, vb, sizeof(b), hipMemcpyDeviceToHost, stream));
+    DR_HIP_TRY(hipStreamSynchronize(stream));
+    for (int a = 0; a < 6; ++a) {
+        const int32_t i = b[a] >= 0 ? b[a] : b[a] ^ 0x7FFFFFFF;
+        float f;
+        std::memcpy(&f, &i, sizeof(f));
+        s->bounds[a] = f;
+    }
+    return DR_OK;
+}
 
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
 {
@@ -931,8 +1092,8 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     DR_HIP_TRY(hipGetLastError());
     if ((st = radix_sort_pairs(ctx, t.keys.as<uint32_t>(), t.vals.as<uint32_t>(), n, 30, stream))) return st;
     k_gather<<<blocks(n), kT, 0, stream>>>(s->verts.as<float>(), s->indices.as<uint32_t>(), t.vals.as<uint32_t>(), n,
-                                           t.rlo, t.rhi, t.rtri, t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(),
-                                           s->tri_v.as<float4>());
+                                           t.rlo, t.rhi, t.rtri, t.rtri ? t.offs.as<uint32_t>() : nullptr,
+                                           t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), s->tri_v.as<float4>());
     DR_HIP_TRY(hipGetLastError());
 
     const float ct = 1.0f, ci = 1.0f;
@@ -980,12 +1141,14 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
             k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), n,
                                                     s->nodes.as<WideNode>(), t.node_lo.as<float4>(), t.node_hi.as<float4>());
             DR_HIP_TRY(hipGetLastError());
+            s->level_ofs = {0u, 1u};
         } else {
             // level-synchronous collapse; counters: [0] wide nodes, [1] queue out, [2] leaves
             k_collapse_init<<<1, 1, 0, stream>>>(counters, t.queue_a.as<int2>());
             uint32_t nin = 1;
             int2 *qa = t.queue_a.as<int2>(), *qb = t.queue_b.as<int2>();
             depth = 0;
+            s->level_ofs = {0u, 1u};  // the root; each level allocates the next one's slots
             while (nin > 0) {
                 k_collapse<<<blocks(nin), kT, 0, stream>>>(qa, nin, qb, counters, t.child_l.as<int32_t>(),
                                                           t.child_r.as<int32_t>(), t.range_first.as<uint32_t>(),
@@ -1001,6 +1164,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
                 nin = c[1];
                 n_nodes = c[0];
                 n_leaves = c[2];
+                if (nin) s->level_ofs.push_back(n_nodes);
                 ++depth;
                 if (c[3]) return fail(DR_E_DEVICE, "BVH collapse exceeded its node bound");
                 int2 *tmp = qa; qa = qb; qb = tmp;

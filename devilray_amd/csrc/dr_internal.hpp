@@ -81,6 +81,9 @@ struct Scene {
     DevBuf nodes;     // WideNode
     uint32_t n_refs = 0;  // tri_v entries: triangles plus split pieces
     uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
+    // wide nodes by depth: level d is [level_ofs[d], level_ofs[d + 1]) (refit)
+    std::vector<uint32_t> level_ofs;
+    DevBuf refit_tmp;  // refit: ordered-int vertex bounds
     double build_ms = 0.0, sah_cost = 0.0;
     bool built = false;
     // traversal overflow stack + flag
@@ -235,6 +238,8 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
 
 // bvh_build.hip
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);
+// new boxes for the built tree over the scene's current vertices
+dr_status refit_bvh(Scene *s, hipStream_t stream);
 
 // trace.hip
 // ray_list (optional): indices of the rays to trace (results go to those

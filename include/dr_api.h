@@ -145,6 +145,16 @@ dr_status dr_scene_get_info(dr_scene *scene, dr_scene_info *info);
  * Replaces generateSimpleBBH (src/models/BBH.cpp:101-115), which is a host
  * median split with one triangle per leaf. */
 dr_status dr_scene_build_bvh(dr_scene *scene, const dr_bvh_params *params, dr_stream stream);
+/* Moves the scene's vertices (host array, the same n_vertices x 3 floats and
+ * the same triangles) and refits the built BVH to them: the tree and its
+ * triangle pieces are kept and every box is recomputed bottom-up, level by
+ * level, so hits stay exact while the tree's quality follows how far the
+ * geometry moved. The light table is rebuilt at the next render. The
+ * reference moves objects by their transform (TriangleMesh::setPosition /
+ * setScale, src/tracing/TriangleMesh.cpp:3-15) over per-object trees that it
+ * never rebuilds; the flattened scene here refits instead. Batches made from
+ * the scene earlier keep their own copy. Synchronises. */
+dr_status dr_scene_refit(dr_scene *scene, const float *vertices, dr_stream stream);
 
 /* Closest hit over a ray batch. Replaces getIntersection / cast
  * (include/tracing/IntersectionTestsImpl.hpp:222-251) for n rays at once. */
