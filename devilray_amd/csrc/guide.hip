@@ -126,8 +126,9 @@ __device__ __forceinline__ bf16x8 pack_relu(f32x4 lo, f32x4 hi)
 }
 
 // Input features 8g .. 8g+7 of one sample: for g < 3, sin and cos of
-// 2^k pi u (k = 0..3) with u the position along axis g normalised to the
-// box; for g = 3 the normal, the outgoing direction, 1 and 0.
+// 2^(k+1) pi u (k = 0..3; periods 1 to 1/8 of the box) with u the position
+// along axis g normalised to the box; for g = 3 the normal, the outgoing
+// direction, 1 and 0.
 __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &box)
 {
     bf16x8 f;
@@ -137,7 +138,7 @@ __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &b
         const float ext = g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2]);
         float u = (x - lo) / ext;
         u = fminf(fmaxf(u, 0.0f), 1.0f);
-        float scale = 0.5f;
+        float scale = 1.0f;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const float a = u * scale;

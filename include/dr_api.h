@@ -340,7 +340,8 @@ dr_status dr_batch_render(dr_batch *batch, const dr_batch_view *views, uint32_t 
  * Lambert bounces from the BSDF only, PathGeneration.hpp:406-420) ----------
  *
  * Network: 32 input features (frequency encoding of the position within the
- * guide's box, the shading normal and the outgoing direction) -> 4 hidden
+ * guide's box: per axis sin and cos of 2^(k+1) pi u, k = 0..3, for u in [0, 1]
+ * across the box; the shading normal and the outgoing direction) -> 4 hidden
  * ReLU layers of 64 -> 48 outputs, of which 41 are used: an 8-lobe von
  * Mises-Fisher mixture (per lobe a mean direction, log concentration and
  * mixture logit) and one selection logit alpha = 0.1 + 0.8 sigmoid(s).

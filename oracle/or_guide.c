@@ -62,18 +62,23 @@ static int head_row(int lobe, int comp)
 }
 #define SEL_ROW 34
 
+/* per axis sin and cos of 2^(k+1) pi u, k = 0..3 (u in [0, 1] across the box),
+ * from the same float reduction of u * 2^k to [0, 1) and the same sin/cos
+ * polynomial as the device, so the features match bit for bit */
 void or_guide_encode(const float bounds[6], const or_guide_sample *s, double feat[32])
 {
     for (int d = 0; d < 3; ++d) {
         const float ext = bounds[3 + d] - bounds[d];
         float u = (s->pos[d] - bounds[d]) / ext;
         u = fminf(fmaxf(u, 0.0f), 1.0f);
-        float scale = 0.5f;
+        float scale = 1.0f;
         for (int k = 0; k < 4; ++k) {
             const float a = u * scale;
             const float t = a - floorf(a);
-            feat[8 * d + 2 * k] = sin(2.0 * ORPI * (double)t);
-            feat[8 * d + 2 * k + 1] = cos(2.0 * ORPI * (double)t);
+            float sn, cs;
+            or_sincos_2pi(t, &sn, &cs);  /* the device's polynomial, bit for bit */
+            feat[8 * d + 2 * k] = sn;
+            feat[8 * d + 2 * k + 1] = cs;
             scale *= 2.0f;
         }
     }

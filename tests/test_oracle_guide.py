@@ -104,13 +104,17 @@ def test_encoding_known_points():
     assert np.allclose(f[24:27], (0, 0.6, 0.8), atol=1e-7)
     assert np.allclose(f[27:30], (1, 0, 0))
     assert f[30] == 1.0 and f[31] == 0.0
+    # sin/cos(2^(k+1) pi u), k = 0..3: periods 1, 1/2, 1/4 and 1/8 of the box
     s["pos"] = [(a + b) / 2 for a, b in zip(BOUNDS[:3], BOUNDS[3:])]  # u = 1/2
     f = oracle.guide_encode(BOUNDS, s)
-    # sin/cos(2^k pi / 2), k = 0..3
-    want = [1, 0, 0, -1, 0, 1, 0, 1]
+    want = [0, -1, 0, 1, 0, 1, 0, 1]
     for d in range(3):
         assert np.allclose(f[8 * d:8 * d + 8], want, atol=1e-7)
     s["pos"] = [a + 0.25 * (b - a) for a, b in zip(BOUNDS[:3], BOUNDS[3:])]  # u = 1/4
+    f = oracle.guide_encode(BOUNDS, s)
+    want = [1, 0, 0, -1, 0, 1, 0, 1]
+    assert np.allclose(f[0:8], want, atol=1e-7)
+    s["pos"] = [a + 0.125 * (b - a) for a, b in zip(BOUNDS[:3], BOUNDS[3:])]  # u = 1/8
     f = oracle.guide_encode(BOUNDS, s)
     r = np.sqrt(0.5)
     want = [r, r, 1, 0, 0, -1, 0, 1]
