@@ -248,77 +248,94 @@ __device__ __forceinline__ V3 vload(const float *v, uint32_t i)
 
 // One wave per 16 queue slots; every lane of a wave reaches the matrix
 // instructions (invalid slots run on neutral inputs and store nothing).
-__global__ __launch_bounds__(kT) void k_guide_infer(Net net, Box box, PathIn in)
+// Grid of the path inference: a few blocks per CU, each copying the forward
+// weights into LDS once and then striding over 64-path tiles (one block per
+// tile read the 34 KB of weights from L2 for every 64 paths).
+constexpr uint32_t kInferBlocks = 1024;
+
+__global__ __launch_bounds__(kT) void k_guide_infer(Net gnet, Box box, PathIn in)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t base = (blockIdx.x * (kT / 64) + wave) * 16u;
+    __shared__ bf16x8 Wf[kFragF * 64];
+    __shared__ float Bs[5][64];
     const uint32_t n = *in.count;
-    if (base >= n) return;  // wave-uniform
+    if ((uint64_t)blockIdx.x * 64u >= n) return;  // block-uniform: no tile for this block
+    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff[k];
+    for (int k = threadIdx.x; k < 5 * 64; k += kT) {
+        const int l = k / 64, o = k % 64;
+        Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
+    }
+    __syncthreads();
+    const Net net{Wf, nullptr, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 15, g = lane >> 4;
-    const uint32_t slot = base + (uint32_t)s;
-    bool valid = slot < n;
-    uint32_t path = 0;
-    V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f};
-    if (valid) {
-        path = in.list[slot];
-        const dr_hit h = in.hits[path];
-        valid = h.id >= 0;
+    for (uint32_t tile = blockIdx.x; (uint64_t)tile * 64u < n; tile += gridDim.x) {
+        const uint32_t base = tile * 64u + (uint32_t)wave * 16u;
+        if (base >= n) continue;  // wave-uniform
+        const uint32_t slot = base + (uint32_t)s;
+        bool valid = slot < n;
+        uint32_t path = 0;
+        V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f};
         if (valid) {
-            const uint32_t tri = (uint32_t)h.id;
-            valid = in.mats[in.tri_mat[tri]].kind == DR_MAT_LAMBERT;
+            path = in.list[slot];
+            const dr_hit h = in.hits[path];
+            valid = h.id >= 0;
             if (valid) {
-                const V3 A = vload(in.verts, in.idx[3 * (size_t)tri]);
-                const V3 B = vload(in.verts, in.idx[3 * (size_t)tri + 1]);
-                const V3 C = vload(in.verts, in.idx[3 * (size_t)tri + 2]);
-                const dr_ray r = in.rays[path];
-                const V3 d{r.dx, r.dy, r.dz};
-                p = bary_point(A, B, C, h.u, h.v);
-                const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
-                nn = dot(ng, d) > 0.0f ? mul(ng, -1.0f) : ng;
-                wo = mul(d, -1.0f);
+                const uint32_t tri = (uint32_t)h.id;
+                valid = in.mats[in.tri_mat[tri]].kind == DR_MAT_LAMBERT;
+                if (valid) {
+                    const V3 A = vload(in.verts, in.idx[3 * (size_t)tri]);
+                    const V3 B = vload(in.verts, in.idx[3 * (size_t)tri + 1]);
+                    const V3 C = vload(in.verts, in.idx[3 * (size_t)tri + 2]);
+                    const dr_ray r = in.rays[path];
+                    const V3 d{r.dx, r.dy, r.dz};
+                    p = bary_point(A, B, C, h.u, h.v);
+                    const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
+                    nn = dot(ng, d) > 0.0f ? mul(ng, -1.0f) : ng;
+                    wo = mul(d, -1.0f);
+                }
             }
         }
-    }
-    const bf16x8 x0 = encode8(g, p, nn, wo, box);
-    bf16x8 xh[4][2];
-    f32x4 y[3];
-    net_forward(net, lane, x0, xh, y);
-    HeadLane h;
-    head_decode(y, lane, h);
-    // a non-finite head (diverged parameters) becomes a neutral one that the
-    // shading kernel never selects (alpha = 0), so images stay finite
-    float chk = h.sig;
+        const bf16x8 x0 = encode8(g, p, nn, wo, box);
+        bf16x8 xh[4][2];
+        f32x4 y[3];
+        net_forward(net, lane, x0, xh, y);
+        HeadLane h;
+        head_decode(y, lane, h);
+        // a non-finite head (diverged parameters) becomes a neutral one that
+        // the shading kernel never selects (alpha = 0), so images stay finite
+        float chk = h.sig;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) chk += h.mu[t][0] + h.mu[t][1] + h.mu[t][2] + h.kap[t] + h.pi[t];
-    int bad = isfinite(chk) ? 0 : 1;
-    bad |= __shfl_xor(bad, 16);
-    bad |= __shfl_xor(bad, 32);
-    if (!valid) return;
-    if (bad) {
+        for (int t = 0; t < 2; ++t) chk += h.mu[t][0] + h.mu[t][1] + h.mu[t][2] + h.kap[t] + h.pi[t];
+        int bad = isfinite(chk) ? 0 : 1;
+        bad |= __shfl_xor(bad, 16);
+        bad |= __shfl_xor(bad, 32);
+        if (!valid) continue;
+        if (bad) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                h.mu[t][0] = 0.0f;
+                h.mu[t][1] = 0.0f;
+                h.mu[t][2] = 1.0f;
+                h.kap[t] = 1.0f;
+                h.pi[t] = 1.0f / kGuideLobes;
+            }
+        }
+        float *o = in.head + (size_t)kHeadFloats * path;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            h.mu[t][0] = 0.0f;
-            h.mu[t][1] = 0.0f;
-            h.mu[t][2] = 1.0f;
-            h.kap[t] = 1.0f;
-            h.pi[t] = 1.0f / kGuideLobes;
+            float *l = o + 5 * (2 * g + t);
+            l[0] = h.mu[t][0];
+            l[1] = h.mu[t][1];
+            l[2] = h.mu[t][2];
+            l[3] = h.kap[t];
+            l[4] = h.pi[t];
         }
-    }
-    float *o = in.head + (size_t)kHeadFloats * path;
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        float *l = o + 5 * (2 * g + t);
-        l[0] = h.mu[t][0];
-        l[1] = h.mu[t][1];
-        l[2] = h.mu[t][2];
-        l[3] = h.kap[t];
-        l[4] = h.pi[t];
-    }
-    if (g == 0) {
-        o[40] = bad ? 0.0f : kAlphaLo + kAlphaSpan * h.sig;
-        o[41] = 0.0f;
-        o[42] = 0.0f;
-        o[43] = 0.0f;
+        if (g == 0) {
+            o[40] = bad ? 0.0f : kAlphaLo + kAlphaSpan * h.sig;
+            o[41] = 0.0f;
+            o[42] = 0.0f;
+            o[43] = 0.0f;
+        }
     }
 }
 
@@ -801,7 +818,7 @@ dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const 
 {
     PathIn in{s->verts.as<float>(), s->indices.as<uint32_t>(), s->tri_mat.as<uint32_t>(),
               s->materials.as<dr_material>(), rays, hits, list, count, head};
-    k_guide_infer<<<cdiv(n_max, 64), kT, 0, stream>>>(net_of(g), box_of(g), in);
+    k_guide_infer<<<std::min<uint32_t>(cdiv(n_max, 64), kInferBlocks), kT, 0, stream>>>(net_of(g), box_of(g), in);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }
