@@ -253,12 +253,13 @@ def guide_train_rate(ctx):
 
 
 def time_to_mse(ctx, sc, cam, ref, guide, spp_per_pass, passes, checkpoint, seed, train_steps=1, max_depth=12,
-                train_passes=None, lr_schedule=None):
+                train_passes=None, lr_schedule=None, guide_after=1):
     """Progressive render; (wall seconds, MSE) at every checkpoint, with the
     image readback outside the timed spans."""
     from devilray_amd.guiding import ProgressiveRenderer
     r = ProgressiveRenderer(sc, cam, guide, spp_per_pass=spp_per_pass, max_depth=max_depth, seed=seed,
-                            train_steps=train_steps, train_passes=train_passes, lr_schedule=lr_schedule)
+                            train_steps=train_steps, train_passes=train_passes, lr_schedule=lr_schedule,
+                            guide_after=guide_after)
     t, curve = 0.0, []
     for k in range(passes):
         ctx.synchronize()
@@ -288,7 +289,7 @@ def time_for(curve, target):
 
 
 def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, checkpoint, train_steps, max_depth,
-                    capacity=1 << 18, train_passes=None, seeds=3, train_batch=0, lr_schedule=None):
+                    capacity=1 << 18, train_passes=None, seeds=3, train_batch=0, lr_schedule=None, guide_after=1):
     """Equal-quality time: the plain sampler renders passes x spp; the target
     is its final MSE against a ref_spp reference (the same plain path, FP64
     accumulated); the guided sampler (trained online while it renders)
@@ -313,8 +314,12 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     for k in range(seeds):
         g = dr.Guide(ctx, s.bounds(), capacity=capacity, seed=3 + k)
         g.set_options(1.0, 64, train_batch)
-        runs.append(time_to_mse(ctx, s, cam, ref, g, spp_per_pass, 2 * passes, checkpoint, 950 + k, train_steps,
-                                max_depth, train_passes, lr_schedule))
+        # the guided run k uses plain run k's seed (common random numbers): the
+        # two arms share camera rays, light choices and every sample the guide
+        # does not change, so their MSE ratio is far less noisy; the ratio
+        # each estimates is unchanged
+        runs.append(time_to_mse(ctx, s, cam, ref, g, spp_per_pass, 2 * passes, checkpoint, 900 + k, train_steps,
+                                max_depth, train_passes, lr_schedule, guide_after))
         g.close()
     guided = avg(runs)
     s.close()

@@ -18,13 +18,14 @@ from . import DR_GUIDE_COLLECT, DR_GUIDE_USE, DrCamera, DrRenderParams, Guide, S
 class ProgressiveRenderer:
     def __init__(self, scene: Scene, camera: DrCamera, guide: Guide | None = None, spp_per_pass: int = 4,
                  max_depth: int = 12, rr_depth: int = 3, seed: int = 0, train_passes: int | None = None,
-                 train_steps: int = 1, lr_schedule=None, stream=None):
+                 train_steps: int = 1, lr_schedule=None, guide_after: int = 1, stream=None):
         self.scene, self.camera, self.guide = scene, camera, guide
         self.spp_per_pass = spp_per_pass
         self.max_depth, self.rr_depth, self.seed = max_depth, rr_depth, seed
         self.train_passes = train_passes  # None: train on every pass
         self.train_steps = train_steps
         self.lr_schedule = lr_schedule  # None, or pass index -> Adam step size for that pass's steps
+        self.guide_after = max(1, guide_after)  # training passes before the guide samples (warm-up)
         self.stream = stream
         self.npix = camera.width * camera.height
         self.accum = scene.ctx.alloc(self.npix * 32)
@@ -45,7 +46,7 @@ class ProgressiveRenderer:
         if self.guide is None:
             self.scene.render_dev(self.camera, p, self.accum.ptr, self.stream)
         else:
-            flags = DR_GUIDE_USE if self.trained > 0 else 0
+            flags = DR_GUIDE_USE if self.trained >= self.guide_after else 0
             train = self._training()
             if train:
                 flags |= DR_GUIDE_COLLECT
