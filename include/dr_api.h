@@ -132,7 +132,8 @@ dr_status dr_memcpy_dtod(dr_context *ctx, void *dst, const void *src, size_t byt
 dr_status dr_synchronize(dr_context *ctx);
 /* Zero bytes of device memory with a kernel (not a memset node, so callers
  * may record it into a graph), asynchronous on `stream`; dst and bytes must
- * be multiples of 16. */
+ * be multiples of 16. Replaces the device side of Buffers::reset
+ * (src/Renderer.cpp:17-21), which clears the image between frames. */
 dr_status dr_fill_zero(dr_context *ctx, void *dst, size_t bytes, dr_stream stream);
 
 /* Scene upload. Replaces convertMeshToTris / viewGpuTris
