@@ -261,7 +261,8 @@ private:
 // render time and the ray casts after every pass, while other threads change
 // the camera or settings (each change clears the image at the start of the
 // next pass) and read getPixels(). An exception from render() stops the loop
-// and is kept for error().
+// and is kept for error(). The loop must be stopped (or destroyed) before the
+// renderer it drives.
 class RenderLoop {
 public:
     explicit RenderLoop(Renderer &renderer);
