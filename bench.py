@@ -19,8 +19,8 @@ Prints one JSON line. `roofline.achieved` uses the algorithmic bytes per ray
 48 + 32*N_node + 40*N_tri with N from the canonical binary SAH BVH counts
 committed in tests/golden/roofline_counts.json, divided by the kernel's mean
 HIP-event duration on its stream; `traffic` is the measured HBM bytes per
-launch from the committed rocprofv3 summary and `traffic_gbs` that amount over
-the same duration. The device BVH cuts long thin triangles into several
+launch from the committed rocprofv3 summary, `traffic_gbs` that amount over
+the same duration and `traffic_frac` its fraction of the HBM peak. The device BVH cuts long thin triangles into several
 references and so makes far fewer tests per ray than the canonical tree, which
 is why `frac` can exceed 1: the algorithmic bytes are fixed by the canonical
 tree, the traffic is what this kernel moves. `cpu_baseline` times the
@@ -551,6 +551,9 @@ def main() -> int:
         if traffic:
             # measured bytes moved per launch over the same time: what the memory system actually carried
             roofline["traffic_gbs"] = round(traffic / (kernel_ms * 1e-3) / 1e9, 1)
+            # the measured bytes against the same peak: how close the kernel
+            # runs to HBM bandwidth (frac compares the canonical tree's bytes)
+            roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
 
     renders = {}
     if not args.no_render:
