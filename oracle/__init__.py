@@ -23,6 +23,9 @@ _i32 = C.c_int32
 PROTOTYPES = {
     "or_philox4x32_10": (None, [_vp, _vp, _vp]),
     "or_u01": (C.c_float, [C.c_uint32]),
+    "or_sample_sphere": (None, [C.c_float, C.c_float, _vp]),
+    "or_sample_cosine": (None, [_vp, C.c_float, C.c_float, _vp]),
+    "or_sample_triangle": (None, [_vp, _vp, _vp, C.c_float, C.c_float, _vp]),
     "or_tri_watertight": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "or_tri_reference": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "or_box_interval_ref": (None, [C.c_float, C.c_float, C.c_float, C.c_float, _vp, _vp]),
@@ -73,6 +76,25 @@ def sincos_2pi(u: float):
     s, c = np.zeros(1, np.float32), np.zeros(1, np.float32)
     lib().or_sincos_2pi(u, _p(s), _p(c))
     return float(s[0]), float(c[0])
+
+
+def sample_sphere(r1: float, r2: float) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().or_sample_sphere(r1, r2, _p(out))
+    return out
+
+
+def sample_cosine(n, r1: float, r2: float) -> np.ndarray:
+    nn, out = np.ascontiguousarray(n, np.float32), np.zeros(3, np.float32)
+    lib().or_sample_cosine(_p(nn), r1, r2, _p(out))
+    return out
+
+
+def sample_triangle(a, b, c, r1: float, r2: float) -> np.ndarray:
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (a, b, c))
+    out = np.zeros(3, np.float32)
+    lib().or_sample_triangle(_p(a), _p(b), _p(c), r1, r2, _p(out))
+    return out
 
 
 def materials_array(mats) -> np.ndarray:

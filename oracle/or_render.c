@@ -163,6 +163,32 @@ static inline v3 cosine_dir(v3 n, float r1, float r2)
     return vdiv(w, sqrtf(l2));
 }
 
+/* area-uniform point of triangle ABC from two uniforms, u + v > 1 folded back
+ * (the reference's uniformTriangleSample, DistributionSamplers.hpp:92-104) */
+static inline v3 triangle_point(v3 a, v3 b, v3 c, float u, float v)
+{
+    if (u + v > 1.0f) { u = 1.0f - u; v = 1.0f - v; }
+    return vadd(vadd(a, vmul(vsub(b, a), u)), vmul(vsub(c, a), v));
+}
+
+void or_sample_sphere(float r1, float r2, float out[3])
+{
+    const v3 w = uniform_sphere(r1, r2);
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+}
+
+void or_sample_cosine(const float n[3], float r1, float r2, float out[3])
+{
+    const v3 w = cosine_dir(mk(n[0], n[1], n[2]), r1, r2);
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+}
+
+void or_sample_triangle(const float a[3], const float b[3], const float c[3], float r1, float r2, float out[3])
+{
+    const v3 p = triangle_point(mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]), mk(c[0], c[1], c[2]), r1, r2);
+    out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+
 static inline float max_abs(v3 a) { return fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))); }
 
 /* offset scale: 1e-5 times the largest vertex coordinate of the triangle */
@@ -303,12 +329,10 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
                 const int32_t lt = take_a ? en.a : en.b;
                 const float psel = take_a ? en.pdf_a : en.pdf_b;
                 if (lt >= 0 && psel > 0.0f) {
-                    float a1 = or_u01(r1[2]), a2 = or_u01(r1[3]);
-                    if (a1 + a2 > 1.0f) { a1 = 1.0f - a1; a2 = 1.0f - a2; }
                     const v3 LA = vtx(rc->verts, rc->tris[3 * (size_t)lt]);
                     const v3 LB = vtx(rc->verts, rc->tris[3 * (size_t)lt + 1]);
                     const v3 LC = vtx(rc->verts, rc->tris[3 * (size_t)lt + 2]);
-                    const v3 xl = vadd(vadd(LA, vmul(vsub(LB, LA), a1)), vmul(vsub(LC, LA), a2));
+                    const v3 xl = triangle_point(LA, LB, LC, or_u01(r1[2]), or_u01(r1[3]));
                     const v3 nl = vnorm(vcross(vsub(LB, LA), vsub(LC, LA)));
                     const float pa = rc->pdf_area[lt];
                     const v3 toL = vsub(xl, p);

@@ -123,6 +123,13 @@ int32_t or_lights_build(const float *verts, const int32_t *tris, int32_t ntris, 
 
 /* sin and cos of 2 pi u (the build's own polynomial). */
 void or_sincos_2pi(float u, float *s, float *c);
+/* The path tracer's samplers, exported for their own tests: a direction
+ * uniform on the sphere, the cosine-weighted direction about n (normalised
+ * sphere sample + n, the reference's PathGeneration.hpp:406-420), and the
+ * area-uniform point of a triangle. r1, r2 in [0, 1). */
+void or_sample_sphere(float r1, float r2, float out[3]);
+void or_sample_cosine(const float n[3], float r1, float r2, float out[3]);
+void or_sample_triangle(const float a[3], const float b[3], const float c[3], float r1, float r2, float out[3]);
 
 /* Path-trace pixels [pixel_begin, pixel_end) adding spp samples each to
  * accum (4 doubles per pixel, in sample order). vertex_hist (optional,
