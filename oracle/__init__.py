@@ -21,6 +21,7 @@ HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("
 _vp = C.c_void_p
 _i32 = C.c_int32
 PROTOTYPES = {
+    "or_camera_ray": (None, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _vp]),
     "or_philox4x32_10": (None, [_vp, _vp, _vp]),
     "or_u01": (C.c_float, [C.c_uint32]),
     "or_sample_sphere": (None, [C.c_float, C.c_float, _vp]),
@@ -76,6 +77,13 @@ def sincos_2pi(u: float):
     s, c = np.zeros(1, np.float32), np.zeros(1, np.float32)
     lib().or_sincos_2pi(u, _p(s), _p(c))
     return float(s[0]), float(c[0])
+
+
+def camera_ray(cam: "Camera", x: int, y: int, sx: float = 0.5, sy: float = 0.5) -> np.ndarray:
+    """One pinhole ray (RAY_DTYPE record) through sensor point (x + sx, y + sy)."""
+    out = np.zeros(1, RAY_DTYPE)
+    lib().or_camera_ray(C.byref(cam), x, y, sx, sy, _p(out))
+    return out[0]
 
 
 def sample_sphere(r1: float, r2: float) -> np.ndarray:

@@ -255,16 +255,11 @@ typedef struct {
 
 /* Pinhole camera ray of a pixel sample (cameraRay,
  * include/tracing/CameraRay.hpp:6-34, with the build's Philox jitter). */
-static or_ray camera_ray(const or_camera *cam, uint64_t seed, uint32_t pixel, uint32_t sample)
+/* pinhole ray through (x + sx, y + sy) of the sensor (cameraRay,
+ * include/tracing/CameraRay.hpp:6-34): origin = translation column,
+ * direction = rotation x normalize(((p * pixel_size - center) / f, 1)) */
+void or_camera_ray(const or_camera *cam, uint32_t x, uint32_t y, float sx, float sy, or_ray *out)
 {
-    const uint32_t x = pixel % cam->width, y = pixel / cam->width;
-    float sx = 0.5f, sy = 0.5f;
-    if (cam->pixel_sampling == 1) {
-        uint32_t r[4];
-        path_rng(pixel, sample, 0xFFFFFFFFu, 0u, seed, r);
-        sx = or_u01(r[0]);
-        sy = or_u01(r[1]);
-    }
     const float pcx = ((float)x + sx) * cam->pixel_size[0] - cam->center[0];
     const float pcy = ((float)y + sy) * cam->pixel_size[1] - cam->center[1];
     const v3 cv = vnorm(mk(pcx / cam->focal_length, pcy / cam->focal_length, 1.0f));
@@ -276,6 +271,21 @@ static or_ray camera_ray(const or_camera *cam, uint64_t seed, uint32_t pixel, ui
     ray.dz = m[8] * cv.x + m[9] * cv.y + m[10] * cv.z;
     ray.tmin = 0.0f;
     ray.tmax = INFINITY;
+    *out = ray;
+}
+
+static or_ray camera_ray(const or_camera *cam, uint64_t seed, uint32_t pixel, uint32_t sample)
+{
+    const uint32_t x = pixel % cam->width, y = pixel / cam->width;
+    float sx = 0.5f, sy = 0.5f;
+    if (cam->pixel_sampling == 1) {
+        uint32_t r[4];
+        path_rng(pixel, sample, 0xFFFFFFFFu, 0u, seed, r);
+        sx = or_u01(r[0]);
+        sy = or_u01(r[1]);
+    }
+    or_ray ray;
+    or_camera_ray(cam, x, y, sx, sy, &ray);
     return ray;
 }
 

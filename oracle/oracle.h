@@ -108,6 +108,9 @@ typedef struct {
     float pixel_size[2];
     uint32_t width, height, pixel_sampling;
 } or_camera;                                                                               /* = dr_camera */
+/* Pinhole ray through sensor point (x + sx, y + sy), sx, sy in [0, 1]
+ * (Center sampling uses 0.5, 0.5). */
+void or_camera_ray(const or_camera *cam, uint32_t x, uint32_t y, float sx, float sy, or_ray *out);
 typedef struct {
     uint32_t spp, sample_offset, max_depth, rr_depth;
     uint64_t seed;
