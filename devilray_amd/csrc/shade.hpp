@@ -102,10 +102,13 @@ __device__ __forceinline__ V3 bary_point(V3 a, V3 b, V3 c, float u, float v)
     return add(add(a, mul(sub(b, a), u)), mul(sub(c, a), v));
 }
 
+// power heuristic (beta = 2); 0 when both densities are 0, as the reference's
+// misWeightedEmission gives no contribution then (test_lighting.cpp:53-59)
 __device__ __forceinline__ float power_heuristic(float a, float b)
 {
     const float a2 = a * a, b2 = b * b;
-    return a2 / (a2 + b2);
+    const float s = a2 + b2;
+    return s > 0.0f ? a2 / s : 0.0f;
 }
 
 // Fresnel reflectance for a ray arriving with cos_i from index n1 into n2.

@@ -21,6 +21,9 @@ HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("
 _vp = C.c_void_p
 _i32 = C.c_int32
 PROTOTYPES = {
+    "or_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
+    "or_emission_weight": (C.c_float, [C.c_int, C.c_float, C.c_float]),
+    "or_direct_light": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_float, _vp, _vp, _vp]),
     "or_camera_ray": (None, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _vp]),
     "or_philox4x32_10": (None, [_vp, _vp, _vp]),
     "or_u01": (C.c_float, [C.c_uint32]),
@@ -84,6 +87,22 @@ def camera_ray(cam: "Camera", x: int, y: int, sx: float = 0.5, sy: float = 0.5) 
     out = np.zeros(1, RAY_DTYPE)
     lib().or_camera_ray(C.byref(cam), x, y, sx, sy, _p(out))
     return out[0]
+
+
+def power_heuristic(a: float, b: float) -> float:
+    return lib().or_power_heuristic(a, b)
+
+
+def emission_weight(spec_prev: bool, pdf_bsdf: float, pdf_light: float) -> float:
+    return lib().or_emission_weight(1 if spec_prev else 0, pdf_bsdf, pdf_light)
+
+
+def direct_light(p, n, albedo, xl, nl, emission, pdf_area: float):
+    """(visible side?, unweighted Ld rgb, light solid-angle pdf, cosine bounce pdf)"""
+    args = [np.ascontiguousarray(x, np.float32) for x in (p, n, albedo, xl, nl, emission)]
+    ld, pl, pb = np.zeros(3, np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32)
+    ok = lib().or_direct_light(*[_p(a) for a in args], pdf_area, _p(ld), _p(pl), _p(pb))
+    return bool(ok), ld, float(pl[0]), float(pb[0])
 
 
 def sample_sphere(r1: float, r2: float) -> np.ndarray:

@@ -203,10 +203,60 @@ static inline v3 bary_point(v3 a, v3 b, v3 c, float u, float v)
     return vadd(vadd(a, vmul(vsub(b, a), u)), vmul(vsub(c, a), v));
 }
 
+/* power heuristic (beta = 2); 0 when both densities are 0 */
 static inline float power_heuristic(float a, float b)
 {
     const float a2 = a * a, b2 = b * b;
-    return a2 / (a2 + b2);
+    const float s = a2 + b2;
+    return s > 0.0f ? a2 / s : 0.0f;
+}
+
+float or_power_heuristic(float a, float b) { return power_heuristic(a, b); }
+
+/* MIS weight of emission reached by a bounce: 1 after a specular vertex or
+ * the camera, else the power heuristic of the bounce's pdf against the
+ * light sampler's solid-angle pdf (misWeightedEmission,
+ * PathGeneration.hpp:306-318) */
+float or_emission_weight(int spec_prev, float pdf_bsdf, float pdf_light)
+{
+    return spec_prev ? 1.0f : power_heuristic(pdf_bsdf, pdf_light);
+}
+
+/* Next-event estimate of one light point xl (normal nl, emission E, area pdf
+ * pa) seen from a Lambert point p (normal n, albedo alb), unoccluded and
+ * unweighted: Ld = E (alb / pi) cos_s |cos_l| / d^2 / pa, with the light's
+ * solid-angle pdf pn = pa d^2 / |cos_l| and the cosine bounce's pb = cos_s / pi
+ * (evaluateDirectLighting, PathGeneration.hpp:336-361). Returns 0 (and no
+ * estimate) when the light is behind the surface. */
+static inline int nee_terms(v3 p, v3 n, v3 alb, v3 xl, v3 nl, const float *E, float pa, v3 *Ld, float *pn,
+                            float *pb, float *dist, v3 *om)
+{
+    const v3 toL = vsub(xl, p);
+    const float dist2 = vdot(toL, toL);
+    *dist = sqrtf(dist2);
+    *om = vdiv(toL, *dist);
+    const float cs = vdot(*om, n);
+    const float acl = fabsf(vdot(*om, nl));
+    if (!(cs > 0.0f && pa > 0.0f && *dist > 0.0f)) return 0;
+    const float G = cs * acl / dist2;
+    *pn = acl < 1e-6f ? 0.0f : pa * dist2 / acl;
+    *pb = cs / kPi;
+    *Ld = mk(E[0] * (alb.x / kPi) * G / pa, E[1] * (alb.y / kPi) * G / pa, E[2] * (alb.z / kPi) * G / pa);
+    return 1;
+}
+
+int or_direct_light(const float p[3], const float n[3], const float albedo[3], const float xl[3], const float nl[3],
+                    const float emission[3], float pdf_area, float ld[3], float *pdf_light, float *pdf_bsdf)
+{
+    v3 L = mk(0.0f, 0.0f, 0.0f), om;
+    float pn = 0.0f, pb = 0.0f, dist;
+    const int ok = nee_terms(mk(p[0], p[1], p[2]), mk(n[0], n[1], n[2]), mk(albedo[0], albedo[1], albedo[2]),
+                             mk(xl[0], xl[1], xl[2]), mk(nl[0], nl[1], nl[2]), emission, pdf_area, &L, &pn, &pb,
+                             &dist, &om);
+    ld[0] = L.x; ld[1] = L.y; ld[2] = L.z;
+    *pdf_light = pn;
+    *pdf_bsdf = pb;
+    return ok;
 }
 
 static inline float fresnel(float cos_i, float n1, float n2, float cos_t, int schlick)
@@ -323,12 +373,12 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
         if (mat.kind == 0) {
             const v3 E = mk(mat.emission[0], mat.emission[1], mat.emission[2]);
             if (E.x != 0.0f || E.y != 0.0f || E.z != 0.0f) {
-                float w = 1.0f;
+                float pl = 0.0f;
                 if (!spec_prev) {
                     const float cl = fabsf(vdot(d, ng));
-                    const float pl = cl < 1e-6f ? 0.0f : rc->pdf_area[tri] * (h.t * h.t) / cl;
-                    w = power_heuristic(pdf_prev, pl);
+                    pl = cl < 1e-6f ? 0.0f : rc->pdf_area[tri] * (h.t * h.t) / cl;
                 }
+                const float w = or_emission_weight(spec_prev, pdf_prev, pl);
                 L = vadd(L, mk(beta.x * E.x * w, beta.y * E.y * w, beta.z * E.z * w));
             }
             const v3 alb = mk(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
@@ -345,23 +395,13 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
                     const v3 xl = triangle_point(LA, LB, LC, or_u01(r1[2]), or_u01(r1[3]));
                     const v3 nl = vnorm(vcross(vsub(LB, LA), vsub(LC, LA)));
                     const float pa = rc->pdf_area[lt];
-                    const v3 toL = vsub(xl, p);
-                    const float dist2 = vdot(toL, toL);
-                    const float dist = sqrtf(dist2);
-                    const v3 om = vdiv(toL, dist);
-                    const float cs = vdot(om, n);
-                    const float acl = fabsf(vdot(om, nl));
-                    if (cs > 0.0f && pa > 0.0f && dist > 0.0f) {
-                        const or_material lm = rc->mats[rc->tri_mat ? rc->tri_mat[lt] : 0];
-                        const float G = cs * acl / dist2;
-                        const float pn = acl < 1e-6f ? 0.0f : pa * dist2 / acl;
-                        const float pb = cs / kPi;
+                    const or_material lm = rc->mats[rc->tri_mat ? rc->tri_mat[lt] : 0];
+                    v3 Ld, om;
+                    float pn = 0.0f, pb = 0.0f, dist;
+                    if (nee_terms(p, n, alb, xl, nl, lm.emission, pa, &Ld, &pn, &pb, &dist, &om)) {
                         const float w = power_heuristic(pn, pb);
                         const float tmax = dist - 2.0f * fmaxf(eps, origin_eps(LA, LB, LC));
                         if (pn > 0.0f && tmax > 0.0f) {
-                            const v3 Ld = mk(lm.emission[0] * (alb.x / kPi) * G / pa,
-                                             lm.emission[1] * (alb.y / kPi) * G / pa,
-                                             lm.emission[2] * (alb.z / kPi) * G / pa);
                             const v3 c = mk(beta.x * Ld.x * w, beta.y * Ld.y * w, beta.z * Ld.z * w);
                             const v3 so = vadd(p, vmul(om, eps));
                             or_ray sr = { so.x, so.y, so.z, 0.0f, om.x, om.y, om.z, tmax };

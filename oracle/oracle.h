@@ -131,6 +131,14 @@ void or_sincos_2pi(float u, float *s, float *c);
  * sphere sample + n, the reference's PathGeneration.hpp:406-420), and the
  * area-uniform point of a triangle. r1, r2 in [0, 1). */
 void or_sample_sphere(float r1, float r2, float out[3]);
+/* Lighting terms of the integrator, exported for their own tests: the power
+ * heuristic, the MIS weight of emission reached by a bounce, and one
+ * next-event estimate (unoccluded, unweighted; returns 0 when the light is
+ * behind the surface). */
+float or_power_heuristic(float a, float b);
+float or_emission_weight(int spec_prev, float pdf_bsdf, float pdf_light);
+int or_direct_light(const float p[3], const float n[3], const float albedo[3], const float xl[3], const float nl[3],
+                    const float emission[3], float pdf_area, float ld[3], float *pdf_light, float *pdf_bsdf);
 void or_sample_cosine(const float n[3], float r1, float r2, float out[3]);
 void or_sample_triangle(const float a[3], const float b[3], const float c[3], float r1, float r2, float out[3]);
 
