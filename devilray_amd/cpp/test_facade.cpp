@@ -3,7 +3,8 @@
 // (test/src/test_intersection.cpp), nearest-of-stacked and cast
 // (test/src/test_tracing.cpp:221-298) and the analytic diffuse reflection
 // render (test/src/test_render.cpp:50-165), the mesh loader
-// (test/src/test_mesh.cpp) and the running average
+// (test/src/test_mesh.cpp), quad and scene structures
+// (test/src/test_scene.cpp:39-77) and the running average
 // (test/src/test_running_average.cpp); then the façade's own cases for the
 // render statistics and the background render loop of the interactive
 // application (renderer/src/RenderThread.cpp:3-12). Prints one line per case
@@ -441,6 +442,42 @@ int main(int argc, char **argv)
         double sum = 0;
         for (int i = 0; i < res.width * res.height; ++i) sum += px[i].x;
         EXPECT(sum == 0.0 && px[0].w == 2.0f && renderer.getSampleCount() == 2);
+    });
+    // ---- test/src/test_scene.cpp, restated ----
+    run("SceneTest.CreateQuadMeshProducesAxisAlignedSquare", [&] {
+        const Mesh quad = createQuadMesh({0, 0, 0}, {0, 0, 1}, {1, 0, 0}, 2.0f);
+        EXPECT(quad.points.size() == 4 && quad.triangles.size() == 2);
+        double area = 0;
+        for (const Vec3 &p : quad.points)
+            EXPECT(std::fabs(std::fabs(p.x) - 1.0f) <= 1e-5f && std::fabs(std::fabs(p.y) - 1.0f) <= 1e-5f &&
+                   std::fabs(p.z) <= 1e-5f);
+        for (const auto &t : quad.triangles) {
+            const Vec3 a = quad.points[t[0]], b = quad.points[t[1]], c = quad.points[t[2]];
+            const double ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z, vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+            const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        }
+        EXPECT_NEAR(area, 4.0, 1e-5);
+    });
+    run("SceneTest.BuildCopiesGeometryAndBuildsStructures", [&] {
+        // a tetrahedron: the device scene holds its triangles, a BVH, and a light
+        // table with one entry per triangle when it emits, none when it does not
+        Mesh tet;
+        tet.points = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        tet.triangles = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
+        for (int emits = 1; emits >= 0; --emits) {
+            Scene s;
+            s.materials.push_back(emits ? Material::diffuse({0.5f, 0.5f, 0.5f}, {1, 1, 1})
+                                        : Material::diffuse({0.5f, 0.5f, 0.5f}));
+            s.objects.push_back({tet, 0});
+            s.build(ctx);
+            dr_scene_info info{};
+            check(dr_scene_get_info(s.handle(), &info));
+            uint32_t n_emit = 0;
+            check(dr_scene_light_count(s.handle(), &n_emit));
+            EXPECT(s.triangleCount() == 4 && info.n_triangles == 4 && info.n_nodes > 0);
+            EXPECT(n_emit == (emits ? 4u : 0u));
+        }
     });
     // ---- test/src/test_running_average.cpp, restated ----
     run("RunningAverageTest.EmptyAverageIsZero", [&] { EXPECT(RunningAverage(5).mean() == 0.0); });
