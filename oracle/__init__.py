@@ -44,6 +44,9 @@ PROTOTYPES = {
     "or_refbbh_build": (_vp, [_vp, _i32, _vp, _i32]),
     "or_refbbh_free": (None, [_vp]),
     "or_refbbh_depth": (_i32, [_vp]),
+    "or_refbbh_node_count": (_i32, [_vp]),
+    "or_refbbh_nodes": (None, [_vp, _vp, _vp]),
+    "or_refbbh_order": (None, [_vp, _i32, _vp]),
     "or_refbbh_trace": (None, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32]),
     "or_lights_build": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "or_sincos_2pi": (None, [C.c_float, _vp, _vp]),
@@ -330,6 +333,15 @@ class RefBBH:
 
     def depth(self) -> int:
         return lib().or_refbbh_depth(self.h)
+
+    def nodes(self):
+        """(boxes float32 [n, 6] lo/hi, links int32 [n, 5] left/right/parent/tb/te, triangle order)"""
+        n = lib().or_refbbh_node_count(self.h)
+        boxes, links = np.zeros((n, 6), np.float32), np.zeros((n, 5), np.int32)
+        lib().or_refbbh_nodes(self.h, _p(boxes), _p(links))
+        order = np.zeros(self.f.shape[0], np.int32)
+        lib().or_refbbh_order(self.h, self.f.shape[0], _p(order))
+        return boxes, links, order
 
     def closest(self, rays, counts: bool = False, nthreads: int | None = None):
         n = rays.shape[0]

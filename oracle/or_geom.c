@@ -749,6 +749,31 @@ void or_refbbh_free(void *h)
 
 int32_t or_refbbh_depth(const void *h) { return ((const rbbh *)h)->depth; }
 
+int32_t or_refbbh_node_count(const void *h) { return ((const rbbh *)h)->n; }
+
+void or_refbbh_nodes(const void *h, float *lo_hi, int32_t *links)
+{
+    const rbbh *b = (const rbbh *)h;
+    for (int32_t i = 0; i < b->n; ++i) {
+        const rnode *nd = b->nodes + i;
+        for (int a = 0; a < 3; ++a) {
+            lo_hi[6 * (size_t)i + a] = nd->lo[a];
+            lo_hi[6 * (size_t)i + 3 + a] = nd->hi[a];
+        }
+        links[5 * (size_t)i] = nd->left;
+        links[5 * (size_t)i + 1] = nd->right;
+        links[5 * (size_t)i + 2] = nd->parent;
+        links[5 * (size_t)i + 3] = nd->tb;
+        links[5 * (size_t)i + 4] = nd->te;
+    }
+}
+
+void or_refbbh_order(const void *h, int32_t ntris, int32_t *order)
+{
+    const rbbh *b = (const rbbh *)h;
+    for (int32_t i = 0; i < ntris; ++i) order[i] = b->order[i];
+}
+
 typedef struct {
     const rbbh *b;
     const float *verts;

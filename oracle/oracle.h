@@ -92,6 +92,13 @@ void or_trace_brute(const float *verts, const int32_t *tris, int32_t ntris, cons
 void *or_refbbh_build(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris);
 void or_refbbh_free(void *bbh);
 int32_t or_refbbh_depth(const void *bbh);
+/* The nodes in the order generateSimpleBBH creates them (depth first, root
+ * 0): lo xyz, hi xyz per node, and left, right (-1: none), parent and the
+ * triangle range [tb, te) over the reordered triangles; order[i] is the
+ * caller's id of reordered triangle i. */
+int32_t or_refbbh_node_count(const void *bbh);
+void or_refbbh_nodes(const void *bbh, float *lo_hi, int32_t *links);
+void or_refbbh_order(const void *bbh, int32_t ntris, int32_t *order);
 /* hits: id = input triangle index or -1, t, u = wB, v = wC. box_tests /
  * tri_tests: the reference HitTests counters (include/tracing/Benchmark.hpp). */
 void or_refbbh_trace(const void *bbh, const float *verts, const int32_t *tris, const or_ray *rays,
