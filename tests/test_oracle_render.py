@@ -64,6 +64,35 @@ def test_alias_table_implied_distribution(n, sigma):
     assert np.allclose(pdf * area, target, rtol=1e-5)
 
 
+# test/src/test_alias.cpp:64-229: single element, uniform, simple biased,
+# alternating high/low and the 200,000-entry log-normal stress case (seed 42);
+# the empty and zero-sum cases are scenes without emitters, tested below
+@pytest.mark.parametrize("case", ["single", "uniform", "biased", "alternating", "stress"])
+def test_alias_table_reference_cases(case):
+    if case == "single":
+        w = np.float64([3.0])
+    elif case == "uniform":
+        w = np.ones(10)
+    elif case == "biased":
+        w = np.float64([1.0, 3.0])
+    elif case == "alternating":
+        w = np.tile([100.0, 1.0], 50)
+    else:
+        w = np.random.default_rng(42).lognormal(0.0, 1.0, 200_000)
+    v, f = weighted_triangles(w)
+    mats = oracle.materials_array([(0, (0, 0, 0), (1, 1, 1), 1.0)])
+    ent, _ = oracle.lights_build(v, f, np.zeros(len(w), np.int32), mats)
+    vv = v.astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(vv[1::3] - vv[0::3], vv[2::3] - vv[0::3]), axis=1)
+    implied = implied_distribution(ent)
+    tol = 1e-3 if case == "stress" else 1e-4     # the reference's bounds
+    assert len(ent) == len(w) and np.max(np.abs(implied - area / area.sum())) < tol / len(w) + 1e-7
+    if case == "single":
+        assert ent["p_a"][0] == 1.0
+    if case == "biased":
+        assert np.allclose(implied, [0.25, 0.75], atol=1e-6)
+
+
 def test_lights_skip_non_emitters_and_dielectrics():
     sc = scenes.cornell_c0()
     ent, pdf = oracle.lights_build(sc.vertices, sc.indices, sc.tri_material, oracle.materials_array(sc.materials))
