@@ -199,7 +199,7 @@ Context::~Context()
 
 Scene::Scene(Scene &&o) noexcept
     : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_),
-      n_points_(o.n_points_)
+      n_points_(o.n_points_), world_(std::move(o.world_)), idx_(std::move(o.idx_)), tri_mat_(std::move(o.tri_mat_))
 {
     o.scene_ = nullptr;
 }
@@ -213,6 +213,9 @@ Scene &Scene::operator=(Scene &&o) noexcept
         scene_ = o.scene_;
         n_tris_ = o.n_tris_;
         n_points_ = o.n_points_;
+        world_ = std::move(o.world_);
+        idx_ = std::move(o.idx_);
+        tri_mat_ = std::move(o.tri_mat_);
         o.scene_ = nullptr;
     }
     return *this;
@@ -250,6 +253,7 @@ void Scene::refit(Context &ctx)
     if (verts.size() != 3ull * n_points_ || tris != n_tris_)
         throw Error(DR_E_INVALID, "refit needs the objects and triangles of the last build");
     check(dr_scene_refit(scene_, verts.data(), nullptr));
+    world_ = verts;
 }
 
 void Scene::build(Context &ctx, uint32_t leaf_max)
@@ -287,6 +291,9 @@ void Scene::build(Context &ctx, uint32_t leaf_max)
     check(dr_scene_build_bvh(scene_, &bp, nullptr));
     n_tris_ = d.n_triangles;
     n_points_ = d.n_vertices;
+    world_ = verts;
+    idx_ = std::move(idx);
+    tri_mat_ = std::move(tri_mat);
 }
 
 namespace {
@@ -313,9 +320,29 @@ std::vector<std::optional<Intersection>> cast(Context &ctx, const Scene &scene, 
     std::vector<dr_hit> hits(n);
     check(dr_memcpy_dtoh(ctx.handle(), hits.data(), dh.ptr, sizeof(dr_hit) * n));
     std::vector<std::optional<Intersection>> out(n);
-    for (size_t i = 0; i < n; ++i)
-        if (hits[i].id >= 0)
-            out[i] = Intersection{hits[i].t, Vec3{1.0f - hits[i].u - hits[i].v, hits[i].u, hits[i].v}, hits[i].id};
+    const std::vector<float> &w = scene.worldVertices();
+    const std::vector<uint32_t> &ix = scene.worldIndices();
+    for (size_t i = 0; i < n; ++i) {
+        if (hits[i].id < 0) continue;
+        const dr_hit &h = hits[i];
+        Intersection x{h.t, Vec3{1.0f - h.u - h.v, h.u, h.v}, h.id};
+        const size_t tri = (size_t)h.id;
+        const float *A = &w[3 * (size_t)ix[3 * tri]], *B = &w[3 * (size_t)ix[3 * tri + 1]],
+                    *C = &w[3 * (size_t)ix[3 * tri + 2]];
+        const float ab[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]}, ac[3] = {A[0] - C[0], A[1] - C[1], A[2] - C[2]};
+        const float nf[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+        const float len = std::sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+        const Vec3 &v = rays[i].v;
+        const float side = nf[0] * v.x + nf[1] * v.y + nf[2] * v.z;
+        x.ccw = side > 0.0f;
+        const float s = (side > 0.0f ? -1.0f : 1.0f) / (len > 0.0f ? len : 1.0f);
+        x.n = Vec3{nf[0] * s, nf[1] * s, nf[2] * s};
+        x.area = 0.5f * len;
+        x.p = Vec3{A[0] + (B[0] - A[0]) * h.u + (C[0] - A[0]) * h.v, A[1] + (B[1] - A[1]) * h.u + (C[1] - A[1]) * h.v,
+                   A[2] + (B[2] - A[2]) * h.u + (C[2] - A[2]) * h.v};
+        x.mat = scene.triangleMaterials()[tri];
+        out[i] = x;
+    }
     return out;
 }
 

@@ -53,11 +53,21 @@ struct Ray {
 
 // Closest hit: t and the barycentric weights (w_A, w_B, w_C) of the
 // triangle's vertices, as the reference's TriangleIntersection
-// (include/tracing/Intersection.hpp:12-20), plus the triangle id.
+// (include/tracing/Intersection.hpp:12-20), plus the triangle id; and the
+// reference's Intersection fields (Intersection.hpp:31-40): the world hit
+// point, the unit face normal turned towards the incoming ray, the world
+// triangle area, ccw = ((A - B) x (A - C)) . v > 0, and the object's material.
+// (The reference scales a model-space normal by 1/S without renormalising;
+// here the normal is that of the placed triangle, unit length.)
 struct Intersection {
     float t;
     Vec3 bari;
     int32_t triangle;
+    Vec3 p{0, 0, 0};
+    Vec3 n{0, 0, 0};
+    float area = 0.0f;
+    bool ccw = false;
+    uint32_t mat = 0;
 };
 
 // DiffuseMaterial / TransparentMaterial (include/tracing/Material.hpp:9-25).
@@ -175,11 +185,18 @@ public:
     void refit(Context &ctx);
     dr_scene *handle() const { return scene_; }
     uint32_t triangleCount() const { return n_tris_; }
+    // the placed triangles as built (or last refitted): xyz per point,
+    // point indices and material per triangle
+    const std::vector<float> &worldVertices() const { return world_; }
+    const std::vector<uint32_t> &worldIndices() const { return idx_; }
+    const std::vector<uint32_t> &triangleMaterials() const { return tri_mat_; }
 
 private:
     std::vector<float> worldPoints() const;
     dr_scene *scene_ = nullptr;
     uint32_t n_tris_ = 0, n_points_ = 0;
+    std::vector<float> world_;
+    std::vector<uint32_t> idx_, tri_mat_;
 };
 
 // Closest hit for each ray (std::nullopt on a miss); triangle ids index the

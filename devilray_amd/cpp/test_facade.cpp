@@ -1,7 +1,7 @@
 // The reference's own GoogleTest cases for the render path, restated against
 // the C++ facade (needs a GPU): triangle intersection cases
-// (test/src/test_intersection.cpp), nearest-of-stacked and cast
-// (test/src/test_tracing.cpp:221-298) and the analytic diffuse reflection
+// (test/src/test_intersection.cpp), mesh intersections, nearest-of-stacked
+// and cast (test/src/test_tracing.cpp:147-298) and the analytic diffuse reflection
 // render (test/src/test_render.cpp:50-165), the mesh loader
 // (test/src/test_mesh.cpp), quad and scene structures
 // (test/src/test_scene.cpp:39-77) and the running average
@@ -442,6 +442,47 @@ int main(int argc, char **argv)
         double sum = 0;
         for (int i = 0; i < res.width * res.height; ++i) sum += px[i].x;
         EXPECT(sum == 0.0 && px[0].w == 2.0f && renderer.getSampleCount() == 2);
+    });
+    // ---- test/src/test_tracing.cpp:147-219 (MeshIntersectionTest), restated ----
+    auto unit_object = [&](Transform tf) {
+        Scene s;
+        s.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
+        s.materials.push_back(Material::diffuse({0.2f, 0.2f, 0.2f}));
+        Mesh m;
+        m.points = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+        m.triangles = {{0, 1, 2}};
+        Scene::Object o{m, 1};
+        o.transform = tf;
+        s.objects.push_back(o);
+        s.build(ctx);
+        return s;
+    };
+    auto near3 = [](Vec3 a, Vec3 b, float tol) {
+        return std::fabs(a.x - b.x) <= tol && std::fabs(a.y - b.y) <= tol && std::fabs(a.z - b.z) <= tol;
+    };
+    run("MeshIntersectionTest.HitsSingleTriangle", [&] {
+        Scene s = unit_object({});
+        const auto hit = cast1(ctx, s, {0.2f, 0.2f, 5}, {0, 0, -1});
+        EXPECT(hit.has_value() && std::fabs(hit->t - 5.0f) <= 1e-5f && near3(hit->p, {0.2f, 0.2f, 0}, 1e-5f));
+        EXPECT(hit && hit->area == 0.5f && hit->mat == 1u);
+        EXPECT(!cast1(ctx, s, {5, 5, 5}, {0, 0, -1}).has_value());
+    });
+    run("MeshIntersectionTest.AppliesModelToWorldTranslation", [&] {
+        Scene s = unit_object(Transform{{1, 1, 1}, {10, 0, 0}});
+        const auto hit = cast1(ctx, s, {10.2f, 0.2f, 5}, {0, 0, -1});
+        EXPECT(hit.has_value() && near3(hit->p, {10.2f, 0.2f, 0}, 1e-5f));
+    });
+    run("MeshIntersectionTest.WorldSpaceAreaScalesWithTransform", [&] {
+        Scene s = unit_object(Transform{{2, 3, 1}, {0, 0, 0}});
+        const auto hit = cast1(ctx, s, {0.2f, 0.2f, 5}, {0, 0, -1});
+        EXPECT(hit.has_value() && std::fabs(hit->area - 3.0f) <= 1e-5f);
+    });
+    run("MeshIntersectionTest.NormalFlipsTowardIncomingRayAndWindingFollowsIt", [&] {
+        Scene s = unit_object({});
+        const auto above = cast1(ctx, s, {0.2f, 0.2f, 5}, {0, 0, -1});
+        const auto below = cast1(ctx, s, {0.2f, 0.2f, -5}, {0, 0, 1});
+        EXPECT(above.has_value() && near3(above->n, {0, 0, 1}, 1e-6f) && !above->ccw);
+        EXPECT(below.has_value() && near3(below->n, {0, 0, -1}, 1e-6f) && below->ccw);
     });
     // ---- test/src/test_scene.cpp, restated ----
     run("SceneTest.CreateQuadMeshProducesAxisAlignedSquare", [&] {
