@@ -199,7 +199,8 @@ Context::~Context()
 
 Scene::Scene(Scene &&o) noexcept
     : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_),
-      n_points_(o.n_points_), world_(std::move(o.world_)), idx_(std::move(o.idx_)), tri_mat_(std::move(o.tri_mat_))
+      n_points_(o.n_points_), built_(o.built_), world_(std::move(o.world_)), idx_(std::move(o.idx_)),
+      tri_mat_(std::move(o.tri_mat_))
 {
     o.scene_ = nullptr;
 }
@@ -213,6 +214,7 @@ Scene &Scene::operator=(Scene &&o) noexcept
         scene_ = o.scene_;
         n_tris_ = o.n_tris_;
         n_points_ = o.n_points_;
+        built_ = o.built_;
         world_ = std::move(o.world_);
         idx_ = std::move(o.idx_);
         tri_mat_ = std::move(o.tri_mat_);
@@ -286,12 +288,24 @@ void Scene::build(Context &ctx, uint32_t leaf_max)
         dr_scene_destroy(scene_);
         scene_ = nullptr;
     }
-    check(dr_scene_create(ctx.handle(), &d, &scene_));
-    dr_bvh_params bp{leaf_max, 0};
-    check(dr_scene_build_bvh(scene_, &bp, nullptr));
     n_tris_ = d.n_triangles;
     n_points_ = d.n_vertices;
     world_ = verts;
+    built_ = true;
+    if (d.n_triangles == 0) return;  // nothing to trace: cast misses (CastTest.EmptySceneMisses)
+    check(dr_scene_create(ctx.handle(), &d, &scene_));
+    dr_bvh_params bp{leaf_max, 0};
+    check(dr_scene_build_bvh(scene_, &bp, nullptr));
+    bool any_debug = false;
+    for (const Material &m : materials) any_debug = any_debug || m.debug_color.has_value();
+    if (any_debug) {
+        std::vector<float> rgba;
+        for (const Material &m : materials) {
+            const Vec4 c = m.debug_color.value_or(Vec4{m.m.albedo[0], m.m.albedo[1], m.m.albedo[2], 0.0f});
+            rgba.insert(rgba.end(), {c.x, c.y, c.z, c.w});
+        }
+        check(dr_scene_set_debug_colors(scene_, rgba.data(), (uint32_t)materials.size()));
+    }
     idx_ = std::move(idx);
     tri_mat_ = std::move(tri_mat);
 }
@@ -307,8 +321,9 @@ struct DeviceBuffer {
 
 std::vector<std::optional<Intersection>> cast(Context &ctx, const Scene &scene, const std::vector<Ray> &rays)
 {
-    if (!scene.handle()) throw Error(DR_E_INVALID, "scene not built");
+    if (!scene.built()) throw Error(DR_E_INVALID, "scene not built");
     const size_t n = rays.size();
+    if (!scene.handle() || n == 0) return std::vector<std::optional<Intersection>>(n);  // no triangles
     std::vector<dr_ray> in(n);
     for (size_t i = 0; i < n; ++i) {
         const Ray &r = rays[i];
@@ -378,8 +393,8 @@ void Renderer::setScene(Scene scene)
     std::scoped_lock frame{frame_mutex_};
     std::scoped_lock guard{render_mutex_};
     scene_ = std::move(scene);
-    if (!scene_.handle()) scene_.build(ctx_);
-    check(dr_scene_build_lights(scene_.handle(), nullptr));
+    if (!scene_.built()) scene_.build(ctx_);
+    if (scene_.handle()) check(dr_scene_build_lights(scene_.handle(), nullptr));
     needs_clear_ = true;
 }
 
@@ -452,11 +467,11 @@ void Renderer::render(uint32_t spp)
     uint32_t encoding;
     {
         std::scoped_lock guard{render_mutex_};
-        if (!scene_.handle()) throw Error(DR_E_INVALID, "no scene set");
+        if (!scene_.built()) throw Error(DR_E_INVALID, "no scene set");
         if (needs_clear_) {
             check(dr_fill_zero(ctx_.handle(), accum_, (size_t)res_.width * res_.height * 4 * sizeof(double), nullptr));
             uint64_t unused;
-            check(dr_scene_ray_casts(scene_.handle(), &unused, 1, nullptr));
+            if (scene_.handle()) check(dr_scene_ray_casts(scene_.handle(), &unused, 1, nullptr));
             samples_ = 0;
             total_casts_ = 0;
             std::scoped_lock stats{stats_mutex_};
@@ -482,8 +497,13 @@ void Renderer::render(uint32_t spp)
         encoding = output_.linearity == OutputLinearity::GammaCorrected ? DR_OUTPUT_SRGB : DR_OUTPUT_LINEAR;
     }
     const auto t0 = std::chrono::steady_clock::now();
-    if (debug == DebugOptions::Off) check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
-    else check(dr_render_debug(scene_.handle(), &c, &p, (uint32_t)debug, (double *)accum_, nullptr));
+    if (!scene_.handle()) {
+        // a scene without triangles: every path misses, the image stays black
+    } else if (debug == DebugOptions::Off) {
+        check(dr_render(scene_.handle(), &c, &p, (double *)accum_, nullptr));
+    } else {
+        check(dr_render_debug(scene_.handle(), &c, &p, (uint32_t)debug, (double *)accum_, nullptr));
+    }
     createPixelsLocked(encoding);  // synchronises
     samples_ += spp;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -492,7 +512,7 @@ void Renderer::render(uint32_t spp)
         render_times_.add(ms);
     }
     uint64_t casts = 0;
-    check(dr_scene_ray_casts(scene_.handle(), &casts, 0, nullptr));
+    if (scene_.handle()) check(dr_scene_ray_casts(scene_.handle(), &casts, 0, nullptr));
     total_casts_ = casts;
 }
 

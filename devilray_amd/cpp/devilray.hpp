@@ -70,11 +70,13 @@ struct Intersection {
     uint32_t mat = 0;
 };
 
-// DiffuseMaterial / TransparentMaterial (include/tracing/Material.hpp:9-25).
+// DiffuseMaterial / TransparentMaterial (include/tracing/Material.hpp:9-25),
+// with the debug colour the UV-checker view shows (the albedo when unset).
 struct Material {
     static Material diffuse(Vec3 reflectance, Vec3 emission = {0, 0, 0});
     static Material transparent(float ior);
     dr_material m{};
+    std::optional<Vec4> debug_color;
 };
 
 // Mesh (include/models/Mesh.hpp:7-27): positions, normals, and per triangle
@@ -185,6 +187,7 @@ public:
     void refit(Context &ctx);
     dr_scene *handle() const { return scene_; }
     uint32_t triangleCount() const { return n_tris_; }
+    bool built() const { return built_; }
     // the placed triangles as built (or last refitted): xyz per point,
     // point indices and material per triangle
     const std::vector<float> &worldVertices() const { return world_; }
@@ -195,6 +198,7 @@ private:
     std::vector<float> worldPoints() const;
     dr_scene *scene_ = nullptr;
     uint32_t n_tris_ = 0, n_points_ = 0;
+    bool built_ = false;  // (a scene built without triangles has no device scene)
     std::vector<float> world_;
     std::vector<uint32_t> idx_, tri_mat_;
 };

@@ -484,6 +484,47 @@ int main(int argc, char **argv)
         EXPECT(above.has_value() && near3(above->n, {0, 0, 1}, 1e-6f) && !above->ccw);
         EXPECT(below.has_value() && near3(below->n, {0, 0, -1}, 1e-6f) && below->ccw);
     });
+    run("CastTest.ReturnsNearestObjectAndEmptySceneMisses", [&] {  // test_tracing.cpp:276-298
+        Scene s;
+        for (int k = 0; k < 23; ++k) s.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
+        Mesh m;
+        m.points = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+        m.triangles = {{0, 1, 2}};
+        s.objects.push_back({m, 11});
+        s.objects.push_back({m, 22});
+        s.objects[1].setPosition({0, 0, -3});
+        s.build(ctx);
+        const auto hit = cast1(ctx, s, {0.2f, 0.2f, 5}, {0, 0, -1});
+        EXPECT(hit.has_value() && hit->mat == 11u && std::fabs(hit->t - 5.0f) <= 1e-5f);
+        Scene empty;
+        empty.build(ctx);
+        EXPECT(!cast1(ctx, empty, {0, 0, 0}, {0, 0, -1}).has_value());
+    });
+    run("RendererTest.DebugRenderSquare", [&] {  // test_render.cpp:168-237
+        const Size2i res{512, 512};
+        Renderer renderer(ctx, res);
+        renderer.setCamera(pinhole(res));
+        Scene scene;
+        Material white = Material::diffuse({1, 1, 1});
+        white.debug_color = Vec4{0.9f, 0.7f, 0.5f, 0.0f};
+        scene.materials.push_back(white);
+        scene.objects.push_back({createQuadMesh({0, 0, 1}, {0, 0, -1}, {1, 0, 0}, 1.0f), 0});
+        renderer.setScene(std::move(scene));
+        renderer.setOutputOptions({OutputLinearity::GammaCorrected});
+        renderer.setDebug(DebugOptions::UVChecker);
+        renderer.render();
+        if (!out_dir.empty()) EXPECT(renderer.saveImage(out_dir + "/debug_render_1_ray.png"));
+        const uint32_t *px = renderer.getPixels();
+        bool ok = true;
+        for (int i = 0; i < res.height; ++i)
+            for (int j = 0; j < res.width; ++j) {
+                const uint32_t p = px[i * res.width + j];
+                const float r = (p & 0xFF) / 255.0f, g = ((p >> 8) & 0xFF) / 255.0f, b = ((p >> 16) & 0xFF) / 255.0f;
+                const bool rim = i < 6 || j < 6 || i >= res.height - 6 || j >= res.width - 6;
+                ok &= rim ? (r == 0.0f && g == 0.0f && b == 0.0f) : (r >= 0.65f && g >= 0.6f && b >= 0.5f);
+            }
+        EXPECT(ok);
+    });
     // ---- test/src/test_scene.cpp, restated ----
     run("SceneTest.CreateQuadMeshProducesAxisAlignedSquare", [&] {
         const Mesh quad = createQuadMesh({0, 0, 0}, {0, 0, 1}, {1, 0, 0}, 2.0f);
