@@ -289,7 +289,8 @@ def time_for(curve, target):
 
 
 def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, checkpoint, train_steps, max_depth,
-                    capacity=1 << 18, train_passes=None, seeds=3, train_batch=0, lr_schedule=None, guide_after=1):
+                    capacity=1 << 18, train_passes=None, seeds=3, train_batch=0, lr_schedule=None, guide_after=1,
+                    selection=dr.DR_GUIDE_SELECT_KL):
     """Equal-quality time: the plain sampler renders passes x spp; the target
     is its final MSE against a ref_spp reference (the same plain path, FP64
     accumulated); the guided sampler (trained online while it renders)
@@ -314,6 +315,7 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     for k in range(seeds):
         g = dr.Guide(ctx, s.bounds(), capacity=capacity, seed=3 + k)
         g.set_options(1.0, 64, train_batch)
+        g.set_selection_objective(selection)
         # the guided run k uses plain run k's seed (common random numbers): the
         # two arms share camera rays, light choices and every sample the guide
         # does not change, so their MSE ratio is far less noisy; the ratio
@@ -335,6 +337,7 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
             "spp_per_pass": spp_per_pass, "plain_spp": spp_per_pass * passes, "train_steps_per_pass": train_steps,
             "train_passes": train_passes, "train_batch": train_batch, "seeds": seeds,
             "lr_schedule": "1e-2 / (1 + pass / 4)" if lr_schedule is not None else "constant",
+            "selection_objective": "second moment" if selection == dr.DR_GUIDE_SELECT_VARIANCE else "KL",
             "guided_over_plain_mse_equal_spp": round(guided[n_ck - 1][1] / target, 3),
             "equal_spp_mse_ratio_per_checkpoint": [round(x, 3) for x in eq],
             "guided_time_per_sample_over_plain": round(cost, 3),

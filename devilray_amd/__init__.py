@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import (  # noqa: F401
     ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
-    DR_GUIDE_COLLECT, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
+    DR_GUIDE_COLLECT, DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
     GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
 )
@@ -442,6 +442,11 @@ class Guide:
 
     def set_learning_rate(self, lr: float):
         check(lib().dr_guide_set_learning_rate(self.handle, lr))
+
+    def set_selection_objective(self, objective: int):
+        """DR_GUIDE_SELECT_KL (default) or DR_GUIDE_SELECT_VARIANCE for the
+        selection logit's training loss."""
+        check(lib().dr_guide_set_selection_objective(self.handle, int(objective)))
 
     def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64, train_batch: int = 0):
         check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices, train_batch))

@@ -86,6 +86,7 @@ class DrGuideParams(C.Structure):
 GUIDE_N_PARAMS = 17712
 GUIDE_HEAD_FLOATS = 44
 DR_GUIDE_USE, DR_GUIDE_COLLECT = 1, 2
+DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE = 0, 1
 DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB = 0, 1
 DR_DEBUG_UV_CHECKER, DR_DEBUG_BARYCENTRIC, DR_DEBUG_WINDING = 1, 2, 3
 GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
@@ -149,6 +150,7 @@ PROTOTYPES = {
     "dr_guide_copy_samples": (_st, [_vp, _vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
     "dr_guide_set_options": (_st, [_vp, C.c_float, _u32, _u32]),
     "dr_guide_set_learning_rate": (_st, [_vp, C.c_float]),
+    "dr_guide_set_selection_objective": (_st, [_vp, C.c_int32]),
     "dr_gen_icosphere": (_st, [_u32, C.c_float, C.c_uint64, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "dr_gen_rays_surface": (_st, [_vp, _vp, _u32, _u32, C.c_uint64, _i32, C.c_float, _vp]),
     "dr_gen_rays_camera": (_st, [_vp, _vp, _vp, C.c_float, _u32, _u32, _vp]),

@@ -168,6 +168,7 @@ struct Guide {
     uint32_t guided_vertices = 64;  // leading path vertices that sample from the guide
     uint32_t train_batch = 0;       // ring samples per step (0: the whole ring), a rotating window
     uint32_t window = 0;
+    uint32_t sel_variance = 0;      // selection logit trained on the second moment instead of the KL
 };
 
 // Network inference for the paths of a wavefront queue: for each listed path

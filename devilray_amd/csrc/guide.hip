@@ -396,6 +396,7 @@ struct TrainIn {
     uint32_t n, cap;
     uint32_t batch, window;  // batch > 0: samples (window * batch + i) mod count, i < batch
     float *partials;        // gridDim.x x (kNP + 1)
+    uint32_t sel_variance;  // selection logit: 0 = KL objective, 1 = second-moment objective
 };
 
 template <int L>
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn i
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint32_t i = tile * 64u + (uint32_t)col;
         V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f}, wi{0.0f, 0.0f, 1.0f};
-        float w = 0.0f;
+        float w = 0.0f, ps = 0.0f;
         if (i < n) {
             const uint32_t src = start + i < avail ? start + i : start + i - avail;
             const dr_guide_sample q = in.smp[src];
@@ -513,6 +514,7 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn i
             wo = V3{q.wo[0], q.wo[1], q.wo[2]};
             wi = V3{q.wi[0], q.wi[1], q.wi[2]};
             w = q.weight;
+            ps = q.pdf;
         }
         if (g == 0) wsum += w;
         const bf16x8 x0 = encode8(g, p, nn, wo, box);
@@ -565,13 +567,19 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn i
             dy[q0 + 1] = (gy - h.mu[t][1] * mg) / h.r[t];
             dy[q0 + 2] = (gz - h.mu[t][2] * mg) / h.r[t];
         }
-        dy[10] = g == 0 ? G * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
+        // second-moment objective sum_i w_i^2 p_i / pm (Rath et al. 2020,
+        // "Variance-aware path guiding"): the KL gradient scaled by w p / pm,
+        // the sample's weight under the current mixture; it trains the
+        // selection row of the output layer only (no delta into the hidden
+        // layers, whose features stay the lobes')
+        const float rs = in.sel_variance ? (pm > 0.0f ? w * ps / pm : 0.0f) : 1.0f;
+        dy[10] = g == 0 ? G * rs * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
         dy[11] = 0.0f;
         bf16x8 dB[2];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             dB[0][j] = (__bf16)dy[j];
-            dB[1][j] = (__bf16)(j < 4 ? dy[8 + j] : 0.0f);
+            dB[1][j] = (__bf16)(j < 4 && !(in.sel_variance && j == 2) ? dy[8 + j] : 0.0f);
         }
 #pragma unroll
         for (int qq = 0; qq < 12; ++qq) D[0][((qq / 4) * 16 + 4 * g + qq % 4) * kLds + col] = (__bf16)dy[qq];
@@ -790,7 +798,7 @@ dr_status run_train(Guide *g, const dr_guide_sample *smp, const uint32_t *count,
 {
     const uint32_t n_max = batch ? std::min(batch, count ? cap : n) : (count ? cap : n);
     const int nblk = (int)std::max<uint32_t>(1u, std::min<uint32_t>(kGuideTrainBlocks, cdiv(n_max, 64)));
-    TrainIn in{smp, count, n, cap, batch, g->window++, g->partials.as<float>()};
+    TrainIn in{smp, count, n, cap, batch, g->window++, g->partials.as<float>(), g->sel_variance};
     k_guide_train<<<nblk, kT, 0, stream>>>(net_of(g), box_of(g), in);
     DR_HIP_TRY(hipGetLastError());
     AdamIn ad{};
@@ -1003,6 +1011,15 @@ dr_status dr_guide_set_options(dr_guide *gd, float collect_fraction, uint32_t gu
     g->collect_fraction = collect_fraction;
     g->guided_vertices = guided_vertices;
     g->train_batch = train_batch;
+    return DR_OK;
+}
+
+dr_status dr_guide_set_selection_objective(dr_guide *gd, int32_t objective)
+{
+    if (!gd) return fail(DR_E_INVALID, "guide is null");
+    if (objective != DR_GUIDE_SELECT_KL && objective != DR_GUIDE_SELECT_VARIANCE)
+        return fail(DR_E_INVALID, "selection objective must be DR_GUIDE_SELECT_KL or DR_GUIDE_SELECT_VARIANCE");
+    reinterpret_cast<Guide *>(gd)->sel_variance = objective == DR_GUIDE_SELECT_VARIANCE ? 1u : 0u;
     return DR_OK;
 }
 

@@ -418,6 +418,14 @@ dr_status dr_guide_set_options(dr_guide *guide, float collect_fraction, uint32_t
 /* Adam step size for the following training steps (> 0), for schedules that
  * decay it as the guide converges. */
 dr_status dr_guide_set_learning_rate(dr_guide *guide, float lr);
+/* Training objective of the selection logit (no reference counterpart).
+ * DR_GUIDE_SELECT_KL (default) descends sum_i weight_i * -log(alpha q + (1 -
+ * alpha) b); DR_GUIDE_SELECT_VARIANCE descends the one-sample estimator's
+ * second moment sum_i weight_i^2 pdf_i / (alpha q + (1 - alpha) b) (Rath et
+ * al. 2020), with pdf_i the density the sample was drawn from, and trains
+ * only the output row of the selection logit. The lobes keep their loss. */
+enum { DR_GUIDE_SELECT_KL = 0, DR_GUIDE_SELECT_VARIANCE = 1 };
+dr_status dr_guide_set_selection_objective(dr_guide *guide, int32_t objective);
 
 /* dr_render with the guide: DR_GUIDE_USE samples Lambert bounces from the
  * one-sample MIS of the guide mixture and the cosine BSDF (with probability

@@ -55,6 +55,8 @@ PROTOTYPES = {
     "or_guide_forward": (None, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "or_guide_loss": (C.c_double, [_vp, _vp, _vp, _vp, _i32, _i32]),
     "or_guide_grad": (C.c_double, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    "or_guide_loss_sel": (C.c_double, [_vp, _vp, _vp, _vp, _i32, _i32, _i32]),
+    "or_guide_grad_sel": (C.c_double, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     "or_vmf_pdf": (C.c_double, [_vp, C.c_double, _vp]),
     "or_vmf_sample": (None, [_vp, C.c_double, C.c_double, C.c_double, _vp]),
     "or_display_thresholds": (None, [_i32, _vp]),
@@ -385,20 +387,22 @@ def guide_forward(params, bounds, samples, emulate_bf16: bool = False):
     return raw, head
 
 
-def guide_loss(params, bounds, samples, emulate_bf16: bool = False, frozen=None) -> float:
+def guide_loss(params, bounds, samples, emulate_bf16: bool = False, frozen=None, objective: int = 0) -> float:
     """Training loss; `frozen` parameters give the mixture inside the
-    selection term (default: params themselves)."""
+    selection term (default: params themselves). objective: 0 KL selection
+    term, 1 second-moment selection term, 2 no selection term."""
     P, b, s = _guide_args(params, bounds, samples)
     F = None if frozen is None else np.ascontiguousarray(frozen, np.float64)
-    return float(lib().or_guide_loss(_p(P), _p(F) if F is not None else None, _p(b), _p(s), len(s),
-                                     int(emulate_bf16)))
+    return float(lib().or_guide_loss_sel(_p(P), _p(F) if F is not None else None, _p(b), _p(s), len(s),
+                                         int(emulate_bf16), int(objective)))
 
 
-def guide_grad(params, bounds, samples, emulate_bf16: bool = False):
-    """(gradient of the loss, sum of sample weights)."""
+def guide_grad(params, bounds, samples, emulate_bf16: bool = False, objective: int = 0):
+    """(gradient of the loss, sum of sample weights); objective 0 or 1 as in
+    guide_loss."""
     P, b, s = _guide_args(params, bounds, samples)
     g = np.zeros(GUIDE_N_PARAMS, np.float64)
-    sw = lib().or_guide_grad(_p(P), _p(b), _p(s), len(s), int(emulate_bf16), _p(g))
+    sw = lib().or_guide_grad_sel(_p(P), _p(b), _p(s), len(s), int(emulate_bf16), int(objective), _p(g))
     return g, float(sw)
 
 

@@ -227,6 +227,12 @@ static double sample_pdf(const head_t *h, const or_guide_sample *s, double *q_ou
 double or_guide_loss(const double *params, const double *frozen, const float bounds[6], const or_guide_sample *s,
                      int32_t n, int32_t emulate_bf16)
 {
+    return or_guide_loss_sel(params, frozen, bounds, s, n, emulate_bf16, 0);
+}
+
+double or_guide_loss_sel(const double *params, const double *frozen, const float bounds[6],
+                         const or_guide_sample *s, int32_t n, int32_t emulate_bf16, int32_t objective)
+{
     acts_t a, a0;
     double loss = 0.0;
     for (int32_t i = 0; i < n; ++i) {
@@ -246,7 +252,8 @@ double or_guide_loss(const double *params, const double *frozen, const float bou
         }
         const double w = (double)s[i].weight;
         loss += w * -log(Q_MIX * q + (1.0 - Q_MIX) * b);
-        loss += w * -log(h.alpha * q0 + (1.0 - h.alpha) * b);
+        if (objective == 0) loss += w * -log(h.alpha * q0 + (1.0 - h.alpha) * b);
+        if (objective == 1) loss += w * w * (double)s[i].pdf / (h.alpha * q0 + (1.0 - h.alpha) * b);
     }
     return loss;
 }
@@ -254,7 +261,7 @@ double or_guide_loss(const double *params, const double *frozen, const float bou
 /* d loss_i / d y (48 raw outputs) for one sample: the lobes get the gradient
  * of the fixed-proportion term, the selection logit that of the learned
  * one-sample MIS pdf with the mixture held fixed. */
-static void head_backward(const head_t *h, const or_guide_sample *s, double *dy)
+static void head_backward(const head_t *h, const or_guide_sample *s, int objective, double *dy)
 {
     for (int k = 0; k < 48; ++k) dy[k] = 0.0;
     const double wgt = s->weight;
@@ -263,7 +270,10 @@ static void head_backward(const head_t *h, const or_guide_sample *s, double *dy)
     const double p = sample_pdf(h, s, &q, &b, f);
     const double G = -wgt / p;
     const double dq = -wgt / (Q_MIX * q + (1.0 - Q_MIX) * b) * Q_MIX;
-    dy[SEL_ROW] = G * (q - b) * A_SPAN * h->sig * (1.0 - h->sig);
+    /* second-moment objective: d/d alpha of w^2 ps / p is the KL's -w (q - b) / p
+     * times w ps / p */
+    const double rs = objective == 1 ? (p > 0.0 ? wgt * (double)s->pdf / p : 0.0) : 1.0;
+    dy[SEL_ROW] = G * rs * (q - b) * A_SPAN * h->sig * (1.0 - h->sig);
     const double w[3] = {s->wi[0], s->wi[1], s->wi[2]};
     for (int j = 0; j < 8; ++j) {
         const double t = h->pi[j] * f[j];
@@ -282,6 +292,12 @@ static void head_backward(const head_t *h, const or_guide_sample *s, double *dy)
 double or_guide_grad(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
                      int32_t emulate_bf16, double *grad)
 {
+    return or_guide_grad_sel(params, bounds, s, n, emulate_bf16, 0, grad);
+}
+
+double or_guide_grad_sel(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
+                         int32_t emulate_bf16, int32_t objective, double *grad)
+{
     const int emu = emulate_bf16;
     memset(grad, 0, sizeof(double) * OR_GUIDE_NP);
     double sumw = 0.0;
@@ -293,7 +309,7 @@ double or_guide_grad(const double *params, const float bounds[6], const or_guide
         forward_one(params, bounds, s + i, emu, &a);
         head_t h;
         decode(a.y, &h);
-        head_backward(&h, s + i, dcur);
+        head_backward(&h, s + i, objective, dcur);
         if (emu)
             for (int k = 0; k < 48; ++k) dcur[k] = bf16r(dcur[k]);
         /* dcur = d loss / d z_{l+1} (pre-activation of layer l's output) */
@@ -307,6 +323,8 @@ double or_guide_grad(const double *params, const float bounds[6], const or_guide
                 for (int k = 0; k < ni; ++k) gW[o * ni + k] += dcur[o] * a.x[l][k];
             }
             if (l == 0) break;
+            /* the second-moment objective trains the selection row alone */
+            if (l == NL - 1 && objective == 1) dcur[SEL_ROW] = 0.0;
             for (int k = 0; k < ni; ++k) {
                 double d = 0.0;
                 for (int o = 0; o < no; ++o) d += (emu ? bf16r(W[o * ni + k]) : W[o * ni + k]) * dcur[o];

@@ -193,6 +193,15 @@ double or_guide_loss(const double *params, const double *frozen, const float bou
  * to BF16 as the device does. */
 double or_guide_grad(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
                      int32_t emulate_bf16, double *grad);
+/* The same with the selection term chosen by `objective`: 0 the KL above,
+ * 1 the one-sample second moment sum_i w_i^2 pdf_i / (alpha q0 + (1 - alpha)
+ * cos+/pi) (Rath et al. 2020), 2 none (lobe term alone). The gradient of
+ * objective 1 trains only the selection row of the output layer: no delta
+ * of that row reaches the hidden layers (objective 2 has no gradient form). */
+double or_guide_loss_sel(const double *params, const double *frozen, const float bounds[6],
+                         const or_guide_sample *s, int32_t n, int32_t emulate_bf16, int32_t objective);
+double or_guide_grad_sel(const double *params, const float bounds[6], const or_guide_sample *s, int32_t n,
+                         int32_t emulate_bf16, int32_t objective, double *grad);
 /* vMF density kappa / (2 pi (1 - e^-2kappa)) e^(kappa (mu.w - 1)) and
  * W. Jakob's stable sampler (2012): w = 1 + log(x1 + (1 - x1) e^-2kappa) / kappa. */
 double or_vmf_pdf(const double mu[3], double kappa, const double w[3]);

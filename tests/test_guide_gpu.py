@@ -115,6 +115,31 @@ def test_gradient_matches_oracle(guide):
     assert np.array_equal(g, g2)
 
 
+def test_second_moment_selection_gradient_matches_oracle(guide):
+    """DR_GUIDE_SELECT_VARIANCE: the device gradient equals the oracle's
+    second-moment objective (selection row trained alone) layer by layer."""
+    rng = np.random.default_rng(24)
+    p = random_params(rng)
+    guide.set_params(p)
+    s = random_samples(rng, 700)
+    s["weight"][::7] = 0.0
+    s["pdf"] = rng.uniform(0.05, 2.0, len(s))
+    guide.set_selection_objective(dr.DR_GUIDE_SELECT_VARIANCE)
+    try:
+        g, _ = guide.gradient(s)
+    finally:
+        guide.set_selection_objective(dr.DR_GUIDE_SELECT_KL)
+    g16, _ = oracle.guide_grad(p.astype(np.float64), BOUNDS, s, emulate_bf16=True, objective=1)
+    g0, _ = oracle.guide_grad(p.astype(np.float64), BOUNDS, s, emulate_bf16=True, objective=0)
+    for l, (ws, bs) in enumerate(layer_slices()):
+        for sl in (ws, bs):
+            err = np.linalg.norm(g[sl] - g16[sl]) / np.linalg.norm(g16[sl])
+            assert err < 2e-2, (l, err)
+    assert np.linalg.norm(g16 - g0) > 0.05 * np.linalg.norm(g0)  # the objectives differ here
+    with pytest.raises(dr.DevilRayError):
+        guide.set_selection_objective(7)
+
+
 def test_adam_step(guide):
     rng = np.random.default_rng(23)
     p = random_params(rng)

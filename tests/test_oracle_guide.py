@@ -168,6 +168,49 @@ def test_gradient_matches_central_differences():
         assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_second_moment_selection_gradient():
+    """Objective 1: the selection row's gradient is the central difference of
+    sum w^2 pdf / (alpha q + (1 - alpha) b); every other parameter gets the
+    gradient of the lobe term alone (objective 2), so the hidden layers see no
+    delta from the selection row; and at pdf = the current mixture pdf the
+    selection gradient equals the KL's times the weight."""
+    rng = np.random.default_rng(17)
+    p = random_params(rng, out_scale=1.0)
+    s = random_samples(rng, 40, BOUNDS)
+    s["pdf"] = rng.uniform(0.05, 2.0, len(s))
+    g1, _ = oracle.guide_grad(p, BOUNDS, s, objective=1)
+    g0, _ = oracle.guide_grad(p, BOUNDS, s, objective=0)
+    offs = layer_offsets()
+    wo4, bo4 = offs[4]
+    sel = [bo4 + 34] + list(wo4 + 64 * 34 + rng.choice(64, 12, replace=False))
+    assert np.any(g1[sel] != g0[sel])
+    picks = list(sel)
+    for l, (wo, bo) in enumerate(offs):
+        picks += list(rng.choice(np.arange(wo, bo), 8, replace=False))
+        picks += list(rng.choice(np.arange(bo, bo + OUT[l]), 3, replace=False))
+    for i in picks:
+        h = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        obj = 1 if i in sel else 2
+        fd = (oracle.guide_loss(pp, BOUNDS, s, frozen=p, objective=obj)
+              - oracle.guide_loss(pm, BOUNDS, s, frozen=p, objective=obj)) / (2 * h)
+        assert abs(fd - g1[i]) <= 1e-5 * max(1.0, abs(fd)), (i, obj, fd, g1[i])
+    # d/dalpha (w^2 ps / pm) = -w (q - b) / pm * (w ps / pm): with ps = pm
+    # (a sample drawn from the current mixture) it is the KL term times w
+    _, head = oracle.guide_forward(p, BOUNDS, s)
+    one = s[:1].copy()
+    q = sum(head[0][5 * j + 4] * oracle.vmf_pdf(head[0][5 * j:5 * j + 3], head[0][5 * j + 3], one["wi"][0])
+            for j in range(8))
+    b = max(0.0, float(np.dot(one["normal"][0], one["wi"][0]))) / np.pi
+    one["pdf"] = head[0][40] * q + (1 - head[0][40]) * b
+    a1, _ = oracle.guide_grad(p, BOUNDS, one, objective=1)
+    a0, _ = oracle.guide_grad(p, BOUNDS, one, objective=0)
+    w = float(one["weight"][0])
+    assert np.isclose(a1[bo4 + 34], w * a0[bo4 + 34], rtol=1e-5, atol=1e-12)
+
+
 def test_unused_outputs_get_no_gradient():
     rng = np.random.default_rng(9)
     p = random_params(rng)
