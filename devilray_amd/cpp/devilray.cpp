@@ -235,9 +235,10 @@ std::vector<float> Scene::worldPoints() const
     for (const Object &o : objects) {
         const Transform &tf = o.transform;
         for (const Vec3 &p : o.mesh.points) {
-            verts.push_back(p.x * tf.s.x + tf.p.x);
-            verts.push_back(p.y * tf.s.y + tf.p.y);
-            verts.push_back(p.z * tf.s.z + tf.p.z);
+            const Vec3 w = tf.applyToPoint(p);
+            verts.push_back(w.x);
+            verts.push_back(w.y);
+            verts.push_back(w.z);
         }
     }
     return verts;
@@ -478,7 +479,7 @@ void Renderer::render(uint32_t spp)
             render_times_.reset();
             needs_clear_ = false;
         }
-        std::memcpy(c.transform, cam_.transform.data(), sizeof c.transform);
+        std::memcpy(c.transform, cam_.transform.values, sizeof c.transform);  // rows 0..2
         c.focal_length = cam_.intrinsics.focal_length;
         c.center[0] = cam_.intrinsics.center.x;
         c.center[1] = cam_.intrinsics.center.y;
@@ -676,6 +677,44 @@ std::exception_ptr RenderLoop::error() const
 {
     std::scoped_lock guard{error_mutex_};
     return error_;
+}
+
+Matrix4x4f Matrix4x4f::rotation(const Vec3 &a, float angle)
+{
+    const float c = std::cos(angle), sn = std::sin(angle), k = 1.0f - c;
+    const float ax[3] = {a.x, a.y, a.z};
+    // [axis]_x: the cross-product matrix, [axis]_x v = axis x v
+    const float cross[3][3] = {{0.0f, -a.z, a.y}, {a.z, 0.0f, -a.x}, {-a.y, a.x, 0.0f}};
+    Matrix4x4f m;
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 3; ++col)
+            m.values[r][col] = (r == col ? c : 0.0f) + sn * cross[r][col] + k * ax[r] * ax[col];
+    return m;
+}
+
+Matrix4x4f Matrix4x4f::operator*(const Matrix4x4f &o) const
+{
+    Matrix4x4f m;
+    for (int r = 0; r < 4; ++r)
+        for (int col = 0; col < 4; ++col) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc += values[r][k] * o.values[k][col];
+            m.values[r][col] = acc;
+        }
+    return m;
+}
+
+Vec3 Matrix4x4f::applyToPosition(const Vec3 &q) const
+{
+    const Vec3 d = applyToDirection(q);
+    return {d.x + values[0][3], d.y + values[1][3], d.z + values[2][3]};
+}
+
+Vec3 Matrix4x4f::applyToDirection(const Vec3 &d) const
+{
+    return {values[0][0] * d.x + values[0][1] * d.y + values[0][2] * d.z,
+            values[1][0] * d.x + values[1][1] * d.y + values[1][2] * d.z,
+            values[2][0] * d.x + values[2][1] * d.y + values[2][2] * d.z};
 }
 
 }  // namespace devilray

@@ -112,6 +112,40 @@ void generateCoarseNormals(Mesh &mesh);
 struct Transform {
     Vec3 s{1.0f, 1.0f, 1.0f};
     Vec3 p{0.0f, 0.0f, 0.0f};
+
+    Vec3 applyToPoint(const Vec3 &q) const { return {q.x * s.x + p.x, q.y * s.y + p.y, q.z * s.z + p.z}; }
+    // A world ray in the object's model space: origin (o - p) / s, direction
+    // v / s (not renormalised), the interval kept.
+    Ray applyInverse(const Ray &r) const
+    {
+        Ray m = r;
+        m.p = {(r.p.x - p.x) / s.x, (r.p.y - p.y) / s.y, (r.p.z - p.z) / s.z};
+        m.v = {r.v.x / s.x, r.v.y / s.y, r.v.z / s.z};
+        return m;
+    }
+};
+
+// Row-major 4x4 affine transform acting on column vectors
+// (include/models/Matrix.hpp:8-126); values[r][3] is the translation.
+struct Matrix4x4f {
+    float values[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+
+    static Matrix4x4f identity() { return {}; }
+    static Matrix4x4f translation(const Vec3 &offset)
+    {
+        Matrix4x4f m;
+        m.values[0][3] = offset.x;
+        m.values[1][3] = offset.y;
+        m.values[2][3] = offset.z;
+        return m;
+    }
+    // Right-handed rotation by `angle` radians about the unit vector `axis`:
+    // R = cos(a) I + sin(a) [axis]_x + (1 - cos(a)) axis axis^T.
+    static Matrix4x4f rotation(const Vec3 &axis, float angle);
+    Matrix4x4f operator*(const Matrix4x4f &o) const;
+    Vec3 applyToPosition(const Vec3 &q) const;   // w = 1: rotation and translation
+    Vec3 applyToDirection(const Vec3 &d) const;  // w = 0: the 3x3 part only
+    Vec3 getOffset() const { return {values[0][3], values[1][3], values[2][3]}; }
 };
 
 enum class PixelSampling : int { Center = 0, UniformRandom = 1 };
@@ -124,10 +158,10 @@ struct Intrinsics {
     float focal_length;
     Vec2 center;
 };
-// Camera (include/tracing/Camera.hpp:6-19): rows 0..2 of the row-major 4x4
-// camera-to-world transform, pinhole intrinsics, physical pixel size.
+// Camera (include/tracing/Camera.hpp:6-19): the camera-to-world transform
+// (its rows 0..2 are used), pinhole intrinsics, physical pixel size.
 struct Camera {
-    std::array<float, 12> transform{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    Matrix4x4f transform;
     Intrinsics intrinsics{1.0f, {0.0f, 0.0f}};
     Size2i resolution{0, 0};
     Vec2 physical_pixel_size{1.0f, 1.0f};

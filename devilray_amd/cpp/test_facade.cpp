@@ -82,7 +82,7 @@ Camera pinhole(Size2i res, float x_offset = 0.0f)
     cam.intrinsics = {50.0f, {res.width / 2.0f * 0.1f, res.height / 2.0f * 0.1f}};
     cam.resolution = res;
     cam.physical_pixel_size = {0.1f, 0.1f};
-    cam.transform[3] = x_offset;
+    cam.transform.values[0][3] = x_offset;
     return cam;
 }
 
@@ -103,6 +103,99 @@ int main(int argc, char **argv)
     const std::string out_dir = argc > 1 ? argv[1] : "";
     Context ctx(0);
     Scene tri = unit_triangle(ctx);
+
+    // test/src/test_transform.cpp:20-89 and test/src/test_matrix.cpp:14-90
+    auto vnear = [](const Vec3 &v, float x, float y, float z) {
+        EXPECT_NEAR(v.x, x, 1e-5f);
+        EXPECT_NEAR(v.y, y, 1e-5f);
+        EXPECT_NEAR(v.z, z, 1e-5f);
+    };
+    run("TransformTest.ApplyToPointScalesThenTranslates", [&] {
+        const Transform t{{2, 3, 4}, {10, 20, 30}};
+        vnear(t.applyToPoint({1, 1, 1}), 12, 23, 34);
+        vnear(t.applyToPoint({0, 0, 0}), 10, 20, 30);
+    });
+    run("TransformTest.ApplyToPointIdentityIsNoOp", [&] { vnear(Transform{}.applyToPoint({5, -2, 7}), 5, -2, 7); });
+    run("TransformTest.ApplyInverseUndoesPointTransformForRayOrigin", [&] {
+        const Transform t{{2, 4, 5}, {1, 2, 3}};
+        Ray world;
+        world.p = {9, 10, 13};
+        world.v = {0, 0, 1};
+        const Ray model = t.applyInverse(world);
+        const Vec3 back = t.applyToPoint(model.p);
+        vnear(back, 9, 10, 13);
+        vnear(model.p, 4, 2, 2);
+    });
+    run("TransformTest.ApplyInverseScalesDirectionWithoutTranslation", [&] {
+        const Transform t{{2, 4, 5}, {100, 200, 300}};
+        Ray r;
+        r.p = {0, 0, 0};
+        r.v = {2, 4, 5};
+        vnear(t.applyInverse(r).v, 1, 1, 1);
+    });
+    run("MatrixTest.IdentityIsDiagonal", [&] {
+        const auto id = Matrix4x4f::identity();
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) EXPECT(id.values[r][c] == (r == c ? 1.0f : 0.0f));
+    });
+    run("MatrixTest.ApplyToDirectionIgnoresTranslation", [&] {
+        vnear(Matrix4x4f::translation({10, 20, 30}).applyToDirection({1, 2, 3}), 1, 2, 3);
+    });
+    run("MatrixTest.ApplyToDirectionRotates", [&] {
+        vnear(Matrix4x4f::rotation({0, 0, 1}, 1.57079632679f).applyToDirection({1, 0, 0}), 0, 1, 0);
+    });
+    run("MatrixTest.GetOffsetReturnsTranslationColumn", [&] {
+        vnear(Matrix4x4f::translation({7, -3, 11}).getOffset(), 7, -3, 11);
+        vnear(Matrix4x4f::identity().getOffset(), 0, 0, 0);
+    });
+    run("MatrixTest.GetOffsetAfterComposition", [&] {
+        vnear((Matrix4x4f::translation({1, 2, 3}) * Matrix4x4f::translation({4, 5, 6})).getOffset(), 5, 7, 9);
+    });
+    run("MatrixTest.MultiplicationByIdentity", [&] {
+        const Matrix4x4f m = Matrix4x4f::translation({1, 2, 3});
+        const Matrix4x4f r = m * Matrix4x4f::identity();
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) EXPECT(r.values[i][j] == m.values[i][j]);
+    });
+    run("MatrixTest.TranslationMatrixComposition", [&] {
+        vnear(Matrix4x4f::translation({5, -2, 10}).applyToPosition({0, 0, 0}), 5, -2, 10);
+    });
+    run("MatrixTest.TranslationDoesNotAffectDirection", [&] {
+        vnear(Matrix4x4f::translation({10, 10, 10}).applyToDirection({1, 0, 0}), 1, 0, 0);
+    });
+    run("MatrixTest.Rotation90DegreesZ", [&] {
+        vnear(Matrix4x4f::rotation({0, 0, 1}, 1.57079632679f).applyToPosition({1, 0, 0}), 0, 1, 0);
+    });
+    run("MatrixTest.RotationZeroAngle", [&] {
+        vnear(Matrix4x4f::rotation({1, 1, 1}, 0.0f).applyToPosition({1, 2, 3}), 1, 2, 3);
+    });
+    run("MatrixTest.ApplyToPositionWithWComponent", [&] {
+        vnear(Matrix4x4f::translation({1, 1, 1}).applyToPosition({2, 2, 2}), 3, 3, 3);
+    });
+    run("MatrixTest.CombinedTransformations", [&] {
+        const Matrix4x4f m = Matrix4x4f::rotation({0, 0, 1}, 1.57079632679f) * Matrix4x4f::translation({1, 0, 0});
+        vnear(m.applyToPosition({0, 0, 0}), 0, 1, 0);
+    });
+    run("MatrixTest.RotationIsOrthonormalAboutAnyAxis", [&] {
+        // R R^T = I and R axis = axis for a unit axis off the coordinate planes
+        const float n = 1.0f / std::sqrt(14.0f);
+        const Vec3 ax{1 * n, -2 * n, 3 * n};
+        const Matrix4x4f r = Matrix4x4f::rotation(ax, 0.7f);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                float d = 0.0f;
+                for (int k = 0; k < 3; ++k) d += r.values[i][k] * r.values[j][k];
+                EXPECT_NEAR(d, i == j ? 1.0f : 0.0f, 1e-6f);
+            }
+        vnear(r.applyToDirection(ax), ax.x, ax.y, ax.z);
+    });
+    run("CameraTest.TransformMatrixMovesTheView", [&] {
+        // a camera translated by (0.5, 0, 0) sees the triangle hit at the
+        // same spot shifted in the image: rays from the camera start at the offset
+        Camera cam;
+        cam.transform = Matrix4x4f::translation({0.5f, 0.0f, 0.0f});
+        EXPECT(cam.transform.values[0][3] == 0.5f && cam.transform.getOffset().x == 0.5f);
+    });
 
     run("IntersectionTest.HitsCenterOfTriangle", [&] {
         auto r = cast1(ctx, tri, {0.2f, 0.2f, 5.0f}, {0, 0, -1});
