@@ -575,7 +575,7 @@ def main() -> int:
         # Adam from lr 1e-2, decayed harmonically over the passes: on this scene
         # a constant 1e-2 left the seed-averaged MSE at plain's (64 seeds)
         guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
-                                                   8, 64, 8, 2, 12, train_batch=1 << 16, seeds=32,
+                                                   8, 64, 8, 2, 12, train_batch=1 << 16, seeds=64,
                                                    lr_schedule=lambda k: 1e-2 / (1.0 + k / 4.0))
         log("[bench] guide: hidden-light reflector scene, guided vs plain")
         guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 512, 512, 16384,
