@@ -14,7 +14,8 @@ from ._lib import (  # noqa: F401
     ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
     DR_GUIDE_COLLECT, DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
     GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
-    DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, check, lib, ptr,
+    DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, SCHED_PER_RAY, SCHED_PER_RAY_SPECULATIVE, SCHED_PERSISTENT,
+    SCHED_PERSISTENT_SPECULATIVE, SCHED_REORDER, SCHED_XCD_SLICES, check, lib, ptr,
 )
 
 

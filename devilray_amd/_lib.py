@@ -53,7 +53,7 @@ class DrSceneInfo(C.Structure):
         ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
         ("depth", C.c_uint32), ("overflow_flag", C.c_uint32), ("simd_stats", C.c_uint32 * 4),
         ("build_ms", C.c_double), ("sah_cost", C.c_double), ("n_references", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("stack_high_water", C.c_uint32),
     ]
 
 
@@ -89,6 +89,8 @@ DR_GUIDE_USE, DR_GUIDE_COLLECT = 1, 2
 DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE = 0, 1
 DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB = 0, 1
 DR_DEBUG_UV_CHECKER, DR_DEBUG_BARYCENTRIC, DR_DEBUG_WINDING = 1, 2, 3
+SCHED_PER_RAY, SCHED_PER_RAY_SPECULATIVE, SCHED_PERSISTENT, SCHED_PERSISTENT_SPECULATIVE = 0, 1, 2, 3
+SCHED_REORDER, SCHED_XCD_SLICES = 4, 8
 GUIDE_SAMPLE_DTYPE = np.dtype([("pos", np.float32, 3), ("weight", np.float32), ("normal", np.float32, 3),
                                ("pdf", np.float32), ("wo", np.float32, 3), ("pad1", np.float32),
                                ("wi", np.float32, 3), ("pad2", np.float32)])

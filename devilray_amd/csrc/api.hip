@@ -250,12 +250,13 @@ dr_status dr_scene_get_info(dr_scene *sc, dr_scene_info *info)
     info->build_ms = s->build_ms;
     info->sah_cost = s->sah_cost;
     info->n_references = s->n_refs;
-    info->reserved = 0;
+    info->stack_high_water = 0;
     if (s->flags.ptr) {
         uint32_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         DR_HIP_TRY(hipMemcpy(f, s->flags.ptr, 32, hipMemcpyDeviceToHost));
         info->overflow_flag = f[0];
         for (int k = 0; k < 4; ++k) info->simd_stats[k] = f[2 + k];
+        info->stack_high_water = f[6];
     }
     return DR_OK;
 }

@@ -1168,7 +1168,9 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
                 ++depth;
                 if (c[3]) return fail(DR_E_DEVICE, "BVH collapse exceeded its node bound");
                 int2 *tmp = qa; qa = qb; qb = tmp;
-                if (depth > 128) return fail(DR_E_DEVICE, "BVH collapse did not terminate");
+                if (depth > kMaxWideDepth)
+                    return fail(DR_E_OVERFLOW, "BVH deeper than " + std::to_string(kMaxWideDepth) +
+                                                   " wide levels: the traversal stack bound");
             }
         }
     }

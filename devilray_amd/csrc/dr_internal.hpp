@@ -51,6 +51,9 @@ static_assert(sizeof(WideNode) == 128, "wide node is 128 B");
 constexpr int32_t DR_EMPTY_REF = -1;  // 0xFFFFFFFF
 constexpr int kLeafMaxCap = 8;
 constexpr uint32_t kMaxLeafRefs = (1u << 28) - 2;  // triangle references per scene or batch
+// wide BVH levels a build may produce: traversal keeps up to 3 entries per
+// level on its stack, and a deeper tree is refused at build time
+constexpr uint32_t kMaxWideDepth = 256;
 
 __host__ __device__ inline int32_t make_leaf_ref(uint32_t first, uint32_t count)
 {

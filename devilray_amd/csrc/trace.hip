@@ -16,8 +16,11 @@
 //
 // Each lane keeps a short stack of child references in LDS (kLdsStack
 // entries, laid out [entry][lane]); deeper entries spill to a per-lane region
-// in global memory, up to kMaxStack in total. At an interior node all four
-// child boxes are tested with the robust slab test; near and far planes are
+// in global memory. A visit pushes at most 3 children, and every stacked entry
+// is a child of a node on the path from the root to the node being visited,
+// so a tree of D wide levels never needs more than 3 D entries: the host sizes
+// the spill region from the built depth, and no ray is ever cut short. At an
+// interior node all four child boxes are tested with the robust slab test; near and far planes are
 // loaded by the ray's octant so no per-slab min/max is needed. Hit children
 // are sorted by entry distance, the nearest is taken next and the rest are
 // pushed far to near. Leaves hold up to 8 triangles stored as 3 float4.
@@ -30,7 +33,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLdsStack = 16;
-constexpr int kMaxStack = 64;
+// the spill region covers schedules up to this many entries per lane at the
+// full per-ray grid; deeper trees run on the resident (persistent) grid
+constexpr uint32_t kPerRaySpillMax = 48;
 constexpr int32_t kNone = DR_EMPTY_REF;
 constexpr uint32_t kCounterSlots = 256;
 
@@ -92,7 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                                                   uint32_t *__restrict__ tri_tests,
                                                   int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
                                                   uint32_t *__restrict__ ray_counter, uint32_t refill_min,
-                                                  const uint32_t *__restrict__ roots, uint32_t slices)
+                                                  const uint32_t *__restrict__ roots, uint32_t slices,
+                                                  uint32_t max_stack)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
     // ray count: fixed at launch, or read from device memory (on-device queues)
@@ -119,7 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
     int32_t node = kNone, leaf = kNone;
     uint32_t nbox = 0;
     uint32_t nx = 0, ny = 32, nz = 64;
-    bool overflow = false;
+    bool overflow = false, any_overflow = false;
+    uint32_t high_water = 0;  // counted variant: deepest stack of the lane
     const char *nbase = reinterpret_cast<const char *>(nodes);
 
     auto pop = [&]() -> int32_t {
@@ -127,11 +134,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
         --sp;
         return sp < kLdsStack ? stack[sp][tid] : ovf[(size_t)(sp - kLdsStack) * nlanes + gl];
     };
+    // max_stack >= 3 x the tree's wide levels, so the last branch is a
+    // guard that never runs; if it did, the ray would end early and the
+    // scene's overflow flag would report it
     auto push = [&](int32_t r) {
         if (sp < kLdsStack) stack[sp][tid] = r;
-        else if (sp < kMaxStack) ovf[(size_t)(sp - kLdsStack) * nlanes + gl] = r;
+        else if ((uint32_t)sp < max_stack) ovf[(size_t)(sp - kLdsStack) * nlanes + gl] = r;
         else { overflow = true; return; }
         ++sp;
+        if (COUNT) high_water = max(high_water, (uint32_t)sp);
     };
 
     for (;;) {
@@ -167,6 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     st.ntri = 0;
                     sp = 0; leaf = kNone; nbox = 0;
                     tri_cur = 0; tri_end = 0;
+                    overflow = false;
                     // a NaN anywhere in the ray hits nothing (the triangle test
                     // rejects it too), and NaN slabs would admit empty slots
                     const bool nan_ray = (r0.x != r0.x) || (r0.y != r0.y) || (r0.z != r0.z) || (r0.w != r0.w) ||
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     node = pop();
                 }
             }
-            if (overflow) { node = kNone; leaf = kNone; sp = 0; tri_cur = tri_end; }
+            if (overflow) { any_overflow = true; node = kNone; leaf = kNone; sp = 0; tri_cur = tri_end; }
             // ---- finished: write the result, free the lane
             if (node == kNone && leaf == kNone && tri_cur >= tri_end) {
                 const uint32_t rr = (uint32_t)ray;
@@ -318,14 +330,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
             }
         }
     }
-    if (overflow) atomicOr(&flags[0], 1u);
+    if (any_overflow) atomicOr(&flags[0], 1u);
     if (COUNT) {
         // SIMD utilisation statistics: [2] wave-level node-loop iterations,
-        // [3] node visits, [4] wave-level triangle-loop iterations, [5] tests
+        // [3] node visits, [4] wave-level triangle-loop iterations, [5] tests;
+        // [6] the deepest traversal stack of any ray
         atomicAdd(&flags[2], wave_node_iters);
         atomicAdd(&flags[3], node_visits);
         atomicAdd(&flags[4], st.wave_tri_iters);
         atomicAdd(&flags[5], tri_total + st.ntri);
+        atomicMax(&flags[6], high_water);
     }
 }
 
@@ -395,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
 using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
                             const uint32_t *, dr_hit *,
                             uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t,
-                            const uint32_t *, uint32_t);
+                            const uint32_t *, uint32_t, uint32_t);
 
 template <bool ANY, bool COUNT>
 TraceKernel pick(uint32_t sched)
@@ -427,7 +441,15 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     if (n == 0) return DR_OK;
     if (!rays || (any_hit ? !occluded : !hits)) return fail(DR_E_INVALID, "null ray or output buffer");
     const bool count = box_tests && tri_tests;
-    const uint32_t loop = sched & 3u;
+    // stack capacity from the built depth (3 entries per wide level, one slot
+    // of slack); a deep tree moves per-ray schedules onto the resident grid so
+    // the spill region stays bounded by the number of resident lanes
+    const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
+    if (s->depth > kMaxWideDepth)
+        return fail(DR_E_OVERFLOW, "BVH deeper than the traversal stack bound of " + std::to_string(kMaxWideDepth) +
+                                       " wide levels");
+    uint32_t loop = sched & 3u;
+    if (max_stack - kLdsStack > kPerRaySpillMax) loop |= 2u;  // PER_RAY -> PERSISTENT, keeping speculation
     const TraceKernel kern = any_hit ? pick<true, false>(loop) : (count ? pick<false, true>(loop) : pick<false, false>(loop));
     const bool persistent = loop == DR_SCHED_PERSISTENT || loop == DR_SCHED_PERSISTENT_SPECULATIVE;
     static int occ_cache[16] = {0};
@@ -442,7 +464,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     const uint32_t resident = (uint32_t)(s->ctx->cu_count * bpc);
     const uint32_t grid = persistent ? (need < resident ? need : resident) : need;
     const size_t nlanes = (size_t)grid * kBlock;
-    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(kMaxStack - kLdsStack) * nlanes)) return st;
+    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(max_stack - kLdsStack) * nlanes + 4)) return st;
     const auto *r4 = reinterpret_cast<const float4 *>(rays);
     const uint32_t *perm = ray_list;
     if (!ray_list && !n_dev && !ray_root && (sched & DR_SCHED_REORDER) && n >= 4096) {
@@ -466,10 +488,11 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     // every slice needs at least one workgroup to drain it
     const uint32_t slices = (persistent && (sched & DR_SCHED_XCD_SLICES) && grid >= 8u) ? 8u : 1u;
     if (count)
-        for (int k = 2; k <= 5; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
+        for (int k = 2; k <= 6; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
-                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices);
+                                      box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices,
+                                      max_stack);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

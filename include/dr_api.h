@@ -102,8 +102,10 @@ typedef struct {
     uint32_t n_triangles;
     uint32_t n_nodes;        /* wide (4-ary) nodes */
     uint32_t n_leaves;
-    uint32_t depth;          /* wide levels */
-    uint32_t overflow_flag;  /* non-zero if a traversal overflowed its stack */
+    uint32_t depth;          /* wide levels (at most 256: a deeper build fails with DR_E_OVERFLOW) */
+    uint32_t overflow_flag;  /* non-zero if a traversal overflowed its stack; traversal sizes its
+                              * stack to 3 x depth entries, which bounds every walk, so this
+                              * stays 0 (a guard that is checked by the tests) */
     /* last counted traversal (dr_trace_closest_counted): wave-level node-loop
      * iterations, node visits, wave-level triangle iterations, triangle tests;
      * visits / (64 * iterations) is the SIMD utilisation of each loop */
@@ -111,7 +113,7 @@ typedef struct {
     double build_ms;         /* device time of the last build */
     double sah_cost;         /* SAH cost of the binary tree before collapse */
     uint32_t n_references;   /* leaf entries: triangles plus split pieces */
-    uint32_t reserved;
+    uint32_t stack_high_water; /* last counted traversal: deepest stack of any ray (<= 3 x depth) */
 } dr_scene_info;
 
 const char *dr_status_string(dr_status s);

@@ -168,5 +168,13 @@ def test_c1_full_size_incoherent_and_coherent(gpu_ctx, c1_oracle, split):
     hit = ref["id"] >= 0
     half["tmax"][hit] = 0.5 * ref["t"][hit]
     assert np.array_equal(scene.trace_any(half), bvh.any(half))
+    if split is None:
+        # the counted traversal (the benchmark's HitTests path) returns the same hits
+        counted, box, tri = scene.trace_closest_counted(rays)
+        assert_hits_equal(counted, ref, "counted incoherent")
+        assert np.all(tri[hit] >= 1) and np.all(box >= 1)
+        info = scene.info()
+        assert int(tri.sum()) == info.simd_stats[3]
+        assert 0 < info.stack_high_water <= 3 * info.depth
     assert scene.info().overflow_flag == 0
     scene.close()
