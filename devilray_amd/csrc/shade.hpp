@@ -112,6 +112,9 @@ __device__ __forceinline__ float power_heuristic(float a, float b)
 }
 
 // Fresnel reflectance for a ray arriving with cos_i from index n1 into n2.
+// Schlick's form is evaluated with the reference's types (schlick,
+// PathGeneration.hpp:75-86): float powers of 1 - cos_i, then a double
+// r0 + (1 - r0) c^5 (its 1.0 literal is a double) rounded once to float.
 __device__ __forceinline__ float fresnel(float cos_i, float n1, float n2, float cos_t, bool schlick)
 {
     if (schlick) {
@@ -121,7 +124,7 @@ __device__ __forceinline__ float fresnel(float cos_i, float n1, float n2, float 
         const float c2 = c * c;
         const float c4 = c2 * c2;
         const float c5 = c4 * c;
-        return r0 + (1.0f - r0) * c5;
+        return (float)((double)r0 + (1.0 - (double)r0) * (double)c5);
     }
     const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
     const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);

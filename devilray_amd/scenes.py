@@ -93,6 +93,32 @@ def cornell_c0() -> SceneData:
                         width=256, height=256))
 
 
+def furnace_enclosure(albedo: float = 0.5, emission: float = 1.0, glass_ior: float | None = None,
+                      glass_subdiv: int = 3) -> SceneData:
+    """A closed box [-1,1]^3 whose six walls are all Lambert with albedo a and
+    emission Le, optionally with a dielectric icosphere (radius 0.45) at the
+    centre; the camera is inside. Every point of the box then sees radiance
+    Le (1 + a + a^2 + ...) = Le / (1 - a) in every direction, and a lossless
+    dielectric in it changes nothing (an isothermal enclosure): a path of at
+    most D Lambert vertices without roulette gives Le (1 - a^D) / (1 - a)."""
+    b = _Builder()
+    wall = b.material(DR_MAT_LAMBERT, (albedo,) * 3, (emission,) * 3)
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            c = [0.0, 0.0, 0.0]
+            c[axis] = sgn
+            n = [0.0, 0.0, 0.0]
+            n[axis] = -sgn
+            r = [0.0, 0.0, 0.0]
+            r[(axis + 1) % 3] = 1.0
+            b.quad(c, n, r, 2, 2, wall)
+    if glass_ior is not None:
+        glass = b.material(DR_MAT_DIELECTRIC, (1, 1, 1), (0, 0, 0), glass_ior)
+        b.sphere(glass_subdiv, (0.0, 0.0, 0.0), 0.45, glass)
+    return b.build(dict(eye=(0.35, 0.25, 0.9), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=60.0,
+                        width=32, height=32))
+
+
 def _random_rotations(rng, n):
     """Uniform random rotation matrices (Shoemake's quaternion method)."""
     u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)

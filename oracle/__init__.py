@@ -22,6 +22,9 @@ _vp = C.c_void_p
 _i32 = C.c_int32
 PROTOTYPES = {
     "or_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
+    "or_fresnel": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32]),
+    "or_dielectric_scatter": (C.c_int32, [_vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_int32, _vp]),
+    "or_roulette": (C.c_int32, [_vp, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32]),
     "or_emission_weight": (C.c_float, [C.c_int, C.c_float, C.c_float]),
     "or_direct_light": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_float, _vp, _vp, _vp]),
     "or_camera_ray": (None, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _vp]),
@@ -110,6 +113,28 @@ def direct_light(p, n, albedo, xl, nl, emission, pdf_area: float):
     return bool(ok), ld, float(pl[0]), float(pb[0])
 
 
+PLANT_RR_NO_REWEIGHT, PLANT_TRANSMIT_WEIGHT = 0x10000, 0x20000
+
+
+def fresnel(cos_i: float, n1: float, n2: float, cos_t: float, schlick: bool) -> float:
+    return lib().or_fresnel(cos_i, n1, n2, cos_t, 1 if schlick else 0)
+
+
+def dielectric_scatter(d, n, entering: bool, ior: float, u: float, schlick: bool):
+    """(reflected?, new direction) of one dielectric event; n faces against d."""
+    dd, nn = np.ascontiguousarray(d, np.float32), np.ascontiguousarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    r = lib().or_dielectric_scatter(_p(dd), _p(nn), 1 if entering else 0, ior, u, 1 if schlick else 0, _p(out))
+    return bool(r), out
+
+
+def roulette(beta, u: float, vertex: int, rr_depth: int, max_depth: int):
+    """(alive?, throughput after the step) of Russian roulette after a vertex."""
+    b = np.array(beta, np.float32)
+    alive = lib().or_roulette(_p(b), u, vertex, rr_depth, max_depth)
+    return bool(alive), b
+
+
 def sample_sphere(r1: float, r2: float) -> np.ndarray:
     out = np.zeros(3, np.float32)
     lib().or_sample_sphere(r1, r2, _p(out))
@@ -150,14 +175,15 @@ def lights_build(vertices, indices, tri_mat, mats):
 
 def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int = 8, rr_depth: int = 3,
            seed: int = 0, fresnel: int = 0, sample_offset: int = 0, pixels=None, accum=None,
-           vertex_hist: bool = False, nthreads: int | None = None):
-    """Build-semantics path tracer; returns (accum[h, w, 4] float64, hist)."""
+           vertex_hist: bool = False, nthreads: int | None = None, plant: int = 0):
+    """Build-semantics path tracer; returns (accum[h, w, 4] float64, hist).
+    plant: PLANT_* bits that insert a known mistake (for the tests' power checks)."""
     tm = np.ascontiguousarray(tri_mat, np.int32)
     m = np.ascontiguousarray(mats, MATERIAL_DTYPE)
     ent, pdf = lights_build(bvh.v, bvh.f, tm, m)
     npix = camera.width * camera.height
     acc = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64).reshape(npix, 4)
-    prm = RenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, 0)
+    prm = RenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, plant)
     lo, hi = (0, npix) if pixels is None else pixels
     hist = np.zeros(max_depth, np.int32) if vertex_hist else None
     lib().or_render(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(m), _p(ent) if len(ent) else None, _p(pdf), len(ent),

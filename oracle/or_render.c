@@ -259,7 +259,12 @@ int or_direct_light(const float p[3], const float n[3], const float albedo[3], c
     return ok;
 }
 
-static inline float fresnel(float cos_i, float n1, float n2, float cos_t, int schlick)
+/* Fresnel reflectance of a ray arriving with cosine cos_i from index n1 into
+ * n2 (cos_t: cosine of the transmitted direction). Exact: the mean of the s-
+ * and p-polarised reflectances. Schlick: the reference's schlick
+ * (PathGeneration.hpp:75-86) with its types: float powers of 1 - cos_i, then
+ * R0 + (1.0 - R0) c^5 in double (its 1.0 literal), rounded once to float. */
+float or_fresnel(float cos_i, float n1, float n2, float cos_t, int32_t schlick)
 {
     if (schlick) {
         const float r = (n1 - n2) / (n1 + n2);
@@ -268,11 +273,87 @@ static inline float fresnel(float cos_i, float n1, float n2, float cos_t, int sc
         const float c2 = c * c;
         const float c4 = c2 * c2;
         const float c5 = c4 * c;
-        return r0 + (1.0f - r0) * c5;
+        return (float)((double)r0 + (1.0 - (double)r0) * (double)c5);
     }
     const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
     const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
     return 0.5f * (rs * rs + rp * rp);
+}
+
+/* Dielectric scattering at a hit (generateNewRay's transparent branch,
+ * PathGeneration.hpp:139-207): d is the ray direction, n the geometric
+ * normal turned against d, `entering` whether d arrived on the front side
+ * (then n1 = 1, n2 = ior, else the reverse; winding replaces the reference's
+ * IOR stack). Total internal reflection when n1 > n2 and
+ * k = 1 - eta^2 (1 - (d.n)^2) < 0; otherwise reflection with probability F
+ * (u < F), else refraction n'(sqrt k - cos_i eta) + eta d with n' = -n.
+ * Returns 1 for reflection; *side is the normal on the outgoing side (the
+ * ray origin moves along it). */
+static inline int dielectric_scatter(v3 d, v3 n, int entering, float ior, float u, int schlick, v3 *nd, v3 *side,
+                                     float *fres)
+{
+    const float n1 = entering ? 1.0f : ior;
+    const float n2 = entering ? ior : 1.0f;
+    const float dd = vdot(d, n);
+    const float eta = n1 / n2;
+    const float k = 1.0f - eta * eta * (1.0f - dd * dd);
+    int reflect_ray = 1;
+    *fres = 1.0f;
+    if (!(n1 > n2 && k < 0.0f)) {
+        const float ci = -dd;
+        const float F = or_fresnel(ci, n1, n2, sqrtf(fmaxf(k, 0.0f)), schlick);
+        *fres = F;
+        reflect_ray = u < F;
+    }
+    if (reflect_ray) {
+        *nd = mk(d.x - n.x * 2.0f * dd, d.y - n.y * 2.0f * dd, d.z - n.z * 2.0f * dd);
+        *side = n;
+    } else {
+        const v3 nn = vmul(n, -1.0f);
+        const float ci = -dd;
+        const float sk = sqrtf(k);
+        *nd = vadd(vmul(nn, sk - ci * eta), vmul(d, eta));
+        *side = nn;
+    }
+    return reflect_ray;
+}
+
+int32_t or_dielectric_scatter(const float d[3], const float n[3], int32_t entering, float ior, float u,
+                              int32_t schlick, float out[3])
+{
+    v3 nd, side;
+    float F;
+    const int r = dielectric_scatter(mk(d[0], d[1], d[2]), mk(n[0], n[1], n[2]), entering, ior, u, schlick, &nd,
+                                     &side, &F);
+    out[0] = nd.x; out[1] = nd.y; out[2] = nd.z;
+    return r;
+}
+
+/* Russian roulette after vertex `vertex` (0-based): from vertex rr_depth - 1
+ * on, a path whose largest throughput component q is below 1 survives with
+ * probability q (u < q) and is divided by q; at max_depth vertices, or with
+ * zero throughput, it ends. Returns whether the path goes on. */
+static inline int roulette(v3 *beta, float u, uint32_t vertex, uint32_t rr_depth, uint32_t max_depth,
+                           int planted_no_reweight)
+{
+    int alive = vertex + 1u < max_depth;
+    if (alive && vertex + 1u >= rr_depth) {
+        const float q = fmaxf(beta->x, fmaxf(beta->y, beta->z));
+        if (q < 1.0f) {
+            if (u >= q) alive = 0;
+            else if (!planted_no_reweight) *beta = mk(beta->x / q, beta->y / q, beta->z / q);
+        }
+    }
+    if (!(beta->x > 0.0f || beta->y > 0.0f || beta->z > 0.0f)) alive = 0;
+    return alive;
+}
+
+int32_t or_roulette(float beta[3], float u, uint32_t vertex, uint32_t rr_depth, uint32_t max_depth)
+{
+    v3 b = mk(beta[0], beta[1], beta[2]);
+    const int alive = roulette(&b, u, vertex, rr_depth, max_depth, 0);
+    beta[0] = b.x; beta[1] = b.y; beta[2] = b.z;
+    return alive;
 }
 
 static inline void path_rng(uint32_t pixel, uint32_t sample, uint32_t vertex, uint32_t stream, uint64_t seed,
@@ -419,41 +500,17 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
             no = vadd(p, vmul(n, eps));
             spec = 0;
         } else {
-            const int entering = !back;
-            const float n1 = entering ? 1.0f : mat.ior;
-            const float n2 = entering ? mat.ior : 1.0f;
-            const float dd = vdot(d, n);
-            const float eta = n1 / n2;
-            const float k = 1.0f - eta * eta * (1.0f - dd * dd);
-            int reflect_ray = 1;
-            if (!(n1 > n2 && k < 0.0f)) {
-                const float ci = -dd;
-                const float F = fresnel(ci, n1, n2, sqrtf(fmaxf(k, 0.0f)), prm->fresnel == 1);
-                reflect_ray = or_u01(r0[3]) < F;
-            }
-            if (reflect_ray) {
-                nd = mk(d.x - n.x * 2.0f * dd, d.y - n.y * 2.0f * dd, d.z - n.z * 2.0f * dd);
-                no = vadd(p, vmul(n, eps));
-            } else {
-                const v3 nn = vmul(n, -1.0f);
-                const float ci = -dd;
-                const float sk = sqrtf(k);
-                nd = vadd(vmul(nn, sk - ci * eta), vmul(d, eta));
-                no = vadd(p, vmul(nn, eps));
-            }
+            v3 side;
+            float F;
+            const int refl = dielectric_scatter(d, n, !back, mat.ior, or_u01(r0[3]), prm->fresnel == 1, &nd, &side, &F);
+            if (!refl && (prm->flags & OR_PLANT_TRANSMIT_WEIGHT)) beta = vmul(beta, 1.0f - F);
+            no = vadd(p, vmul(side, eps));
             pdf = 0.0f;
             spec = 1;
         }
-        int alive = vertex + 1u < prm->max_depth;
-        if (alive && vertex + 1u >= prm->rr_depth) {
-            const float q = fmaxf(beta.x, fmaxf(beta.y, beta.z));
-            if (q < 1.0f) {
-                if (or_u01(r0[2]) >= q) alive = 0;
-                else beta = mk(beta.x / q, beta.y / q, beta.z / q);
-            }
-        }
-        if (!(beta.x > 0.0f || beta.y > 0.0f || beta.z > 0.0f)) alive = 0;
-        if (!alive) break;
+        if (!roulette(&beta, or_u01(r0[2]), vertex, prm->rr_depth, prm->max_depth,
+                      (prm->flags & OR_PLANT_RR_NO_REWEIGHT) != 0))
+            break;
         spec_prev = spec;
         pdf_prev = pdf;
         ray.ox = no.x; ray.oy = no.y; ray.oz = no.z; ray.tmin = 0.0f;

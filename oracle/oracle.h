@@ -123,6 +123,13 @@ typedef struct {
     uint64_t seed;
     uint32_t fresnel, flags;
 } or_render_params;                                                                        /* = dr_render_params */
+/* flags bits read by the oracle only: each plants a known mistake, so that
+ * the tests can show their checks would catch it. RR_NO_REWEIGHT: roulette
+ * survivors keep their throughput; TRANSMIT_WEIGHT: refracted throughput is
+ * multiplied by 1 - F although refraction was already chosen with that
+ * probability. */
+#define OR_PLANT_RR_NO_REWEIGHT 0x10000u
+#define OR_PLANT_TRANSMIT_WEIGHT 0x20000u
 
 /* Alias table over emissive triangles, weights area x luminance(E) (FP64),
  * built by a sequential sweep in exact integer units. entries: one per
@@ -147,6 +154,15 @@ float or_emission_weight(int spec_prev, float pdf_bsdf, float pdf_light);
 int or_direct_light(const float p[3], const float n[3], const float albedo[3], const float xl[3], const float nl[3],
                     const float emission[3], float pdf_area, float ld[3], float *pdf_light, float *pdf_bsdf);
 void or_sample_cosine(const float n[3], float r1, float r2, float out[3]);
+/* Scattering steps of the integrator, exported for their own tests: the
+ * Fresnel reflectance (exact, or the reference's Schlick), one dielectric
+ * scattering event (returns 1 for reflection, out = the new direction), and
+ * Russian roulette after a vertex (returns whether the path goes on; beta is
+ * divided by the survival probability). */
+float or_fresnel(float cos_i, float n1, float n2, float cos_t, int32_t schlick);
+int32_t or_dielectric_scatter(const float d[3], const float n[3], int32_t entering, float ior, float u,
+                              int32_t schlick, float out[3]);
+int32_t or_roulette(float beta[3], float u, uint32_t vertex, uint32_t rr_depth, uint32_t max_depth);
 void or_sample_triangle(const float a[3], const float b[3], const float c[3], float r1, float r2, float out[3]);
 
 /* Path-trace pixels [pixel_begin, pixel_end) adding spp samples each to

@@ -1,10 +1,15 @@
 """GPU render path against the oracle and the reference's own fixtures.
 
 - light table: bit-identical to the oracle's sequential sweep;
-- Cornell C0 at low resolution: per-pixel FP64 sums equal to the oracle's
-  (same Philox numbers, same float expressions; a pixel may differ only where
-  a sqrt/division-free branch flips, so >= 99% of pixels must be bit-equal and
-  the image means agree within 3 standard errors);
+- Cornell C0 at 64 x 64 and at its full 256 x 256 x 16 spp, with exact and
+  Schlick Fresnel on the glass sphere: per-pixel FP64 sums equal to the
+  oracle's (same Philox numbers, same float expressions; a pixel may differ
+  only where a branch flips on a last-bit difference, so >= 99% of pixels must
+  be bit-equal and the image means agree within 3 standard errors);
+- a closed box of emitting, reflecting walls with a glass sphere inside
+  (radiance Le / (1 - a) everywhere; the oracle is pinned by that closed form
+  on the CPU): the device equals the oracle with both Fresnel models and
+  roulette on, and equals the closed form without the sphere;
 - the reference's analytic furnace (test_render.cpp:50-165);
 - the reference's 'triangle umbrella' image (test_input/triangle_umbrella.png,
   100 spp, linear output): MAE <= 1.5/255 (test_render.cpp:239-433), rendered
@@ -56,23 +61,59 @@ def test_light_table_matches_oracle(gpu_ctx):
     s.close()
 
 
-def test_cornell_matches_oracle(gpu_ctx):
-    print("[case] cornell vs oracle", flush=True)
-    sc = scenes.cornell_c0()
-    s = make_scene(gpu_ctx, sc)
-    c = sc.camera
-    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], 64, 64)
-    got = s.render(cam, spp=8, max_depth=8, rr_depth=3, seed=0)
-    ref, _ = oracle.render(oracle.BVH(sc.vertices, sc.indices), sc.tri_material, oracle.materials_array(sc.materials),
-                           oracle.Camera.from_buffer_copy(bytes(cam)), spp=8, max_depth=8, rr_depth=3, seed=0)
+def assert_matches_oracle(got, ref, label):
+    """>= 99% of pixels bit-identical (same Philox numbers, same float
+    expressions) and the image means within 3 standard errors."""
     assert np.array_equal(got[..., 3], ref[..., 3])
     same = np.all(got[..., :3].view(np.uint64) == ref[..., :3].view(np.uint64), axis=2)
-    print(f"bit-identical pixels: {same.mean():.5f}")
+    print(f"{label}: bit-identical pixels {same.mean():.5f}", flush=True)
     assert same.mean() >= 0.99
     g = got[..., :3] / got[..., 3:]
     r = ref[..., :3] / ref[..., 3:]
     se = r.std() / np.sqrt(r.size)
     assert abs(g.mean() - r.mean()) < 3 * se + 1e-9
+
+
+# (size, spp): a quick case and the full Cornell workload (256 x 256, 16 spp,
+# 8 vertices, roulette from vertex 3, seed 0); fresnel 0 = exact, 1 = Schlick
+# (the reference's, and the one the glass sphere then uses)
+@pytest.mark.parametrize("size,spp,fresnel", [(64, 8, 0), (64, 8, 1), (256, 16, 0), (256, 16, 1)])
+def test_cornell_matches_oracle(gpu_ctx, size, spp, fresnel):
+    print(f"[case] cornell vs oracle {size}^2 x {spp} spp, fresnel {fresnel}", flush=True)
+    sc = scenes.cornell_c0()
+    s = make_scene(gpu_ctx, sc)
+    c = sc.camera
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], size, size)
+    got = s.render(cam, spp=spp, max_depth=8, rr_depth=3, seed=0, fresnel=fresnel)
+    ref, _ = oracle.render(oracle.BVH(sc.vertices, sc.indices), sc.tri_material, oracle.materials_array(sc.materials),
+                           oracle.Camera.from_buffer_copy(bytes(cam)), spp=spp, max_depth=8, rr_depth=3, seed=0,
+                           fresnel=fresnel)
+    assert_matches_oracle(got, ref, f"cornell {size} fresnel {fresnel}")
+    s.close()
+
+
+@pytest.mark.parametrize("glass,fresnel", [(None, 0), (1.5, 0), (1.5, 1)])
+def test_isothermal_enclosure(gpu_ctx, glass, fresnel):
+    """A closed box of walls with albedo 0.5 and emission 1 (radiance 2
+    everywhere), optionally with a glass sphere inside, roulette from vertex
+    2, 64 vertices: the device equals the oracle (whose estimator is pinned
+    by the closed form on the CPU), and without the sphere it gives 2 within
+    3.5 standard errors (0.5^64 is far below the noise; with the sphere, paths
+    circulating inside it by total internal reflection are cut at 64 vertices,
+    about 0.4% of the radiance)."""
+    print(f"[case] enclosure glass {glass} fresnel {fresnel}", flush=True)
+    sc = scenes.furnace_enclosure(0.5, 1.0, glass)
+    s = make_scene(gpu_ctx, sc)
+    c = sc.camera
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], 32, 32)
+    got = s.render(cam, spp=256, max_depth=64, rr_depth=2, seed=0, fresnel=fresnel)
+    ref, _ = oracle.render(oracle.BVH(sc.vertices, sc.indices), sc.tri_material, oracle.materials_array(sc.materials),
+                           oracle.Camera.from_buffer_copy(bytes(cam)), spp=256, max_depth=64, rr_depth=2, seed=0,
+                           fresnel=fresnel)
+    assert_matches_oracle(got, ref, f"enclosure glass {glass}")
+    if glass is None:
+        px = (got[..., 0] / got[..., 3]).ravel()
+        assert abs(px.mean() - 2.0) < 3.5 * px.std() / np.sqrt(px.size)
     s.close()
 
 
