@@ -126,6 +126,7 @@ PROTOTYPES = {
     "dr_scene_build_bvh": (_st, [_vp, C.POINTER(DrBvhParams), _vp]),
     "dr_scene_refit": (_st, [_vp, _vp, _vp]),
     "dr_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_trace_closest_noalloc": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "dr_scene_set_trace_schedule": (_st, [_vp, _u32, _u32]),

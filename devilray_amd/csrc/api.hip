@@ -96,6 +96,10 @@ dr_status dr_context_create(int32_t device, dr_context **out)
     if (!ctx) return fail(DR_E_OOM, "host allocation failed");
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
+    if (dr_status st = noalloc_init(ctx)) {
+        delete ctx;
+        return st;
+    }
     *out = reinterpret_cast<dr_context *>(ctx);
     return DR_OK;
 }
@@ -309,6 +313,12 @@ dr_status dr_trace_closest(dr_scene *sc, const dr_ray *rays, uint32_t n, dr_hit 
     if (!sc) return fail(DR_E_INVALID, "scene is null");
     return launch_trace(reinterpret_cast<Scene *>(sc), rays, n, hits, nullptr, nullptr, nullptr, false,
                         reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_trace_closest_noalloc(dr_scene *sc, const dr_ray *rays, uint32_t n, dr_hit *hits, dr_stream stream)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    return launch_trace_noalloc(reinterpret_cast<Scene *>(sc), rays, n, hits, reinterpret_cast<hipStream_t>(stream));
 }
 
 dr_status dr_trace_any(dr_scene *sc, const dr_ray *rays, uint32_t n, uint32_t *occluded, dr_stream stream)

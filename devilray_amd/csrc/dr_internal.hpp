@@ -68,6 +68,9 @@ struct Context {
     hipStream_t stream = nullptr;  // owned helper stream (unused by the async API)
     DevBuf scratch_sort[6];         // radix-sort / scan temporaries
     DevBuf display_thr;             // 2 x 255 FP64 decision thresholds of the 8-bit display encodings
+    // this device's copy of the zero-allocation traversal's static scratch
+    // array (its address from hipGetSymbolAddress at context creation)
+    int32_t *noalloc_scratch = nullptr;
 };
 
 struct Scene {
@@ -256,6 +259,10 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
                           const uint32_t *ray_root = nullptr);
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
+// closest hit with every scratch buffer in the static device array (no
+// allocation); noalloc_init looks up this device's copy of it
+dr_status noalloc_init(Context *ctx);
+dr_status launch_trace_noalloc(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, hipStream_t stream);
 
 // sort.hip
 dr_status radix_sort_pairs(Context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,

@@ -422,7 +422,82 @@ TraceKernel pick(uint32_t sched)
     }
 }
 
+// Blocks per CU of a kernel variant at kBlock threads (occupancy limit),
+// looked up once per variant.
+template <bool ANY, bool COUNT>
+dr_status blocks_per_cu(uint32_t loop, TraceKernel kern, int &bpc)
+{
+    static int cache[3][4] = {};
+    int &c = cache[ANY ? 0 : (COUNT ? 1 : 2)][loop & 3u];
+    if (!c) {
+        int b = 0;
+        DR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBlock, 0));
+        c = b > 0 ? b : 1;
+    }
+    bpc = c;
+    return DR_OK;
+}
+
+// ---- zero-allocation closest hit ------------------------------------------
+// The scratch of a persistent closest-hit launch (the overflow flag, the ray
+// counter and the per-lane stack spill region) in one static device array,
+// for callers that cannot allocate. Its launch shape is capped so the array
+// covers it: at most kNoAllocBlocks resident blocks, and trees of at most
+// kNoAllocMaxDepth wide levels (3 x 21 + 1 = 64 stack entries, 16 of them in
+// LDS). Words [0, 64): [0] overflow flag, [8, 16) ray counters; then the
+// spill region, kNoAllocSpill entries x kNoAllocBlocks x kBlock lanes.
+constexpr uint32_t kNoAllocBlocks = 1024;
+constexpr uint32_t kNoAllocMaxDepth = 21;
+constexpr uint32_t kNoAllocSpill = 3u * kNoAllocMaxDepth + 1u - (uint32_t)kLdsStack;
+constexpr size_t kNoAllocHeadWords = 64;
+constexpr size_t kScratchWords = kNoAllocHeadWords + (size_t)kNoAllocSpill * kNoAllocBlocks * kBlock;
+
 }  // namespace
+
+static __device__ __align__(256) int32_t scratch[kScratchWords];  // 48 MiB + 256 B per device
+
+dr_status noalloc_init(Context *ctx)
+{
+    // the array's name is a host-side stand-in: kernels get this device
+    // address instead
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(DR_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ctx->noalloc_scratch = static_cast<int32_t *>(p);
+    return DR_OK;
+}
+
+dr_status launch_trace_noalloc(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, hipStream_t stream)
+{
+    if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
+    if (n == 0) return DR_OK;
+    if (!rays || !hits) return fail(DR_E_INVALID, "null ray or output buffer");
+    if (s->depth > kNoAllocMaxDepth)
+        return fail(DR_E_OVERFLOW, "dr_trace_closest_noalloc: the BVH has " + std::to_string(s->depth) +
+                                       " wide levels, the static scratch covers at most " +
+                                       std::to_string(kNoAllocMaxDepth) + " (use dr_trace_closest)");
+    int32_t *base = s->ctx->noalloc_scratch;
+    if (!base) return fail(DR_E_DEVICE, "zero-allocation scratch not set up on this context");
+    const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
+    // the resident-grid form of the scene's closest-hit schedule, no reordering
+    const uint32_t loop = (s->sched_closest & 1u) | 2u;
+    const TraceKernel kern = pick<false, false>(loop);
+    int occ = 1;
+    if (dr_status st = blocks_per_cu<false, false>(loop, kern, occ)) return st;
+    const int bpc = s->blocks_per_cu ? std::min<int>((int)s->blocks_per_cu, occ) : occ;
+    const uint32_t need = (n + kBlock - 1) / kBlock;
+    const uint32_t grid = std::min<uint32_t>({need, (uint32_t)(s->ctx->cu_count * bpc), kNoAllocBlocks});
+    uint32_t *flags = reinterpret_cast<uint32_t *>(base);
+    uint32_t *counter = flags + 8;
+    k_zero_counters8<<<1, 64, 0, stream>>>(counter);
+    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(),
+                                      reinterpret_cast<const float4 *>(rays), nullptr, n, nullptr, hits, nullptr,
+                                      nullptr, nullptr, base + kNoAllocHeadWords, flags, counter, s->refill_min,
+                                      nullptr, 1u, max_stack);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
 
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream)
@@ -452,15 +527,13 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     if (max_stack - kLdsStack > kPerRaySpillMax) loop |= 2u;  // PER_RAY -> PERSISTENT, keeping speculation
     const TraceKernel kern = any_hit ? pick<true, false>(loop) : (count ? pick<false, true>(loop) : pick<false, false>(loop));
     const bool persistent = loop == DR_SCHED_PERSISTENT || loop == DR_SCHED_PERSISTENT_SPECULATIVE;
-    static int occ_cache[16] = {0};
-    const int variant = (any_hit ? 0 : (count ? 1 : 2)) * 4 + (int)loop;
-    if (!occ_cache[variant]) {
-        int bpc = 0;
-        DR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kBlock, 0));
-        occ_cache[variant] = bpc > 0 ? bpc : 1;
-    }
+    int occ = 1;
+    if (dr_status st = any_hit ? blocks_per_cu<true, false>(loop, kern, occ)
+                               : (count ? blocks_per_cu<false, true>(loop, kern, occ)
+                                        : blocks_per_cu<false, false>(loop, kern, occ)))
+        return st;
     const uint32_t need = (n + kBlock - 1) / kBlock;
-    const int bpc = s->blocks_per_cu ? std::min<int>((int)s->blocks_per_cu, occ_cache[variant]) : occ_cache[variant];
+    const int bpc = s->blocks_per_cu ? std::min<int>((int)s->blocks_per_cu, occ) : occ;
     const uint32_t resident = (uint32_t)(s->ctx->cu_count * bpc);
     const uint32_t grid = persistent ? (need < resident ? need : resident) : need;
     const size_t nlanes = (size_t)grid * kBlock;

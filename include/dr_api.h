@@ -164,6 +164,19 @@ dr_status dr_scene_refit(dr_scene *scene, const float *vertices, dr_stream strea
 dr_status dr_trace_closest(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_hit *hits,
                            dr_stream stream);
 
+/* dr_trace_closest without any allocation. No counterpart in the reference:
+ * for callers that cannot call hipMalloc (HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation). Same hits as dr_trace_closest,
+ * bit for bit. Its scratch (ray counter and traversal stack spill) is one
+ * static device array of 48 MiB + 256 B that every GPU running the library
+ * holds until the process ends; it covers the resident grid (at most 1024
+ * blocks of 256 lanes) over trees of at most 21 wide levels, and a deeper
+ * tree returns DR_E_OVERFLOW (dr_scene_info.depth tells). Calls on one GPU
+ * share that array: calls that overlap in time on one GPU (different
+ * streams, threads or scenes) race and must be serialised by the caller. */
+dr_status dr_trace_closest_noalloc(dr_scene *scene, const dr_ray *rays, uint32_t n, dr_hit *hits,
+                                   dr_stream stream);
+
 /* Any hit (occlusion): occluded[i] = 1 if some triangle has tmin <= t <= tmax.
  * Replaces visible() (include/tracing/PathGeneration.hpp:213-224), which the
  * reference answers with a closest-hit cast. */
