@@ -21,6 +21,14 @@ class SceneData:
     tri_material: np.ndarray
     materials: list = field(default_factory=list)
     camera: dict = field(default_factory=dict)
+    # the reference's objects (TriangleMesh: one material, own BBH) as
+    # contiguous triangle ranges (first, count); None = one object
+    objects: np.ndarray | None = None
+
+    def object_ranges(self) -> np.ndarray:
+        if self.objects is None:
+            return np.array([[0, self.indices.shape[0]]], np.int32)
+        return np.asarray(self.objects, np.int32).reshape(-1, 2)
 
     def dr_materials(self):
         out = []
@@ -38,6 +46,8 @@ class _Builder:
     def __init__(self):
         self.v, self.f, self.m, self.mats = [], [], [], []
         self.nv = 0
+        self.nt = 0
+        self.objects = []  # (first triangle, count) per mesh
 
     def material(self, kind, albedo=(0, 0, 0), emission=(0, 0, 0), ior=1.0) -> int:
         self.mats.append((kind, tuple(float(x) for x in albedo), tuple(float(x) for x in emission), float(ior)))
@@ -49,7 +59,9 @@ class _Builder:
         self.v.append(verts)
         self.f.append(tris + self.nv)
         self.m.append(np.full(tris.shape[0], mat, np.uint32))
+        self.objects.append((self.nt, tris.shape[0]))
         self.nv += verts.shape[0]
+        self.nt += tris.shape[0]
 
     def quad(self, center, normal, right, size_u, size_v, mat):
         """Quad like createQuadMesh (src/GpuTris.cpp:87-107): up = right x normal,
@@ -66,7 +78,8 @@ class _Builder:
 
     def build(self, camera=None) -> SceneData:
         return SceneData(np.concatenate(self.v).astype(np.float32), np.concatenate(self.f).astype(np.uint32),
-                         np.concatenate(self.m).astype(np.uint32), self.mats, camera or {})
+                         np.concatenate(self.m).astype(np.uint32), self.mats, camera or {},
+                         np.array(self.objects, np.int32).reshape(-1, 2))
 
 
 def cornell_c0() -> SceneData:
@@ -214,8 +227,12 @@ def many_lights(n_spheres: int = 400, subdiv: int = 4, n_emissive: int = 100_000
             b_m.append(wmat)
     cam = dict(eye=(0.0, 0.0, 3.2 * extent), target=(0.0, -0.6 * extent if partition else 0.0, 0.0),
                up=(0.0, 1.0, 0.0), fov=45.0, width=1920, height=1080)
+    # objects: one per sphere, then one per room quad
+    objects = [(k * nt, nt) for k in range(n_spheres)]
+    if partition:
+        objects += [(n_spheres * nt + first, count) for first, count in room.objects]
     return SceneData(np.concatenate(b_v).astype(np.float32), np.concatenate(b_f).astype(np.uint32),
-                     np.concatenate(b_m).astype(np.uint32), mats, cam)
+                     np.concatenate(b_m).astype(np.uint32), mats, cam, np.array(objects, np.int32).reshape(-1, 2))
 
 
 def hidden_light_reflector() -> SceneData:

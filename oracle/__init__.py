@@ -21,6 +21,9 @@ HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("
 _vp = C.c_void_p
 _i32 = C.c_int32
 PROTOTYPES = {
+    "or_refscene_build": (_vp, [_vp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32]),
+    "or_refscene_free": (None, [_vp]),
+    "or_render_ref": (None, [_vp, _vp, _vp, _vp, C.c_int32, _vp, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_int32]),
     "or_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
     "or_fresnel": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32]),
     "or_dielectric_scatter": (C.c_int32, [_vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_int32, _vp]),
@@ -189,6 +192,43 @@ def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int =
     lib().or_render(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(m), _p(ent) if len(ent) else None, _p(pdf), len(ent),
                     C.byref(camera), C.byref(prm), lo, hi, _p(acc), _p(hist), nthreads or threads())
     return acc.reshape(camera.height, camera.width, 4), hist
+
+class RefScene:
+    """The reference's scene model for its integrator: objects (contiguous
+    triangle ranges, scenes.SceneData.object_ranges()) with one median-split
+    BBH each."""
+
+    def __init__(self, vertices, indices, objects):
+        self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.f = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        self.objects = np.ascontiguousarray(objects, np.int32).reshape(-1, 2)
+        self.h = lib().or_refscene_build(_p(self.v), self.v.shape[0], _p(self.f), self.f.shape[0],
+                                         _p(self.objects), self.objects.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_refscene_free(self.h)
+            self.h = None
+
+
+def render_ref(scene: RefScene, tri_mat, mats, camera: Camera, spp: int, max_depth: int = 10, seed: int = 0,
+               sample_offset: int = 0, pixels=None, accum=None, nthreads: int | None = None, counts: bool = False):
+    """The reference's integrator (sampleColor, reference semantics); returns
+    accum[h, w, 4] float64, plus (extension casts, shadow casts, box tests,
+    triangle tests) when counts is set."""
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    m = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+    ent, _ = lights_build(scene.v, scene.f, tm, m)
+    npix = camera.width * camera.height
+    acc = np.zeros((npix, 4), np.float64) if accum is None else np.ascontiguousarray(accum, np.float64).reshape(npix, 4)
+    prm = RenderParams(spp, sample_offset, max_depth, max_depth, seed, 1, 0)
+    lo, hi = (0, npix) if pixels is None else pixels
+    cnt = np.zeros(4, np.int64)
+    lib().or_render_ref(scene.h, _p(tm), _p(m), _p(ent) if len(ent) else None, len(ent), C.byref(camera),
+                        C.byref(prm), lo, hi, _p(acc), _p(cnt), nthreads or threads())
+    out = acc.reshape(camera.height, camera.width, 4)
+    return (out, cnt) if counts else out
+
 
 def display_thresholds(srgb: bool) -> np.ndarray:
     """The 255 decision thresholds of the 8-bit linear or sRGB encoding."""

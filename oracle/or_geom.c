@@ -789,69 +789,87 @@ typedef struct {
  * on a miss or a leaf climb through parent links until an unvisited far
  * child appears. Boxes are skipped when their entry distance is not below
  * the best hit. Leaf triangles keep the first strictly nearer hit. */
+int or_refbbh_closest_one(const void *bbh, const float *verts, const int32_t *tris, const float p[3],
+                          const float d[3], int32_t *id_out, float *t_out, float bari_out[3], int64_t *box_tests,
+                          int64_t *tri_tests)
+{
+    const rbbh *b = (const rbbh *)bbh;
+    int has = 0;
+    float bt = 0, bb[3] = { 0, 0, 0 };
+    int32_t bid = -1;
+    int64_t nbox = 0, ntri = 0;
+    uint32_t trail = 0, depth = 0;
+    int32_t cur = b->n > 0 ? 0 : -1;
+    while (cur >= 0 && cur < b->n) {
+        const rnode *nd = &b->nodes[cur];
+        ++nbox;
+        float tb;
+        if (or_box_ref(nd->lo, nd->hi, p, d, &tb)) {
+            if (!has || tb < bt) {
+                if (nd->left == -1 && nd->right == -1) {
+                    for (int32_t j = nd->tb; j < nd->te; ++j) {
+                        const int32_t id = b->order[j];
+                        const int32_t *tri = tris + 3 * (size_t)id;
+                        float t, bari[3];
+                        ++ntri;
+                        if (!or_tri_reference(p, d, verts + 3 * (size_t)tri[0], verts + 3 * (size_t)tri[1],
+                                              verts + 3 * (size_t)tri[2], &t, bari)) continue;
+                        if (!has || bt > t) {
+                            has = 1; bt = t; bid = id;
+                            bb[0] = bari[0]; bb[1] = bari[1]; bb[2] = bari[2];
+                        }
+                    }
+                } else {
+                    const uint32_t axis = depth % 3;
+                    const int right = d[axis] > 0.0f;
+                    const int32_t nearc = right ? cur + 1 : nd->right;
+                    trail &= ~(1u << depth);
+                    cur = nearc;
+                    ++depth;
+                    continue;
+                }
+            }
+        }
+        for (;;) {
+            if (cur == 0) { cur = -1; break; }
+            const int32_t pi = b->nodes[cur].parent;
+            const rnode *pn = &b->nodes[pi];
+            --depth;
+            if (!(trail & (1u << depth))) {
+                const uint32_t axis = depth % 3;
+                const int right = d[axis] > 0.0f;
+                const int32_t farc = right ? pn->right : pi + 1;
+                trail |= (1u << depth);
+                cur = farc;
+                ++depth;
+                break;
+            }
+            cur = pi;
+        }
+    }
+    *id_out = has ? bid : -1;
+    *t_out = has ? bt : INFINITY;
+    bari_out[0] = bb[0]; bari_out[1] = bb[1]; bari_out[2] = bb[2];
+    if (box_tests) *box_tests += nbox;
+    if (tri_tests) *tri_tests += ntri;
+    return has;
+}
+
 static void rtrace_range(void *vc, int32_t begin, int32_t end)
 {
     rtrace_ctx *c = (rtrace_ctx *)vc;
-    const rbbh *b = c->b;
     for (int32_t i = begin; i < end; ++i) {
         const or_ray *r = &c->rays[i];
         const float p[3] = { r->ox, r->oy, r->oz };
         const float d[3] = { r->dx, r->dy, r->dz };
-        int has = 0;
-        float bt = 0, bu = 0, bv = 0;
-        int32_t bid = -1;
+        int32_t id;
+        float t, bari[3];
         int64_t nbox = 0, ntri = 0;
-        uint32_t trail = 0, depth = 0;
-        int32_t cur = b->n > 0 ? 0 : -1;
-        while (cur >= 0 && cur < b->n) {
-            const rnode *nd = &b->nodes[cur];
-            ++nbox;
-            float tb;
-            if (or_box_ref(nd->lo, nd->hi, p, d, &tb)) {
-                if (!has || tb < bt) {
-                    if (nd->left == -1 && nd->right == -1) {
-                        for (int32_t j = nd->tb; j < nd->te; ++j) {
-                            const int32_t id = b->order[j];
-                            const int32_t *tri = c->tris + 3 * (size_t)id;
-                            float t, bari[3];
-                            ++ntri;
-                            if (!or_tri_reference(p, d, c->verts + 3 * (size_t)tri[0],
-                                                  c->verts + 3 * (size_t)tri[1],
-                                                  c->verts + 3 * (size_t)tri[2], &t, bari)) continue;
-                            if (!has || bt > t) { has = 1; bt = t; bid = id; bu = bari[1]; bv = bari[2]; }
-                        }
-                    } else {
-                        const uint32_t axis = depth % 3;
-                        const int right = d[axis] > 0.0f;
-                        const int32_t nearc = right ? cur + 1 : nd->right;
-                        trail &= ~(1u << depth);
-                        cur = nearc;
-                        ++depth;
-                        continue;
-                    }
-                }
-            }
-            for (;;) {
-                if (cur == 0) { cur = -1; break; }
-                const int32_t pi = b->nodes[cur].parent;
-                const rnode *pn = &b->nodes[pi];
-                --depth;
-                if (!(trail & (1u << depth))) {
-                    const uint32_t axis = depth % 3;
-                    const int right = d[axis] > 0.0f;
-                    const int32_t farc = right ? pn->right : pi + 1;
-                    trail |= (1u << depth);
-                    cur = farc;
-                    ++depth;
-                    break;
-                }
-                cur = pi;
-            }
-        }
-        c->hits[i].id = has ? bid : -1;
-        c->hits[i].t = has ? bt : INFINITY;
-        c->hits[i].u = has ? bu : 0.0f;
-        c->hits[i].v = has ? bv : 0.0f;
+        const int has = or_refbbh_closest_one(c->b, c->verts, c->tris, p, d, &id, &t, bari, &nbox, &ntri);
+        c->hits[i].id = id;
+        c->hits[i].t = t;
+        c->hits[i].u = has ? bari[1] : 0.0f;
+        c->hits[i].v = has ? bari[2] : 0.0f;
         if (c->box_tests) c->box_tests[i] = nbox;
         if (c->tri_tests) c->tri_tests[i] = ntri;
     }

@@ -420,6 +420,16 @@ static or_ray camera_ray(const or_camera *cam, uint64_t seed, uint32_t pixel, ui
     return ray;
 }
 
+void or_path_rng(uint32_t pixel, uint32_t sample, uint32_t vertex, uint32_t stream, uint64_t seed, uint32_t out[4])
+{
+    path_rng(pixel, sample, vertex, stream, seed, out);
+}
+
+void or_camera_sample(const or_camera *cam, uint64_t seed, uint32_t pixel, uint32_t sample, or_ray *out)
+{
+    *out = camera_ray(cam, seed, pixel, sample);
+}
+
 /* One path: returns its radiance. */
 static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
 {

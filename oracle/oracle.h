@@ -173,6 +173,24 @@ void or_render(const void *bvh, const float *verts, const int32_t *tris, const i
                const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
                double *accum, int32_t *vertex_hist, int32_t threads);
 
+/* ---- the reference's integrator (reference semantics, or_refrender.c) ------ */
+/* An object: a contiguous range of triangles (the reference's TriangleMesh,
+ * one material, its own BBH). */
+typedef struct { int32_t first, count; } or_object;
+/* Per-object median-split BBHs over verts / tris (kept by the caller). */
+void *or_refscene_build(const float *verts, int32_t nverts, const int32_t *tris, int32_t ntris,
+                        const or_object *objects, int32_t n_objects);
+void or_refscene_free(void *scene);
+/* sampleColor (PathGeneration.hpp:363-515) for pixels [pixel_begin,
+ * pixel_end), prm->spp samples each (prm->max_depth vertices, 10 in the
+ * reference; rr_depth and fresnel are ignored: no roulette, Schlick), added
+ * to accum in sample order. alias / n_emit: the emissive-triangle table of
+ * or_lights_build. counts (optional, int64[4], added to): extension casts,
+ * shadow casts, BBH box tests, triangle tests. */
+void or_render_ref(const void *scene, const int32_t *tri_mat, const or_material *mats,
+                   const or_alias_entry *alias, int32_t n_emit, const or_camera *cam, const or_render_params *prm,
+                   int32_t pixel_begin, int32_t pixel_end, double *accum, int64_t *counts, int32_t threads);
+
 /* ---- display and debug views ------------------------------------------------ */
 /* 8-bit decision thresholds of the linear (srgb = 0) or sRGB encoding. */
 void or_display_thresholds(int32_t srgb, double thr[255]);
