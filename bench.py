@@ -15,17 +15,27 @@ scene (240x135, reference 65,536 spp of the plain path in FP64) and on the
 hidden-light reflector scene; and batched rendering of 64 uneven scenes
 (one dr_render per scene against one autotuned dr_batch_render).
 
-Prints one JSON line. `roofline.achieved` uses the algorithmic bytes per ray
-48 + 32*N_node + 40*N_tri with N from the canonical binary SAH BVH counts
-committed in tests/golden/roofline_counts.json, divided by the kernel's mean
-HIP-event duration on its stream; `traffic` is the measured HBM bytes per
-launch from the committed rocprofv3 summary, `traffic_gbs` that amount over
-the same duration and `traffic_frac` its fraction of the HBM peak. The device BVH cuts long thin triangles into several
-references and so makes far fewer tests per ray than the canonical tree, which
-is why `frac` can exceed 1: the algorithmic bytes are fixed by the canonical
-tree, the traffic is what this kernel moves. `cpu_baseline` times the
-oracle's reference-semantics traversal (median-split BBH, stackless walk) on a
-bounded sample of the same rays on the host cores.
+Prints one JSON line. `roofline` is the timed kernel's memory traffic: the
+calibrated FETCH_SIZE + WRITE_SIZE bytes per launch from the committed
+rocprofv3 counter runs (profiles/trace_closest_c1_incoherent_traffic.json),
+over the kernel's mean HIP-event duration on its stream, against the 8 TB/s
+HBM peak. The counter runs carry a hash of the kernel sources; if the
+sources changed since, the traffic is reported as stale (null). Beside it,
+`roofline.algorithmic` gives the canonical figure: 48 + 32*N_node + 40*N_tri
+bytes per ray with N_node the interior-node visits and N_tri the triangle
+tests of the oracle's canonical binary SAH BVH on the same rays
+(tests/golden/roofline_counts.json). The device BVH cuts long thin triangles
+into several references and so does far less work than that canonical tree,
+which is why the algorithmic figure can reach or exceed the peak.
+`cpu_baseline` times the oracle's reference-semantics traversal (median-split
+BBH, stackless walk, the reference's triangle test) over the whole ray batch
+on all host threads allotted to the process (1 warm-up, median of 5).
+`extra.render` adds the many-lights scene at its full configuration
+(1920x1080, 64 spp, 12 vertices) with its path-sample roofline (the sum of a
+sample's ray bytes on the canonical tree plus 64 B of FP64 accumulation,
+from the oracle's counts), and the reference-semantics CPU integrator's
+samples/s on the Cornell box (its full 256 x 256 x 16 spp) and on a
+480 x 270 x 4 spp sample of the many-lights view.
 
 --probe: runs only the traversal launches and a 256 MiB device copy (the
 known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
@@ -55,6 +65,19 @@ HBM_PEAK_GBS = 8000.0
 SUBDIV, SIGMA, SEED, NRAYS = 8, 0.05, 0, 1 << 20
 COUNTS_FIXTURE = os.path.join(ROOT, "tests", "golden", "roofline_counts.json")
 TRAFFIC_FIXTURE = os.path.join(ROOT, "profiles", "trace_closest_c1_incoherent_traffic.json")
+# the sources that decide the traversal kernel's memory traffic (the kernel
+# and the tree it walks); counter runs record their hash
+TRAFFIC_SOURCES = ["devilray_amd/csrc/trace.hip", "devilray_amd/csrc/dr_device.hpp",
+                   "devilray_amd/csrc/dr_internal.hpp", "devilray_amd/csrc/bvh_build.hip"]
+
+
+def sources_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def log(msg: str) -> None:
@@ -77,28 +100,90 @@ def timed_launches(fn, steps: int, warmup: int, ctx) -> tuple[float, list[float]
     return wall, per
 
 
+def host_threads() -> int:
+    """Threads allotted to this process: its CPU affinity, capped by
+    OMP_NUM_THREADS where the machine sets it (the per-GPU CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
+
+
+def median_time(fn, runs: int = 5) -> float:
+    fn(warm=True)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn(warm=False)
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2]
+
+
 def cpu_baseline(v, f, rays_host, sample: int):
     import oracle  # CPU baseline leg only
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    nthreads = host_threads()
     log(f"[bench] cpu baseline: building reference-semantics BBH over {f.shape[0]} tris")
+    t0 = time.perf_counter()
     bbh = oracle.RefBBH(v, f)
+    build_s = time.perf_counter() - t0
     sub = rays_host[:sample].copy()
-    bbh.closest(sub[:1024], nthreads=nthreads)  # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        bbh.closest(sub, nthreads=nthreads)
-        times.append(time.perf_counter() - t0)
-    med = sorted(times)[len(times) // 2]
+    log(f"[bench] cpu baseline: {sample} rays, {nthreads} threads, warm-up + median of 5")
+    med = median_time(lambda warm: bbh.closest(sub[:16384] if warm else sub, nthreads=nthreads))
     return {
         "value": round(sample / med / 1e6, 4),
         "unit": "Mrays/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"{sample} of the {NRAYS} incoherent rays, closest-hit through the oracle's restatement of "
-                  f"the reference median-split BBH + stackless walk + plane triangle test, median of 3 runs, "
-                  f"{nthreads} threads",
+        "sample": f"{sample} of the {NRAYS} incoherent rays (seed 0), closest hit through the oracle's restatement "
+                  f"of the reference's median-split BBH, stackless walk and plane triangle test; 1 warm-up, median "
+                  f"of 5 runs on {nthreads} threads (all the process is allotted); BBH build {build_s:.1f} s "
+                  f"single-threaded, not included",
+        "host": host_info(),
     }
+
+
+def cpu_render_baselines():
+    """The oracle's restatement of the reference's integrator (10 vertices,
+    no roulette, Schlick, objects with median-split BBHs and a cast over all
+    of them) on the host: the Cornell box at its full 256 x 256 x 16 spp, and
+    the many-lights scene on a 480 x 270 x 4 spp sample of its view (about a
+    second per run on 16 threads)."""
+    import oracle  # CPU baseline leg only
+    nthreads = host_threads()
+    out = {}
+    for name, sc, w, h, spp in (("cornell_256_16spp", scenes.cornell_c0(), 256, 256, 16),
+                                ("many_lights_480x270_4spp", scenes.many_lights(), 480, 270, 4)):
+        log(f"[bench] cpu render baseline: {name}")
+        t0 = time.perf_counter()
+        rs = oracle.RefScene(sc.vertices, sc.indices, sc.object_ranges())
+        build_s = time.perf_counter() - t0
+        c = sc.camera
+        cam = oracle.Camera.from_buffer_copy(bytes(dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], w, h)))
+        mats = oracle.materials_array(sc.materials)
+        npix = w * h
+        med = median_time(lambda warm: oracle.render_ref(rs, sc.tri_material, mats, cam, spp=1 if warm else spp,
+                                                         pixels=(0, npix // 16) if warm else None, nthreads=nthreads))
+        out[name] = {"samples_per_s": round(npix * spp / med, 1), "cores": nthreads, "kind": "port",
+                     "median_s": round(med, 3), "objects": int(sc.object_ranges().shape[0]),
+                     "bbh_build_s": round(build_s, 2),
+                     "sample": f"{w}x{h}x{spp} spp, 10 vertices, no roulette, Schlick; 1 warm-up, median of 5"}
+    return out
 
 
 def render_rate(ctx, sc, spp, max_depth, rr_depth, reps, width=None, height=None):
@@ -475,12 +560,62 @@ def probe_guide() -> int:
     return 0
 
 
+def load_counts() -> dict:
+    if not os.path.exists(COUNTS_FIXTURE):
+        return {}
+    with open(COUNTS_FIXTURE) as fh:
+        return json.load(fh)
+
+
+def traversal_roofline(kernel_ms: float) -> dict:
+    """The headline kernel against the HBM roof: its measured traffic per
+    launch (calibrated counters, valid only for the sources it was measured
+    on) over its mean launch time; the canonical algorithmic bytes beside."""
+    kernel_s = kernel_ms * 1e-3
+    roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+            "kernel_ms": round(kernel_ms, 4),
+            "basis": "calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch of the timed kernel "
+                     "(profiles/trace_closest_c1_incoherent_traffic.json) over its mean HIP-event time"}
+    if os.path.exists(TRAFFIC_FIXTURE):
+        with open(TRAFFIC_FIXTURE) as fh:
+            tf = json.load(fh)
+        cur = sources_hash()
+        roof["traffic_sources_hash"] = tf.get("sources_hash")
+        if tf.get("sources_hash") == cur:
+            traffic = int(tf["hbm_bytes_per_launch"])
+            gbs = traffic / kernel_s / 1e9
+            roof.update({"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic})
+        else:
+            roof["traffic_stale"] = f"measured on sources {tf.get('sources_hash')}, built from {cur}"
+    c = load_counts().get("c1_incoherent_closest")
+    if c:
+        bpr = 48.0 + 32.0 * c["mean_node_visits"] + 40.0 * c["mean_tri_tests"]
+        gbs = bpr * NRAYS / kernel_s / 1e9
+        roof["algorithmic"] = {"bytes_per_ray": round(bpr, 2), "achieved": round(gbs, 1),
+                               "frac": round(gbs / HBM_PEAK_GBS, 4),
+                               "basis": "48 + 32 N_node + 40 N_tri, N_node = interior-node visits and N_tri = "
+                                        "triangle tests per ray of the oracle's canonical binary SAH BVH "
+                                        "(32 bins, leaf <= 4) on the same rays"}
+    return roof
+
+
+def render_roofline(samples_per_s: float) -> dict | None:
+    c = load_counts().get("c2_render_path_sample")
+    if not c:
+        return None
+    gbs = samples_per_s * c["bytes_per_sample"] / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_sample": c["bytes_per_sample"],
+            "basis": "per path sample: its extension and shadow rays' bytes on the oracle's canonical BVH "
+                     "(48 or 36 + 32 N_node + 40 N_tri each) + 64 B of FP64 accumulation"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample", type=int, default=NRAYS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-guide", action="store_true")
@@ -536,34 +671,16 @@ def main() -> int:
     log("[bench] BVH refit to a new displacement of the same mesh")
     refit = refit_rate(ctx, v, f, args)
 
-    roofline = None
-    counts = None
-    if os.path.exists(COUNTS_FIXTURE):
-        with open(COUNTS_FIXTURE) as fh:
-            counts = json.load(fh).get("c1_incoherent_closest")
-    if counts:
-        bytes_per_ray = 48.0 + 32.0 * counts["mean_node_tests"] + 40.0 * counts["mean_tri_tests"]
-        achieved = bytes_per_ray * NRAYS / (kernel_ms * 1e-3) / 1e9
-        traffic = None
-        if os.path.exists(TRAFFIC_FIXTURE):
-            with open(TRAFFIC_FIXTURE) as fh:
-                traffic = json.load(fh).get("hbm_bytes_per_launch")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "bytes_per_ray": round(bytes_per_ray, 2), "kernel_ms": round(kernel_ms, 4)}
-        if traffic:
-            # measured bytes moved per launch over the same time: what the memory system actually carried
-            roofline["traffic_gbs"] = round(traffic / (kernel_ms * 1e-3) / 1e9, 1)
-            # the measured bytes against the same peak: how close the kernel
-            # runs to HBM bandwidth (frac compares the canonical tree's bytes)
-            roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    roofline = traversal_roofline(kernel_ms)
 
     renders = {}
     if not args.no_render:
         log("[bench] render: Cornell box 256x256")
         renders["cornell_256_16spp"] = render_rate(ctx, scenes.cornell_c0(), 16, 8, 3, 5)
-        log("[bench] render: many-lights scene 1920x1080")
-        renders["many_lights_1080p"] = render_rate(ctx, scenes.many_lights(), 2, 12, 3, 2)
+        log("[bench] render: many-lights scene 1920x1080, 64 spp")
+        ml = render_rate(ctx, scenes.many_lights(), 64, 12, 3, 1)
+        ml["roofline"] = render_roofline(ml["samples_per_s"])
+        renders["many_lights_1080p_64spp"] = ml
         log("[bench] display resolve and debug view at 1920x1080")
         renders["display"] = display_rate(ctx, scenes.many_lights())
 
@@ -588,7 +705,9 @@ def main() -> int:
 
     base = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline(v, f, rays, args.cpu_sample)
+        base = cpu_baseline(v, f, rays, min(args.cpu_sample, NRAYS))
+        if not args.no_render:
+            renders["cpu_reference_integrator"] = cpu_render_baselines()
 
     out = {
         "metric": "Mrays/s incoherent closest-hit; ms to target MSE, guided vs plain",

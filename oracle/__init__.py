@@ -56,7 +56,7 @@ PROTOTYPES = {
     "or_refbbh_trace": (None, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32]),
     "or_lights_build": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "or_sincos_2pi": (None, [C.c_float, _vp, _vp]),
-    "or_render": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32]),
+    "or_render": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _i32]),
     "or_guide_encode": (None, [_vp, _vp, _vp]),
     "or_guide_forward": (None, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "or_guide_loss": (C.c_double, [_vp, _vp, _vp, _vp, _i32, _i32]),
@@ -178,9 +178,12 @@ def lights_build(vertices, indices, tri_mat, mats):
 
 def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int = 8, rr_depth: int = 3,
            seed: int = 0, fresnel: int = 0, sample_offset: int = 0, pixels=None, accum=None,
-           vertex_hist: bool = False, nthreads: int | None = None, plant: int = 0):
-    """Build-semantics path tracer; returns (accum[h, w, 4] float64, hist).
-    plant: PLANT_* bits that insert a known mistake (for the tests' power checks)."""
+           vertex_hist: bool = False, nthreads: int | None = None, plant: int = 0, counts: bool = False):
+    """Build-semantics path tracer; returns (accum[h, w, 4] float64, hist),
+    plus the traversal work (extension rays, their interior-node visits and
+    triangle tests in the canonical BVH, then the same for shadow rays) when
+    counts is set. plant: PLANT_* bits that insert a known mistake (for the
+    tests' power checks)."""
     tm = np.ascontiguousarray(tri_mat, np.int32)
     m = np.ascontiguousarray(mats, MATERIAL_DTYPE)
     ent, pdf = lights_build(bvh.v, bvh.f, tm, m)
@@ -189,8 +192,12 @@ def render(bvh: "BVH", tri_mat, mats, camera: Camera, spp: int, max_depth: int =
     prm = RenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, plant)
     lo, hi = (0, npix) if pixels is None else pixels
     hist = np.zeros(max_depth, np.int32) if vertex_hist else None
+    cnt = np.zeros(6, np.int64)
     lib().or_render(bvh.h, _p(bvh.v), _p(bvh.f), _p(tm), _p(m), _p(ent) if len(ent) else None, _p(pdf), len(ent),
-                    C.byref(camera), C.byref(prm), lo, hi, _p(acc), _p(hist), nthreads or threads())
+                    C.byref(camera), C.byref(prm), lo, hi, _p(acc), _p(hist), _p(cnt) if counts else None,
+                    nthreads or threads())
+    if counts:
+        return acc.reshape(camera.height, camera.width, 4), hist, cnt
     return acc.reshape(camera.height, camera.width, 4), hist
 
 class RefScene:

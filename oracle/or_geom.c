@@ -580,6 +580,12 @@ void or_trace_one(const void *bvh, const float *verts, const int32_t *tris, cons
     trace_ray((const obvh *)bvh, verts, tris, ray, any, hit, occluded, NULL, NULL);
 }
 
+void or_trace_one_counted(const void *bvh, const float *verts, const int32_t *tris, const or_ray *ray, int any,
+                          or_hit *hit, uint32_t *occluded, int32_t *box_tests, int32_t *tri_tests)
+{
+    trace_ray((const obvh *)bvh, verts, tris, ray, any, hit, occluded, box_tests, tri_tests);
+}
+
 static void trace_range(void *vc, int32_t begin, int32_t end)
 {
     trace_ctx *c = (trace_ctx *)vc;

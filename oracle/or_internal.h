@@ -7,6 +7,10 @@
 /* One ray through the canonical BVH (build semantics). */
 void or_trace_one(const void *bvh, const float *verts, const int32_t *tris, const or_ray *ray, int any,
                   or_hit *hit, uint32_t *occluded);
+/* The same, also returning its box tests (1 for the root, then 2 per
+ * interior node visited, so visits = (box_tests - 1) / 2) and triangle tests. */
+void or_trace_one_counted(const void *bvh, const float *verts, const int32_t *tris, const or_ray *ray, int any,
+                          or_hit *hit, uint32_t *occluded, int32_t *box_tests, int32_t *tri_tests);
 
 /* One ray through a reference-semantics BBH (the stackless bit-trail walk,
  * getIntersectionImpl, IntersectionTestsImpl.hpp:141-220): the first

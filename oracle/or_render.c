@@ -382,7 +382,24 @@ typedef struct {
     const or_render_params *prm;
     double *accum;
     int32_t *vertex_hist;  /* optional: paths reaching each vertex count */
+    int64_t *counts;       /* optional: traversal work, see or_render */
 } render_ctx;
+
+/* one traversal, its work added to c[0..2] (rays, interior-node visits,
+ * triangle tests) when c is given */
+static inline void trace_counted(const render_ctx *rc, const or_ray *ray, int any, or_hit *h, uint32_t *occ,
+                                 int64_t *c)
+{
+    if (!c) {
+        or_trace_one(rc->bvh, rc->verts, rc->tris, ray, any, h, occ);
+        return;
+    }
+    int32_t nb = 0, nt = 0;
+    or_trace_one_counted(rc->bvh, rc->verts, rc->tris, ray, any, h, occ, &nb, &nt);
+    c[0] += 1;
+    c[1] += nb > 0 ? (nb - 1) / 2 : 0;
+    c[2] += nt;
+}
 
 /* Pinhole camera ray of a pixel sample (cameraRay,
  * include/tracing/CameraRay.hpp:6-34, with the build's Philox jitter). */
@@ -431,7 +448,7 @@ void or_camera_sample(const or_camera *cam, uint64_t seed, uint32_t pixel, uint3
 }
 
 /* One path: returns its radiance. */
-static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
+static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample, int64_t *counts)
 {
     const or_render_params *prm = rc->prm;
     or_ray ray = camera_ray(rc->cam, prm->seed, pixel, sample);
@@ -441,7 +458,7 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
     float pdf_prev = 0.0f;
     for (uint32_t vertex = 0; vertex < prm->max_depth; ++vertex) {
         or_hit h;
-        or_trace_one(rc->bvh, rc->verts, rc->tris, &ray, 0, &h, NULL);
+        trace_counted(rc, &ray, 0, &h, NULL, counts);
         if (h.id < 0) break;
         if (rc->vertex_hist) __atomic_fetch_add(&rc->vertex_hist[vertex], 1, __ATOMIC_RELAXED);
         const int32_t tri = h.id;
@@ -497,7 +514,7 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
                             const v3 so = vadd(p, vmul(om, eps));
                             or_ray sr = { so.x, so.y, so.z, 0.0f, om.x, om.y, om.z, tmax };
                             uint32_t occ = 0;
-                            or_trace_one(rc->bvh, rc->verts, rc->tris, &sr, 1, NULL, &occ);
+                            trace_counted(rc, &sr, 1, NULL, &occ, counts ? counts + 3 : NULL);
                             if (!occ) L = vadd(L, c);
                         }
                     }
@@ -529,16 +546,16 @@ static v3 trace_path(const render_ctx *rc, uint32_t pixel, uint32_t sample)
     return L;
 }
 
-typedef struct { const render_ctx *rc; int32_t begin, end; } render_job;
+typedef struct { const render_ctx *rc; int32_t begin, end; int64_t counts[6]; } render_job;
 
 static void *render_thread(void *arg)
 {
-    const render_job *j = (const render_job *)arg;
+    render_job *j = (render_job *)arg;
     const render_ctx *rc = j->rc;
     for (int32_t pixel = j->begin; pixel < j->end; ++pixel) {
         double *a = rc->accum + 4 * (size_t)pixel;
         for (uint32_t s = 0; s < rc->prm->spp; ++s) {
-            const v3 L = trace_path(rc, (uint32_t)pixel, rc->prm->sample_offset + s);
+            const v3 L = trace_path(rc, (uint32_t)pixel, rc->prm->sample_offset + s, rc->counts ? j->counts : NULL);
             a[0] = a[0] + (double)L.x;
             a[1] = a[1] + (double)L.y;
             a[2] = a[2] + (double)L.z;
@@ -551,14 +568,15 @@ static void *render_thread(void *arg)
 void or_render(const void *bvh, const float *verts, const int32_t *tris, const int32_t *tri_mat,
                const or_material *mats, const or_alias_entry *alias, const float *pdf_area, int32_t n_emit,
                const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
-               double *accum, int32_t *vertex_hist, int32_t threads)
+               double *accum, int32_t *vertex_hist, int64_t *counts, int32_t threads)
 {
-    render_ctx rc = { bvh, verts, tris, tri_mat, mats, alias, pdf_area, n_emit, cam, prm, accum, vertex_hist };
+    render_ctx rc = { bvh, verts, tris, tri_mat, mats, alias, pdf_area, n_emit, cam, prm, accum, vertex_hist, counts };
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
     const int32_t n = pixel_end - pixel_begin;
     pthread_t th[256];
     render_job jobs[256];
+    memset(jobs, 0, sizeof(jobs));
     int started = 0;
     /* interleaved row blocks keep the threads' loads even */
     const int32_t chunk = (n + threads - 1) / threads;
@@ -571,6 +589,9 @@ void or_render(const void *bvh, const float *verts, const int32_t *tris, const i
         ++started;
     }
     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+    if (counts)
+        for (int t = 0; t < started; ++t)
+            for (int k = 0; k < 6; ++k) counts[k] += jobs[t].counts[k];
 }
 
 /* ---- display and debug views --------------------------------------------- */

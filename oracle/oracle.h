@@ -167,11 +167,13 @@ void or_sample_triangle(const float a[3], const float b[3], const float c[3], fl
 
 /* Path-trace pixels [pixel_begin, pixel_end) adding spp samples each to
  * accum (4 doubles per pixel, in sample order). vertex_hist (optional,
- * max_depth ints) counts paths reaching each vertex. */
+ * max_depth ints) counts paths reaching each vertex. counts (optional,
+ * int64[6], added to): extension rays, their interior-node visits and
+ * triangle tests in the canonical BVH, then the same for shadow rays. */
 void or_render(const void *bvh, const float *verts, const int32_t *tris, const int32_t *tri_mat,
                const or_material *mats, const or_alias_entry *alias, const float *pdf_area, int32_t n_emit,
                const or_camera *cam, const or_render_params *prm, int32_t pixel_begin, int32_t pixel_end,
-               double *accum, int32_t *vertex_hist, int32_t threads);
+               double *accum, int32_t *vertex_hist, int64_t *counts, int32_t threads);
 
 /* ---- the reference's integrator (reference semantics, or_refrender.c) ------ */
 /* An object: a contiguous range of triangles (the reference's TriangleMesh,

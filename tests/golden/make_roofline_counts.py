@@ -1,9 +1,17 @@
-"""Write tests/golden/roofline_counts.json: the per-ray node and triangle test
-counts of the oracle's canonical binary BVH (binned SAH, 32 bins, C_trav =
-C_isect = 1, <= 4 triangles per leaf; front-to-back, prune on
+"""Write tests/golden/roofline_counts.json: the per-ray interior-node visits
+and triangle tests of the oracle's canonical binary BVH (binned SAH, 32
+bins, C_trav = C_isect = 1, <= 4 triangles per leaf; front-to-back, prune on
 t_entry > t_best) on the traversal benchmark's ray batches. These counts fix
 the algorithmic bytes per ray (48 + 32*N_node + 40*N_tri closest-hit,
-36 + 32*N_node + 40*N_tri any-hit) independently of the GPU's BVH.
+36 + 32*N_node + 40*N_tri any-hit, N_node = interior-node visits)
+independently of the GPU's BVH. Box tests (two per visit, plus the root's)
+are recorded beside them.
+
+For the 2,048,000-triangle many-lights render (depth 12, roulette from
+vertex 3, seed 0), the build integrator's extension and shadow rays over the
+same kind of canonical BVH, sampled at 240 x 135 x 4 spp with the render's
+field of view: the algorithmic bytes of one path sample are the sum of its
+rays' bytes plus 64 B of FP64 accumulation read-modify-write.
 
 Also records the reference-semantics BBH counters (median split, one
 triangle per leaf, stackless walk) as context.
@@ -29,10 +37,33 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "roofline_counts.
 
 
 def stats(nt, tt, base):
-    n_node = float(np.mean(nt))
+    """nt: box tests per ray (1 for the root + 2 per interior-node visit)."""
+    visits = float(np.mean((np.asarray(nt, np.int64) - 1) // 2))
     n_tri = float(np.mean(tt))
-    return {"mean_node_tests": round(n_node, 4), "mean_tri_tests": round(n_tri, 4),
-            "bytes_per_ray": round(base + 32 * n_node + 40 * n_tri, 2)}
+    return {"mean_node_visits": round(visits, 4), "mean_box_tests": round(float(np.mean(nt)), 4),
+            "mean_tri_tests": round(n_tri, 4), "bytes_per_ray": round(base + 32 * visits + 40 * n_tri, 2)}
+
+
+def render_counts():
+    from devilray_amd import scenes
+    sc = scenes.many_lights()
+    bvh = oracle.BVH(sc.vertices, sc.indices, max_leaf=4, bins=32)
+    c = sc.camera
+    w, h, spp = 240, 135, 4
+    cam = oracle.Camera.from_buffer_copy(bytes(dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], w, h)))
+    _, _, cnt = oracle.render(bvh, sc.tri_material, oracle.materials_array(sc.materials), cam, spp=spp,
+                              max_depth=12, rr_depth=3, seed=0, counts=True)
+    ns = w * h * spp
+    ext, ev, et, sh, sv, st = (int(x) for x in cnt)
+    per = (48 * ext + 32 * ev + 40 * et + 36 * sh + 32 * sv + 40 * st) / ns + 64
+    return {"scene": "many_lights: 2,048,000 tris, 100k emitters; depth 12, roulette from 3, seed 0",
+            "sample": f"{w}x{h}x{spp} spp, the 1920x1080 view's field of view",
+            "extension_rays_per_sample": round(ext / ns, 4),
+            "extension_node_visits_per_ray": round(ev / ext, 4), "extension_tri_tests_per_ray": round(et / ext, 4),
+            "shadow_rays_per_sample": round(sh / ns, 4),
+            "shadow_node_visits_per_ray": round(sv / max(sh, 1), 4),
+            "shadow_tri_tests_per_ray": round(st / max(sh, 1), 4),
+            "bytes_per_sample": round(per, 2)}
 
 
 def main():
@@ -51,6 +82,7 @@ def main():
     out["c1_incoherent_any"] = stats(nt, tt, 36) | {"occluded_fraction": float(np.mean(o))}
     h, nt, tt = bvh.closest(cam, counts=True)
     out["c1_coherent_closest"] = stats(nt, tt, 48) | {"hit_fraction": float(np.mean(h["id"] >= 0))}
+    out["c2_render_path_sample"] = render_counts()
     sub = rays[:1 << 16].copy()
     ref = oracle.RefBBH(v, f)
     hr, bt, tr = ref.closest(sub, counts=True)
