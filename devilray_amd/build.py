@@ -73,7 +73,10 @@ BENCH_CLI = os.path.join(CPP, "devil_ray_benchmark")  # the reference's benchmar
 def build_host(verbose: bool = True) -> None:
     """The C++ facade (Renderer / Scene / cast over the C ABI) as
     libdevilray_cpp.so, and its test program; plain g++, no HIP headers."""
-    common = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}"]
+    # C++20: the facade keeps the reference's aggregates, designated
+    # initialisers of materials and std::span views
+    common = ["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-missing-field-initializers",
+              f"-I{os.path.join(ROOT, 'include')}"]
     link = [f"-L{PKG}", "-ldevilray", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
     steps = [
         common + ["-shared", "-o", HOST_LIB, os.path.join(CPP, "devilray.cpp")] + link,

@@ -107,8 +107,9 @@ int main(int argc, char **argv)
         Context ctx(0);
         const MeshBounds bounds = calculateMeshBounds(mesh);
         Scene scene;
-        scene.materials.push_back(Material::diffuse({0.5f, 0.5f, 0.5f}));
-        scene.objects.push_back({std::move(mesh), 0});
+        scene.materials.push_back(DiffuseMaterial{{}, Vec4{0, 0, 0, 0}, Vec4{0.5f, 0.5f, 0.5f, 0.0f}});
+        scene.mesh_storage.push_back(convertMeshToTris(mesh, false));
+        scene.objects.push_back(viewGpuTris(scene.mesh_storage.back()));
         scene.build(ctx);
 
         SplitMix rng{0};

@@ -7,9 +7,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <numeric>
 #include <sstream>
-#include <fstream>
 
 namespace devilray {
 
@@ -18,46 +18,46 @@ void check(dr_status s)
     if (s != DR_OK) throw Error(s, std::string(dr_status_string(s)) + ": " + dr_last_error());
 }
 
-Material Material::diffuse(Vec3 reflectance, Vec3 emission)
+// ---- materials ----------------------------------------------------------------
+Vec4 getDebugColor(const Material &mat)
 {
-    Material r;
-    r.m.kind = DR_MAT_LAMBERT;
-    r.m.albedo[0] = reflectance.x;
-    r.m.albedo[1] = reflectance.y;
-    r.m.albedo[2] = reflectance.z;
-    r.m.emission[0] = emission.x;
-    r.m.emission[1] = emission.y;
-    r.m.emission[2] = emission.z;
-    r.m.ior = 1.0f;
-    return r;
-}
-
-Material Material::transparent(float ior)
-{
-    Material r;
-    r.m.kind = DR_MAT_DIELECTRIC;
-    r.m.albedo[0] = r.m.albedo[1] = r.m.albedo[2] = 1.0f;
-    r.m.ior = ior;
-    return r;
-}
-
-Mesh createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size)
-{
-    // up = right x normal; corners (-,-), (+,-), (+,+), (-,+); triangles (0,2,1), (0,3,2)
-    const Vec3 up{right.y * normal.z - right.z * normal.y, right.z * normal.x - right.x * normal.z,
-                  right.x * normal.y - right.y * normal.x};
-    const float h = size / 2.0f;
-    auto corner = [&](float a, float b) {
-        return Vec3{center.x + right.x * a + up.x * b, center.y + right.y * a + up.y * b,
-                    center.z + right.z * a + up.z * b};
-    };
-    Mesh m;
-    m.points = {corner(-h, -h), corner(h, -h), corner(h, h), corner(-h, h)};
-    m.triangles = {{0, 2, 1}, {0, 3, 2}};
-    return m;
+    return std::visit([](const auto &m) { return m.debug_color; }, mat);
 }
 
 namespace {
+
+dr_material to_dr(const Material &mat)
+{
+    dr_material d{};
+    if (const auto *diffuse = std::get_if<DiffuseMaterial>(&mat)) {
+        d.kind = DR_MAT_LAMBERT;
+        d.albedo[0] = diffuse->diffuse_reflectance.x;
+        d.albedo[1] = diffuse->diffuse_reflectance.y;
+        d.albedo[2] = diffuse->diffuse_reflectance.z;
+        d.emission[0] = diffuse->emission.x;
+        d.emission[1] = diffuse->emission.y;
+        d.emission[2] = diffuse->emission.z;
+        d.ior = 1.0f;
+    } else {
+        d.kind = DR_MAT_DIELECTRIC;
+        d.albedo[0] = d.albedo[1] = d.albedo[2] = 1.0f;
+        d.ior = std::get<TransparentMaterial>(mat).inside_medium.ior;
+    }
+    return d;
+}
+
+// luminance(radiantExitance(material)): pi E for a diffuse material, 0 for a
+// transparent one (include/tracing/LightSampling.hpp:7-13, Utils.hpp:226-229)
+float exitance_luminance(const Material &mat)
+{
+    const auto *diffuse = std::get_if<DiffuseMaterial>(&mat);
+    if (!diffuse) return 0.0f;
+    const float pi = 3.14159265358979f;
+    return 0.2126f * diffuse->emission.x * pi + 0.7152f * diffuse->emission.y * pi +
+           0.0722f * diffuse->emission.z * pi;
+}
+
+float triangle_area(const Vec3 &A, const Vec3 &B, const Vec3 &C) { return (A - B).cross(A - C).length() / 2; }
 
 void trim_right(std::string &line)
 {
@@ -101,10 +101,10 @@ Vec3 read_vec3(std::istringstream &ss)
 
 }  // namespace
 
-Mesh loadMesh(const std::string &path)
+Mesh loadMesh(const std::string &fileName)
 {
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("File " + path + " does not exist");
+    std::ifstream in(fileName);
+    if (!in) throw std::runtime_error("File " + fileName + " does not exist");
     Mesh mesh;
     mesh.name = "Mesh";
     std::vector<std::array<std::array<int, 2>, 3>> faces;
@@ -134,7 +134,7 @@ Mesh loadMesh(const std::string &path)
     if (mesh.normals.empty()) mesh.normals.push_back(Vec3{0.0f, 0.0f, 0.0f});
     const int np = (int)mesh.points.size(), nn = (int)mesh.normals.size();
     for (size_t i = 0; i < faces.size(); ++i) {
-        std::array<uint32_t, 3> t{}, tn{};
+        Vertex v[3];
         for (int k = 0; k < 3; ++k) {
             int pi = faces[i][k][0], ni = faces[i][k][1];
             pi = pi < 0 ? np + pi : pi - 1;  // 1-based, or relative to the end
@@ -145,11 +145,9 @@ Mesh loadMesh(const std::string &path)
             if (ni < 0 || ni >= nn)
                 throw std::runtime_error("Invalid normal index on triangle " + std::to_string(i) + ": " +
                                          std::to_string(ni) + ". Number of normals is " + std::to_string(nn) + ".");
-            t[k] = (uint32_t)pi;
-            tn[k] = (uint32_t)ni;
+            v[k] = Vertex{(uint32_t)pi, (uint32_t)ni};
         }
-        mesh.triangles.push_back(t);
-        mesh.normal_indices.push_back(tn);
+        mesh.triangles.push_back(Triangle{v[0], v[1], v[2]});
     }
     return mesh;
 }
@@ -157,38 +155,84 @@ Mesh loadMesh(const std::string &path)
 MeshBounds calculateMeshBounds(const Mesh &mesh)
 {
     Vec3 sum{0.0f, 0.0f, 0.0f};
-    for (const Vec3 &p : mesh.points) { sum.x += p.x; sum.y += p.y; sum.z += p.z; }
-    const float n = (float)mesh.points.size();
-    const Vec3 c{sum.x / n, sum.y / n, sum.z / n};
+    for (const Vec3 &p : mesh.points) sum += p;
+    const Vec3 c = sum / (float)mesh.points.size();
     float extent = 0.0f;
-    for (const Vec3 &p : mesh.points) {
-        const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
-        extent = std::max(extent, std::sqrt(dx * dx + dy * dy + dz * dz));
-    }
+    for (const Vec3 &p : mesh.points) extent = std::max(extent, (p - c).length());
     return MeshBounds{c, extent};
 }
 
 void normalizeMeshSize(Mesh &mesh)
 {
     const MeshBounds b = calculateMeshBounds(mesh);
-    for (Vec3 &p : mesh.points)
-        p = Vec3{(p.x - b.center.x) / b.extent, (p.y - b.center.y) / b.extent, (p.z - b.center.z) / b.extent};
+    for (Vec3 &p : mesh.points) p = (p - b.center) / b.extent;
 }
 
 void generateCoarseNormals(Mesh &mesh)
 {
     mesh.normals.clear();
-    mesh.normal_indices.clear();
-    for (const auto &t : mesh.triangles) {
-        const Vec3 A = mesh.points[t[0]], B = mesh.points[t[1]], C = mesh.points[t[2]];
-        const Vec3 e1{A.x - B.x, A.y - B.y, A.z - B.z}, e2{A.x - C.x, A.y - C.y, A.z - C.z};
-        Vec3 n{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
-        const float l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
-        n = Vec3{n.x / l, n.y / l, n.z / l};
+    for (Triangle &t : mesh.triangles) {
+        const Vec3 A = mesh.points[t.a.pi], B = mesh.points[t.b.pi], C = mesh.points[t.c.pi];
         const uint32_t k = (uint32_t)mesh.normals.size();
-        mesh.normals.push_back(n);
-        mesh.normal_indices.push_back({k, k, k});
+        mesh.normals.push_back((A - B).cross(A - C).normalized());
+        t.a.ni = t.b.ni = t.c.ni = k;
     }
+}
+
+void RunningAverage::add(double value)
+{
+    m_samples.push_back(value);
+    while ((int)m_samples.size() > m_sampleCount) m_samples.pop_front();
+}
+
+double RunningAverage::mean() const
+{
+    if (m_samples.empty()) return 0.0;
+    return std::accumulate(m_samples.begin(), m_samples.end(), 0.0) / (double)m_samples.size();
+}
+
+// ---- scene storage and views ------------------------------------------------------
+GpuTris convertMeshToTris(Mesh &mesh, bool)
+{
+    return GpuTris{mesh.points, mesh.normals, mesh.triangles, mesh.name};
+}
+
+GpuTris createQuadMesh(Vec3 center, Vec3 normal, Vec3 right, float size)
+{
+    const Vec3 up = right.cross(normal);
+    const float half = size / 2.0f;
+    Mesh mesh;
+    mesh.name = "quad";
+    mesh.points = {center + right * (-half) + up * (-half), center + right * half + up * (-half),
+                   center + right * half + up * half, center + right * (-half) + up * half};
+    mesh.normals = {normal};
+    mesh.triangles = {Triangle{{0, 0}, {2, 0}, {1, 0}}, Triangle{{0, 0}, {3, 0}, {2, 0}}};
+    return convertMeshToTris(mesh);
+}
+
+TriangleMesh viewGpuTris(GpuTris &tris)
+{
+    TriangleMesh obj;
+    obj.points = tris.points.data();
+    obj.normals = tris.normals.data();
+    obj.triangles = tris.triangles.data();
+    obj.triangle_count = (int)tris.triangles.size();
+    obj.point_count = (int)tris.points.size();
+    double total = 0;  // FP64 sum of the triangle areas (src/GpuTris.cpp:39-56)
+    for (const Triangle &t : tris.triangles)
+        total += triangle_area(tris.points[t.a.pi], tris.points[t.b.pi], tris.points[t.c.pi]);
+    obj.base_surface_area = (float)total;
+    obj.surface_area = obj.base_surface_area;
+    return obj;
+}
+
+void TriangleMesh::setPosition(const Vec3 &pos) { model_to_world.p = pos; }
+
+void TriangleMesh::setScale(const Vec3 &scale)
+{
+    model_to_world.s = scale;
+    const float vol = scale.x * scale.y * scale.z;
+    surface_area = base_surface_area * std::cbrt(vol * vol);
 }
 
 Context::Context(int device) { check(dr_context_create(device, &ctx_)); }
@@ -197,10 +241,196 @@ Context::~Context()
     if (ctx_) dr_context_destroy(ctx_);
 }
 
+Context &defaultContext()
+{
+    // created on first use and kept until the process ends, so that no
+    // renderer or scene can outlive it in static destruction
+    static Context *ctx = new Context(0);
+    return *ctx;
+}
+
+namespace {
+
+// number of points a view's triangles refer to
+int view_points(const TriangleMesh &o)
+{
+    if (o.point_count > 0) return o.point_count;
+    uint32_t m = 0;
+    for (int i = 0; i < o.triangle_count; ++i)
+        m = std::max({m, o.triangles[i].a.pi + 1, o.triangles[i].b.pi + 1, o.triangles[i].c.pi + 1});
+    return (int)m;
+}
+
+// The placed objects flattened for the device: world points (p * s +
+// position), point indices rebased per object, and each triangle's material.
+struct Flat {
+    std::vector<float> verts;
+    std::vector<uint32_t> idx, tri_mat;
+    std::vector<uint32_t> first_tri;  // per object
+};
+
+Flat flatten(std::span<const TriangleMesh> objects, size_t n_materials)
+{
+    Flat f;
+    uint32_t base = 0;
+    for (const TriangleMesh &o : objects) {
+        if (o.material < 0 || (size_t)o.material >= std::max<size_t>(n_materials, 1))
+            throw Error(DR_E_INVALID, "object material index out of range");
+        const int np = view_points(o);
+        f.first_tri.push_back((uint32_t)f.tri_mat.size());
+        for (int k = 0; k < np; ++k) {
+            const Vec3 w = o.model_to_world.applyToPoint(o.points[k]);
+            f.verts.insert(f.verts.end(), {w.x, w.y, w.z});
+        }
+        for (int i = 0; i < o.triangle_count; ++i) {
+            const Triangle &t = o.triangles[i];
+            for (uint32_t k : {t.a.pi, t.b.pi, t.c.pi}) {
+                if (k >= (uint32_t)np) throw Error(DR_E_INVALID, "triangle vertex index out of range");
+                f.idx.push_back(base + k);
+            }
+            f.tri_mat.push_back((uint32_t)o.material);
+        }
+        base += (uint32_t)np;
+    }
+    return f;
+}
+
+dr_scene *create_device_scene(Context &ctx, const Flat &f, const std::vector<dr_material> &mats, uint32_t leaf_max)
+{
+    if (f.tri_mat.empty()) return nullptr;
+    dr_scene_desc d{};
+    d.vertices = f.verts.data();
+    d.n_vertices = (uint32_t)(f.verts.size() / 3);
+    d.indices = f.idx.data();
+    d.n_triangles = (uint32_t)f.tri_mat.size();
+    d.tri_material = f.tri_mat.data();
+    d.materials = mats.data();
+    d.n_materials = (uint32_t)mats.size();
+    dr_scene *s = nullptr;
+    check(dr_scene_create(ctx.handle(), &d, &s));
+    const dr_bvh_params bp{leaf_max, 0};
+    const dr_status st = dr_scene_build_bvh(s, &bp, nullptr);
+    if (st != DR_OK) {
+        dr_scene_destroy(s);
+        check(st);
+    }
+    return s;
+}
+
+struct DeviceBuffer {
+    Context &ctx;
+    void *ptr = nullptr;
+    DeviceBuffer(Context &c, size_t bytes) : ctx(c) { check(dr_device_alloc(c.handle(), bytes ? bytes : 16, &ptr)); }
+    ~DeviceBuffer() { dr_device_free(ctx.handle(), ptr); }
+};
+
+struct SceneHandle {
+    dr_scene *s = nullptr;
+    ~SceneHandle()
+    {
+        if (s) dr_scene_destroy(s);
+    }
+};
+
+// Closest hits of a ray batch on a device scene (counters when given).
+std::vector<dr_hit> trace(Context &ctx, dr_scene *s, const std::vector<Ray> &rays, std::vector<uint32_t> *box = nullptr,
+                          std::vector<uint32_t> *tri = nullptr)
+{
+    const size_t n = rays.size();
+    std::vector<dr_hit> hits(n, dr_hit{-1, INFINITY, 0.0f, 0.0f});
+    if (!s || n == 0) return hits;
+    std::vector<dr_ray> in(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Ray &r = rays[i];
+        in[i] = dr_ray{r.p.x, r.p.y, r.p.z, r.tmin, r.v.x, r.v.y, r.v.z, r.tmax};
+    }
+    DeviceBuffer dr(ctx, sizeof(dr_ray) * n), dh(ctx, sizeof(dr_hit) * n);
+    check(dr_memcpy_htod(ctx.handle(), dr.ptr, in.data(), sizeof(dr_ray) * n));
+    if (box && tri) {
+        DeviceBuffer db(ctx, 4 * n), dt(ctx, 4 * n);
+        check(dr_trace_closest_counted(s, (const dr_ray *)dr.ptr, (uint32_t)n, (dr_hit *)dh.ptr, (uint32_t *)db.ptr,
+                                       (uint32_t *)dt.ptr, nullptr));
+        box->resize(n);
+        tri->resize(n);
+        check(dr_memcpy_dtoh(ctx.handle(), box->data(), db.ptr, 4 * n));
+        check(dr_memcpy_dtoh(ctx.handle(), tri->data(), dt.ptr, 4 * n));
+    } else {
+        check(dr_trace_closest(s, (const dr_ray *)dr.ptr, (uint32_t)n, (dr_hit *)dh.ptr, nullptr));
+    }
+    check(dr_memcpy_dtoh(ctx.handle(), hits.data(), dh.ptr, sizeof(dr_hit) * n));
+    return hits;
+}
+
+// The reference's Intersection of a device hit on the flattened triangles:
+// hit point from the weights, unit face normal turned against the ray,
+// world area, winding flag, material.
+Intersection describe(const dr_hit &h, const Ray &ray, const std::vector<float> &w, const std::vector<uint32_t> &ix,
+                      const std::vector<uint32_t> &tri_mat)
+{
+    const size_t tri = (size_t)h.id;
+    const Vec3 A{w[3 * ix[3 * tri]], w[3 * ix[3 * tri] + 1], w[3 * ix[3 * tri] + 2]};
+    const Vec3 B{w[3 * ix[3 * tri + 1]], w[3 * ix[3 * tri + 1] + 1], w[3 * ix[3 * tri + 1] + 2]};
+    const Vec3 C{w[3 * ix[3 * tri + 2]], w[3 * ix[3 * tri + 2] + 1], w[3 * ix[3 * tri + 2] + 2]};
+    const Vec3 nf = (A - B).cross(A - C);
+    const float len = nf.length();
+    const bool ccw = nf.dot(ray.v) > 0.0f;
+    Intersection x{};
+    x.t = h.t;
+    x.p = A + (B - A) * h.u + (C - A) * h.v;
+    x.uv = Vec2f{0.0f, 0.0f};
+    x.n = nf * ((ccw ? -1.0f : 1.0f) / (len > 0.0f ? len : 1.0f));
+    x.mat = (int)tri_mat[tri];
+    x.object = nullptr;
+    x.triangle = TriangleHitData{Vec3{1.0f - h.u - h.v, h.u, h.v}, 0.5f * len, ccw};
+    x.triangle_id = h.id;
+    return x;
+}
+
+std::vector<dr_material> device_materials(size_t n_needed)
+{
+    // objects traced without a scene's material list: one grey diffuse
+    // material stands for every index the objects use
+    dr_material m{};
+    m.kind = DR_MAT_LAMBERT;
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.5f;
+    m.ior = 1.0f;
+    return std::vector<dr_material>(std::max<size_t>(n_needed, 1), m);
+}
+
+// One ray against a set of objects on the device: the hit, which object it
+// belongs to, and (optionally) the traversal counters.
+std::optional<Intersection> trace_objects(const Ray &ray, std::span<const TriangleMesh> objects,
+                                          benchmark::HitTests *counts)
+{
+    if (objects.empty()) return std::nullopt;
+    int max_mat = 0;
+    for (const TriangleMesh &o : objects) max_mat = std::max(max_mat, o.material);
+    const Flat f = flatten(objects, (size_t)max_mat + 1);
+    Context &ctx = defaultContext();
+    SceneHandle sh;
+    sh.s = create_device_scene(ctx, f, device_materials((size_t)max_mat + 1), 4);
+    std::vector<uint32_t> box, tri;
+    const std::vector<dr_hit> h = trace(ctx, sh.s, {ray}, counts ? &box : nullptr, counts ? &tri : nullptr);
+    if (counts && !box.empty()) {
+        counts->bbox_tests += box[0];
+        counts->triangle_tests += tri[0];
+    }
+    if (h[0].id < 0) return std::nullopt;
+    Intersection x = describe(h[0], ray, f.verts, f.idx, f.tri_mat);
+    const size_t k = (size_t)(std::upper_bound(f.first_tri.begin(), f.first_tri.end(), (uint32_t)h[0].id) -
+                              f.first_tri.begin()) - 1;
+    x.object = &objects[k];
+    x.triangle_id = h[0].id - (int32_t)f.first_tri[k];
+    return x;
+}
+
+}  // namespace
+
+// ---- the scene on the device ----------------------------------------------------------
 Scene::Scene(Scene &&o) noexcept
-    : materials(std::move(o.materials)), objects(std::move(o.objects)), scene_(o.scene_), n_tris_(o.n_tris_),
-      n_points_(o.n_points_), built_(o.built_), world_(std::move(o.world_)), idx_(std::move(o.idx_)),
-      tri_mat_(std::move(o.tri_mat_))
+    : info(o.info), objects(std::move(o.objects)), materials(std::move(o.materials)),
+      mesh_storage(std::move(o.mesh_storage)), scene_(o.scene_), n_tris_(o.n_tris_), n_points_(o.n_points_),
+      built_(o.built_), world_(std::move(o.world_)), idx_(std::move(o.idx_)), tri_mat_(std::move(o.tri_mat_))
 {
     o.scene_ = nullptr;
 }
@@ -209,8 +439,10 @@ Scene &Scene::operator=(Scene &&o) noexcept
 {
     if (this != &o) {
         if (scene_) dr_scene_destroy(scene_);
-        materials = std::move(o.materials);
+        info = o.info;
         objects = std::move(o.objects);
+        materials = std::move(o.materials);
+        mesh_storage = std::move(o.mesh_storage);
         scene_ = o.scene_;
         n_tris_ = o.n_tris_;
         n_points_ = o.n_points_;
@@ -228,147 +460,105 @@ Scene::~Scene()
     if (scene_) dr_scene_destroy(scene_);
 }
 
-// every object's points placed by its transform, p * s + position, in object order
-std::vector<float> Scene::worldPoints() const
-{
-    std::vector<float> verts;
-    for (const Object &o : objects) {
-        const Transform &tf = o.transform;
-        for (const Vec3 &p : o.mesh.points) {
-            const Vec3 w = tf.applyToPoint(p);
-            verts.push_back(w.x);
-            verts.push_back(w.y);
-            verts.push_back(w.z);
-        }
-    }
-    return verts;
-}
-
 void Scene::refit(Context &ctx)
 {
     if (!scene_) {
         build(ctx);
         return;
     }
-    const std::vector<float> verts = worldPoints();
-    uint32_t tris = 0;
-    for (const Object &o : objects) tris += (uint32_t)o.mesh.triangles.size();
-    if (verts.size() != 3ull * n_points_ || tris != n_tris_)
+    Flat f = flatten(objects, materials.size());
+    if (f.verts.size() != 3ull * n_points_ || f.tri_mat.size() != n_tris_)
         throw Error(DR_E_INVALID, "refit needs the objects and triangles of the last build");
-    check(dr_scene_refit(scene_, verts.data(), nullptr));
-    world_ = verts;
+    check(dr_scene_refit(scene_, f.verts.data(), nullptr));
+    world_ = std::move(f.verts);
 }
 
 void Scene::build(Context &ctx, uint32_t leaf_max)
 {
-    const std::vector<float> verts = worldPoints();
-    std::vector<uint32_t> idx, tri_mat;
-    uint32_t base = 0;
-    for (const Object &o : objects) {
-        if (o.material >= materials.size()) throw Error(DR_E_INVALID, "object material index out of range");
-        for (const auto &t : o.mesh.triangles) {
-            for (uint32_t k : t) {
-                if (k >= o.mesh.points.size()) throw Error(DR_E_INVALID, "triangle vertex index out of range");
-                idx.push_back(base + k);
-            }
-            tri_mat.push_back(o.material);
-        }
-        base += (uint32_t)o.mesh.points.size();
-    }
+    Flat f = flatten(objects, materials.size());
     std::vector<dr_material> mats;
-    for (const Material &m : materials) mats.push_back(m.m);
-    dr_scene_desc d{};
-    d.vertices = verts.data();  // (the scene copies them in)
-    d.n_vertices = (uint32_t)(verts.size() / 3);
-    d.indices = idx.data();
-    d.n_triangles = (uint32_t)tri_mat.size();
-    d.tri_material = tri_mat.data();
-    d.materials = mats.data();
-    d.n_materials = (uint32_t)mats.size();
+    for (const Material &m : materials) mats.push_back(to_dr(m));
     if (scene_) {
         dr_scene_destroy(scene_);
         scene_ = nullptr;
     }
-    n_tris_ = d.n_triangles;
-    n_points_ = d.n_vertices;
-    world_ = verts;
+    // calculateLightWeights (src/Renderer.cpp:113-152): object weight =
+    // surface area x luminance of the radiant exitance
+    double power = 0.0;
+    for (const TriangleMesh &o : objects) power += (double)o.surface_area * exitance_luminance(materials[o.material]);
+    info.total_radiant_power = (float)power;
+    n_tris_ = (uint32_t)f.tri_mat.size();
+    n_points_ = (uint32_t)(f.verts.size() / 3);
     built_ = true;
-    if (d.n_triangles == 0) return;  // nothing to trace: cast misses (CastTest.EmptySceneMisses)
-    check(dr_scene_create(ctx.handle(), &d, &scene_));
-    dr_bvh_params bp{leaf_max, 0};
-    check(dr_scene_build_bvh(scene_, &bp, nullptr));
-    bool any_debug = false;
-    for (const Material &m : materials) any_debug = any_debug || m.debug_color.has_value();
-    if (any_debug) {
+    scene_ = create_device_scene(ctx, f, mats, leaf_max);  // none without triangles: cast misses
+    world_ = std::move(f.verts);
+    idx_ = std::move(f.idx);
+    tri_mat_ = std::move(f.tri_mat);
+    if (scene_ && !materials.empty()) {
         std::vector<float> rgba;
         for (const Material &m : materials) {
-            const Vec4 c = m.debug_color.value_or(Vec4{m.m.albedo[0], m.m.albedo[1], m.m.albedo[2], 0.0f});
+            const Vec4 c = getDebugColor(m);
             rgba.insert(rgba.end(), {c.x, c.y, c.z, c.w});
         }
         check(dr_scene_set_debug_colors(scene_, rgba.data(), (uint32_t)materials.size()));
     }
-    idx_ = std::move(idx);
-    tri_mat_ = std::move(tri_mat);
 }
 
-namespace {
-struct DeviceBuffer {
-    Context &ctx;
-    void *ptr = nullptr;
-    DeviceBuffer(Context &c, size_t bytes) : ctx(c) { check(dr_device_alloc(c.handle(), bytes ? bytes : 16, &ptr)); }
-    ~DeviceBuffer() { dr_device_free(ctx.handle(), ptr); }
-};
-}  // namespace
+// ---- tracing -------------------------------------------------------------------------------
+std::optional<TriangleIntersection> testTriangleIntersection(const Ray &ray, const TriangleVertices &triangle)
+{
+    GpuTris tris;
+    tris.points = {triangle.a, triangle.b, triangle.c};
+    tris.triangles = {Triangle{{0, 0}, {1, 0}, {2, 0}}};
+    const TriangleMesh view = viewGpuTris(tris);
+    const auto hit = trace_objects(ray, std::span<const TriangleMesh>(&view, 1), nullptr);
+    if (!hit) return std::nullopt;
+    return TriangleIntersection{hit->t, hit->triangle.bari};
+}
+
+std::optional<Intersection> getIntersection(const Ray &ray, const TriangleMesh &tris)
+{
+    return trace_objects(ray, std::span<const TriangleMesh>(&tris, 1), nullptr);
+}
+
+std::optional<Intersection> getIntersectionBenchmark(const Ray &ray, const TriangleMesh &tris,
+                                                     benchmark::HitTests &benchmark)
+{
+    return trace_objects(ray, std::span<const TriangleMesh>(&tris, 1), &benchmark);
+}
+
+std::optional<Intersection> cast(const Ray &ray, std::span<const TriangleMesh> objects, const ObjectsInfo &)
+{
+    return trace_objects(ray, objects, nullptr);
+}
 
 std::vector<std::optional<Intersection>> cast(Context &ctx, const Scene &scene, const std::vector<Ray> &rays)
 {
     if (!scene.built()) throw Error(DR_E_INVALID, "scene not built");
     const size_t n = rays.size();
-    if (!scene.handle() || n == 0) return std::vector<std::optional<Intersection>>(n);  // no triangles
-    std::vector<dr_ray> in(n);
-    for (size_t i = 0; i < n; ++i) {
-        const Ray &r = rays[i];
-        in[i] = dr_ray{r.p.x, r.p.y, r.p.z, r.tmin, r.v.x, r.v.y, r.v.z, r.tmax};
-    }
-    DeviceBuffer dr(ctx, sizeof(dr_ray) * n), dh(ctx, sizeof(dr_hit) * n);
-    check(dr_memcpy_htod(ctx.handle(), dr.ptr, in.data(), sizeof(dr_ray) * n));
-    check(dr_trace_closest(scene.handle(), (const dr_ray *)dr.ptr, (uint32_t)n, (dr_hit *)dh.ptr, nullptr));
-    std::vector<dr_hit> hits(n);
-    check(dr_memcpy_dtoh(ctx.handle(), hits.data(), dh.ptr, sizeof(dr_hit) * n));
     std::vector<std::optional<Intersection>> out(n);
-    const std::vector<float> &w = scene.worldVertices();
-    const std::vector<uint32_t> &ix = scene.worldIndices();
-    for (size_t i = 0; i < n; ++i) {
-        if (hits[i].id < 0) continue;
-        const dr_hit &h = hits[i];
-        Intersection x{h.t, Vec3{1.0f - h.u - h.v, h.u, h.v}, h.id};
-        const size_t tri = (size_t)h.id;
-        const float *A = &w[3 * (size_t)ix[3 * tri]], *B = &w[3 * (size_t)ix[3 * tri + 1]],
-                    *C = &w[3 * (size_t)ix[3 * tri + 2]];
-        const float ab[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]}, ac[3] = {A[0] - C[0], A[1] - C[1], A[2] - C[2]};
-        const float nf[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
-        const float len = std::sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
-        const Vec3 &v = rays[i].v;
-        const float side = nf[0] * v.x + nf[1] * v.y + nf[2] * v.z;
-        x.ccw = side > 0.0f;
-        const float s = (side > 0.0f ? -1.0f : 1.0f) / (len > 0.0f ? len : 1.0f);
-        x.n = Vec3{nf[0] * s, nf[1] * s, nf[2] * s};
-        x.area = 0.5f * len;
-        x.p = Vec3{A[0] + (B[0] - A[0]) * h.u + (C[0] - A[0]) * h.v, A[1] + (B[1] - A[1]) * h.u + (C[1] - A[1]) * h.v,
-                   A[2] + (B[2] - A[2]) * h.u + (C[2] - A[2]) * h.v};
-        x.mat = scene.triangleMaterials()[tri];
-        out[i] = x;
-    }
+    if (!scene.handle() || n == 0) return out;  // no triangles
+    const std::vector<dr_hit> hits = trace(ctx, scene.handle(), rays);
+    for (size_t i = 0; i < n; ++i)
+        if (hits[i].id >= 0)
+            out[i] = describe(hits[i], rays[i], scene.worldVertices(), scene.worldIndices(), scene.triangleMaterials());
     return out;
 }
+
+// ---- the renderer --------------------------------------------------------------------------
+Renderer::Renderer(Size2i resolution) : Renderer(defaultContext(), resolution) {}
 
 Renderer::Renderer(Context &ctx, Size2i resolution) : ctx_(ctx), res_(resolution)
 {
     if (resolution.width <= 0 || resolution.height <= 0) throw Error(DR_E_INVALID, "empty resolution");
     const size_t npix = (size_t)resolution.width * resolution.height;
     check(dr_device_alloc(ctx.handle(), npix * 4 * sizeof(double), &accum_));
+    check(dr_fill_zero(ctx.handle(), accum_, npix * 4 * sizeof(double), nullptr));
     pixels_.assign(npix, 0u);
     display_.assign(npix, 0u);
+    buffers_.resolution = resolution;
+    buffers_.color = static_cast<const double *>(accum_);
+    buffers_.casts = &total_casts_;
 }
 
 Renderer::~Renderer()
@@ -377,26 +567,26 @@ Renderer::~Renderer()
     if (rgba_) dr_device_free(ctx_.handle(), rgba_);
 }
 
-void RunningAverage::add(double value)
-{
-    values_.push_back(value);
-    while (values_.size() > window_) values_.pop_front();
-}
-
-double RunningAverage::mean() const
-{
-    if (values_.empty()) return 0.0;
-    return std::accumulate(values_.begin(), values_.end(), 0.0) / (double)values_.size();
-}
-
 void Renderer::setScene(Scene scene)
 {
     std::scoped_lock frame{frame_mutex_};
     std::scoped_lock guard{render_mutex_};
     scene_ = std::move(scene);
     if (!scene_.built()) scene_.build(ctx_);
-    if (scene_.handle()) check(dr_scene_build_lights(scene_.handle(), nullptr));
+    calculateLightWeightsLocked();
     needs_clear_ = true;
+}
+
+void Renderer::calculateLightWeights()
+{
+    std::scoped_lock frame{frame_mutex_};
+    std::scoped_lock guard{render_mutex_};
+    calculateLightWeightsLocked();
+}
+
+void Renderer::calculateLightWeightsLocked()
+{
+    if (scene_.handle()) check(dr_scene_build_lights(scene_.handle(), nullptr));
 }
 
 void Renderer::setCamera(Camera cam)
@@ -483,8 +673,8 @@ void Renderer::render(uint32_t spp)
         c.focal_length = cam_.intrinsics.focal_length;
         c.center[0] = cam_.intrinsics.center.x;
         c.center[1] = cam_.intrinsics.center.y;
-        c.pixel_size[0] = cam_.physical_pixel_size.x;
-        c.pixel_size[1] = cam_.physical_pixel_size.y;
+        c.pixel_size[0] = cam_.physical_pixel_size.width;
+        c.pixel_size[1] = cam_.physical_pixel_size.height;
         c.width = (uint32_t)res_.width;
         c.height = (uint32_t)res_.height;
         c.pixel_sampling = (uint32_t)sampling_;
@@ -597,7 +787,7 @@ void png_chunk(std::vector<uint8_t> &o, const char *type, const std::vector<uint
 
 }  // namespace
 
-bool Renderer::saveImage(const std::string &path)
+bool Renderer::saveImage(const std::filesystem::path &path)
 {
     // PNG with the image data in stored (uncompressed) deflate blocks: a
     // valid zlib stream that needs no compressor
@@ -635,10 +825,10 @@ bool Renderer::saveImage(const std::string &path)
     png_chunk(out, "IHDR", ihdr);
     png_chunk(out, "IDAT", z);
     png_chunk(out, "IEND", {});
-    FILE *f = std::fopen(path.c_str(), "wb");
+    FILE *f = std::fopen(path.string().c_str(), "wb");
     const bool ok = f && std::fwrite(out.data(), 1, out.size(), f) == out.size();
     if (f) std::fclose(f);
-    if (!ok) std::fprintf(stderr, "WARN: Failed to save %s\n", path.c_str());
+    if (!ok) std::fprintf(stderr, "WARN: Failed to save %s\n", path.string().c_str());
     return ok;
 }
 
@@ -677,44 +867,6 @@ std::exception_ptr RenderLoop::error() const
 {
     std::scoped_lock guard{error_mutex_};
     return error_;
-}
-
-Matrix4x4f Matrix4x4f::rotation(const Vec3 &a, float angle)
-{
-    const float c = std::cos(angle), sn = std::sin(angle), k = 1.0f - c;
-    const float ax[3] = {a.x, a.y, a.z};
-    // [axis]_x: the cross-product matrix, [axis]_x v = axis x v
-    const float cross[3][3] = {{0.0f, -a.z, a.y}, {a.z, 0.0f, -a.x}, {-a.y, a.x, 0.0f}};
-    Matrix4x4f m;
-    for (int r = 0; r < 3; ++r)
-        for (int col = 0; col < 3; ++col)
-            m.values[r][col] = (r == col ? c : 0.0f) + sn * cross[r][col] + k * ax[r] * ax[col];
-    return m;
-}
-
-Matrix4x4f Matrix4x4f::operator*(const Matrix4x4f &o) const
-{
-    Matrix4x4f m;
-    for (int r = 0; r < 4; ++r)
-        for (int col = 0; col < 4; ++col) {
-            float acc = 0.0f;
-            for (int k = 0; k < 4; ++k) acc += values[r][k] * o.values[k][col];
-            m.values[r][col] = acc;
-        }
-    return m;
-}
-
-Vec3 Matrix4x4f::applyToPosition(const Vec3 &q) const
-{
-    const Vec3 d = applyToDirection(q);
-    return {d.x + values[0][3], d.y + values[1][3], d.z + values[2][3]};
-}
-
-Vec3 Matrix4x4f::applyToDirection(const Vec3 &d) const
-{
-    return {values[0][0] * d.x + values[0][1] * d.y + values[0][2] * d.z,
-            values[1][0] * d.x + values[1][1] * d.y + values[1][2] * d.z,
-            values[2][0] * d.x + values[2][1] * d.y + values[2][2] * d.z};
 }
 
 }  // namespace devilray

@@ -24,7 +24,7 @@ def test_facade_reference_cases(tmp_path):
     res = subprocess.run([BIN, str(tmp_path), models], capture_output=True, text=True, timeout=240)
     print(res.stdout)
     lines = [ln for ln in res.stdout.splitlines() if ln.startswith(("PASS", "FAIL"))]
-    assert len(lines) >= 63
+    assert len(lines) >= 76
     assert res.returncode == 0, res.stdout + res.stderr
     assert all(ln.startswith("PASS") for ln in lines)
     # Renderer::saveImage: a PNG whose pixels are the display pixels
