@@ -13,7 +13,7 @@ import numpy as np
 from ._lib import (  # noqa: F401
     ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
     DR_GUIDE_COLLECT, DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
-    GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
+    GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, SCHED_PER_RAY, SCHED_PER_RAY_SPECULATIVE, SCHED_PERSISTENT,
     SCHED_PERSISTENT_SPECULATIVE, SCHED_REORDER, SCHED_XCD_SLICES, check, lib, ptr,
 )
@@ -58,8 +58,8 @@ def camera_reference(width: int, height: int, focal_length: float, pixel_size: f
     return cam
 
 __all__ = [
-    "Context", "Scene", "DeviceArray", "gen_icosphere", "gen_rays_surface", "gen_rays_camera",
-    "RAY_DTYPE", "HIT_DTYPE", "DevilRayError",
+    "Context", "Scene", "Instances", "DeviceArray", "gen_icosphere", "gen_rays_surface", "gen_rays_camera",
+    "RAY_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "DevilRayError",
 ]
 
 
@@ -452,6 +452,75 @@ class Guide:
     def set_options(self, collect_fraction: float = 1.0, guided_vertices: int = 64, train_batch: int = 0):
         check(lib().dr_guide_set_options(self.handle, collect_fraction, guided_vertices, train_batch))
         self.collect_fraction, self.guided_vertices, self.train_batch = collect_fraction, guided_vertices, train_batch
+
+
+def make_instances(scene, scale, translate) -> np.ndarray:
+    """INSTANCE_DTYPE array from per-instance scene indices, scales (3) and
+    translations (3); scalars broadcast."""
+    sc = np.atleast_1d(np.asarray(scene, np.uint32))
+    n = max(sc.shape[0], np.atleast_2d(scale).shape[0], np.atleast_2d(translate).shape[0])
+    out = np.zeros(n, INSTANCE_DTYPE)
+    out["scene"] = np.broadcast_to(sc, (n,))
+    out["scale"] = np.broadcast_to(np.asarray(scale, np.float32), (n, 3))
+    out["translate"] = np.broadcast_to(np.asarray(translate, np.float32), (n, 3))
+    return out
+
+
+class Instances:
+    """Built scenes placed with a scale and a translation each (dr_instances):
+    a top-level tree over the instances above each scene's own tree. Hit ids
+    number the instances' triangles in instance order."""
+
+    def __init__(self, ctx: Context, scenes, instances):
+        self.ctx = ctx
+        self._inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        arr = (C.c_void_p * len(scenes))(*[s.handle.value if hasattr(s.handle, "value") else s.handle
+                                           for s in scenes])
+        h = C.c_void_p()
+        check(lib().dr_instances_create(ctx.handle, arr, len(scenes), self._inst.ctypes.data_as(C.c_void_p),
+                                        self._inst.shape[0], C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().dr_instances_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> DrSceneInfo:
+        i = DrSceneInfo()
+        check(lib().dr_instances_get_info(self.handle, C.byref(i)))
+        return i
+
+    def set_trace_schedule(self, closest: int, any_hit: int):
+        check(lib().dr_instances_set_trace_schedule(self.handle, closest, any_hit))
+
+    def trace_closest_dev(self, rays_ptr: int, n: int, hits_ptr: int, stream=None):
+        check(lib().dr_instances_trace_closest(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr), stream))
+
+    def trace_any_dev(self, rays_ptr: int, n: int, occ_ptr: int, stream=None):
+        check(lib().dr_instances_trace_any(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(occ_ptr), stream))
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        dr = self.ctx.upload(rays)
+        dh = self.ctx.alloc(max(n, 1) * HIT_DTYPE.itemsize)
+        self.trace_closest_dev(dr.ptr, n, dh.ptr)
+        self.ctx.synchronize()
+        return dh.to_numpy(HIT_DTYPE, n)
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        dr = self.ctx.upload(rays)
+        do = self.ctx.alloc(max(n, 1) * 4)
+        self.trace_any_dev(dr.ptr, n, do.ptr)
+        self.ctx.synchronize()
+        return do.to_numpy(np.uint32, n)
 
 
 class Batch:

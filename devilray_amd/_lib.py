@@ -100,6 +100,8 @@ ALIAS_DTYPE = np.dtype([("p_a", np.float32), ("pdf_a", np.float32), ("pdf_b", np
 
 RAY_DTYPE = np.dtype([(n, np.float32) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")])
 HIT_DTYPE = np.dtype([("id", np.int32), ("t", np.float32), ("u", np.float32), ("v", np.float32)])
+# dr_instance: world = model * scale + translate
+INSTANCE_DTYPE = np.dtype([("scene", np.uint32), ("scale", np.float32, 3), ("translate", np.float32, 3)])
 
 _vp = C.c_void_p
 _u32 = C.c_uint32
@@ -142,6 +144,12 @@ PROTOTYPES = {
     "dr_batch_create": (_st, [_vp, _vp, _u32, C.POINTER(_vp)]),
     "dr_batch_destroy": (_st, [_vp]),
     "dr_batch_render": (_st, [_vp, _vp, _u32, C.POINTER(DrRenderParams), C.POINTER(DrBatchOptions), _vp]),
+    "dr_instances_create": (_st, [_vp, _vp, _u32, _vp, _u32, C.POINTER(_vp)]),
+    "dr_instances_destroy": (_st, [_vp]),
+    "dr_instances_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_instances_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
+    "dr_instances_set_trace_schedule": (_st, [_vp, _u32, _u32]),
+    "dr_instances_get_info": (_st, [_vp, C.POINTER(DrSceneInfo)]),
     "dr_render_guided": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _u32, _vp, _vp]),
     "dr_guide_create": (_st, [_vp, C.POINTER(DrGuideParams), C.POINTER(_vp)]),
     "dr_guide_destroy": (_st, [_vp]),

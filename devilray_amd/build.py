@@ -18,7 +18,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdevilray.so")
 
 SOURCES = ["api.hip", "sort.hip", "bvh_build.hip", "trace.hip", "light.hip", "render.hip", "guide.hip", "batch.hip",
-           "display.hip", "gen.cpp"]
+           "display.hip", "instance.hip", "gen.cpp"]
 HEADERS = ["dr_internal.hpp", "dr_device.hpp", "philox.hpp", "shade.hpp", "guide_device.hpp"]
 
 # -ffp-contract=off: products and sums round as written, so the triangle test

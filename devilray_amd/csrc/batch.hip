@@ -71,6 +71,15 @@ __global__ __launch_bounds__(kT) void k_rebase_alias(const AliasEntryB *__restri
 
 }  // namespace
 
+dr_status rebase_nodes(const WideNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, WideNode *dst,
+                       hipStream_t stream)
+{
+    if (n == 0) return DR_OK;
+    k_rebase_nodes<<<blocks(n), kT, 0, stream>>>(src, n, node_off, ref_off, dst);
+    DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
 struct Batch {
     Context *ctx = nullptr;
     Scene merged;
@@ -146,8 +155,9 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
             DR_HIP_TRY(hipMemcpyAsync(m.light_pdf_area.as<float>() + to, s->light_pdf_area.ptr, 4ull * s->n_tris,
                                       hipMemcpyDeviceToDevice, s0));
         }
-        k_rebase_nodes<<<blocks(s->n_nodes), kT, 0, s0>>>(s->nodes.as<WideNode>(), s->n_nodes, (uint32_t)no,
-                                                          (uint32_t)ro, m.nodes.as<WideNode>() + no);
+        if (dr_status st2 = rebase_nodes(s->nodes.as<WideNode>(), s->n_nodes, (uint32_t)no, (uint32_t)ro,
+                                         m.nodes.as<WideNode>() + no, s0))
+            return st2;
         if (s->n_emit)
             k_rebase_alias<<<blocks(s->n_emit), kT, 0, s0>>>(s->alias.as<AliasEntryB>(), s->n_emit, (uint32_t)to,
                                                              m.alias.as<AliasEntryB>() + eo);

@@ -62,6 +62,18 @@ __host__ __device__ inline int32_t make_leaf_ref(uint32_t first, uint32_t count)
 __host__ __device__ inline uint32_t leaf_first(int32_t ref) { return (uint32_t)ref & 0x0FFFFFFFu; }
 __host__ __device__ inline uint32_t leaf_count(int32_t ref) { return (((uint32_t)ref >> 28) & 0x7u) + 1u; }
 
+// One instance of a two-level forest: the root of its scene's tree and the
+// inverse of its scale-and-translate transform. A ray enters model space as
+// ((o - t) * inv_s, d * inv_s), so its t stays world t; hits report
+// id_delta + the scene-local triangle id.
+struct InstRec {
+    int32_t root;
+    float isx, isy, isz;
+    float tx, ty, tz;
+    uint32_t id_delta;
+};
+static_assert(sizeof(InstRec) == 32, "instance record is 32 B");
+
 struct Context {
     int device = 0;
     int cu_count = 0;
@@ -86,6 +98,11 @@ struct Scene {
     DevBuf tri_v;     // 3 float4 per triangle reference, leaf order; w of the first = id bits
     DevBuf nodes;     // WideNode
     uint32_t n_refs = 0;  // tri_v entries: triangles plus split pieces
+    // two-level forest (instance.hip): nodes [0, n_top) are a tree over the
+    // instances whose leaf references name an InstRec, then the instanced
+    // scenes' trees; 0 for an ordinary scene
+    uint32_t n_inst = 0, n_top = 0;
+    DevBuf inst_recs;  // InstRec * n_inst
     uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
     // wide nodes by depth: level d is [level_ofs[d], level_ofs[d + 1]) (refit)
     std::vector<uint32_t> level_ofs;
@@ -242,6 +259,11 @@ dr_status build_lights(Scene *s, hipStream_t stream);
 // render.hip
 dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
                  double *accum, hipStream_t stream);
+
+// batch.hip: copy n wide nodes, moving interior references by node_off and
+// leaf references by ref_off
+dr_status rebase_nodes(const WideNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, WideNode *dst,
+                       hipStream_t stream);
 
 // bvh_build.hip
 dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream);

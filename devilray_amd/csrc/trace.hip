@@ -24,6 +24,16 @@
 // loaded by the ray's octant so no per-slab min/max is needed. Hit children
 // are sorted by entry distance, the nearest is taken next and the rest are
 // pushed far to near. Leaves hold up to 8 triangles stored as 3 float4.
+//
+// Instances (INST). A two-level forest starts with a tree over instance
+// boxes in world space whose leaves name an instance. A lane that reaches
+// one enters it at once: it pushes an exit marker, moves its ray into the
+// instance's model space ((o - t) / s, d / s, so t stays world t) and
+// descends the instance's tree; popping the marker restores the world ray. A
+// model-space leaf parked by the speculative loop is intersected before the
+// marker is taken, so a parked leaf always belongs to the current instance.
+// The stack bound is 3 entries per level of the top tree and of the deepest
+// instanced tree, plus the marker.
 #include "dr_device.hpp"
 #include "dr_internal.hpp"
 
@@ -38,6 +48,9 @@ constexpr int kLdsStack = 16;
 constexpr uint32_t kPerRaySpillMax = 48;
 constexpr int32_t kNone = DR_EMPTY_REF;
 constexpr uint32_t kCounterSlots = 256;
+// stack marker of an instance's model space (a leaf reference of count 8 at
+// first 2^28 - 2, beyond every valid leaf)
+constexpr int32_t kInstExit = (int32_t)0xFFFFFFFEu;
 
 __device__ __forceinline__ void cswap(float &ta, int32_t &ra, float &tb, int32_t &rb)
 {
@@ -47,7 +60,11 @@ __device__ __forceinline__ void cswap(float &ta, int32_t &ra, float &tb, int32_t
     }
 }
 
-__device__ __forceinline__ bool is_leaf(int32_t r) { return r < 0 && r != kNone; }
+template <bool INST = false>
+__device__ __forceinline__ bool is_leaf(int32_t r)
+{
+    return r < 0 && r != kNone && (!INST || r != kInstExit);
+}
 
 struct LeafState {
     float bt, bu, bv;
@@ -66,7 +83,7 @@ __device__ __forceinline__ bool leads_wave()
 
 template <bool ANY, bool COUNT, int KZ>
 __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v, int32_t leaf, const RayPre &p,
-                                               LeafState &st)
+                                               LeafState &st, uint32_t id_add = 0u)
 {
     const uint32_t first = leaf_first(leaf), cnt = leaf_count(leaf);
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -78,7 +95,7 @@ __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v,
             if (leads_wave()) ++st.wave_tri_iters;
         }
         if (!tri_watertight<KZ>(p, a, b, c, t, u, v)) continue;
-        const uint32_t id = (uint32_t)__float_as_int(a.w);
+        const uint32_t id = (uint32_t)__float_as_int(a.w) + id_add;
         if (ANY) {
             if (t >= p.tmin && t <= p.tmax) { st.occluded = true; return; }
         } else if (accept_hit(t, id, p.tmin, st.bt, st.bid)) {
@@ -87,7 +104,7 @@ __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v,
     }
 }
 
-template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE>
+template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE, bool INST = false>
 __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ nodes,
                                                   const float4 *__restrict__ tri_v,
                                                   const float4 *__restrict__ rays, const uint32_t *__restrict__ perm,
@@ -98,8 +115,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                                                   int32_t *__restrict__ ovf, uint32_t *__restrict__ flags,
                                                   uint32_t *__restrict__ ray_counter, uint32_t refill_min,
                                                   const uint32_t *__restrict__ roots, uint32_t slices,
-                                                  uint32_t max_stack)
+                                                  uint32_t max_stack, const InstRec *__restrict__ inst)
 {
+    static_assert(!INST || (SPECULATIVE && !COUNT), "instances run the speculative loop, uncounted");
     __shared__ int32_t stack[kLdsStack][kBlock];
     // ray count: fixed at launch, or read from device memory (on-device queues)
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
@@ -128,6 +146,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
     bool overflow = false, any_overflow = false;
     uint32_t high_water = 0;  // counted variant: deepest stack of the lane
     const char *nbase = reinterpret_cast<const char *>(nodes);
+    // instances: the world ray, whether the lane is in an instance's model
+    // space, and that instance's triangle id offset
+    float wox = 0.0f, woy = 0.0f, woz = 0.0f, wdx = 0.0f, wdy = 0.0f, wdz = 0.0f;
+    bool in_inst = false;
+    uint32_t id_add = 0;
+    // byte offsets of the near slab planes for the ray's octant
+    auto set_octant = [&]() {
+        nx = signbit(p.ix) ? 16u : 0u;
+        ny = signbit(p.iy) ? 48u : 32u;
+        nz = signbit(p.iz) ? 80u : 64u;
+    };
 
     auto pop = [&]() -> int32_t {
         if (sp == 0) return kNone;
@@ -185,10 +214,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                                          (r1.x != r1.x) || (r1.y != r1.y) || (r1.z != r1.z) || (r1.w != r1.w);
                     // the root is always an interior node
                     node = nan_ray ? kNone : (roots ? (int32_t)roots[r] : 0);
-                    // byte offsets of the near slab planes for this ray's octant
-                    nx = signbit(p.ix) ? 16u : 0u;
-                    ny = signbit(p.iy) ? 48u : 32u;
-                    nz = signbit(p.iz) ? 80u : 64u;
+                    set_octant();
+                    if (INST) {
+                        wox = r0.x; woy = r0.y; woz = r0.z;
+                        wdx = r1.x; wdy = r1.y; wdz = r1.z;
+                        in_inst = false;
+                        id_add = 0u;
+                    }
                 } else if (!PERSISTENT || parts_seen >= slices) {
                     exhausted = true;
                 }
@@ -258,9 +290,34 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
             };
             if (SPECULATIVE) {
                 // ---- interior nodes, leaves postponed
-                while (node >= 0) {
+                for (;;) {
+                    if (INST) {
+                        // (a failed push leaves the stack without its marker:
+                        // stop here and let the overflow path end the ray)
+                        if (overflow) break;
+                        // leave the instance once its parked leaf is done
+                        if (node == kInstExit && leaf == kNone) {
+                            ray_prepare(wox, woy, woz, p.tmin, wdx, wdy, wdz, p.tmax, p);
+                            set_octant();
+                            in_inst = false;
+                            id_add = 0u;
+                            node = pop();
+                        }
+                        // an instance leaf of the top tree: enter it
+                        if (!in_inst && is_leaf<true>(node)) {
+                            const InstRec ir = inst[leaf_first(node)];
+                            push(kInstExit);
+                            ray_prepare((wox - ir.tx) * ir.isx, (woy - ir.ty) * ir.isy, (woz - ir.tz) * ir.isz,
+                                        p.tmin, wdx * ir.isx, wdy * ir.isy, wdz * ir.isz, p.tmax, p);
+                            set_octant();
+                            in_inst = true;
+                            id_add = ir.id_delta;
+                            node = ir.root;
+                        }
+                    }
+                    if (node < 0) break;
                     node = visit(node);
-                    if (is_leaf(node) && leaf == kNone) {
+                    if (is_leaf<INST>(node) && leaf == kNone && (!INST || in_inst)) {
                         leaf = node;
                         node = pop();
                     }
@@ -270,17 +327,19 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                 // the specialised triangle test
                 const int kz0 = __builtin_amdgcn_readfirstlane(p.kz);
                 const bool uniform = __ballot(p.kz != kz0) == 0ull;
-                while (is_leaf(leaf)) {
+                // (a parked leaf is in the model space of the lane's instance,
+                // and so is a leaf taken from its stack below)
+                while (is_leaf<INST>(leaf) && !(INST && overflow)) {
                     if (uniform) {
-                        if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, leaf, p, st);
-                        else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, leaf, p, st);
-                        else intersect_leaf<ANY, COUNT, 2>(tri_v, leaf, p, st);
+                        if (kz0 == 0) intersect_leaf<ANY, COUNT, 0>(tri_v, leaf, p, st, id_add);
+                        else if (kz0 == 1) intersect_leaf<ANY, COUNT, 1>(tri_v, leaf, p, st, id_add);
+                        else intersect_leaf<ANY, COUNT, 2>(tri_v, leaf, p, st, id_add);
                     } else {
-                        intersect_leaf<ANY, COUNT, -1>(tri_v, leaf, p, st);
+                        intersect_leaf<ANY, COUNT, -1>(tri_v, leaf, p, st, id_add);
                     }
                     if (ANY && st.occluded) { leaf = kNone; node = kNone; sp = 0; break; }
                     leaf = kNone;
-                    if (is_leaf(node)) {
+                    if (is_leaf<INST>(node)) {
                         leaf = node;
                         node = pop();
                     }
@@ -320,7 +379,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ n
                     const bool hit = st.bid != 0xFFFFFFFFu;
                     dr_hit h;
                     h.id = hit ? (int32_t)st.bid : -1;
-                    h.t = hit ? st.bt : INFINITY;
+                    // + 0 reports a hit at distance zero as +0 whatever the
+                    // sign of the test's determinant
+                    h.t = hit ? st.bt + 0.0f : INFINITY;
                     h.u = hit ? st.bu : 0.0f;
                     h.v = hit ? st.bv : 0.0f;
                     hits[rr] = h;
@@ -409,11 +470,13 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
 using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
                             const uint32_t *, dr_hit *,
                             uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t,
-                            const uint32_t *, uint32_t, uint32_t);
+                            const uint32_t *, uint32_t, uint32_t, const InstRec *);
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool INST = false>
 TraceKernel pick(uint32_t sched)
 {
+    if (INST)  // the speculative loops only (the launcher promotes the others)
+        return (sched & 2u) ? k_trace<ANY, false, true, true, true> : k_trace<ANY, false, false, true, true>;
     switch (sched & 3u) {
     case DR_SCHED_PER_RAY: return k_trace<ANY, COUNT, false, false>;
     case DR_SCHED_PER_RAY_SPECULATIVE: return k_trace<ANY, COUNT, false, true>;
@@ -424,11 +487,11 @@ TraceKernel pick(uint32_t sched)
 
 // Blocks per CU of a kernel variant at kBlock threads (occupancy limit),
 // looked up once per variant.
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool INST = false>
 dr_status blocks_per_cu(uint32_t loop, TraceKernel kern, int &bpc)
 {
-    static int cache[3][4] = {};
-    int &c = cache[ANY ? 0 : (COUNT ? 1 : 2)][loop & 3u];
+    static int cache[5][4] = {};
+    int &c = cache[INST ? 3 + ANY : (ANY ? 0 : (COUNT ? 1 : 2))][loop & 3u];
     if (!c) {
         int b = 0;
         DR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBlock, 0));
@@ -494,7 +557,7 @@ dr_status launch_trace_noalloc(Scene *s, const dr_ray *rays, uint32_t n, dr_hit 
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(),
                                       reinterpret_cast<const float4 *>(rays), nullptr, n, nullptr, hits, nullptr,
                                       nullptr, nullptr, base + kNoAllocHeadWords, flags, counter, s->refill_min,
-                                      nullptr, 1u, max_stack);
+                                      nullptr, 1u, max_stack, nullptr);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }
@@ -516,8 +579,12 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     if (n == 0) return DR_OK;
     if (!rays || (any_hit ? !occluded : !hits)) return fail(DR_E_INVALID, "null ray or output buffer");
     const bool count = box_tests && tri_tests;
+    const bool inst = s->n_inst != 0;
+    if (inst && (count || ray_root))
+        return fail(DR_E_UNSUPPORTED, "instances: no counted traversal and no per-ray roots");
     // stack capacity from the built depth (3 entries per wide level, one slot
-    // of slack); a deep tree moves per-ray schedules onto the resident grid so
+    // of slack; for instances the depth counts both levels and the slot holds
+    // the marker); a deep tree moves per-ray schedules onto the resident grid so
     // the spill region stays bounded by the number of resident lanes
     const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
     if (s->depth > kMaxWideDepth)
@@ -525,12 +592,17 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
                                        " wide levels");
     uint32_t loop = sched & 3u;
     if (max_stack - kLdsStack > kPerRaySpillMax) loop |= 2u;  // PER_RAY -> PERSISTENT, keeping speculation
-    const TraceKernel kern = any_hit ? pick<true, false>(loop) : (count ? pick<false, true>(loop) : pick<false, false>(loop));
+    if (inst) loop |= 1u;  // instances run the speculative loops
+    const TraceKernel kern = inst ? (any_hit ? pick<true, false, true>(loop) : pick<false, false, true>(loop))
+                             : any_hit ? pick<true, false>(loop)
+                             : (count ? pick<false, true>(loop) : pick<false, false>(loop));
     const bool persistent = loop == DR_SCHED_PERSISTENT || loop == DR_SCHED_PERSISTENT_SPECULATIVE;
     int occ = 1;
-    if (dr_status st = any_hit ? blocks_per_cu<true, false>(loop, kern, occ)
-                               : (count ? blocks_per_cu<false, true>(loop, kern, occ)
-                                        : blocks_per_cu<false, false>(loop, kern, occ)))
+    if (dr_status st = inst ? (any_hit ? blocks_per_cu<true, false, true>(loop, kern, occ)
+                                       : blocks_per_cu<false, false, true>(loop, kern, occ))
+                       : any_hit ? blocks_per_cu<true, false>(loop, kern, occ)
+                       : (count ? blocks_per_cu<false, true>(loop, kern, occ)
+                                : blocks_per_cu<false, false>(loop, kern, occ)))
         return st;
     const uint32_t need = (n + kBlock - 1) / kBlock;
     const int bpc = s->blocks_per_cu ? std::min<int>((int)s->blocks_per_cu, occ) : occ;
@@ -565,7 +637,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     int32_t *ovf = s->ovf_stack.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
                                       box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices,
-                                      max_stack);
+                                      max_stack, inst ? s->inst_recs.as<InstRec>() : nullptr);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

@@ -45,8 +45,9 @@ typedef void *dr_stream;
  * tmin <= t <= tmax; tmin must be >= 0. The direction need not be unit. */
 typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } dr_ray;
 
-/* 16-byte hit: triangle id (-1 on miss), distance t, barycentric weights of
- * the triangle's 2nd (u) and 3rd (v) vertex. Ties on t resolve to the lower id. */
+/* 16-byte hit: triangle id (-1 on miss), distance t (+inf on miss; a hit at
+ * distance zero reports +0), barycentric weights of the triangle's 2nd (u)
+ * and 3rd (v) vertex. Ties on t resolve to the lower id. */
 typedef struct { int32_t id; float t, u, v; } dr_hit;
 
 /* Materials: the reference's Material variant (include/tracing/Material.hpp:9-25):
@@ -351,6 +352,52 @@ dr_status dr_batch_destroy(dr_batch *batch);
  * at entry (the view table is rewritten), then runs asynchronously. */
 dr_status dr_batch_render(dr_batch *batch, const dr_batch_view *views, uint32_t n_views,
                           const dr_render_params *params, const dr_batch_options *options, dr_stream stream);
+
+/* ---- instancing: one scene placed several times ----------------------------
+ *
+ * Replaces the reference's objects that share one GpuTris storage with a
+ * transform each (viewGpuTris, src/GpuTris.cpp:61-85; TriangleMesh's
+ * model_to_world, include/tracing/TriangleMesh.hpp:9-27) and its per-object
+ * loop in cast (include/tracing/IntersectionTestsImpl.hpp:234-251). A ray is
+ * tested against an instance in model space, as getIntersectionImpl does
+ * (IntersectionTestsImpl.hpp:143-151): o' = (o - translate) * inv_s,
+ * d' = d * inv_s with inv_s = 1 / scale per axis (Transform::applyInverse,
+ * include/Transform.hpp:14-21), d' not renormalised, so t is world t. A
+ * top-level 4-wide tree over the instances' world boxes selects the instances
+ * a ray enters; each scene's tree is stored once.
+ *
+ * Hit ids number the instances' triangles in instance order: instance i's
+ * triangle j is (triangles of instances 0 .. i-1) + j, as in a scene that
+ * concatenates the transformed copies in that order. Ties on t go to the
+ * lower id. */
+typedef struct dr_instances dr_instances;
+
+typedef struct {
+    uint32_t scene;      /* index into the scenes passed to dr_instances_create */
+    float scale[3];      /* per axis, finite and non-zero (negative mirrors) */
+    float translate[3];  /* world = model * scale + translate */
+} dr_instance;
+
+/* Scenes must have their BVH built; the instance set copies their trees, so
+ * they may be destroyed afterwards (a later rebuild or refit of a scene does
+ * not reach the set). At most 2^28 - 2 instances, 2^31 - 1 instanced
+ * triangles and 2^28 - 2 triangle references in the scenes. */
+dr_status dr_instances_create(dr_context *ctx, dr_scene *const *scenes, uint32_t n_scenes,
+                              const dr_instance *instances, uint32_t n_instances, dr_instances **out);
+dr_status dr_instances_destroy(dr_instances *inst);
+/* Closest hit and occlusion as dr_trace_closest / dr_trace_any. Instances
+ * run the speculative schedules: DR_SCHED_PER_RAY becomes
+ * DR_SCHED_PER_RAY_SPECULATIVE and DR_SCHED_PERSISTENT becomes
+ * DR_SCHED_PERSISTENT_SPECULATIVE (default: persistent speculative). */
+dr_status dr_instances_trace_closest(dr_instances *inst, const dr_ray *rays, uint32_t n, dr_hit *hits,
+                                     dr_stream stream);
+dr_status dr_instances_trace_any(dr_instances *inst, const dr_ray *rays, uint32_t n, uint32_t *occluded,
+                                 dr_stream stream);
+dr_status dr_instances_set_trace_schedule(dr_instances *inst, uint32_t closest_sched, uint32_t any_sched);
+/* n_triangles: instanced triangles; n_nodes: top-level plus scene nodes;
+ * depth: top-level levels plus the deepest scene's levels (the traversal
+ * stack holds 3 x depth + 1 entries); overflow_flag as for scenes. */
+dr_status dr_instances_get_info(dr_instances *inst, dr_scene_info *info);
 
 /* ---- learned path guide (no reference counterpart: the reference samples
  * Lambert bounces from the BSDF only, PathGeneration.hpp:406-420) ----------

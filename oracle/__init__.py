@@ -47,6 +47,7 @@ PROTOTYPES = {
     "or_bvh_validate": (_i32, [_vp, _vp, _vp]),
     "or_trace_bvh": (None, [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _i32]),
     "or_trace_brute": (None, [_vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32]),
+    "or_trace_instances_brute": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32]),
     "or_refbbh_build": (_vp, [_vp, _i32, _vp, _i32]),
     "or_refbbh_free": (None, [_vp]),
     "or_refbbh_depth": (_i32, [_vp]),
@@ -391,6 +392,38 @@ def brute_any(vertices, indices, rays, nthreads=None):
     occ = np.empty(n, np.uint32)
     lib().or_trace_brute(_p(v), _p(f), f.shape[0], _p(rays), n, 1, None, _p(occ), nthreads or threads())
     return occ
+
+
+def _instances_call(scenes, instances, rays, any_hit, nthreads):
+    """scenes: [(vertices, indices)]; instances: a structured array with
+    fields scene (u32), scale (3 f32), translate (3 f32), 28 B each."""
+    vs = [np.ascontiguousarray(v, np.float32).reshape(-1, 3) for v, _ in scenes]
+    fs = [np.ascontiguousarray(f, np.int32).reshape(-1, 3) for _, f in scenes]
+    vofs = np.cumsum([0] + [v.shape[0] for v in vs[:-1]]).astype(np.int32)
+    tofs = np.cumsum([0] + [f.shape[0] for f in fs[:-1]]).astype(np.int32)
+    ntri = np.array([f.shape[0] for f in fs], np.int32)
+    v = np.ascontiguousarray(np.concatenate(vs))
+    f = np.ascontiguousarray(np.concatenate(fs))
+    inst = np.ascontiguousarray(instances)
+    if inst.dtype.itemsize != 28 or int(np.max(inst["scene"], initial=0)) >= len(scenes):
+        raise ValueError("instances: 28-byte records naming scenes in range")
+    n = rays.shape[0]
+    hits = None if any_hit else np.empty(n, HIT_DTYPE)
+    occ = np.empty(n, np.uint32) if any_hit else None
+    lib().or_trace_instances_brute(_p(v), _p(f), _p(vofs), _p(tofs), _p(ntri), _p(inst), inst.shape[0],
+                                   _p(rays), n, int(any_hit), _p(hits) if hits is not None else None,
+                                   _p(occ) if occ is not None else None, nthreads or threads())
+    return occ if any_hit else hits
+
+
+def instances_closest(scenes, instances, rays, nthreads=None):
+    """Closest hit over instanced scenes by brute force in model space."""
+    return _instances_call(scenes, instances, rays, False, nthreads)
+
+
+def instances_any(scenes, instances, rays, nthreads=None):
+    """Occlusion over instanced scenes by brute force in model space."""
+    return _instances_call(scenes, instances, rays, True, nthreads)
 
 
 class RefBBH:

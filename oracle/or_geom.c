@@ -566,7 +566,7 @@ static void trace_ray(const obvh *b, const float *verts, const int32_t *tris, co
         if (occ_out) *occ_out = (uint32_t)occluded;
     } else if (hit_out) {
         hit_out->id = best;
-        hit_out->t = best >= 0 ? tbest : INFINITY;
+        hit_out->t = best >= 0 ? tbest + 0.0f : INFINITY;  /* a zero distance as +0 */
         hit_out->u = best >= 0 ? bu : 0.0f;
         hit_out->v = best >= 0 ? bv : 0.0f;
     }
@@ -635,7 +635,7 @@ static void brute_range(void *vc, int32_t begin, int32_t end)
         if (c->any) c->occ[i] = (uint32_t)occ;
         else {
             c->hits[i].id = best;
-            c->hits[i].t = best >= 0 ? tbest : INFINITY;
+            c->hits[i].t = best >= 0 ? tbest + 0.0f : INFINITY;  /* a zero distance as +0 */
             c->hits[i].u = best >= 0 ? bu : 0.0f;
             c->hits[i].v = best >= 0 ? bv : 0.0f;
         }
@@ -647,6 +647,72 @@ void or_trace_brute(const float *verts, const int32_t *tris, int32_t ntris, cons
 {
     brute_ctx c = { verts, tris, ntris, rays, any_hit, hits, occluded };
     parallel_for(n, threads, brute_range, &c);
+}
+
+typedef struct {
+    const float *verts;
+    const int32_t *tris, *vofs, *tofs, *ntris;
+    const or_instance *inst;
+    int32_t n_inst;
+    const or_ray *rays;
+    int32_t any;
+    or_hit *hits;
+    uint32_t *occ;
+} inst_ctx;
+
+/* include/tracing/IntersectionTestsImpl.hpp:143-151 (ray to model space per
+ * object) and :234-251 (nearest over all objects) */
+static void inst_range(void *vc, int32_t begin, int32_t end)
+{
+    inst_ctx *c = (inst_ctx *)vc;
+    for (int32_t i = begin; i < end; ++i) {
+        const or_ray *w = &c->rays[i];
+        float tbest = w->tmax, bu = 0, bv = 0;
+        int32_t best = -1;
+        int occ = 0;
+        uint32_t id_off = 0;
+        for (int32_t k = 0; k < c->n_inst && !occ; ++k) {
+            const or_instance *in = &c->inst[k];
+            const float isx = 1.0f / in->scale[0], isy = 1.0f / in->scale[1], isz = 1.0f / in->scale[2];
+            const or_ray m = { (w->ox - in->translate[0]) * isx, (w->oy - in->translate[1]) * isy,
+                               (w->oz - in->translate[2]) * isz, w->tmin,
+                               w->dx * isx, w->dy * isy, w->dz * isz, w->tmax };
+            ray_pre p;
+            ray_prepare(&m, &p);
+            const int32_t s = (int32_t)in->scene;
+            const float *vb = c->verts + 3 * (size_t)c->vofs[s];
+            const int32_t *tb = c->tris + 3 * (size_t)c->tofs[s];
+            for (int32_t j = 0; j < c->ntris[s]; ++j) {
+                const int32_t *tri = tb + 3 * (size_t)j;
+                float t, u, v;
+                if (!tri_wt(&p, vb + 3 * (size_t)tri[0], vb + 3 * (size_t)tri[1], vb + 3 * (size_t)tri[2],
+                            &t, &u, &v)) continue;
+                const int32_t id = (int32_t)(id_off + (uint32_t)j);
+                if (c->any) {
+                    if (t >= p.tmin && t <= p.tmax) { occ = 1; break; }
+                } else if (accept_hit(t, id, p.tmin, tbest, best)) {
+                    tbest = t; best = id; bu = u; bv = v;
+                }
+            }
+            id_off += (uint32_t)c->ntris[s];
+        }
+        if (c->any) c->occ[i] = (uint32_t)occ;
+        else {
+            c->hits[i].id = best;
+            c->hits[i].t = best >= 0 ? tbest + 0.0f : INFINITY;  /* a zero distance as +0 */
+            c->hits[i].u = best >= 0 ? bu : 0.0f;
+            c->hits[i].v = best >= 0 ? bv : 0.0f;
+        }
+    }
+}
+
+void or_trace_instances_brute(const float *verts, const int32_t *tris, const int32_t *scene_vofs,
+                              const int32_t *scene_tofs, const int32_t *scene_ntris,
+                              const or_instance *inst, int32_t n_inst, const or_ray *rays, int32_t n,
+                              int32_t any_hit, or_hit *hits, uint32_t *occluded, int32_t threads)
+{
+    inst_ctx c = { verts, tris, scene_vofs, scene_tofs, scene_ntris, inst, n_inst, rays, any_hit, hits, occluded };
+    parallel_for(n, threads, inst_range, &c);
 }
 
 /* ------------------------------------------------------------------------ */

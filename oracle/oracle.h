@@ -83,6 +83,20 @@ void or_trace_bvh(const void *bvh, const float *verts, const int32_t *tris, cons
 void or_trace_brute(const float *verts, const int32_t *tris, int32_t ntris, const or_ray *rays,
                     int32_t n, int32_t any_hit, or_hit *hits, uint32_t *occluded, int32_t threads);
 
+/* Brute force over instances, each a scene placed by p * scale + translate.
+ * A ray is tested in each instance's model space as the reference's
+ * getIntersectionImpl does (include/tracing/IntersectionTestsImpl.hpp:143-151
+ * with Transform::applyInverse, include/Transform.hpp:14-21):
+ * o' = (o - translate) * (1 / scale), d' = d * (1 / scale), per axis, t
+ * unchanged. Scene k's triangles are tris[scene_tofs[k] .. + scene_ntris[k])
+ * indexing verts from scene_vofs[k]. Hit ids number the instances'
+ * triangles in instance order; ties on t go to the lower id. */
+typedef struct { uint32_t scene; float scale[3]; float translate[3]; } or_instance;  /* = dr_instance */
+void or_trace_instances_brute(const float *verts, const int32_t *tris, const int32_t *scene_vofs,
+                              const int32_t *scene_tofs, const int32_t *scene_ntris,
+                              const or_instance *inst, int32_t n_inst, const or_ray *rays, int32_t n,
+                              int32_t any_hit, or_hit *hits, uint32_t *occluded, int32_t threads);
+
 /* ---- reference-semantics BBH (median split, 1 tri / leaf) ----------------- */
 /* generateSimpleBBH / generateBBHLayer (src/models/BBH.cpp:38-115) and the
  * stackless bit-trail walk getIntersectionImpl
