@@ -94,10 +94,12 @@ __host__ __device__ inline int param_layer(int p, int &o, int &c)
 }
 
 struct Net {
-    const bf16x8 *ff;      // forward fragments
+    const bf16x8 *ff0;     // all forward fragments, from layer 0's first
+    const bf16x8 *ff;      // forward fragments from layer 1's first (ff0 + kFrag0 * 64)
     const bf16x8 *fb;      // transposed fragments
     const float *bias[5];  // FP32 biases of each layer
 };
+constexpr int kFrag0 = 4;  // forward fragments of layer 0
 
 struct Box {
     float lo[3], ext[3];
@@ -114,16 +116,36 @@ __device__ __forceinline__ f32x4 bias4(const Net &net, int l, int ob, int g)
     return f32x4{b[0], b[1], b[2], b[3]};
 }
 
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// two floats to a packed BF16 pair (one conversion instruction)
+__device__ __forceinline__ unsigned int pk_bf16(float a, float b)
+{
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+// eight floats (lo then hi) as a BF16 operand
+__device__ __forceinline__ bf16x8 to_bf16x8(f32x4 lo, f32x4 hi)
+{
+    const u32x4 w = {pk_bf16(lo[0], lo[1]), pk_bf16(lo[2], lo[3]), pk_bf16(hi[0], hi[1]), pk_bf16(hi[2], hi[3])};
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+// BF16 conversion, then ReLU on the bit patterns: as 16-bit integers the
+// negative values and -0 are exactly the negative patterns, so a packed
+// integer max with 0 rounds as relu-then-round does and yields +0 for them.
 __device__ __forceinline__ bf16x8 pack_relu(f32x4 lo, f32x4 hi)
 {
-    bf16x8 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        r[i] = (__bf16)fmaxf(lo[i], 0.0f);
-        r[4 + i] = (__bf16)fmaxf(hi[i], 0.0f);
-    }
-    return r;
+    const s16x8 v = __builtin_elementwise_max(__builtin_bit_cast(s16x8, to_bf16x8(lo, hi)), (s16x8)0);
+    return __builtin_bit_cast(bf16x8, v);
 }
+
+// e^x as one exp2 instruction (relative error about 1e-7 (1 + |x|)): the
+// loss head's exponentials, compared with their references at BF16 tolerance
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
 
 // Input features 8g .. 8g+7 of one sample: for g < 3, sin and cos of
 // 2^(k+1) pi u (k = 0..3; periods 1 to 1/8 of the box) with u the position
@@ -138,16 +160,18 @@ __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &b
         const float ext = g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2]);
         float u = (x - lo) / ext;
         u = fminf(fmaxf(u, 0.0f), 1.0f);
-        float scale = 1.0f;
+        // sin and cos of 2 pi u, then of each doubled angle:
+        // sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)
+        // (a few ulps after three doublings; the features are rounded to BF16)
+        float s, c;
+        sincos_2pi(u - floorf(u), s, c);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float a = u * scale;
-            const float t = a - floorf(a);
-            float s, c;
-            sincos_2pi(t, s, c);
             f[2 * k] = (__bf16)s;
             f[2 * k + 1] = (__bf16)c;
-            scale *= 2.0f;
+            const float s2 = 2.0f * s * c;
+            c = (c - s) * (c + s);
+            s = s2;
         }
     } else {
         f[0] = (__bf16)n.x; f[1] = (__bf16)n.y; f[2] = (__bf16)n.z;
@@ -165,7 +189,7 @@ __device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0,
     const int g = lane >> 4;
     f32x4 acc[4];
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff[(f_base(0) + ob) * 64 + lane], x0, bias4(net, 0, ob, g));
+    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff0[ob * 64 + lane], x0, bias4(net, 0, ob, g));
     xh[0][0] = pack_relu(acc[0], acc[1]);
     xh[0][1] = pack_relu(acc[2], acc[3]);
 #pragma unroll
@@ -174,7 +198,8 @@ __device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0,
         for (int ob = 0; ob < 4; ++ob) {
             acc[ob] = bias4(net, l, ob, g);
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) acc[ob] = mfma(net.ff[(f_base(l) + ob * 2 + ks) * 64 + lane], xh[l - 1][ks], acc[ob]);
+            for (int ks = 0; ks < 2; ++ks)
+                acc[ob] = mfma(net.ff[(f_base(l) - kFrag0 + ob * 2 + ks) * 64 + lane], xh[l - 1][ks], acc[ob]);
         }
         xh[l][0] = pack_relu(acc[0], acc[1]);
         xh[l][1] = pack_relu(acc[2], acc[3]);
@@ -183,7 +208,7 @@ __device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0,
     for (int ob = 0; ob < 3; ++ob) {
         y[ob] = bias4(net, 4, ob, g);
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) y[ob] = mfma(net.ff[(f_base(4) + ob * 2 + ks) * 64 + lane], xh[3][ks], y[ob]);
+        for (int ks = 0; ks < 2; ++ks) y[ob] = mfma(net.ff[(f_base(4) - kFrag0 + ob * 2 + ks) * 64 + lane], xh[3][ks], y[ob]);
     }
 }
 
@@ -208,24 +233,26 @@ __device__ __forceinline__ void head_decode(const f32x4 (&y)[3], int lane, HeadL
         const int q0 = 5 * t;
         const float mx = yl[q0], my = yl[q0 + 1], mz = yl[q0 + 2];
         const float rr = fmaxf(sqrtf(mx * mx + my * my + mz * mz), kMuMinNorm);
+        const float ir = __builtin_amdgcn_rcpf(rr);
         h.r[t] = rr;
-        h.mu[t][0] = mx / rr;
-        h.mu[t][1] = my / rr;
-        h.mu[t][2] = mz / rr;
+        h.mu[t][0] = mx * ir;
+        h.mu[t][1] = my * ir;
+        h.mu[t][2] = mz * ir;
         h.kr[t] = yl[q0 + 3];
-        h.kap[t] = expf(fminf(fmaxf(h.kr[t], kLogKappaLo), kLogKappaHi));
+        h.kap[t] = fexp(fminf(fmaxf(h.kr[t], kLogKappaLo), kLogKappaHi));
         w[t] = yl[q0 + 4];
     }
     float m = fmaxf(w[0], w[1]);
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
-    const float e0 = expf(w[0] - m), e1 = expf(w[1] - m);
+    const float e0 = fexp(w[0] - m), e1 = fexp(w[1] - m);
     float sum = e0 + e1;
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
-    h.pi[0] = e0 / sum;
-    h.pi[1] = e1 / sum;
-    const float sl = 1.0f / (1.0f + expf(-yl[10]));
+    const float isum = __builtin_amdgcn_rcpf(sum);
+    h.pi[0] = e0 * isum;
+    h.pi[1] = e1 * isum;
+    const float sl = __builtin_amdgcn_rcpf(1.0f + fexp(-yl[10]));
     h.sig = __shfl(sl, lane & 15);
 }
 
@@ -259,13 +286,13 @@ __global__ __launch_bounds__(kT) void k_guide_infer(Net gnet, Box box, PathIn in
     __shared__ float Bs[5][64];
     const uint32_t n = *in.count;
     if ((uint64_t)blockIdx.x * 64u >= n) return;  // block-uniform: no tile for this block
-    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff[k];
+    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff0[k];
     for (int k = threadIdx.x; k < 5 * 64; k += kT) {
         const int l = k / 64, o = k % 64;
         Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
     }
     __syncthreads();
-    const Net net{Wf, nullptr, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
+    const Net net{Wf, Wf + kFrag0 * 64, nullptr, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 15, g = lane >> 4;
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * 64u < n; tile += gridDim.x) {
@@ -379,16 +406,33 @@ __global__ __launch_bounds__(kT) void k_guide_eval(Net net, Box box, const dr_gu
 }
 
 // ---- training: fused forward + backward, partial gradients per block -------
-constexpr int kLds = 72;                     // BF16 per LDS row: 64 samples + pad
-__host__ __device__ constexpr int x_row(int l) { return l == 0 ? 0 : 32 + 64 * (l - 1); }
-constexpr int kXRows = 32 + 4 * 64;
+// A block of 8 waves (two per SIMD) trains on tiles of 128 samples (16 per
+// wave). LDS holds all weight fragments, the biases and two 64-row x
+// 128-sample BF16 tiles each of a layer's input activations X and output
+// deltas D (138,528 B). The activations stay in registers from the forward
+// pass; the backward pass visits one layer per phase: the waves write X_L and
+// D_L into the phase's buffer, meet at a barrier, and accumulate their share
+// of dW_L = D_L X_L^T on the matrix cores while they compute the next delta.
+// Phases alternate buffers, so a buffer is rewritten only after every wave
+// has passed the next barrier, i.e. finished reading it: one barrier per
+// layer. Each wave keeps its dW tiles in registers across all tiles.
+constexpr int kTrainWaves = 8;
+constexpr int kTrainThreads = 64 * kTrainWaves;
+constexpr int kTileCols = 16 * kTrainWaves;  // samples per tile
+constexpr int kTilePitch = kTileCols + 8;    // BF16 per tile row: a 16-byte pad shifts each row's banks
+constexpr int kTileElems = 64 * kTilePitch;  // BF16 per X or D tile
+constexpr int kTileKs = kTileCols / 32;      // K steps of a dW product
+// element (row, sample) of a tile; with the row and the sample split into a
+// per-lane part and a compile-time part, every access is one lane base plus
+// an immediate offset
+__device__ __forceinline__ int tile_idx(int row, int sample) { return row * kTilePitch + sample; }
 // accumulator tiles per wave: layer l has out-blocks x (in-blocks + bias)
-// tiles dealt round-robin to the 4 waves
+// tiles dealt round-robin to the waves
 __host__ __device__ constexpr int n_tiles(int l) { return layer_ob(l) * (layer_in(l) / 16 + 1); }
-__host__ __device__ constexpr int n_wtiles(int l) { return (n_tiles(l) + 3) / 4; }
+__host__ __device__ constexpr int n_wtiles(int l) { return (n_tiles(l) + kTrainWaves - 1) / kTrainWaves; }
 __host__ __device__ constexpr int acc_base(int l) { return l == 4 ? 0 : acc_base(l + 1) + n_wtiles(l + 1); }
 constexpr int kAcc = acc_base(0) + n_wtiles(0);
-static_assert(kAcc == 22, "accumulator tiles per wave");
+static_assert(kAcc == 13, "accumulator tiles per wave");
 
 struct TrainIn {
     const dr_guide_sample *smp;
@@ -399,27 +443,29 @@ struct TrainIn {
     uint32_t sel_variance;  // selection logit: 0 = KL objective, 1 = second-moment objective
 };
 
+// The wave's share of dW_L += D_L X_L^T over the tile's samples (K).
 template <int L>
-__device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dbuf, const __bf16 *X, int wave, int lane)
+__device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dt, const __bf16 *Xt, int wave, int lane,
+                                         bf16x8 ones)
 {
     constexpr int IB = layer_in(L) / 16;
     constexpr int T = n_tiles(L);
     const int i = lane & 15, g = lane >> 4;
-    bf16x8 ones;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
 #pragma unroll
     for (int u = 0; u < n_wtiles(L); ++u) {
-        const int t = wave + 4 * u;
+        const int t = wave + kTrainWaves * u;
         if (t < T) {
             const int ob = t / (IB + 1), ib = t % (IB + 1);
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Dbuf + (ob * 16 + i) * kLds + 32 * ks + 8 * g);
-                const bf16x8 b = ib < IB
-                                     ? *reinterpret_cast<const bf16x8 *>(X + (x_row(L) + ib * 16 + i) * kLds + 32 * ks + 8 * g)
-                                     : ones;
-                acc[acc_base(L) + u] = mfma(a, b, acc[acc_base(L) + u]);
+            for (int ks = 0; ks < kTileKs; ++ks) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Dt + tile_idx(ob * 16 + i, 32 * ks + 8 * g));
+                // (separate products, so neither operand is copied into a shared register tuple)
+                if (ib < IB)
+                    acc[acc_base(L) + u] = mfma(
+                        a, *reinterpret_cast<const bf16x8 *>(Xt + tile_idx(ib * 16 + i, 32 * ks + 8 * g)),
+                        acc[acc_base(L) + u]);
+                else
+                    acc[acc_base(L) + u] = mfma(a, ones, acc[acc_base(L) + u]);
             }
         }
     }
@@ -430,80 +476,112 @@ __device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int
 {
     constexpr int IB = layer_in(L) / 16;
     constexpr int T = n_tiles(L);
+    constexpr int kW = w_off(L), kB = b_off(L);
     const int j = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int u = 0; u < n_wtiles(L); ++u) {
-        const int t = wave + 4 * u;
+        const int t = wave + kTrainWaves * u;
         if (t < T) {
             const int ob = t / (IB + 1), ib = t % (IB + 1);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = ob * 16 + 4 * g + r;
-                if (ib < IB) P[w_off(L) + row * layer_in(L) + ib * 16 + j] = acc[acc_base(L) + u][r];
-                else if (j == 0) P[b_off(L) + row] = acc[acc_base(L) + u][r];
+                if (ib < IB) P[kW + row * layer_in(L) + ib * 16 + j] = acc[acc_base(L) + u][r];
+                else if (j == 0) P[kB + row] = acc[acc_base(L) + u][r];
             }
         }
     }
 }
 
-// delta of hidden layer L (1..4) from the delta of layer L+1 (B operands in
-// dB): W_L^T dB, masked by the ReLU of activation L; written to Dout and
-// returned as the next B operands.
-template <int L>
-__device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&dB)[2], const bf16x8 (&xl)[2],
-                                            __bf16 *Dout, int col)
+// A hidden layer's values in B-operand order (element j of operand ks is
+// feature (2 ks + j / 4) * 16 + 4 g + j % 4 of the lane's sample) into a tile.
+__device__ __forceinline__ void store_hidden(__bf16 *t, const bf16x8 (&v)[2], int g, int col)
 {
-    const int g = lane >> 4;
-    bf16x8 nb[2];
 #pragma unroll
-    for (int ib = 0; ib < 4; ++ib) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) c = mfma(net.fb[((L - 1) * 8 + ib * 2 + ks) * 64 + lane], dB[ks], c);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = (ib % 2) * 4 + r;
-            const float xv = (float)xl[ib / 2][j];
-            const __bf16 d = (__bf16)(xv > 0.0f ? c[r] : 0.0f);
-            nb[ib / 2][j] = d;
-            Dout[(ib * 16 + 4 * g + r) * kLds + col] = d;
-        }
-    }
-    dB[0] = nb[0];
-    dB[1] = nb[1];
+        for (int j = 0; j < 8; ++j) t[tile_idx((2 * ks + j / 4) * 16 + 4 * g + j % 4, col)] = v[ks][j];
 }
 
-__global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn in)
+// Zero the BF16 elements of d whose ReLU activation x is not positive. x
+// comes from pack_relu, so its patterns are 16-bit integers in [0, 0x7FFF]:
+// 0 - x is negative exactly where x > 0, and its arithmetic shift by 15 is
+// then all ones. Written as two packed 16-bit instructions per pair (the
+// compiler otherwise turns the expression into per-element compares).
+__device__ __forceinline__ bf16x8 relu_mask(bf16x8 d, bf16x8 x)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 X[kXRows * kLds];
-    __shared__ __attribute__((aligned(16))) __bf16 D[2][64 * kLds];
-    __shared__ float red[kT / 64];
-    // the weights stay in LDS for the whole kernel: every tile reads all of
-    // them, and from L2 each of those reads was an exposed latency
+    u32x4 dw = __builtin_bit_cast(u32x4, d);
+    const u32x4 xw = __builtin_bit_cast(u32x4, x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        unsigned int m;
+        asm("v_pk_sub_u16 %0, %1, %2\n\tv_pk_ashrrev_i16 %0, %3, %0"
+            : "=&v"(m)
+            : "v"(0u), "v"(xw[k]), "v"(0x000F000Fu));
+        dw[k] &= m;
+    }
+    return __builtin_bit_cast(bf16x8, dw);
+}
+
+// delta of hidden layer L (1..4) from the delta of layer L+1 (B operands in
+// dB): W_L^T dB, masked by the ReLU of activation L; returned in dB as the
+// next B operands.
+template <int L>
+__device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&dB)[2], const bf16x8 (&xl)[2])
+{
+    f32x4 c[4];
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+        c[ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) c[ib] = mfma(net.fb[((L - 1) * 8 + ib * 2 + ks) * 64 + lane], dB[ks], c[ib]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) dB[h] = relu_mask(to_bf16x8(c[2 * h], c[2 * h + 1]), xl[h]);
+}
+
+// Fast reciprocal for the loss head (the gradient is compared with its
+// reference at BF16 tolerance; ~1 ulp).
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+__global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box, TrainIn in)
+{
     __shared__ bf16x8 Wf[kFragF * 64];
     __shared__ bf16x8 Wb[kFragB * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kTileElems];
+    __shared__ __attribute__((aligned(16))) __bf16 Dt[2][kTileElems];
     __shared__ float Bs[5][64];
-    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff[k];
-    for (int k = threadIdx.x; k < kFragB * 64; k += kT) Wb[k] = gnet.fb[k];
-    for (int k = threadIdx.x; k < 5 * 64; k += kT) {
+    __shared__ float red[kTrainWaves];
+    for (int k = threadIdx.x; k < kFragF * 64; k += kTrainThreads) Wf[k] = gnet.ff0[k];
+    for (int k = threadIdx.x; k < kFragB * 64; k += kTrainThreads) Wb[k] = gnet.fb[k];
+    for (int k = threadIdx.x; k < 5 * 64; k += kTrainThreads) {
         const int l = k / 64, o = k % 64;
         Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
     }
     __syncthreads();
-    const Net net{Wf, Wb, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const Net net{Wf, Wf + kFrag0 * 64, Wb, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
+    const int lane = threadIdx.x & 63;
+    // wave-uniform, so each wave's share of the dW tiles is chosen by scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int s = lane & 15, g = lane >> 4;
     const int col = wave * 16 + s;
     const uint32_t avail = in.count ? min(*in.count, in.cap) : in.n;
     const uint32_t n = in.batch ? min(avail, in.batch) : avail;
     const uint32_t start = (in.batch && avail) ? (uint32_t)(((uint64_t)in.window * in.batch) % avail) : 0u;
-    const uint32_t ntiles = (n + 63u) / 64u;
+    const uint32_t ntiles = (n + (kTileCols - 1)) / kTileCols;
     f32x4 acc[kAcc];
 #pragma unroll
     for (int k = 0; k < kAcc; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f;
+    int ph = 0;  // backward phase counter: phase ph uses X / D buffer ph & 1
+    // BF16 ones, the B operand of the bias-gradient products; the empty asm
+    // hides the value, so the compiler keeps it in registers instead of
+    // rematerialising the constant before every product
+    u32x4 ones_w = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};
+    asm volatile("" : "+v"(ones_w));
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, ones_w);
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint32_t i = tile * 64u + (uint32_t)col;
+        const uint32_t i = tile * (uint32_t)kTileCols + (uint32_t)col;
         V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f}, wi{0.0f, 0.0f, 1.0f};
         float w = 0.0f, ps = 0.0f;
         if (i < n) {
@@ -518,88 +596,112 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn i
         }
         if (g == 0) wsum += w;
         const bf16x8 x0 = encode8(g, p, nn, wo, box);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) X[(8 * g + j) * kLds + col] = x0[j];
         bf16x8 xh[4][2];
         f32x4 y[3];
         net_forward(net, lane, x0, xh, y);
-#pragma unroll
-        for (int l = 0; l < 4; ++l)
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    X[(x_row(l + 1) + (2 * ks + j / 4) * 16 + 4 * g + j % 4) * kLds + col] = xh[l][ks][j];
 
         // head: loss w * [-log(q/2 + cos+/(2 pi)) - log(alpha q' + (1 - alpha) cos+/pi)]
         HeadLane h;
         head_decode(y, lane, h);
         const float alpha = kAlphaLo + kAlphaSpan * h.sig;
         float f[2], c[2];
+        float em[2];  // e^(2 kappa) - 1 per lobe
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
+            const float k = h.kap[t];
             c[t] = h.mu[t][0] * wi.x + h.mu[t][1] * wi.y + h.mu[t][2] * wi.z;
-            f[t] = vmf_density(h.kap[t], c[t]);
+            // vMF density k / (2 pi (1 - e^-2k)) e^(k (c - 1))
+            em[t] = fexp(2.0f * k) - 1.0f;
+            f[t] = fdiv(k * fexp(k * (c[t] - 1.0f)), kTwoPi * (1.0f - fexp(-2.0f * k)));
         }
         float q = h.pi[0] * f[0] + h.pi[1] * f[1];
         q += __shfl_xor(q, 16);
         q += __shfl_xor(q, 32);
         const float cb = dot(nn, wi);
-        const float b = cb > 0.0f ? cb / kPi : 0.0f;
+        const float b = cb > 0.0f ? cb * (1.0f / kPi) : 0.0f;
         // lobes: KL of the fixed half-and-half proposal; selection: KL of the
         // learned one-sample MIS pdf with the mixture held fixed
         const float pm = alpha * q + (1.0f - alpha) * b;
-        const float G = w != 0.0f ? -w / pm : 0.0f;
-        const float dq = w != 0.0f ? -w / (kLobeShare * q + (1.0f - kLobeShare) * b) * kLobeShare : 0.0f;
+        const float G = w != 0.0f ? -fdiv(w, pm) : 0.0f;
+        const float dq = w != 0.0f ? -fdiv(w, kLobeShare * q + (1.0f - kLobeShare) * b) * kLobeShare : 0.0f;
         float dy[12];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int q0 = 5 * t;
             const float tt = h.pi[t] * f[t];
             const float k = h.kap[t];
+            const float ir = __builtin_amdgcn_rcpf(h.r[t]);
             dy[q0 + 4] = dq * (tt - h.pi[t] * q);
             dy[q0 + 3] = (h.kr[t] > kLogKappaLo && h.kr[t] < kLogKappaHi)
-                             ? dq * tt * (1.0f - 2.0f * k / expm1f(2.0f * k) + k * (c[t] - 1.0f))
+                             ? dq * tt * (1.0f - fdiv(2.0f * k, em[t]) + k * (c[t] - 1.0f))
                              : 0.0f;
             const float gx = dq * tt * k * wi.x, gy = dq * tt * k * wi.y, gz = dq * tt * k * wi.z;
             const float mg = h.mu[t][0] * gx + h.mu[t][1] * gy + h.mu[t][2] * gz;
-            dy[q0] = (gx - h.mu[t][0] * mg) / h.r[t];
-            dy[q0 + 1] = (gy - h.mu[t][1] * mg) / h.r[t];
-            dy[q0 + 2] = (gz - h.mu[t][2] * mg) / h.r[t];
+            dy[q0] = (gx - h.mu[t][0] * mg) * ir;
+            dy[q0 + 1] = (gy - h.mu[t][1] * mg) * ir;
+            dy[q0 + 2] = (gz - h.mu[t][2] * mg) * ir;
         }
         // second-moment objective sum_i w_i^2 p_i / pm (Rath et al. 2020,
         // "Variance-aware path guiding"): the KL gradient scaled by w p / pm,
         // the sample's weight under the current mixture; it trains the
         // selection row of the output layer only (no delta into the hidden
         // layers, whose features stay the lobes')
-        const float rs = in.sel_variance ? (pm > 0.0f ? w * ps / pm : 0.0f) : 1.0f;
+        const float rs = in.sel_variance ? (pm > 0.0f ? fdiv(w * ps, pm) : 0.0f) : 1.0f;
         dy[10] = g == 0 ? G * rs * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
         dy[11] = 0.0f;
-        bf16x8 dB[2];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            dB[0][j] = (__bf16)dy[j];
-            dB[1][j] = (__bf16)(j < 4 && !(in.sel_variance && j == 2) ? dy[8 + j] : 0.0f);
-        }
-#pragma unroll
-        for (int qq = 0; qq < 12; ++qq) D[0][((qq / 4) * 16 + 4 * g + qq % 4) * kLds + col] = (__bf16)dy[qq];
-        __syncthreads();
+        const float d10 = in.sel_variance ? 0.0f : dy[10];
+        bf16x8 dB[2] = {
+            __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[0], dy[1]), pk_bf16(dy[2], dy[3]), pk_bf16(dy[4], dy[5]),
+                                               pk_bf16(dy[6], dy[7])}),
+            __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[8], dy[9]), pk_bf16(d10, dy[11]), 0u, 0u})};
 
-        // backward, one layer per phase; D[0] / D[1] alternate
-        dw_layer<4>(acc, D[0], X, wave, lane);
-        delta_layer<4>(net, lane, dB, xh[3], D[1], col);
-        __syncthreads();
-        dw_layer<3>(acc, D[1], X, wave, lane);
-        delta_layer<3>(net, lane, dB, xh[2], D[0], col);
-        __syncthreads();
-        dw_layer<2>(acc, D[0], X, wave, lane);
-        delta_layer<2>(net, lane, dB, xh[1], D[1], col);
-        __syncthreads();
-        dw_layer<1>(acc, D[1], X, wave, lane);
-        delta_layer<1>(net, lane, dB, xh[0], D[0], col);
-        __syncthreads();
-        dw_layer<0>(acc, D[0], X, wave, lane);
-        __syncthreads();
+        // backward, one layer per phase through alternating X / D buffers
+        {
+            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+#pragma unroll
+            for (int qq = 0; qq < 12; ++qq) D[tile_idx((qq / 4) * 16 + 4 * g + qq % 4, col)] = (__bf16)dy[qq];
+            store_hidden(X, xh[3], g, col);
+            __syncthreads();
+            dw_layer<4>(acc, D, X, wave, lane, ones);
+            delta_layer<4>(net, lane, dB, xh[3]);
+            ++ph;
+        }
+        {
+            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+            store_hidden(D, dB, g, col);
+            store_hidden(X, xh[2], g, col);
+            __syncthreads();
+            dw_layer<3>(acc, D, X, wave, lane, ones);
+            delta_layer<3>(net, lane, dB, xh[2]);
+            ++ph;
+        }
+        {
+            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+            store_hidden(D, dB, g, col);
+            store_hidden(X, xh[1], g, col);
+            __syncthreads();
+            dw_layer<2>(acc, D, X, wave, lane, ones);
+            delta_layer<2>(net, lane, dB, xh[1]);
+            ++ph;
+        }
+        {
+            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+            store_hidden(D, dB, g, col);
+            store_hidden(X, xh[0], g, col);
+            __syncthreads();
+            dw_layer<1>(acc, D, X, wave, lane, ones);
+            delta_layer<1>(net, lane, dB, xh[0]);
+            ++ph;
+        }
+        {
+            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+            store_hidden(D, dB, g, col);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) X[tile_idx(8 * g + j, col)] = x0[j];
+            __syncthreads();
+            dw_layer<0>(acc, D, X, wave, lane, ones);
+            ++ph;
+        }
     }
     float *P = in.partials + (size_t)blockIdx.x * (kNP + 1);
     dw_store<4>(acc, P, wave, lane);
@@ -611,7 +713,12 @@ __global__ __launch_bounds__(kT) void k_guide_train(Net gnet, Box box, TrainIn i
     for (int o = 32; o >= 1; o >>= 1) wsum += __shfl_xor(wsum, o);
     if (lane == 0) red[wave] = wsum;
     __syncthreads();
-    if (threadIdx.x == 0) P[kNP] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kTrainWaves; ++k) t += red[k];
+        P[kNP] = t;
+    }
 }
 
 __device__ __forceinline__ void pack_param(int p, float w, __bf16 *ff, __bf16 *fb)
@@ -752,7 +859,8 @@ inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 Net net_of(const Guide *g)
 {
     const float *m = g->master.as<const float>();
-    return Net{g->frag_f.as<const bf16x8>(), g->frag_b.as<const bf16x8>(),
+    const bf16x8 *ff = g->frag_f.as<const bf16x8>();
+    return Net{ff, ff + kFrag0 * 64, g->frag_b.as<const bf16x8>(),
                {m + b_off(0), m + b_off(1), m + b_off(2), m + b_off(3), m + b_off(4)}};
 }
 
@@ -797,9 +905,9 @@ dr_status run_train(Guide *g, const dr_guide_sample *smp, const uint32_t *count,
                     float *grad, hipStream_t stream, uint32_t batch = 0)
 {
     const uint32_t n_max = batch ? std::min(batch, count ? cap : n) : (count ? cap : n);
-    const int nblk = (int)std::max<uint32_t>(1u, std::min<uint32_t>(kGuideTrainBlocks, cdiv(n_max, 64)));
+    const int nblk = (int)std::max<uint32_t>(1u, std::min<uint32_t>(kGuideTrainBlocks, cdiv(n_max, kTileCols)));
     TrainIn in{smp, count, n, cap, batch, g->window++, g->partials.as<float>(), g->sel_variance};
-    k_guide_train<<<nblk, kT, 0, stream>>>(net_of(g), box_of(g), in);
+    k_guide_train<<<nblk, kTrainThreads, 0, stream>>>(net_of(g), box_of(g), in);
     DR_HIP_TRY(hipGetLastError());
     AdamIn ad{};
     if (!grad) {

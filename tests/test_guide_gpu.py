@@ -115,6 +115,23 @@ def test_gradient_matches_oracle(guide):
     assert np.array_equal(g, g2)
 
 
+def test_gradient_is_additive_over_tiles(guide):
+    """A batch of more than 256 tiles makes each training block loop over
+    several tiles; its gradient equals the sum of the gradients of 4096-sample
+    parts (one tile per block) up to FP32 summation order."""
+    rng = np.random.default_rng(26)
+    guide.set_params(random_params(rng))
+    s = random_samples(rng, 2 * 32768 + 777)
+    s["weight"][::11] = 0.0
+    g, sw = guide.gradient(s)
+    parts = [guide.gradient(s[k:k + 4096]) for k in range(0, len(s), 4096)]
+    gs = np.sum([q[0] for q in parts], axis=0, dtype=np.float64)
+    sws = float(sum(q[1] for q in parts))
+    assert abs(sw - sws) <= 1e-4 * sws
+    err = np.linalg.norm(g - gs) / np.linalg.norm(gs)
+    assert err < 1e-4, err
+
+
 def test_second_moment_selection_gradient_matches_oracle(guide):
     """DR_GUIDE_SELECT_VARIANCE: the device gradient equals the oracle's
     second-moment objective (selection row trained alone) layer by layer."""
