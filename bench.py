@@ -421,6 +421,7 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
     return {"resolution": [width, height], "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
             "spp_per_pass": spp_per_pass, "plain_spp": spp_per_pass * passes, "train_steps_per_pass": train_steps,
             "train_passes": train_passes, "train_batch": train_batch, "seeds": seeds,
+            "train_samples_per_step": train_batch or capacity, "lr": 1e-2,
             "lr_schedule": "1e-2 / (1 + pass / 4)" if lr_schedule is not None else "constant",
             "selection_objective": "second moment" if selection == dr.DR_GUIDE_SELECT_VARIANCE else "KL",
             "guided_over_plain_mse_equal_spp": round(guided[n_ck - 1][1] / target, 3),
@@ -689,11 +690,16 @@ def main() -> int:
         log("[bench] guide: training step throughput")
         guide["train_step"] = guide_train_rate(ctx)
         log("[bench] guide: indirect-lit partition scene (2,048,000 sphere tris), guided vs plain")
-        # Adam from lr 1e-2, decayed harmonically over the passes: on this scene
-        # a constant 1e-2 left the seed-averaged MSE at plain's (64 seeds)
+        # the stated configuration: every Adam step on the whole 2^18-sample
+        # training ring, constant lr 1e-2; the reference is the plain device
+        # path at 65,536 spp, the path that matches the oracle on this scene
+        # (at least 99% of pixels bit-identical at 160x90x16, equal means)
         guide["partition_scene"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
-                                                   8, 64, 8, 2, 12, train_batch=1 << 16, seeds=64,
-                                                   lr_schedule=lambda k: 1e-2 / (1.0 + k / 4.0))
+                                                   8, 64, 8, 2, 12, train_batch=0, seeds=64)
+        log("[bench] guide: partition scene, tuned training (2^16-sample windows, lr 1e-2 / (1 + pass / 4))")
+        guide["partition_scene_tuned"] = guided_vs_plain(ctx, scenes.many_lights(partition=True), 240, 135, 65536,
+                                                         8, 64, 8, 2, 12, train_batch=1 << 16, seeds=64,
+                                                         lr_schedule=lambda k: 1e-2 / (1.0 + k / 4.0))
         log("[bench] guide: hidden-light reflector scene, guided vs plain")
         guide["hidden_light_scene"] = guided_vs_plain(ctx, scenes.hidden_light_reflector(), 512, 512, 16384,
                                                       16, 64, 8, 16, 8, capacity=1 << 16, train_passes=8)

@@ -184,32 +184,76 @@ __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &b
 // Forward pass of one wave's 16 samples. xh[l] = hidden activation l+1 as
 // B operands (kept for the backward masks); y = output tiles (rows ob*16 +
 // 4g + r of the lane's sample).
-__device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0, bf16x8 (&xh)[4][2], f32x4 (&y)[3])
+// Forward pass of NG groups of 16 samples in one wave; each weight fragment
+// is loaded once and applied to every group. xh[q][l] = hidden activation
+// l+1 of group q as B operands (kept for the backward masks); y[q] = output
+// tiles (rows ob*16 + 4g + r of the lane's sample).
+template <int NG>
+__device__ __forceinline__ void net_forward_g(const Net &net, int lane, const bf16x8 (&x0)[NG],
+                                              bf16x8 (&xh)[NG][4][2], f32x4 (&y)[NG][3])
 {
     const int g = lane >> 4;
-    f32x4 acc[4];
+    f32x4 acc[NG][4];
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(net.ff0[ob * 64 + lane], x0, bias4(net, 0, ob, g));
-    xh[0][0] = pack_relu(acc[0], acc[1]);
-    xh[0][1] = pack_relu(acc[2], acc[3]);
+    for (int ob = 0; ob < 4; ++ob) {
+        const bf16x8 a = net.ff0[ob * 64 + lane];
+        const f32x4 b0 = bias4(net, 0, ob, g);
+#pragma unroll
+        for (int q = 0; q < NG; ++q) acc[q][ob] = mfma(a, x0[q], b0);
+    }
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        xh[q][0][0] = pack_relu(acc[q][0], acc[q][1]);
+        xh[q][0][1] = pack_relu(acc[q][2], acc[q][3]);
+    }
 #pragma unroll
     for (int l = 1; l < 4; ++l) {
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) {
-            acc[ob] = bias4(net, l, ob, g);
+            const f32x4 b0 = bias4(net, l, ob, g);
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks)
-                acc[ob] = mfma(net.ff[(f_base(l) - kFrag0 + ob * 2 + ks) * 64 + lane], xh[l - 1][ks], acc[ob]);
+            for (int q = 0; q < NG; ++q) acc[q][ob] = b0;
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = net.ff[(f_base(l) - kFrag0 + ob * 2 + ks) * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < NG; ++q) acc[q][ob] = mfma(a, xh[q][l - 1][ks], acc[q][ob]);
+            }
         }
-        xh[l][0] = pack_relu(acc[0], acc[1]);
-        xh[l][1] = pack_relu(acc[2], acc[3]);
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            xh[q][l][0] = pack_relu(acc[q][0], acc[q][1]);
+            xh[q][l][1] = pack_relu(acc[q][2], acc[q][3]);
+        }
     }
 #pragma unroll
     for (int ob = 0; ob < 3; ++ob) {
-        y[ob] = bias4(net, 4, ob, g);
+        const f32x4 b0 = bias4(net, 4, ob, g);
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) y[ob] = mfma(net.ff[(f_base(4) - kFrag0 + ob * 2 + ks) * 64 + lane], xh[3][ks], y[ob]);
+        for (int q = 0; q < NG; ++q) y[q][ob] = b0;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 a = net.ff[(f_base(4) - kFrag0 + ob * 2 + ks) * 64 + lane];
+#pragma unroll
+            for (int q = 0; q < NG; ++q) y[q][ob] = mfma(a, xh[q][3][ks], y[q][ob]);
+        }
     }
+}
+
+// One group of 16 samples (inference and evaluation).
+__device__ __forceinline__ void net_forward(const Net &net, int lane, bf16x8 x0, bf16x8 (&xh)[4][2], f32x4 (&y)[3])
+{
+    const bf16x8 xs[1] = {x0};
+    bf16x8 xq[1][4][2];
+    f32x4 yq[1][3];
+    net_forward_g<1>(net, lane, xs, xq, yq);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        xh[l][0] = xq[0][l][0];
+        xh[l][1] = xq[0][l][1];
+    }
+#pragma unroll
+    for (int ob = 0; ob < 3; ++ob) y[ob] = yq[0][ob];
 }
 
 // Decoded lobes 2g, 2g+1 of the lane's sample (local output slot q = 4 ob + r:
@@ -406,33 +450,37 @@ __global__ __launch_bounds__(kT) void k_guide_eval(Net net, Box box, const dr_gu
 }
 
 // ---- training: fused forward + backward, partial gradients per block -------
-// A block of 8 waves (two per SIMD) trains on tiles of 128 samples (16 per
-// wave). LDS holds all weight fragments, the biases and two 64-row x
-// 128-sample BF16 tiles each of a layer's input activations X and output
-// deltas D (138,528 B). The activations stay in registers from the forward
-// pass; the backward pass visits one layer per phase: the waves write X_L and
-// D_L into the phase's buffer, meet at a barrier, and accumulate their share
-// of dW_L = D_L X_L^T on the matrix cores while they compute the next delta.
-// Phases alternate buffers, so a buffer is rewritten only after every wave
-// has passed the next barrier, i.e. finished reading it: one barrier per
-// layer. Each wave keeps its dW tiles in registers across all tiles.
+// A block of 8 waves (two per SIMD) trains on tiles of 256 samples. Each wave
+// takes two groups of 16 samples and applies every weight fragment it loads
+// to both: half the fragment reads per sample, and two independent chains of
+// matrix products to cover each other's latency. LDS holds all weight
+// fragments, the biases and one 64-row x 256-sample BF16 tile each of a
+// layer's input activations X and output deltas D (136,480 B). The
+// activations stay in registers from the forward pass; the backward pass
+// visits one layer per phase: the waves write X_L and D_L, meet at a barrier,
+// accumulate their share of dW_L = D_L X_L^T on the matrix cores while they
+// compute the next delta, and meet again before the next phase rewrites the
+// tiles. Each wave keeps its dW tiles in registers across all tiles.
 constexpr int kTrainWaves = 8;
 constexpr int kTrainThreads = 64 * kTrainWaves;
-constexpr int kTileCols = 16 * kTrainWaves;  // samples per tile
-constexpr int kTilePitch = kTileCols + 8;    // BF16 per tile row: a 16-byte pad shifts each row's banks
-constexpr int kTileElems = 64 * kTilePitch;  // BF16 per X or D tile
-constexpr int kTileKs = kTileCols / 32;      // K steps of a dW product
-// element (row, sample) of a tile; with the row and the sample split into a
-// per-lane part and a compile-time part, every access is one lane base plus
-// an immediate offset
+constexpr int kGroups = 2;                             // 16-sample groups per wave
+constexpr int kTileCols = 16 * kGroups * kTrainWaves;  // samples per tile
+constexpr int kTilePitch = kTileCols + 8;              // BF16 per tile row: a 16-byte pad shifts each row's banks
+constexpr int kTileElems = 64 * kTilePitch;            // BF16 per X or D tile
+constexpr int kTileKs = kTileCols / 32;                // K steps of a dW product
+// element (row, sample) of a tile; every access is one lane base plus an
+// immediate offset
 __device__ __forceinline__ int tile_idx(int row, int sample) { return row * kTilePitch + sample; }
-// accumulator tiles per wave: layer l has out-blocks x (in-blocks + bias)
-// tiles dealt round-robin to the waves
-__host__ __device__ constexpr int n_tiles(int l) { return layer_ob(l) * (layer_in(l) / 16 + 1); }
-__host__ __device__ constexpr int n_wtiles(int l) { return (n_tiles(l) + kTrainWaves - 1) / kTrainWaves; }
-__host__ __device__ constexpr int acc_base(int l) { return l == 4 ? 0 : acc_base(l + 1) + n_wtiles(l + 1); }
-constexpr int kAcc = acc_base(0) + n_wtiles(0);
-static_assert(kAcc == 13, "accumulator tiles per wave");
+
+// dW tiles of a wave: out block ob = wave % 4 (layer 4 has three, so waves 3
+// and 7 take none there); in blocks 0 .. IB/2 - 1 plus the bias tile for the
+// first four waves, IB/2 .. IB - 1 for the others. The wave reads its D rows
+// once per K step for all its tiles.
+__host__ __device__ constexpr int n_in_half(int l) { return layer_in(l) / 32; }       // in blocks per wave
+__host__ __device__ constexpr int n_slots(int l) { return n_in_half(l) + 1; }         // with the bias tile
+__host__ __device__ constexpr int acc_base(int l) { return l == 4 ? 0 : acc_base(l + 1) + n_slots(l + 1); }
+constexpr int kAcc = acc_base(0) + n_slots(0);
+static_assert(kAcc == 14, "accumulator tiles per wave");
 
 struct TrainIn {
     const dr_guide_sample *smp;
@@ -445,51 +493,38 @@ struct TrainIn {
 
 // The wave's share of dW_L += D_L X_L^T over the tile's samples (K).
 template <int L>
-__device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dt, const __bf16 *Xt, int wave, int lane,
-                                         bf16x8 ones)
+__device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dt, const __bf16 *Xt, int ob, int half,
+                                         int lane, bf16x8 ones)
 {
-    constexpr int IB = layer_in(L) / 16;
-    constexpr int T = n_tiles(L);
+    constexpr int NI = n_in_half(L);
+    if (ob >= layer_ob(L)) return;  // wave-uniform
     const int i = lane & 15, g = lane >> 4;
+    const __bf16 *dp = Dt + tile_idx(ob * 16 + i, 8 * g);
+    const __bf16 *xp = Xt + tile_idx(half * NI * 16 + i, 8 * g);
 #pragma unroll
-    for (int u = 0; u < n_wtiles(L); ++u) {
-        const int t = wave + kTrainWaves * u;
-        if (t < T) {
-            const int ob = t / (IB + 1), ib = t % (IB + 1);
+    for (int ks = 0; ks < kTileKs; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dp + 32 * ks);
 #pragma unroll
-            for (int ks = 0; ks < kTileKs; ++ks) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Dt + tile_idx(ob * 16 + i, 32 * ks + 8 * g));
-                // (separate products, so neither operand is copied into a shared register tuple)
-                if (ib < IB)
-                    acc[acc_base(L) + u] = mfma(
-                        a, *reinterpret_cast<const bf16x8 *>(Xt + tile_idx(ib * 16 + i, 32 * ks + 8 * g)),
-                        acc[acc_base(L) + u]);
-                else
-                    acc[acc_base(L) + u] = mfma(a, ones, acc[acc_base(L) + u]);
-            }
-        }
+        for (int u = 0; u < NI; ++u)
+            acc[acc_base(L) + u] = mfma(a, *reinterpret_cast<const bf16x8 *>(xp + u * 16 * kTilePitch + 32 * ks),
+                                        acc[acc_base(L) + u]);
+        if (half == 0) acc[acc_base(L) + NI] = mfma(a, ones, acc[acc_base(L) + NI]);
     }
 }
 
 template <int L>
-__device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int wave, int lane)
+__device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int ob, int half, int lane)
 {
-    constexpr int IB = layer_in(L) / 16;
-    constexpr int T = n_tiles(L);
+    constexpr int NI = n_in_half(L);
     constexpr int kW = w_off(L), kB = b_off(L);
+    if (ob >= layer_ob(L)) return;
     const int j = lane & 15, g = lane >> 4;
 #pragma unroll
-    for (int u = 0; u < n_wtiles(L); ++u) {
-        const int t = wave + kTrainWaves * u;
-        if (t < T) {
-            const int ob = t / (IB + 1), ib = t % (IB + 1);
+    for (int r = 0; r < 4; ++r) {
+        const int row = ob * 16 + 4 * g + r;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = ob * 16 + 4 * g + r;
-                if (ib < IB) P[kW + row * layer_in(L) + ib * 16 + j] = acc[acc_base(L) + u][r];
-                else if (j == 0) P[kB + row] = acc[acc_base(L) + u][r];
-            }
-        }
+        for (int u = 0; u < NI; ++u) P[kW + row * layer_in(L) + (half * NI + u) * 16 + j] = acc[acc_base(L) + u][r];
+        if (half == 0 && j == 0) P[kB + row] = acc[acc_base(L) + NI][r];
     }
 }
 
@@ -523,33 +558,101 @@ __device__ __forceinline__ bf16x8 relu_mask(bf16x8 d, bf16x8 x)
     return __builtin_bit_cast(bf16x8, dw);
 }
 
-// delta of hidden layer L (1..4) from the delta of layer L+1 (B operands in
-// dB): W_L^T dB, masked by the ReLU of activation L; returned in dB as the
-// next B operands.
+// deltas of hidden layer L (1..4) of both groups from the deltas of layer L+1
+// (B operands in dB): W_L^T dB, masked by the ReLU of activation L; returned
+// in dB as the next B operands. Each transposed fragment is loaded once.
 template <int L>
-__device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&dB)[2], const bf16x8 (&xl)[2])
+__device__ __forceinline__ void delta_layer(const Net &net, int lane, bf16x8 (&dB)[kGroups][2],
+                                            const bf16x8 (&xh)[kGroups][4][2])
 {
-    f32x4 c[4];
+    f32x4 c[kGroups][4];
 #pragma unroll
     for (int ib = 0; ib < 4; ++ib) {
-        c[ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) c[ib] = mfma(net.fb[((L - 1) * 8 + ib * 2 + ks) * 64 + lane], dB[ks], c[ib]);
+        for (int q = 0; q < kGroups; ++q) c[q][ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 a = net.fb[((L - 1) * 8 + ib * 2 + ks) * 64 + lane];
+#pragma unroll
+            for (int q = 0; q < kGroups; ++q) c[q][ib] = mfma(a, dB[q][ks], c[q][ib]);
+        }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) dB[h] = relu_mask(to_bf16x8(c[2 * h], c[2 * h + 1]), xl[h]);
+    for (int q = 0; q < kGroups; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) dB[q][h] = relu_mask(to_bf16x8(c[q][2 * h], c[q][2 * h + 1]), xh[q][L - 1][h]);
 }
 
 // Fast reciprocal for the loss head (the gradient is compared with its
 // reference at BF16 tolerance; ~1 ulp).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// Loss gradient at the network's outputs for the lane's sample of one group:
+// dy (slot q = 4 ob + r of output block ob) and the output layer's deltas as
+// B operands.
+// loss w * [-log(q/2 + cos+/(2 pi)) - log(alpha q' + (1 - alpha) cos+/pi)]
+__device__ __forceinline__ void train_head(const f32x4 (&y)[3], int lane, V3 nn, V3 wi, float w, float ps,
+                                           uint32_t sel_variance, float (&dy)[12], bf16x8 (&dB)[2])
+{
+    const int g = lane >> 4;
+    HeadLane h;
+    head_decode(y, lane, h);
+    const float alpha = kAlphaLo + kAlphaSpan * h.sig;
+    float f[2], c[2], em[2];  // em: e^(2 kappa) - 1 per lobe
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const float k = h.kap[t];
+        c[t] = h.mu[t][0] * wi.x + h.mu[t][1] * wi.y + h.mu[t][2] * wi.z;
+        // vMF density k / (2 pi (1 - e^-2k)) e^(k (c - 1))
+        em[t] = fexp(2.0f * k) - 1.0f;
+        f[t] = fdiv(k * fexp(k * (c[t] - 1.0f)), kTwoPi * (1.0f - fexp(-2.0f * k)));
+    }
+    float q = h.pi[0] * f[0] + h.pi[1] * f[1];
+    q += __shfl_xor(q, 16);
+    q += __shfl_xor(q, 32);
+    const float cb = dot(nn, wi);
+    const float b = cb > 0.0f ? cb * (1.0f / kPi) : 0.0f;
+    // lobes: KL of the fixed half-and-half proposal; selection: KL of the
+    // learned one-sample MIS pdf with the mixture held fixed
+    const float pm = alpha * q + (1.0f - alpha) * b;
+    const float G = w != 0.0f ? -fdiv(w, pm) : 0.0f;
+    const float dq = w != 0.0f ? -fdiv(w, kLobeShare * q + (1.0f - kLobeShare) * b) * kLobeShare : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int q0 = 5 * t;
+        const float tt = h.pi[t] * f[t];
+        const float k = h.kap[t];
+        const float ir = __builtin_amdgcn_rcpf(h.r[t]);
+        dy[q0 + 4] = dq * (tt - h.pi[t] * q);
+        dy[q0 + 3] = (h.kr[t] > kLogKappaLo && h.kr[t] < kLogKappaHi)
+                         ? dq * tt * (1.0f - fdiv(2.0f * k, em[t]) + k * (c[t] - 1.0f))
+                         : 0.0f;
+        const float gx = dq * tt * k * wi.x, gy = dq * tt * k * wi.y, gz = dq * tt * k * wi.z;
+        const float mg = h.mu[t][0] * gx + h.mu[t][1] * gy + h.mu[t][2] * gz;
+        dy[q0] = (gx - h.mu[t][0] * mg) * ir;
+        dy[q0 + 1] = (gy - h.mu[t][1] * mg) * ir;
+        dy[q0 + 2] = (gz - h.mu[t][2] * mg) * ir;
+    }
+    // second-moment objective sum_i w_i^2 p_i / pm (Rath et al. 2020,
+    // "Variance-aware path guiding"): the KL gradient scaled by w p / pm, the
+    // sample's weight under the current mixture; it trains the selection row
+    // of the output layer only (no delta into the hidden layers, whose
+    // features stay the lobes')
+    const float rs = sel_variance ? (pm > 0.0f ? fdiv(w * ps, pm) : 0.0f) : 1.0f;
+    dy[10] = g == 0 ? G * rs * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
+    dy[11] = 0.0f;
+    const float d10 = sel_variance ? 0.0f : dy[10];
+    dB[0] = __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[0], dy[1]), pk_bf16(dy[2], dy[3]), pk_bf16(dy[4], dy[5]),
+                                               pk_bf16(dy[6], dy[7])});
+    dB[1] = __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[8], dy[9]), pk_bf16(d10, dy[11]), 0u, 0u});
+}
+
 __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box, TrainIn in)
 {
     __shared__ bf16x8 Wf[kFragF * 64];
     __shared__ bf16x8 Wb[kFragB * 64];
-    __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kTileElems];
-    __shared__ __attribute__((aligned(16))) __bf16 Dt[2][kTileElems];
+    __shared__ __attribute__((aligned(16))) __bf16 Xt[kTileElems];
+    __shared__ __attribute__((aligned(16))) __bf16 Dt[kTileElems];
     __shared__ float Bs[5][64];
     __shared__ float red[kTrainWaves];
     for (int k = threadIdx.x; k < kFragF * 64; k += kTrainThreads) Wf[k] = gnet.ff0[k];
@@ -561,10 +664,12 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     __syncthreads();
     const Net net{Wf, Wf + kFrag0 * 64, Wb, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
     const int lane = threadIdx.x & 63;
-    // wave-uniform, so each wave's share of the dW tiles is chosen by scalar branches
+    // wave-uniform, so the wave's dW tiles are chosen by scalar branches
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // waves go to the SIMDs round-robin: out block = wave % 4, so each SIMD
+    // runs one wave of each half (the first half also takes the bias tiles)
+    const int ob = wave & 3, half = wave >> 2;
     const int s = lane & 15, g = lane >> 4;
-    const int col = wave * 16 + s;
     const uint32_t avail = in.count ? min(*in.count, in.cap) : in.n;
     const uint32_t n = in.batch ? min(avail, in.batch) : avail;
     const uint32_t start = (in.batch && avail) ? (uint32_t)(((uint64_t)in.window * in.batch) % avail) : 0u;
@@ -572,143 +677,104 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     f32x4 acc[kAcc];
 #pragma unroll
     for (int k = 0; k < kAcc; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float wsum = 0.0f;
-    int ph = 0;  // backward phase counter: phase ph uses X / D buffer ph & 1
     // BF16 ones, the B operand of the bias-gradient products; the empty asm
     // hides the value, so the compiler keeps it in registers instead of
     // rematerialising the constant before every product
     u32x4 ones_w = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};
     asm volatile("" : "+v"(ones_w));
     const bf16x8 ones = __builtin_bit_cast(bf16x8, ones_w);
+    float wsum = 0.0f;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint32_t i = tile * (uint32_t)kTileCols + (uint32_t)col;
-        V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f}, wi{0.0f, 0.0f, 1.0f};
-        float w = 0.0f, ps = 0.0f;
-        if (i < n) {
-            const uint32_t src = start + i < avail ? start + i : start + i - avail;
-            const dr_guide_sample q = in.smp[src];
-            p = V3{q.pos[0], q.pos[1], q.pos[2]};
-            nn = V3{q.normal[0], q.normal[1], q.normal[2]};
-            wo = V3{q.wo[0], q.wo[1], q.wo[2]};
-            wi = V3{q.wi[0], q.wi[1], q.wi[2]};
-            w = q.weight;
-            ps = q.pdf;
+        int col[kGroups];
+        bf16x8 x0[kGroups];
+        V3 nn[kGroups], wi[kGroups];
+        float w[kGroups], ps[kGroups];
+#pragma unroll
+        for (int q = 0; q < kGroups; ++q) {
+            col[q] = (wave * kGroups + q) * 16 + s;
+            const uint32_t i = tile * (uint32_t)kTileCols + (uint32_t)col[q];
+            V3 p{0.0f, 0.0f, 0.0f}, wo{0.0f, 0.0f, 1.0f};
+            nn[q] = V3{0.0f, 0.0f, 1.0f};
+            wi[q] = V3{0.0f, 0.0f, 1.0f};
+            w[q] = 0.0f;
+            ps[q] = 0.0f;
+            if (i < n) {
+                const uint32_t src = start + i < avail ? start + i : start + i - avail;
+                const dr_guide_sample smp = in.smp[src];
+                p = V3{smp.pos[0], smp.pos[1], smp.pos[2]};
+                nn[q] = V3{smp.normal[0], smp.normal[1], smp.normal[2]};
+                wo = V3{smp.wo[0], smp.wo[1], smp.wo[2]};
+                wi[q] = V3{smp.wi[0], smp.wi[1], smp.wi[2]};
+                w[q] = smp.weight;
+                ps[q] = smp.pdf;
+            }
+            if (g == 0) wsum += w[q];
+            x0[q] = encode8(g, p, nn[q], wo, box);
         }
-        if (g == 0) wsum += w;
-        const bf16x8 x0 = encode8(g, p, nn, wo, box);
-        bf16x8 xh[4][2];
-        f32x4 y[3];
-        net_forward(net, lane, x0, xh, y);
+        bf16x8 xh[kGroups][4][2];
+        f32x4 y[kGroups][3];
+        net_forward_g<kGroups>(net, lane, x0, xh, y);
+        float dy[kGroups][12];
+        bf16x8 dB[kGroups][2];
+#pragma unroll
+        for (int q = 0; q < kGroups; ++q) train_head(y[q], lane, nn[q], wi[q], w[q], ps[q], in.sel_variance, dy[q], dB[q]);
 
-        // head: loss w * [-log(q/2 + cos+/(2 pi)) - log(alpha q' + (1 - alpha) cos+/pi)]
-        HeadLane h;
-        head_decode(y, lane, h);
-        const float alpha = kAlphaLo + kAlphaSpan * h.sig;
-        float f[2], c[2];
-        float em[2];  // e^(2 kappa) - 1 per lobe
+        // backward, one layer per phase through the X / D tiles (the first
+        // barrier of a phase waits for every wave to finish reading the tiles)
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const float k = h.kap[t];
-            c[t] = h.mu[t][0] * wi.x + h.mu[t][1] * wi.y + h.mu[t][2] * wi.z;
-            // vMF density k / (2 pi (1 - e^-2k)) e^(k (c - 1))
-            em[t] = fexp(2.0f * k) - 1.0f;
-            f[t] = fdiv(k * fexp(k * (c[t] - 1.0f)), kTwoPi * (1.0f - fexp(-2.0f * k)));
-        }
-        float q = h.pi[0] * f[0] + h.pi[1] * f[1];
-        q += __shfl_xor(q, 16);
-        q += __shfl_xor(q, 32);
-        const float cb = dot(nn, wi);
-        const float b = cb > 0.0f ? cb * (1.0f / kPi) : 0.0f;
-        // lobes: KL of the fixed half-and-half proposal; selection: KL of the
-        // learned one-sample MIS pdf with the mixture held fixed
-        const float pm = alpha * q + (1.0f - alpha) * b;
-        const float G = w != 0.0f ? -fdiv(w, pm) : 0.0f;
-        const float dq = w != 0.0f ? -fdiv(w, kLobeShare * q + (1.0f - kLobeShare) * b) * kLobeShare : 0.0f;
-        float dy[12];
+        for (int q = 0; q < kGroups; ++q) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int q0 = 5 * t;
-            const float tt = h.pi[t] * f[t];
-            const float k = h.kap[t];
-            const float ir = __builtin_amdgcn_rcpf(h.r[t]);
-            dy[q0 + 4] = dq * (tt - h.pi[t] * q);
-            dy[q0 + 3] = (h.kr[t] > kLogKappaLo && h.kr[t] < kLogKappaHi)
-                             ? dq * tt * (1.0f - fdiv(2.0f * k, em[t]) + k * (c[t] - 1.0f))
-                             : 0.0f;
-            const float gx = dq * tt * k * wi.x, gy = dq * tt * k * wi.y, gz = dq * tt * k * wi.z;
-            const float mg = h.mu[t][0] * gx + h.mu[t][1] * gy + h.mu[t][2] * gz;
-            dy[q0] = (gx - h.mu[t][0] * mg) * ir;
-            dy[q0 + 1] = (gy - h.mu[t][1] * mg) * ir;
-            dy[q0 + 2] = (gz - h.mu[t][2] * mg) * ir;
+            for (int qq = 0; qq < 12; ++qq)
+                Dt[tile_idx((qq / 4) * 16 + 4 * g + qq % 4, col[q])] = (__bf16)dy[q][qq];
+            store_hidden(Xt, xh[q][3], g, col[q]);
         }
-        // second-moment objective sum_i w_i^2 p_i / pm (Rath et al. 2020,
-        // "Variance-aware path guiding"): the KL gradient scaled by w p / pm,
-        // the sample's weight under the current mixture; it trains the
-        // selection row of the output layer only (no delta into the hidden
-        // layers, whose features stay the lobes')
-        const float rs = in.sel_variance ? (pm > 0.0f ? fdiv(w * ps, pm) : 0.0f) : 1.0f;
-        dy[10] = g == 0 ? G * rs * (q - b) * kAlphaSpan * h.sig * (1.0f - h.sig) : 0.0f;
-        dy[11] = 0.0f;
-        const float d10 = in.sel_variance ? 0.0f : dy[10];
-        bf16x8 dB[2] = {
-            __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[0], dy[1]), pk_bf16(dy[2], dy[3]), pk_bf16(dy[4], dy[5]),
-                                               pk_bf16(dy[6], dy[7])}),
-            __builtin_bit_cast(bf16x8, (u32x4){pk_bf16(dy[8], dy[9]), pk_bf16(d10, dy[11]), 0u, 0u})};
-
-        // backward, one layer per phase through alternating X / D buffers
-        {
-            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
+        __syncthreads();
+        dw_layer<4>(acc, Dt, Xt, ob, half, lane, ones);
+        delta_layer<4>(net, lane, dB, xh);
+        __syncthreads();
 #pragma unroll
-            for (int qq = 0; qq < 12; ++qq) D[tile_idx((qq / 4) * 16 + 4 * g + qq % 4, col)] = (__bf16)dy[qq];
-            store_hidden(X, xh[3], g, col);
-            __syncthreads();
-            dw_layer<4>(acc, D, X, wave, lane, ones);
-            delta_layer<4>(net, lane, dB, xh[3]);
-            ++ph;
+        for (int q = 0; q < kGroups; ++q) {
+            store_hidden(Dt, dB[q], g, col[q]);
+            store_hidden(Xt, xh[q][2], g, col[q]);
         }
-        {
-            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
-            store_hidden(D, dB, g, col);
-            store_hidden(X, xh[2], g, col);
-            __syncthreads();
-            dw_layer<3>(acc, D, X, wave, lane, ones);
-            delta_layer<3>(net, lane, dB, xh[2]);
-            ++ph;
-        }
-        {
-            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
-            store_hidden(D, dB, g, col);
-            store_hidden(X, xh[1], g, col);
-            __syncthreads();
-            dw_layer<2>(acc, D, X, wave, lane, ones);
-            delta_layer<2>(net, lane, dB, xh[1]);
-            ++ph;
-        }
-        {
-            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
-            store_hidden(D, dB, g, col);
-            store_hidden(X, xh[0], g, col);
-            __syncthreads();
-            dw_layer<1>(acc, D, X, wave, lane, ones);
-            delta_layer<1>(net, lane, dB, xh[0]);
-            ++ph;
-        }
-        {
-            __bf16 *D = Dt[ph & 1], *X = Xt[ph & 1];
-            store_hidden(D, dB, g, col);
+        __syncthreads();
+        dw_layer<3>(acc, Dt, Xt, ob, half, lane, ones);
+        delta_layer<3>(net, lane, dB, xh);
+        __syncthreads();
 #pragma unroll
-            for (int j = 0; j < 8; ++j) X[tile_idx(8 * g + j, col)] = x0[j];
-            __syncthreads();
-            dw_layer<0>(acc, D, X, wave, lane, ones);
-            ++ph;
+        for (int q = 0; q < kGroups; ++q) {
+            store_hidden(Dt, dB[q], g, col[q]);
+            store_hidden(Xt, xh[q][1], g, col[q]);
         }
+        __syncthreads();
+        dw_layer<2>(acc, Dt, Xt, ob, half, lane, ones);
+        delta_layer<2>(net, lane, dB, xh);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kGroups; ++q) {
+            store_hidden(Dt, dB[q], g, col[q]);
+            store_hidden(Xt, xh[q][0], g, col[q]);
+        }
+        __syncthreads();
+        dw_layer<1>(acc, Dt, Xt, ob, half, lane, ones);
+        delta_layer<1>(net, lane, dB, xh);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kGroups; ++q) {
+            store_hidden(Dt, dB[q], g, col[q]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Xt[tile_idx(8 * g + j, col[q])] = x0[q][j];
+        }
+        __syncthreads();
+        dw_layer<0>(acc, Dt, Xt, ob, half, lane, ones);
     }
     float *P = in.partials + (size_t)blockIdx.x * (kNP + 1);
-    dw_store<4>(acc, P, wave, lane);
-    dw_store<3>(acc, P, wave, lane);
-    dw_store<2>(acc, P, wave, lane);
-    dw_store<1>(acc, P, wave, lane);
-    dw_store<0>(acc, P, wave, lane);
+    dw_store<4>(acc, P, ob, half, lane);
+    dw_store<3>(acc, P, ob, half, lane);
+    dw_store<2>(acc, P, ob, half, lane);
+    dw_store<1>(acc, P, ob, half, lane);
+    dw_store<0>(acc, P, ob, half, lane);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) wsum += __shfl_xor(wsum, o);
     if (lane == 0) red[wave] = wsum;
