@@ -67,8 +67,9 @@ COUNTS_FIXTURE = os.path.join(ROOT, "tests", "golden", "roofline_counts.json")
 TRAFFIC_FIXTURE = os.path.join(ROOT, "profiles", "trace_closest_c1_incoherent_traffic.json")
 # the sources that decide the traversal kernel's memory traffic (the kernel
 # and the tree it walks); counter runs record their hash
-TRAFFIC_SOURCES = ["devilray_amd/csrc/trace.hip", "devilray_amd/csrc/dr_device.hpp",
-                   "devilray_amd/csrc/dr_internal.hpp", "devilray_amd/csrc/bvh_build.hip"]
+TRAFFIC_SOURCES = ["devilray_amd/csrc/trace.hip", "devilray_amd/csrc/trace_device.hpp",
+                   "devilray_amd/csrc/dr_device.hpp", "devilray_amd/csrc/dr_internal.hpp",
+                   "devilray_amd/csrc/bvh_build.hip"]
 
 
 def sources_hash() -> str:
@@ -438,8 +439,10 @@ def batched_vs_separate(ctx, n_scenes=64, max_depth=12, rr_depth=3):
     """The batched workload (64 scenes, triangles log-uniform in [1e3, 2e6],
     pixels log-uniform in [128^2, 1024^2], spp in {4, 16, 64}): every view
     rendered by one dr_render call per scene, against one dr_batch_render
-    over all views with autotuned launch options. Both are timed on their
-    second run (the first sizes the buffers) with HIP events."""
+    over all views with autotuned launch options, and against the whole
+    batch in one persistent launch (the sweep's best persistent shape). All
+    are timed on their second run (the first sizes the buffers) with HIP
+    events."""
     from devilray_amd import autotune
     work = scenes.batch_c3(n_scenes)
     scs, cams, npix = [], [], []
@@ -492,6 +495,30 @@ def batched_vs_separate(ctx, n_scenes=64, max_depth=12, rr_depth=3):
     clear()
     bat_ms = batched()
     same = bool(np.array_equal(acc.to_numpy(np.float64, 4 * sum(npix)), sep_img))
+    # the whole batch in one persistent launch, with the sweep's best
+    # persistent shape, at the full spp
+    sweep = tuned.get("_sweep") or []
+    pers = [(ms, o) for ms, o in sweep if o.get("launch") == dr.DR_BATCH_LAUNCH_PERSISTENT]
+    seq = [(ms, o) for ms, o in sweep if o.get("launch") != dr.DR_BATCH_LAUNCH_PERSISTENT]
+    one = None
+    if pers:
+        popts = min(pers, key=lambda x: x[0])[1]
+
+        def persistent():
+            e0, e1 = Event(), Event()
+            e0.record()
+            b.render_dev(views, max_depth=max_depth, rr_depth=rr_depth, seed=0, **popts)
+            e1.record()
+            e1.synchronize()
+            return e0.elapsed_ms(e1)
+
+        clear()
+        persistent()
+        clear()
+        p_ms = persistent()
+        one = {"options": popts, "ms": round(p_ms, 2), "speedup_over_separate": round(sep_ms / p_ms, 3),
+               "samples_per_s": round(paths / p_ms * 1e3, 1),
+               "images_identical": bool(np.array_equal(acc.to_numpy(np.float64, 4 * sum(npix)), sep_img))}
     b.close()
     for s in scs:
         s.close()
@@ -501,7 +528,10 @@ def batched_vs_separate(ctx, n_scenes=64, max_depth=12, rr_depth=3):
             "speedup": round(sep_ms / bat_ms, 3),
             "separate_samples_per_s": round(paths / sep_ms * 1e3, 1),
             "batched_samples_per_s": round(paths / bat_ms * 1e3, 1),
-            "images_identical": same, "tuned": opts, "sweep": tuned.get("_sweep")}
+            "images_identical": same, "tuned": opts,
+            "sweep_best_sequence_ms": round(min(ms for ms, _ in seq), 3) if seq else None,
+            "sweep_best_persistent_ms": round(min(ms for ms, _ in pers), 3) if pers else None,
+            "one_persistent_launch": one, "sweep": sweep}
 
 
 def probe() -> int:

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    ALIAS_DTYPE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
+    ALIAS_DTYPE, DR_BATCH_LAUNCH_PERSISTENT, DR_BATCH_LAUNCH_SEQUENCE, DrBatchOptions, DrBatchView, DR_DEBUG_BARYCENTRIC, DR_DEBUG_UV_CHECKER, DR_DEBUG_WINDING,
     DR_GUIDE_COLLECT, DR_GUIDE_SELECT_KL, DR_GUIDE_SELECT_VARIANCE, DR_GUIDE_USE, DR_MAT_DIELECTRIC, DR_MAT_LAMBERT, DR_OUTPUT_LINEAR, DR_OUTPUT_SRGB, GUIDE_HEAD_FLOATS,
     GUIDE_N_PARAMS, GUIDE_SAMPLE_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, RAY_DTYPE, DevilRayError, DrBvhParams, DrCamera, DrGuideParams,
     DrMaterial, DrRenderParams, DrSceneDesc, DrSceneInfo, SCHED_PER_RAY, SCHED_PER_RAY_SPECULATIVE, SCHED_PERSISTENT,
@@ -549,13 +549,39 @@ class Batch:
 
     def render_dev(self, views, max_depth: int = 12, rr_depth: int = 3, seed: int = 0, fresnel: int = 0,
                    chunk_paths: int = 1 << 21, trace_sched: int = 1, refill_min: int = 16, use_graph: bool = True,
-                   stream=None):
-        """views: [(scene index, DrCamera, spp, sample_offset, accum device pointer)]."""
+                   launch: int = 0, blocks_per_cu: int = 0, stream=None):
+        """views: [(scene index, DrCamera, spp, sample_offset, accum device pointer)]. launch:
+        DR_BATCH_LAUNCH_SEQUENCE (a launch per stage, per chunk) or DR_BATCH_LAUNCH_PERSISTENT (one
+        launch for the whole render, blocks_per_cu resident blocks per CU, 0 = occupancy limit)."""
         vs = (DrBatchView * len(views))()
         for i, (sc, cam, spp, off, acc) in enumerate(views):
             vs[i].scene, vs[i].spp, vs[i].sample_offset = sc, spp, off
             vs[i].camera = cam
             vs[i].accum = acc
         p = DrRenderParams(0, 0, max_depth, rr_depth, seed, fresnel, 0)
-        o = DrBatchOptions(chunk_paths, trace_sched, refill_min, int(use_graph))
+        o = DrBatchOptions(chunk_paths, trace_sched, refill_min, int(use_graph), launch, blocks_per_cu)
         check(lib().dr_batch_render(self.handle, vs, len(views), C.byref(p), C.byref(o), stream))
+
+    def autotune(self, views, max_depth: int = 12, rr_depth: int = 3, seed: int = 0, fresnel: int = 0,
+                 budget_spp: int = 4, reps: int = 1):
+        """dr_batch_autotune over views as for render_dev (their accumulation
+        pointers are not written): (best options as render_dev keyword
+        arguments, [(milliseconds, options)] for every candidate timed)."""
+        vs = (DrBatchView * len(views))()
+        for i, (sc, cam, spp, off, acc) in enumerate(views):
+            vs[i].scene, vs[i].spp, vs[i].sample_offset = sc, spp, off
+            vs[i].camera = cam
+            vs[i].accum = acc
+        p = DrRenderParams(0, 0, max_depth, rr_depth, seed, fresnel, 0)
+        best = DrBatchOptions()
+        cap = 256
+        tried = (DrBatchOptions * cap)()
+        ms = (C.c_float * cap)()
+        n = C.c_uint32()
+        check(lib().dr_batch_autotune(self.handle, vs, len(views), C.byref(p), budget_spp, reps, C.byref(best),
+                                      tried, ms, cap, C.byref(n)))
+
+        def kw(o):
+            return dict(chunk_paths=o.chunk_paths, trace_sched=o.trace_sched, refill_min=o.refill_min,
+                        use_graph=bool(o.use_graph), launch=o.launch, blocks_per_cu=o.blocks_per_cu)
+        return kw(best), [(float(ms[k]), kw(tried[k])) for k in range(min(n.value, cap))]

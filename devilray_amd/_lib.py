@@ -75,7 +75,11 @@ class DrBatchView(C.Structure):
 
 class DrBatchOptions(C.Structure):
     _fields_ = [("chunk_paths", C.c_uint32), ("trace_sched", C.c_uint32), ("refill_min", C.c_uint32),
-                ("use_graph", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("launch", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
+DR_BATCH_LAUNCH_SEQUENCE = 0
+DR_BATCH_LAUNCH_PERSISTENT = 1
 
 
 class DrGuideParams(C.Structure):
@@ -144,6 +148,8 @@ PROTOTYPES = {
     "dr_batch_create": (_st, [_vp, _vp, _u32, C.POINTER(_vp)]),
     "dr_batch_destroy": (_st, [_vp]),
     "dr_batch_render": (_st, [_vp, _vp, _u32, C.POINTER(DrRenderParams), C.POINTER(DrBatchOptions), _vp]),
+    "dr_batch_autotune": (_st, [_vp, _vp, _u32, C.POINTER(DrRenderParams), _u32, _u32, C.POINTER(DrBatchOptions),
+                                _vp, _vp, _u32, C.POINTER(_u32)]),
     "dr_instances_create": (_st, [_vp, _vp, _u32, _vp, _u32, C.POINTER(_vp)]),
     "dr_instances_destroy": (_st, [_vp]),
     "dr_instances_trace_closest": (_st, [_vp, _vp, _u32, _vp, _vp]),

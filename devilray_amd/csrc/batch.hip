@@ -6,7 +6,9 @@
 #include "dr_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <new>
+#include <string>
 #include <vector>
 
 namespace dr {
@@ -242,11 +244,13 @@ dr_status dr_batch_render(dr_batch *bt, const dr_batch_view *views, uint32_t n_v
     if (!bt || !params || (n_views && !views)) return fail(DR_E_INVALID, "null argument");
     auto *b = reinterpret_cast<Batch *>(bt);
     DR_HIP_TRY(hipSetDevice(b->ctx->device));
-    dr_batch_options opt{kPathCap, DR_SCHED_PER_RAY_SPECULATIVE, 16u, 1u};
+    dr_batch_options opt{kPathCap, DR_SCHED_PER_RAY_SPECULATIVE, 16u, 1u, DR_BATCH_LAUNCH_SEQUENCE, 0u};
     if (options) opt = *options;
     if (opt.chunk_paths < 4096 || opt.chunk_paths > (1u << 23)) return fail(DR_E_INVALID, "chunk_paths out of range");
     if (opt.trace_sched > 15u) return fail(DR_E_INVALID, "unknown trace schedule");
     if (opt.refill_min < 1u || opt.refill_min > 64u) return fail(DR_E_INVALID, "refill_min must be 1..64");
+    if (opt.launch > DR_BATCH_LAUNCH_PERSISTENT) return fail(DR_E_INVALID, "unknown launch mode");
+    if (opt.blocks_per_cu > 16u) return fail(DR_E_INVALID, "blocks_per_cu must be 0..16");
     std::vector<ViewInfo> vt(n_views);
     uint64_t total = 0;
     for (uint32_t v = 0; v < n_views; ++v) {
@@ -281,8 +285,10 @@ dr_status dr_batch_render(dr_batch *bt, const dr_batch_view *views, uint32_t n_v
     j.cap = (uint32_t)std::min<uint64_t>(total, opt.chunk_paths);
     j.sched = opt.trace_sched;
     j.graph = opt.use_graph != 0;
+    j.persistent = opt.launch == DR_BATCH_LAUNCH_PERSISTENT;
+    j.blocks_per_cu = opt.blocks_per_cu;
     hipStream_t run = user;
-    if (j.graph && user == nullptr) {
+    if (j.graph && !j.persistent && user == nullptr) {
         // the null stream cannot be recorded: run on the batch's own stream
         if (!b->stream) {
             DR_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -298,6 +304,87 @@ dr_status dr_batch_render(dr_batch *bt, const dr_batch_view *views, uint32_t n_v
         DR_HIP_TRY(hipEventRecord(b->ev_out, run));
         DR_HIP_TRY(hipStreamWaitEvent(user, b->ev_out, 0));
     }
+    return st;
+}
+
+dr_status dr_batch_autotune(dr_batch *bt, const dr_batch_view *views, uint32_t n_views,
+                            const dr_render_params *params, uint32_t budget_spp, uint32_t reps,
+                            dr_batch_options *best, dr_batch_options *tried, float *ms, uint32_t max_tried,
+                            uint32_t *n_tried)
+{
+    if (!bt || !views || !n_views || !params || !best) return fail(DR_E_INVALID, "null argument or no views");
+    if (budget_spp < 1 || reps < 1 || reps > 16) return fail(DR_E_INVALID, "budget_spp >= 1 and reps 1..16");
+    if (max_tried && (!tried || !ms)) return fail(DR_E_INVALID, "null candidate output arrays");
+    auto *b = reinterpret_cast<Batch *>(bt);
+    DR_HIP_TRY(hipSetDevice(b->ctx->device));
+    // the trial renders: each view at min(spp, budget_spp) into one scratch
+    // accumulation buffer (the caller's images are not touched)
+    std::vector<dr_batch_view> tv(views, views + n_views);
+    uint64_t npix = 0;
+    for (const dr_batch_view &v : tv) npix += (uint64_t)v.camera.width * v.camera.height;
+    DevBuf scratch;
+    if (dr_status st = scratch.ensure(32ull * std::max<uint64_t>(npix, 1))) return st;
+    DR_HIP_TRY(hipMemset(scratch.ptr, 0, 32ull * std::max<uint64_t>(npix, 1)));
+    uint64_t off = 0;
+    for (dr_batch_view &v : tv) {
+        v.spp = std::min(v.spp, budget_spp);
+        v.accum = scratch.as<double>() + 4 * off;
+        off += (uint64_t)v.camera.width * v.camera.height;
+    }
+    // candidates: the launch sequence over queue sizes, traversal schedules
+    // (per-ray or resident-grid speculative; the refill threshold for the
+    // latter) and graph recording; the persistent launch over queue sizes,
+    // the resident-grid schedule with one ray counter or one per XCD, refill
+    // thresholds and resident blocks per CU (more cost more at its barriers)
+    std::vector<dr_batch_options> cand;
+    for (uint32_t chunk : {1u << 20, 1u << 21, 1u << 22, 1u << 23})
+        for (uint32_t graph : {0u, 1u}) {
+            cand.push_back(dr_batch_options{chunk, DR_SCHED_PER_RAY_SPECULATIVE, 16u, graph,
+                                            DR_BATCH_LAUNCH_SEQUENCE, 0u});
+            for (uint32_t refill : {8u, 16u, 32u})
+                cand.push_back(dr_batch_options{chunk, DR_SCHED_PERSISTENT_SPECULATIVE, refill, graph,
+                                                DR_BATCH_LAUNCH_SEQUENCE, 0u});
+        }
+    for (uint32_t chunk : {1u << 22, 1u << 23})
+        for (uint32_t sched : {(uint32_t)DR_SCHED_PERSISTENT_SPECULATIVE,
+                               (uint32_t)DR_SCHED_PERSISTENT_SPECULATIVE | DR_SCHED_XCD_SLICES})
+            for (uint32_t refill : {32u, 64u})
+                for (uint32_t bpc : {1u, 2u})
+                    cand.push_back(dr_batch_options{chunk, sched, refill, 0u, DR_BATCH_LAUNCH_PERSISTENT, bpc});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DR_HIP_TRY(hipEventCreate(&e0));
+    DR_HIP_TRY(hipEventCreate(&e1));
+    float best_ms = INFINITY;
+    dr_status st = DR_OK;
+    uint32_t k = 0;
+    for (const dr_batch_options &c : cand) {
+        std::vector<float> t;
+        for (uint32_t r = 0; r <= reps && st == DR_OK; ++r) {  // the first run sizes buffers, untimed
+            hipError_t e = hipEventRecord(e0, nullptr);
+            if (e == hipSuccess && (st = dr_batch_render(bt, tv.data(), n_views, params, &c, nullptr)) != DR_OK) break;
+            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float m = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+            if (e != hipSuccess) st = fail(DR_E_DEVICE, std::string("autotune timing: ") + hipGetErrorString(e));
+            if (r > 0) t.push_back(m);
+        }
+        if (st) break;
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (k < max_tried) {
+            tried[k] = c;
+            ms[k] = med;
+        }
+        ++k;
+        if (med < best_ms) {
+            best_ms = med;
+            *best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (n_tried) *n_tried = k;
     return st;
 }
 

@@ -141,6 +141,7 @@ struct Scene {
     DevBuf rs_shadow_rays, rs_shadow_occ, rs_shadow_contrib;
     DevBuf rs_view, rs_root, rs_sroot, rs_chunk, rs_views;  // per-path view / BVH root, chunk cursor, view table
     DevBuf rs_guide_head;   // DR_GUIDE_HEAD_FLOATS floats per path (guided renders)
+    DevBuf rs_grid;         // persistent render: grid barrier state and trace ray counters
     DevBuf rs_pending;      // u32 per path: training sample awaiting its radiance snapshot
     // u64: extension rays (camera and bounce rays, not shadow rays) traced by
     // the renders of this scene since it was created or the count was reset
@@ -244,6 +245,10 @@ struct RenderJob {
     uint32_t cap = kPathCap;  // paths per chunk
     uint32_t sched = DR_SCHED_PER_RAY_SPECULATIVE;
     bool graph = false;  // record the bounce loop once, replay it per chunk
+    // run every chunk and bounce in one persistent launch (plain renders),
+    // with this many resident blocks per CU (0: the occupancy limit)
+    bool persistent = false;
+    uint32_t blocks_per_cu = 0;
 };
 
 CamParams cam_params(const dr_camera &c);

@@ -20,6 +20,10 @@
 #include "dr_internal.hpp"
 #include "guide_device.hpp"
 #include "shade.hpp"
+#include "trace_device.hpp"
+
+#include <algorithm>
+#include <string>
 
 namespace dr {
 
@@ -78,14 +82,18 @@ __global__ void k_chunk_init(ChunkState *cs, uint64_t total, uint32_t cap)
     cs->cap = cap;
 }
 
-__global__ void k_next_chunk(ChunkState *cs)
+__device__ __forceinline__ void next_chunk(ChunkState *cs)
 {
-    if (threadIdx.x != 0) return;
     const uint64_t item0 = (uint64_t)cs->next * cs->cap;
     cs->item0 = item0;
     const uint64_t left = item0 < cs->total ? cs->total - item0 : 0ull;
     cs->n = left < cs->cap ? (uint32_t)left : cs->cap;
     cs->next = cs->next + 1u;
+}
+__global__ void k_next_chunk(ChunkState *cs)
+{
+    if (threadIdx.x != 0) return;
+    next_chunk(cs);
 }
 
 __global__ void k_set_view(ViewInfo *dst, ViewInfo v)
@@ -93,13 +101,15 @@ __global__ void k_set_view(ViewInfo *dst, ViewInfo v)
     if (threadIdx.x == 0) *dst = v;
 }
 
-__global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict__ rays,
-                                               float4 *__restrict__ beta, float4 *__restrict__ rad,
-                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
-                                               uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
-                                               uint32_t *__restrict__ proot)
+// Each stage is written for one item (path slot, queue slot or shadow ray)
+// and run either as its own kernel, one thread per item, or as a
+// grid-stride loop of the persistent render kernel.
+__device__ __forceinline__ void camera_item(uint32_t p, const PathParams &pp, dr_ray *__restrict__ rays,
+                                            float4 *__restrict__ beta, float4 *__restrict__ rad,
+                                            uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
+                                            uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
+                                            uint32_t *__restrict__ proot)
 {
-    const uint32_t p = blockIdx.x * kT + threadIdx.x;
     const uint32_t n = pp.chunk->n;
     if (p == 0) { counters[0] = n; counters[1] = 0u; counters[2] = 0u; }
     if (p >= n) return;
@@ -118,14 +128,28 @@ __global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict
     if (pending) pending[p] = kGuideNoSample;
 }
 
+__global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict__ rays,
+                                               float4 *__restrict__ beta, float4 *__restrict__ rad,
+                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
+                                               uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
+                                               uint32_t *__restrict__ proot)
+{
+    camera_item(blockIdx.x * kT + threadIdx.x, pp, rays, beta, rad, list, counters, pending, pview, proot);
+}
+
 // also adds this vertex's extension rays (the input queue) to the scene's
 // ray-cast count
-__global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
-                               unsigned long long *casts)
+__device__ __forceinline__ void reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
+                                             unsigned long long *casts)
 {
     *out_count = 0u;
     *shadow_count = 0u;
     *casts += *in_count;
+}
+__global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
+                               unsigned long long *casts)
+{
+    reset_counts(out_count, shadow_count, in_count, casts);
 }
 
 __global__ void k_add_casts(unsigned long long *casts, unsigned long long n, int zero)
@@ -161,17 +185,16 @@ __device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
     return V3{sv.verts[3 * (size_t)vi], sv.verts[3 * (size_t)vi + 1], sv.verts[3 * (size_t)vi + 2]};
 }
 
-__global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint32_t vertex,
-                                              const uint32_t *__restrict__ in_list,
-                                              const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
-                                              uint32_t *__restrict__ out_count, dr_ray *__restrict__ rays,
-                                              const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
-                                              float4 *__restrict__ rad, dr_ray *__restrict__ srays,
-                                              float4 *__restrict__ scontrib, uint32_t *__restrict__ scount,
-                                              GuideView gv, const uint32_t *__restrict__ pview,
-                                              uint32_t *__restrict__ sroot)
+__device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, const PathParams &pp, uint32_t vertex,
+                                           const uint32_t *__restrict__ in_list,
+                                           const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
+                                           uint32_t *__restrict__ out_count, dr_ray *__restrict__ rays,
+                                           const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
+                                           float4 *__restrict__ rad, dr_ray *__restrict__ srays,
+                                           float4 *__restrict__ scontrib, uint32_t *__restrict__ scount,
+                                           const GuideView &gv, const uint32_t *__restrict__ pview,
+                                           uint32_t *__restrict__ sroot)
 {
-    const uint32_t slot = blockIdx.x * kT + threadIdx.x;
     if (slot >= *in_count) return;
     const uint32_t path = in_list[slot];
     const dr_hit h = hits[path];
@@ -383,11 +406,24 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
     out_list[atomicAdd(out_count, 1u)] = path;
 }
 
-__global__ __launch_bounds__(kT) void k_shadow_apply(const uint32_t *__restrict__ scount,
-                                                     const uint32_t *__restrict__ occ,
-                                                     const float4 *__restrict__ scontrib, float4 *__restrict__ rad)
+__global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint32_t vertex,
+                                              const uint32_t *__restrict__ in_list,
+                                              const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
+                                              uint32_t *__restrict__ out_count, dr_ray *__restrict__ rays,
+                                              const dr_hit *__restrict__ hits, float4 *__restrict__ beta,
+                                              float4 *__restrict__ rad, dr_ray *__restrict__ srays,
+                                              float4 *__restrict__ scontrib, uint32_t *__restrict__ scount,
+                                              GuideView gv, const uint32_t *__restrict__ pview,
+                                              uint32_t *__restrict__ sroot)
 {
-    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    shade_item(blockIdx.x * kT + threadIdx.x, sv, pp, vertex, in_list, in_count, out_list, out_count, rays, hits, beta,
+               rad, srays, scontrib, scount, gv, pview, sroot);
+}
+
+__device__ __forceinline__ void shadow_apply_item(uint32_t i, const uint32_t *__restrict__ scount,
+                                                  const uint32_t *__restrict__ occ,
+                                                  const float4 *__restrict__ scontrib, float4 *__restrict__ rad)
+{
     if (i >= *scount || occ[i]) return;
     const float4 c = scontrib[i];
     const uint32_t path = (uint32_t)__float_as_int(c.w);
@@ -397,14 +433,19 @@ __global__ __launch_bounds__(kT) void k_shadow_apply(const uint32_t *__restrict_
     l.z = l.z + c.z;
     rad[path] = l;
 }
+__global__ __launch_bounds__(kT) void k_shadow_apply(const uint32_t *__restrict__ scount,
+                                                     const uint32_t *__restrict__ occ,
+                                                     const float4 *__restrict__ scontrib, float4 *__restrict__ rad)
+{
+    shadow_apply_item(blockIdx.x * kT + threadIdx.x, scount, occ, scontrib, rad);
+}
 
 // Lane p resolves path p if it is the chunk's first sample of its pixel and
 // adds that pixel's samples of the chunk in increasing sample order (the
 // next sample of a pixel is npix items later, within the same view).
-__global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__restrict__ rad,
-                                                const uint32_t *__restrict__ pview)
+__device__ __forceinline__ void resolve_item(uint32_t p, const PathParams &pp, const float4 *__restrict__ rad,
+                                             const uint32_t *__restrict__ pview)
 {
-    const uint32_t p = blockIdx.x * kT + threadIdx.x;
     const uint32_t n = pp.chunk->n;
     if (p >= n) return;
     const ViewInfo &V = pp.views[pview[p]];
@@ -422,6 +463,11 @@ __global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__r
         w = w + 1.0;
     }
     a[0] = r; a[1] = g; a[2] = b; a[3] = w;
+}
+__global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__restrict__ rad,
+                                                const uint32_t *__restrict__ pview)
+{
+    resolve_item(blockIdx.x * kT + threadIdx.x, pp, rad, pview);
 }
 
 inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
@@ -483,6 +529,251 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
         if (dr_status st = guide_finalize(j.guide, rad, stream)) return st;
     k_resolve<<<blocks(cap), kT, 0, stream>>>(pp, rad, pview);
     DR_HIP_TRY(hipGetLastError());
+    return DR_OK;
+}
+
+// ---- one persistent launch ------------------------------------------------
+// The whole render (every chunk, its camera rays and every bounce) in one
+// launch of resident blocks: each wavefront stage is a grid-stride loop (the
+// traversal stages its persistent schedule) and the stages are separated by
+// grid-wide barriers. The stages and their order are those of the launch
+// sequence, so the images are the same.
+//
+// Grid state: [0] top-level arrivals, [1] barrier generation, [2] abort
+// flag; [4, 12) the closest-hit stage's ray counters, [12, 20) the any-hit
+// stage's; from word 32 the barrier groups, 16 words (one 64-byte line) each:
+// [0] arrivals, [1] generation.
+constexpr uint32_t kBarrierGroups = 32;
+constexpr uint32_t kGridWords = 32 + 16 * kBarrierGroups;
+// barrier wait bound: about 2^23 x 512 cycles, a few seconds
+constexpr uint32_t kBarrierSpins = 1u << 23;
+
+struct PersistArgs {
+    PathParams pp;
+    SceneView sv;
+    dr_ray *rays;
+    dr_hit *hits;
+    float4 *beta, *rad;
+    uint32_t *list_a, *list_b, *cnt;
+    dr_ray *srays;
+    uint32_t *socc;
+    float4 *scon;
+    uint32_t *pview, *proot, *sroot;
+    unsigned long long *casts;
+    ChunkState *chunk;
+    const WideNode *nodes;
+    const float4 *tri_v;
+    int32_t *ovf;
+    uint32_t *flags;
+    uint32_t *grid;
+    uint32_t refill_min, slices, max_stack, cap, has_emit;
+};
+
+__device__ __forceinline__ uint32_t load_agent(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until *gen differs from `seen`; bounded: on time-out or an abort by
+// another block it sets the abort flag and returns false.
+__device__ __forceinline__ bool wait_generation(const uint32_t *gen, uint32_t seen, uint32_t *abort_flag)
+{
+    uint32_t spins = 0;
+    while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == seen) {
+        if (load_agent(abort_flag) != 0u || ++spins > kBarrierSpins) {
+            atomicOr(abort_flag, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+    return true;
+}
+
+// Arrival at a counter of `size` participants: true for the last one, which
+// resets the counter; the others see false. `gen` is the generation read
+// before arriving.
+__device__ __forceinline__ bool arrive(uint32_t *count, uint32_t size)
+{
+    if (atomicAdd(count, 1u) + 1u != size) return false;
+    __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Grid-wide barrier for a launch whose blocks are all resident, in two
+// levels: block b arrives at group b % 32 (at most ceil(grid / 32)
+// arrivals on one cache line), the last block of each group at the top
+// counter, and the release runs back down through the groups' generations,
+// so no address sees more than a few dozen atomics or pollers. One thread
+// per block arrives and fences (the block's barriers order its other threads'
+// accesses around it). Every wait is bounded: a block that waits too long,
+// or sees the abort flag, sets the flag and the barrier returns false (the
+// same value for every thread of the block), so a launch whose blocks
+// disagree ends instead of hanging.
+__device__ bool grid_barrier(uint32_t *g)
+{
+    __shared__ uint32_t ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's writes, device-wide
+        const uint32_t nb = gridDim.x;
+        const uint32_t ng = nb < kBarrierGroups ? nb : kBarrierGroups;
+        const uint32_t k = blockIdx.x % kBarrierGroups;
+        const uint32_t gsize = (nb - k + kBarrierGroups - 1u) / kBarrierGroups;
+        uint32_t *grp = g + 32 + 16 * k;
+        uint32_t good = 1u;
+        const uint32_t ggen = __hip_atomic_load(grp + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrive(grp, gsize)) {
+            const uint32_t tgen = __hip_atomic_load(g + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (arrive(g, ng))
+                __hip_atomic_store(g + 1, tgen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                good = wait_generation(g + 1, tgen, g + 2) ? 1u : 0u;
+            __hip_atomic_store(grp + 1, ggen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            good = wait_generation(grp + 1, ggen, g + 2) ? 1u : 0u;
+        }
+        if (load_agent(g + 2) != 0u) good = 0u;
+        __threadfence();  // the other blocks' writes
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0u;
+}
+
+// Block size of the persistent launch. The traversal stages treat each
+// kBlock-thread part of a block as a block of their own, so larger blocks
+// work too; measured, 1024-thread blocks (one per CU) were slower than
+// 256-thread ones at the same number of waves (66 against 55 ms on a
+// 1024 x 1024 x 4 spp render): the barrier and the traversal's refill
+// counter see more contention.
+constexpr int kPersistThreads = 256;
+constexpr int kPersistParts = kPersistThreads / kBlock;
+
+template <bool SPEC>
+__global__ __launch_bounds__(kPersistThreads) void k_render_persistent(PersistArgs a)
+{
+    __shared__ int32_t stacks[kPersistParts][kLdsStack][kBlock];
+    const uint32_t gtid = blockIdx.x * kPersistThreads + threadIdx.x, nthr = gridDim.x * kPersistThreads;
+    const uint32_t part = threadIdx.x / kBlock, ptid = threadIdx.x % kBlock;
+    const uint32_t vbid = blockIdx.x * kPersistParts + part, vnblk = gridDim.x * kPersistParts;
+    int32_t(*stack)[kBlock] = stacks[part];
+    uint32_t *ctr_closest = a.grid + 4, *ctr_any = a.grid + 12;
+    const GuideView gv{};
+    for (;;) {
+        if (gtid == 0) {
+            next_chunk(a.chunk);
+            for (int k = 0; k < 8; ++k) ctr_closest[k] = 0u;
+        }
+        if (!grid_barrier(a.grid)) return;
+        const uint32_t n = load_agent(&a.chunk->n);
+        if (n == 0u) return;  // every block reads the same count
+        for (uint32_t p = gtid; p < n; p += nthr)
+            camera_item(p, a.pp, a.rays, a.beta, a.rad, a.list_a, a.cnt, nullptr, a.pview, a.proot);
+        if (!grid_barrier(a.grid)) return;
+        for (uint32_t v = 0; v < a.pp.max_depth; ++v) {
+            uint32_t *lin = (v & 1u) ? a.list_b : a.list_a;
+            uint32_t *lout = (v & 1u) ? a.list_a : a.list_b;
+            uint32_t *cin = a.cnt + (v & 1u);
+            uint32_t *cout = a.cnt + ((v + 1u) & 1u);
+            const uint32_t nin = load_agent(cin);
+            if (nin == 0u) break;  // no path left in this chunk (the same count everywhere)
+            // extension rays; the output queue and the shadow queue restart
+            if (gtid == 0) reset_counts(cout, a.cnt + 2, cin, a.casts);
+            trace_body<false, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
+                                                        reinterpret_cast<const float4 *>(a.rays), lin, a.cap, cin,
+                                                        a.hits, nullptr, nullptr, nullptr, a.ovf, a.flags,
+                                                        ctr_closest, a.refill_min, a.proot, a.slices, a.max_stack,
+                                                        nullptr);
+            if (!grid_barrier(a.grid)) return;
+            if (gtid == 0)
+                for (int k = 0; k < 8; ++k) {
+                    ctr_any[k] = 0u;
+                    ctr_closest[k] = 0u;
+                }
+            for (uint32_t slot = gtid; slot < nin; slot += nthr)
+                shade_item(slot, a.sv, a.pp, v, lin, cin, lout, cout, a.rays, a.hits, a.beta, a.rad, a.srays, a.scon,
+                           a.cnt + 2, gv, a.pview, a.sroot);
+            if (!grid_barrier(a.grid)) return;
+            const uint32_t ns = a.has_emit ? load_agent(a.cnt + 2) : 0u;
+            if (ns) {
+                trace_body<true, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
+                                                           reinterpret_cast<const float4 *>(a.srays), nullptr, a.cap,
+                                                           a.cnt + 2, nullptr, a.socc, nullptr, nullptr, a.ovf,
+                                                           a.flags, ctr_any, a.refill_min, a.sroot, a.slices,
+                                                           a.max_stack, nullptr);
+                if (!grid_barrier(a.grid)) return;
+                for (uint32_t i = gtid; i < ns; i += nthr) shadow_apply_item(i, a.cnt + 2, a.socc, a.scon, a.rad);
+                if (!grid_barrier(a.grid)) return;
+            }
+        }
+        for (uint32_t p = gtid; p < n; p += nthr) resolve_item(p, a.pp, a.rad, a.pview);
+        // the next chunk rewrites the path state
+        if (!grid_barrier(a.grid)) return;
+    }
+}
+
+__global__ void k_grid_init(uint32_t *g)
+{
+    for (uint32_t k = threadIdx.x; k < kGridWords; k += blockDim.x) g[k] = 0u;
+}
+
+// One cooperative launch of resident blocks for the whole job (plain renders;
+// the traversal runs its persistent schedule, speculative or not as j.sched
+// says). Waits for the launch and reports an aborted barrier as an error.
+dr_status run_persistent(const RenderJob &j, const PathParams &pp, const SceneView &sv, hipStream_t stream)
+{
+    Scene *s = j.s;
+    if (s->depth > kMaxWideDepth)
+        return fail(DR_E_OVERFLOW, "BVH deeper than the traversal stack bound of " + std::to_string(kMaxWideDepth) +
+                                       " wide levels");
+    const bool spec = (j.sched & 1u) != 0u;
+    const void *kern = spec ? reinterpret_cast<const void *>(&k_render_persistent<true>)
+                            : reinterpret_cast<const void *>(&k_render_persistent<false>);
+    int occ = 0;
+    DR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kPersistThreads, 0));
+    if (occ < 1) return fail(DR_E_DEVICE, "persistent render kernel does not fit on a CU");
+    const int bpc = j.blocks_per_cu ? std::min<int>((int)j.blocks_per_cu, occ) : occ;
+    const uint32_t grid = (uint32_t)(s->ctx->cu_count * bpc);
+    const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
+    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(max_stack - kLdsStack) * grid * kPersistThreads + 4))
+        return st;
+    if (dr_status st = s->rs_grid.ensure(sizeof(uint32_t) * kGridWords)) return st;
+    k_grid_init<<<1, 64, 0, stream>>>(s->rs_grid.as<uint32_t>());
+    DR_HIP_TRY(hipGetLastError());
+    PersistArgs a;
+    a.pp = pp;
+    a.sv = sv;
+    a.rays = s->rs_rays.as<dr_ray>();
+    a.hits = s->rs_hits.as<dr_hit>();
+    a.beta = s->rs_beta.as<float4>();
+    a.rad = s->rs_radiance.as<float4>();
+    a.list_a = s->rs_list_a.as<uint32_t>();
+    a.list_b = s->rs_list_b.as<uint32_t>();
+    a.cnt = s->rs_counters.as<uint32_t>();
+    a.srays = s->rs_shadow_rays.as<dr_ray>();
+    a.socc = s->rs_shadow_occ.as<uint32_t>();
+    a.scon = s->rs_shadow_contrib.as<float4>();
+    a.pview = s->rs_view.as<uint32_t>();
+    a.proot = j.scenes ? s->rs_root.as<uint32_t>() : nullptr;
+    a.sroot = j.scenes ? s->rs_sroot.as<uint32_t>() : nullptr;
+    a.casts = s->ray_casts.as<unsigned long long>();
+    a.chunk = s->rs_chunk.as<ChunkState>();
+    a.nodes = s->nodes.as<WideNode>();
+    a.tri_v = s->tri_v.as<float4>();
+    a.ovf = s->ovf_stack.as<int32_t>();
+    a.flags = s->flags.as<uint32_t>();
+    a.grid = s->rs_grid.as<uint32_t>();
+    a.refill_min = s->refill_min;
+    a.slices = ((j.sched & DR_SCHED_XCD_SLICES) && grid * kPersistParts >= 8u) ? 8u : 1u;
+    a.max_stack = max_stack;
+    a.cap = j.cap;
+    a.has_emit = s->n_emit ? 1u : 0u;
+    void *args[] = {&a};
+    DR_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kPersistThreads), args, 0, stream));
+    uint32_t abort_flag = 0;
+    DR_HIP_TRY(hipMemcpyAsync(&abort_flag, s->rs_grid.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, stream));
+    DR_HIP_TRY(hipStreamSynchronize(stream));
+    if (abort_flag) return fail(DR_E_DEVICE, "persistent render: a grid barrier timed out (the launch was stopped)");
     return DR_OK;
 }
 
@@ -590,7 +881,10 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     k_chunk_init<<<1, 64, 0, stream>>>(s->rs_chunk.as<ChunkState>(), j.total, cap);
     DR_HIP_TRY(hipGetLastError());
     const uint64_t n_chunks = (j.total + cap - 1) / cap;
-    if (!j.graph || n_chunks == 1) {
+    if (j.persistent) {
+        if (use_guide || collect) return fail(DR_E_UNSUPPORTED, "the persistent launch renders without a guide");
+        if (dr_status st = run_persistent(j, pp, sv, stream)) return st;
+    } else if (!j.graph || n_chunks == 1) {
         for (uint64_t c = 0; c < n_chunks; ++c)
             if (dr_status st = run_chunk(j, pp, sv, gv, stream)) return st;
     } else {

@@ -333,12 +333,26 @@ typedef struct {
 } dr_batch_view;
 
 /* Launch shape (the autotuner's knobs). NULL = defaults {1 << 21,
- * DR_SCHED_PER_RAY_SPECULATIVE, 16, 1}. */
+ * DR_SCHED_PER_RAY_SPECULATIVE, 16, 1, DR_BATCH_LAUNCH_SEQUENCE, 0}.
+ *
+ * DR_BATCH_LAUNCH_SEQUENCE: per chunk of paths, one launch per wavefront
+ * stage (camera, then per bounce trace, shade, shadow trace, shadow apply,
+ * then resolve), the bounce loop optionally recorded as a HIP graph.
+ * DR_BATCH_LAUNCH_PERSISTENT: the whole render in one cooperative launch of
+ * resident blocks (blocks_per_cu per CU, 0 = the occupancy limit) that runs
+ * the same stages in the same order, separated by grid-wide barriers; the
+ * traversal uses its persistent schedule (speculative if trace_sched has
+ * bit 0). Same images as the sequence. The call then waits for the launch,
+ * and fails with DR_E_DEVICE if a barrier wait ran out (it is bounded). */
+#define DR_BATCH_LAUNCH_SEQUENCE 0u
+#define DR_BATCH_LAUNCH_PERSISTENT 1u
 typedef struct {
-    uint32_t chunk_paths;  /* paths in flight per chunk (4096 .. 1 << 23) */
-    uint32_t trace_sched;  /* DR_SCHED_* for the extension and shadow traces */
-    uint32_t refill_min;   /* persistent traversal refill threshold (1..64) */
-    uint32_t use_graph;    /* record the bounce loop once, replay it per chunk */
+    uint32_t chunk_paths;   /* paths in flight per chunk (4096 .. 1 << 23) */
+    uint32_t trace_sched;   /* DR_SCHED_* for the extension and shadow traces */
+    uint32_t refill_min;    /* persistent traversal refill threshold (1..64) */
+    uint32_t use_graph;     /* sequence: record the bounce loop once, replay it per chunk */
+    uint32_t launch;        /* DR_BATCH_LAUNCH_* */
+    uint32_t blocks_per_cu; /* persistent launch: resident blocks per CU (0 = occupancy limit, at most 16) */
 } dr_batch_options;
 
 /* Scenes must have their BVH built (their light tables are built here if
@@ -352,6 +366,23 @@ dr_status dr_batch_destroy(dr_batch *batch);
  * at entry (the view table is rewritten), then runs asynchronously. */
 dr_status dr_batch_render(dr_batch *batch, const dr_batch_view *views, uint32_t n_views,
                           const dr_render_params *params, const dr_batch_options *options, dr_stream stream);
+/* Launch-shape autotuner: renders the views with each view's spp cut to
+ * budget_spp into a scratch accumulation buffer (the views' own images are
+ * not touched), reps + 1 times per candidate (the first untimed), and writes
+ * the candidate of least median time to *best. Candidates: the launch
+ * sequence over chunk_paths 2^20 .. 2^23, the per-ray and the resident-grid
+ * speculative schedules (refill_min 8, 16, 32 for the latter) and graph
+ * recording off and on; the persistent launch over chunk_paths 2^22 and
+ * 2^23, the resident-grid speculative schedule with one ray counter and with
+ * one per XCD (DR_SCHED_XCD_SLICES), refill_min 32 and 64 and blocks_per_cu
+ * 1 and 2. If max_tried > 0,
+ * the first max_tried candidates and their median milliseconds go to tried
+ * and ms; *n_tried (may be NULL) is the number of candidates timed.
+ * Synchronous. */
+dr_status dr_batch_autotune(dr_batch *batch, const dr_batch_view *views, uint32_t n_views,
+                            const dr_render_params *params, uint32_t budget_spp, uint32_t reps,
+                            dr_batch_options *best, dr_batch_options *tried, float *ms, uint32_t max_tried,
+                            uint32_t *n_tried);
 
 /* ---- instancing: one scene placed several times ----------------------------
  *

@@ -69,6 +69,43 @@ def test_batch_views_equal_single_renders(gpu_ctx, three_scenes, graph, sched):
     b.close()
 
 
+@pytest.mark.parametrize("sched,chunk,bpc", [(1, 5000, 0), (0, 5000, 1), (1, 1 << 20, 2), (3, 4096, 0)])
+def test_persistent_launch_equals_single_renders(gpu_ctx, three_scenes, sched, chunk, bpc):
+    """DR_BATCH_LAUNCH_PERSISTENT: the whole render in one launch (chunks,
+    bounces and stages separated by grid barriers) gives every view's
+    accumulation bit for bit, as the single renders do; with several chunks,
+    one chunk, and different numbers of resident blocks per CU."""
+    sds, scs = three_scenes
+    views = [(0, 40, 32, 4, 0), (1, 64, 36, 3, 5), (2, 32, 32, 6, 0), (1, 20, 12, 16, 2)]
+    print(f"[case] persistent launch sched {sched} chunk {chunk} blocks/CU {bpc}", flush=True)
+    b = dr.Batch(gpu_ctx, scs)
+    accs, cams = [], []
+    for sc, w, h, spp, off in views:
+        a = gpu_ctx.alloc(w * h * 32)
+        a.copy_from(np.zeros(w * h * 4))
+        accs.append(a)
+        cams.append(_cam(sds[sc], w, h))
+    seed = 0x5151
+    b.render_dev([(v[0], cams[i], v[3], v[4], accs[i].ptr) for i, v in enumerate(views)], max_depth=8, seed=seed,
+                 chunk_paths=chunk, trace_sched=sched, launch=dr.DR_BATCH_LAUNCH_PERSISTENT, blocks_per_cu=bpc)
+    gpu_ctx.synchronize()
+    for i, (sc, w, h, spp, off) in enumerate(views):
+        got = accs[i].to_numpy(np.float64, w * h * 4).reshape(h, w, 4)
+        ref = scs[sc].render(cams[i], spp=spp, max_depth=8, seed=seed ^ (i << 32), sample_offset=off)
+        assert np.array_equal(got, ref), i
+    b.close()
+
+
+def test_persistent_launch_rejects_bad_options(gpu_ctx, three_scenes):
+    sds, scs = three_scenes
+    b = dr.Batch(gpu_ctx, scs[:1])
+    a = gpu_ctx.alloc(16 * 16 * 32)
+    for kw in ({"launch": 2}, {"launch": 1, "blocks_per_cu": 17}):
+        with pytest.raises(dr.DevilRayError):
+            b.render_dev([(0, _cam(sds[0], 16, 16), 1, 0, a.ptr)], **kw)
+    b.close()
+
+
 def test_graph_renders_back_to_back_then_destroy(gpu_ctx, three_scenes):
     """Two graph renders queued without a synchronisation in between (the
     second records a new graph while the first one's replays may still be

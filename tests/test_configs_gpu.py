@@ -67,7 +67,8 @@ def test_partition_scene_matches_oracle(gpu_ctx):
 
 
 def test_batch_of_64_uneven_scenes(gpu_ctx):
-    print("[case] 64-scene batch: batched == separate, smallest view vs oracle", flush=True)
+    print("[case] 64-scene batch: batched (launch sequence and one persistent launch) == separate, smallest view "
+          "vs oracle", flush=True)
     work = scenes.batch_c3(64)
     scs, cams, npix = [], [], []
     for sd, w, h, spp in work:
@@ -93,6 +94,13 @@ def test_batch_of_64_uneven_scenes(gpu_ctx):
     a_bat = bat.to_numpy(np.float64, 4 * total)
     print(f"  {len(work)} scenes, {sum(sd.indices.shape[0] for sd, *_ in work)} tris, {total} pixels", flush=True)
     assert np.array_equal(a_sep, a_bat), "batched views differ from their single-scene renders"
+    # the same batch in one persistent launch
+    print("  one persistent launch", flush=True)
+    bat.copy_from(np.zeros(4 * total))
+    b.render_dev(views, max_depth=12, rr_depth=3, seed=0, chunk_paths=1 << 22, trace_sched=1,
+                 launch=dr.DR_BATCH_LAUNCH_PERSISTENT)
+    gpu_ctx.synchronize()
+    assert np.array_equal(a_sep, bat.to_numpy(np.float64, 4 * total)), "persistent launch differs"
     # the view with the fewest samples, against the oracle at its full size and spp
     k = int(np.argmin([n * w[3] for n, w in zip(npix, work)]))
     sd, w, h, spp = work[k]
