@@ -185,6 +185,8 @@ struct Guide {
     DevBuf aux;      // GuideAux x stage_cap
     DevBuf ring;     // training ring: the latest `capacity` samples of non-zero weight
     DevBuf flags;    // u32 x (stage_cap + 1): compaction ranks
+    DevBuf keys;     // u32 x 2 x stage_cap: each staged sample's order key, low words then high words
+    DevBuf perm;     // u32 x stage_cap: stage slots in key order (valid after a collecting render)
     DevBuf counters; // u32[4]: [0] samples offered to the stage, [1] ring head, [2] ring fill
     uint32_t stage_cap = 0;
     uint32_t step = 0;
@@ -202,9 +204,11 @@ dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const 
                             const uint32_t *count, uint32_t n_max, float *head, hipStream_t stream);
 // Complete the training samples of a finished chunk from the path radiance.
 dr_status guide_finalize(Guide *g, const float4 *radiance, hipStream_t stream);
-// Append the staged samples of non-zero weight to the training ring (scan
-// order, deterministic).
-dr_status guide_compact(Guide *g, hipStream_t stream);
+// Append the staged samples of non-zero weight to the training ring in the
+// order of their keys ((path item << 6) | vertex, unique per sample), so the
+// ring does not depend on the order in which the stage's slots were taken.
+// total_items: path items of the render (bounds the keys).
+dr_status guide_compact(Guide *g, uint64_t total_items, hipStream_t stream);
 dr_status guide_reset_counters(Guide *g, hipStream_t stream);
 
 // ---- rendering (render.hip) ------------------------------------------------

@@ -885,20 +885,49 @@ __global__ __launch_bounds__(kT) void k_guide_finalize(dr_guide_sample *__restri
     aux[i].path = kGuideNoSample;
 }
 
+// Sort input over the whole stage: perm = identity, and the slots no sample
+// took get the sentinel key, above every sample's, so that after the sort
+// the first n positions of perm are the n staged samples in key order.
+__global__ __launch_bounds__(kT) void k_guide_sort_init(uint32_t *__restrict__ key_lo, uint32_t *__restrict__ key_hi,
+                                                        uint32_t *__restrict__ perm,
+                                                        const uint32_t *__restrict__ counters, uint32_t stage_cap,
+                                                        uint32_t sent_lo, uint32_t sent_hi)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= stage_cap) return;
+    perm[i] = i;
+    if (i >= min(counters[0], stage_cap)) {
+        key_lo[i] = sent_lo;
+        key_hi[i] = sent_hi;
+    }
+}
+
+// Second digit group of a 64-bit key: the high words in the order of the
+// first sort (a stable sort by them then orders by the whole key).
+__global__ __launch_bounds__(kT) void k_guide_gather_hi(uint32_t *__restrict__ dst, const uint32_t *__restrict__ key_hi,
+                                                        const uint32_t *__restrict__ perm, uint32_t stage_cap)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i < stage_cap) dst[i] = key_hi[perm[i]];
+}
+
 __global__ __launch_bounds__(kT) void k_guide_flag(const dr_guide_sample *__restrict__ smp,
+                                                   const uint32_t *__restrict__ perm,
                                                    const uint32_t *__restrict__ counters, uint32_t stage_cap,
                                                    uint32_t *__restrict__ flags)
 {
     const uint32_t i = blockIdx.x * kT + threadIdx.x;
     if (i > stage_cap) return;
     const uint32_t n = min(counters[0], stage_cap);
-    flags[i] = (i < n && smp[i].weight > 0.0f) ? 1u : 0u;
+    flags[i] = (i < n && smp[perm[i]].weight > 0.0f) ? 1u : 0u;
 }
 
-// flags holds exclusive ranks (flags[stage_cap] = number of non-zero samples);
-// sample of rank r goes to ring slot (head + r) mod ring_cap. Only the first
-// ring_cap ranks are written, so every slot has one writer.
+// flags holds exclusive ranks over the key order (flags[stage_cap] = number
+// of non-zero samples); the sample of rank r goes to ring slot (head + r) mod
+// ring_cap. Only the first ring_cap ranks are written, so every slot has one
+// writer.
 __global__ __launch_bounds__(kT) void k_guide_scatter(const dr_guide_sample *__restrict__ smp,
+                                                      const uint32_t *__restrict__ perm,
                                                       const uint32_t *__restrict__ counters, uint32_t stage_cap,
                                                       const uint32_t *__restrict__ flags, dr_guide_sample *ring,
                                                       uint32_t ring_cap)
@@ -908,7 +937,7 @@ __global__ __launch_bounds__(kT) void k_guide_scatter(const dr_guide_sample *__r
     if (i >= n) return;
     const uint32_t r = flags[i];
     if (flags[i + 1] == r || r >= ring_cap) return;  // zero weight, or beyond one ring's worth
-    ring[(counters[1] + r) % ring_cap] = smp[i];
+    ring[(counters[1] + r) % ring_cap] = smp[perm[i]];
 }
 
 __global__ void k_guide_ring_advance(uint32_t *counters, const uint32_t *__restrict__ flags, uint32_t stage_cap,
@@ -1005,16 +1034,30 @@ dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const 
     return DR_OK;
 }
 
-dr_status guide_compact(Guide *g, hipStream_t stream)
+dr_status guide_compact(Guide *g, uint64_t total_items, hipStream_t stream)
 {
     const uint32_t S = g->stage_cap;
-    k_guide_flag<<<cdiv(S + 1, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), g->counters.as<uint32_t>(), S,
-                                                    g->flags.as<uint32_t>());
+    uint32_t *lo = g->keys.as<uint32_t>(), *hi = lo + S, *perm = g->perm.as<uint32_t>();
+    // keys are below total_items << 6; B bits hold them and the sentinel 2^B - 1
+    int B = 1;
+    while ((1ull << B) <= (total_items << 6)) ++B;
+    const uint64_t sent = (1ull << B) - 1;
+    k_guide_sort_init<<<cdiv(S, kT), kT, 0, stream>>>(lo, hi, perm, g->counters.as<uint32_t>(), S, (uint32_t)sent,
+                                                      (uint32_t)(sent >> 32));
+    DR_HIP_TRY(hipGetLastError());
+    if (dr_status st = radix_sort_pairs(g->ctx, lo, perm, S, std::min(B, 32), stream)) return st;
+    if (B > 32) {
+        k_guide_gather_hi<<<cdiv(S, kT), kT, 0, stream>>>(lo, hi, perm, S);
+        DR_HIP_TRY(hipGetLastError());
+        if (dr_status st = radix_sort_pairs(g->ctx, lo, perm, S, B - 32, stream)) return st;
+    }
+    k_guide_flag<<<cdiv(S + 1, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), perm,
+                                                    g->counters.as<uint32_t>(), S, g->flags.as<uint32_t>());
     DR_HIP_TRY(hipGetLastError());
     if (dr_status st = exclusive_scan_u32(g->ctx, g->flags.as<uint32_t>(), S + 1, stream)) return st;
-    k_guide_scatter<<<cdiv(S, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), g->counters.as<uint32_t>(), S,
-                                                   g->flags.as<uint32_t>(), g->ring.as<dr_guide_sample>(),
-                                                   g->prm.capacity);
+    k_guide_scatter<<<cdiv(S, kT), kT, 0, stream>>>(g->samples.as<dr_guide_sample>(), perm,
+                                                   g->counters.as<uint32_t>(), S, g->flags.as<uint32_t>(),
+                                                   g->ring.as<dr_guide_sample>(), g->prm.capacity);
     DR_HIP_TRY(hipGetLastError());
     k_guide_ring_advance<<<1, 64, 0, stream>>>(g->counters.as<uint32_t>(), g->flags.as<uint32_t>(), S,
                                                g->prm.capacity);
@@ -1075,6 +1118,8 @@ dr_status dr_guide_create(dr_context *cx, const dr_guide_params *params, dr_guid
     need(g->aux, sizeof(GuideAux) * (size_t)g->stage_cap);
     need(g->ring, sizeof(dr_guide_sample) * (size_t)p.capacity);
     need(g->flags, 4ull * ((size_t)g->stage_cap + 1));
+    need(g->keys, 8ull * g->stage_cap);
+    need(g->perm, 4ull * g->stage_cap);
     need(g->counters, 16);
     if (st) {
         delete g;
@@ -1171,9 +1216,19 @@ dr_status dr_guide_copy_samples(dr_guide *gd, dr_guide_sample *host, uint32_t ma
     const uint32_t kept = std::min(c[0], g->stage_cap);
     if (n_kept) *n_kept = kept;
     if (n_total) *n_total = c[0];
-    if (host && max_n)
-        DR_HIP_TRY(hipMemcpy(host, g->samples.ptr, sizeof(dr_guide_sample) * std::min(kept, max_n),
-                             hipMemcpyDeviceToHost));
+    if (host && max_n && kept) {
+        // in key order: the staged samples occupy slots [0, kept), and the
+        // first kept entries of perm are those slots sorted by key
+        std::vector<dr_guide_sample> stage(kept);
+        std::vector<uint32_t> order(kept);
+        DR_HIP_TRY(hipMemcpy(stage.data(), g->samples.ptr, sizeof(dr_guide_sample) * kept, hipMemcpyDeviceToHost));
+        DR_HIP_TRY(hipMemcpy(order.data(), g->perm.ptr, 4ull * kept, hipMemcpyDeviceToHost));
+        const uint32_t m = std::min(kept, max_n);
+        for (uint32_t i = 0; i < m; ++i) {
+            if (order[i] >= kept) return fail(DR_E_DEVICE, "stage order out of range");
+            host[i] = stage[order[i]];
+        }
+    }
     return DR_OK;
 }
 

@@ -169,11 +169,13 @@ struct SceneView {
 
 // Guide state seen by the shading kernel: head != NULL samples from the
 // guide (records written by the inference kernel for this vertex);
-// smp != NULL collects training samples.
+// smp != NULL collects training samples, and key[k] / key[cap + k] receive
+// the low / high word of slot k's order key.
 struct GuideView {
     const float *head;
     dr_guide_sample *smp;
     GuideAux *aux;
+    uint32_t *key;
     uint32_t *counter;
     uint32_t *pending;
     uint32_t cap;
@@ -398,6 +400,10 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
             x.beta[0] = beta3.x; x.beta[1] = beta3.y; x.beta[2] = beta3.z;
             x.lsnap[0] = x.lsnap[1] = x.lsnap[2] = 0.0f;
             gv.aux[k] = x;
+            // (path item, vertex) names the sample whatever slot it took
+            const uint64_t key = ((pp.chunk->item0 + path) << 6) | vertex;
+            gv.key[k] = (uint32_t)key;
+            gv.key[gv.cap + k] = (uint32_t)(key >> 32);
             gv.pending[path] = k;
         }
     }
@@ -823,6 +829,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     const bool use_guide = (j.guide_flags & DR_GUIDE_USE) != 0;
     const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
     if (j.total == 0) return DR_OK;
+    if (collect && j.total >= (1ull << 57)) return fail(DR_E_INVALID, "a collecting render takes fewer than 2^57 samples");
     const uint32_t cap = j.cap;
     if (s->rs_capacity < cap) {
         dr_status st = DR_OK;
@@ -857,6 +864,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     if (collect) {
         gv.smp = j.guide->samples.as<dr_guide_sample>();
         gv.aux = j.guide->aux.as<GuideAux>();
+        gv.key = j.guide->keys.as<uint32_t>();
         gv.counter = j.guide->counters.as<uint32_t>();
         gv.pending = s->rs_pending.as<uint32_t>();
         gv.cap = j.guide->stage_cap;
@@ -916,7 +924,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         DR_HIP_TRY(ee);
     }
     if (collect)
-        if (dr_status st = guide_compact(j.guide, stream)) return st;
+        if (dr_status st = guide_compact(j.guide, j.total, stream)) return st;
     return DR_OK;
 }
 

@@ -7,6 +7,14 @@ carried radiance enter the guide's training ring) and `train_steps` Adam
 steps on the ring follow on the device. Every pass's samples are unbiased
 estimates whatever the state of the network, so all passes are accumulated
 into the image. The plain sampler is the same loop with no guide.
+
+The collection fraction is sized so that the stage (4 x the guide's
+capacity) does not fill: the first collecting pass uses a fraction at which
+even paths that all reach max_depth would offer 3/4 of the stage, and the
+passes after it a fraction that offers 3/4 of the stage at the first pass's
+measured rate. The ring is filled in sample order, so with a stage that
+never fills, training on the same scene and seeds gives the same network
+bit for bit.
 """
 from __future__ import annotations
 
@@ -33,6 +41,7 @@ class ProgressiveRenderer:
         self.passes = 0
         self.trained = 0
         self._fraction_set = False
+        self._fraction = 1.0
 
     def _training(self) -> bool:
         return self.guide is not None and (self.train_passes is None or self.passes < self.train_passes)
@@ -50,14 +59,20 @@ class ProgressiveRenderer:
             train = self._training()
             if train:
                 flags |= DR_GUIDE_COLLECT
+            stage = 4 * self.guide.capacity
+            if train and not self._fraction_set:
+                # at most max_depth - 1 vertices of a path are eligible
+                bound = self.npix * self.spp_per_pass * max(1, self.max_depth - 1)
+                self._fraction = min(1.0, 0.75 * stage / bound)
+                self.guide.set_options(self._fraction, self.guide.guided_vertices, self.guide.train_batch)
             self.scene.render_guided_dev(self.camera, p, self.guide, flags, self.accum.ptr, self.stream)
             if train:
                 if not self._fraction_set:
                     _, total = self.guide.copy_samples(0)
                     if total > 0:
-                        self.guide.set_options(min(1.0, 4 * self.guide.capacity / total), self.guide.guided_vertices,
-                                               self.guide.train_batch)
-                    self._fraction_set = True
+                        self._fraction = min(1.0, 0.75 * stage * self._fraction / total)
+                        self.guide.set_options(self._fraction, self.guide.guided_vertices, self.guide.train_batch)
+                        self._fraction_set = True
                 if self.lr_schedule is not None:
                     self.guide.set_learning_rate(float(self.lr_schedule(self.passes)))
                 for _ in range(self.train_steps):

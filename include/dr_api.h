@@ -498,7 +498,12 @@ dr_status dr_guide_gradient(dr_guide *guide, const dr_guide_sample *samples, uin
 dr_status dr_guide_train_step(dr_guide *guide, const dr_guide_sample *samples, uint32_t n, dr_stream stream);
 /* Samples staged by the last collecting render, weights included (each such
  * render starts a new stage of at most 4 x capacity; *n_total counts all
- * offered). host may be NULL to query the counts. Synchronises. */
+ * offered). host may be NULL to query the counts. Synchronises.
+ * Order: by (path sample, vertex), the order in which they also enter the
+ * training ring, so a collecting render whose stage did not fill
+ * (*n_total <= 4 x capacity) stages and trains the same way on every run.
+ * When it fills, which of the offered samples it kept depends on the
+ * device's scheduling. */
 dr_status dr_guide_copy_samples(dr_guide *guide, dr_guide_sample *host, uint32_t max_n, uint32_t *n_kept,
                                 uint32_t *n_total);
 /* Fraction of eligible vertices a collecting render records (default 1);
