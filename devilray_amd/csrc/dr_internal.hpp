@@ -161,6 +161,31 @@ struct Scene {
 
 // ---- learned path guide (guide.hip) ----------------------------------------
 constexpr uint32_t kGuideNoSample = 0xFFFFFFFFu;
+
+// Smallest b with 2^b >= v (0 for v <= 1).
+__host__ __device__ inline uint32_t ceil_log2_u64(uint64_t v)
+{
+    uint32_t b = 0;
+    while (b < 64 && (1ull << b) < v) ++b;
+    return b;
+}
+
+// Order key of a staged training sample: a bijection of the path item on
+// [0, 2^item_bits) (odd multipliers and xor-shifts, each invertible modulo
+// 2^item_bits), then the vertex in the low vertex_bits. Keys are unique, and
+// the leading samples in key order are spread over the image rather than
+// taken from its first rows.
+__host__ __device__ inline uint64_t guide_sample_key(uint64_t item, uint32_t vertex, uint32_t item_bits,
+                                                     uint32_t vertex_bits)
+{
+    const uint64_t mask = item_bits >= 64 ? ~0ull : (1ull << item_bits) - 1ull;
+    const uint32_t sh = (item_bits + 1u) / 2u;
+    uint64_t x = (item * 0x9E3779B97F4A7C15ull) & mask;
+    x ^= x >> sh;
+    x = (x * 0xBF58476D1CE4E5B9ull) & mask;
+    x ^= x >> sh;
+    return (x << vertex_bits) | vertex;
+}
 constexpr int kGuideTrainBlocks = 256;  // partial-gradient slots (deterministic reduction)
 
 // Per training sample, what the render needs to complete it: the path slot
@@ -205,10 +230,10 @@ dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const 
 // Complete the training samples of a finished chunk from the path radiance.
 dr_status guide_finalize(Guide *g, const float4 *radiance, hipStream_t stream);
 // Append the staged samples of non-zero weight to the training ring in the
-// order of their keys ((path item << 6) | vertex, unique per sample), so the
-// ring does not depend on the order in which the stage's slots were taken.
-// total_items: path items of the render (bounds the keys).
-dr_status guide_compact(Guide *g, uint64_t total_items, hipStream_t stream);
+// order of their keys (guide_sample_key, unique per sample), so the ring does
+// not depend on the order in which the stage's slots were taken. key_bits:
+// item_bits + vertex_bits of the render's keys.
+dr_status guide_compact(Guide *g, uint32_t key_bits, hipStream_t stream);
 dr_status guide_reset_counters(Guide *g, hipStream_t stream);
 
 // ---- rendering (render.hip) ------------------------------------------------

@@ -886,8 +886,10 @@ __global__ __launch_bounds__(kT) void k_guide_finalize(dr_guide_sample *__restri
 }
 
 // Sort input over the whole stage: perm = identity, and the slots no sample
-// took get the sentinel key, above every sample's, so that after the sort
-// the first n positions of perm are the n staged samples in key order.
+// took get the sentinel key, the largest key of the key width. A sample's key
+// may equal it, but the sort is stable and such a sample's slot is below
+// every unused slot, so after the sort the first n positions of perm are the
+// n staged samples in key order.
 __global__ __launch_bounds__(kT) void k_guide_sort_init(uint32_t *__restrict__ key_lo, uint32_t *__restrict__ key_hi,
                                                         uint32_t *__restrict__ perm,
                                                         const uint32_t *__restrict__ counters, uint32_t stage_cap,
@@ -1034,14 +1036,13 @@ dr_status guide_infer_paths(Guide *g, const Scene *s, const dr_ray *rays, const 
     return DR_OK;
 }
 
-dr_status guide_compact(Guide *g, uint64_t total_items, hipStream_t stream)
+dr_status guide_compact(Guide *g, uint32_t key_bits, hipStream_t stream)
 {
     const uint32_t S = g->stage_cap;
     uint32_t *lo = g->keys.as<uint32_t>(), *hi = lo + S, *perm = g->perm.as<uint32_t>();
-    // keys are below total_items << 6; B bits hold them and the sentinel 2^B - 1
-    int B = 1;
-    while ((1ull << B) <= (total_items << 6)) ++B;
-    const uint64_t sent = (1ull << B) - 1;
+    if (key_bits > 64) return fail(DR_E_INVALID, "guide sample keys wider than 64 bits");
+    const int B = std::max(1, (int)key_bits);
+    const uint64_t sent = B >= 64 ? ~0ull : (1ull << B) - 1ull;
     k_guide_sort_init<<<cdiv(S, kT), kT, 0, stream>>>(lo, hi, perm, g->counters.as<uint32_t>(), S, (uint32_t)sent,
                                                       (uint32_t)(sent >> 32));
     DR_HIP_TRY(hipGetLastError());

@@ -180,6 +180,7 @@ struct GuideView {
     uint32_t *pending;
     uint32_t cap;
     float frac;
+    uint32_t item_bits, vertex_bits;
 };
 
 __device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
@@ -401,7 +402,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
             x.lsnap[0] = x.lsnap[1] = x.lsnap[2] = 0.0f;
             gv.aux[k] = x;
             // (path item, vertex) names the sample whatever slot it took
-            const uint64_t key = ((pp.chunk->item0 + path) << 6) | vertex;
+            const uint64_t key = guide_sample_key(pp.chunk->item0 + path, vertex, gv.item_bits, gv.vertex_bits);
             gv.key[k] = (uint32_t)key;
             gv.key[gv.cap + k] = (uint32_t)(key >> 32);
             gv.pending[path] = k;
@@ -829,7 +830,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     const bool use_guide = (j.guide_flags & DR_GUIDE_USE) != 0;
     const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
     if (j.total == 0) return DR_OK;
-    if (collect && j.total >= (1ull << 57)) return fail(DR_E_INVALID, "a collecting render takes fewer than 2^57 samples");
+    if (collect && j.total > (1ull << 58)) return fail(DR_E_INVALID, "a collecting render takes at most 2^58 samples");
     const uint32_t cap = j.cap;
     if (s->rs_capacity < cap) {
         dr_status st = DR_OK;
@@ -869,6 +870,8 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         gv.pending = s->rs_pending.as<uint32_t>();
         gv.cap = j.guide->stage_cap;
         gv.frac = j.guide->collect_fraction;
+        gv.item_bits = ceil_log2_u64(j.total);
+        gv.vertex_bits = ceil_log2_u64(j.prm->max_depth);
     }
     SceneView sv;
     sv.verts = s->verts.as<float>();
@@ -924,7 +927,7 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         DR_HIP_TRY(ee);
     }
     if (collect)
-        if (dr_status st = guide_compact(j.guide, j.total, stream)) return st;
+        if (dr_status st = guide_compact(j.guide, gv.item_bits + gv.vertex_bits, stream)) return st;
     return DR_OK;
 }
 

@@ -499,8 +499,10 @@ dr_status dr_guide_train_step(dr_guide *guide, const dr_guide_sample *samples, u
 /* Samples staged by the last collecting render, weights included (each such
  * render starts a new stage of at most 4 x capacity; *n_total counts all
  * offered). host may be NULL to query the counts. Synchronises.
- * Order: by (path sample, vertex), the order in which they also enter the
- * training ring, so a collecting render whose stage did not fill
+ * Order: by a key of (path sample, vertex) that hashes the path sample, the
+ * order in which they also enter the training ring (when a render offers
+ * more than the ring holds, the ring keeps the first in this order, spread
+ * over the image). So a collecting render whose stage did not fill
  * (*n_total <= 4 x capacity) stages and trains the same way on every run.
  * When it fills, which of the offered samples it kept depends on the
  * device's scheduling. */

@@ -333,6 +333,14 @@ def test_stage_order_is_reproducible(gpu_ctx):
         assert total == len(smp) and 0 < total <= 4 << 16  # the stage did not fill
         runs.append(smp)
     assert runs[0].tobytes() == runs[1].tobytes()
+    # the order hashes the path sample: its first half is spread over the
+    # image like its second half (in raster order the first half would be the
+    # top rows, whose first hits lie higher in the box)
+    s = runs[0]
+    h = len(s) // 2
+    lo, hi = sc.bounds()[:3], sc.bounds()[3:]
+    d = np.abs(s["pos"][:h].mean(0) - s["pos"][h:].mean(0)) / (hi - lo)
+    assert np.all(d < 0.02), d
     g.close()
     sc.close()
 
