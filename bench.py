@@ -67,7 +67,7 @@ COUNTS_FIXTURE = os.path.join(ROOT, "tests", "golden", "roofline_counts.json")
 TRAFFIC_FIXTURE = os.path.join(ROOT, "profiles", "trace_closest_c1_incoherent_traffic.json")
 # the sources that decide the traversal kernel's memory traffic (the kernel
 # and the tree it walks); counter runs record their hash
-TRAFFIC_SOURCES = ["devilray_amd/csrc/trace.hip", "devilray_amd/csrc/trace_device.hpp",
+TRAFFIC_SOURCES = ["devilray_amd/csrc/trace.hip", "devilray_amd/csrc/trace_device.hpp", "devilray_amd/csrc/node_quant.hpp",
                    "devilray_amd/csrc/dr_device.hpp", "devilray_amd/csrc/dr_internal.hpp",
                    "devilray_amd/csrc/bvh_build.hip"]
 

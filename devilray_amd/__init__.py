@@ -243,7 +243,7 @@ class Scene:
     def set_schedule(self, closest: int, any_hit: int | None = None):
         check(lib().dr_scene_set_trace_schedule(self.handle, closest, closest if any_hit is None else any_hit))
 
-    def set_tuning(self, refill_min: int = 16, blocks_per_cu: int = 5, sort_key: int = 0):
+    def set_tuning(self, refill_min: int = 16, blocks_per_cu: int = 6, sort_key: int = 0):
         check(lib().dr_scene_set_trace_tuning(self.handle, refill_min, blocks_per_cu, sort_key))
 
     def build_lights(self, stream=None):

@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "libdevilray.so")
 
 SOURCES = ["api.hip", "sort.hip", "bvh_build.hip", "trace.hip", "light.hip", "render.hip", "guide.hip", "batch.hip",
            "display.hip", "instance.hip", "gen.cpp"]
-HEADERS = ["dr_internal.hpp", "dr_device.hpp", "trace_device.hpp", "philox.hpp", "shade.hpp", "guide_device.hpp"]
+HEADERS = ["dr_internal.hpp", "dr_device.hpp", "trace_device.hpp", "node_quant.hpp", "philox.hpp", "shade.hpp", "guide_device.hpp"]
 
 # -ffp-contract=off: products and sums round as written, so the triangle test
 # matches a CPU evaluation bit for bit; FMAs are written explicitly.

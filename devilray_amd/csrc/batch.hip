@@ -46,12 +46,12 @@ __global__ __launch_bounds__(kT) void k_rebase_triv(const float4 *__restrict__ s
 }
 
 // interior references move by node_off, leaf references by ref_off
-__global__ __launch_bounds__(kT) void k_rebase_nodes(const WideNode *__restrict__ src, uint32_t n, uint32_t node_off,
-                                                     uint32_t ref_off, WideNode *__restrict__ dst)
+__global__ __launch_bounds__(kT) void k_rebase_nodes(const QNode *__restrict__ src, uint32_t n, uint32_t node_off,
+                                                     uint32_t ref_off, QNode *__restrict__ dst)
 {
     const uint32_t i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
-    WideNode w = src[i];
+    QNode w = src[i];
     for (int c = 0; c < 4; ++c) {
         const int32_t r = w.child[c];
         if (r >= 0) w.child[c] = r + (int32_t)node_off;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kT) void k_rebase_alias(const AliasEntryB *__restri
 
 }  // namespace
 
-dr_status rebase_nodes(const WideNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, WideNode *dst,
+dr_status rebase_nodes(const QNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, QNode *dst,
                        hipStream_t stream)
 {
     if (n == 0) return DR_OK;
@@ -129,7 +129,7 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
     need(m.tri_mat, 4ull * nt);
     need(m.materials, sizeof(dr_material) * nm);
     need(m.tri_v, 48ull * nr);
-    need(m.nodes, sizeof(WideNode) * nn);
+    need(m.nodes, sizeof(QNode) * nn);
     need(m.light_pdf_area, 4ull * nt);
     need(m.alias, sizeof(AliasEntryB) * std::max<uint64_t>(ne, 1));
     need(m.flags, 2048);
@@ -157,8 +157,8 @@ dr_status merge(Batch *b, Scene *const *sc, uint32_t n)
             DR_HIP_TRY(hipMemcpyAsync(m.light_pdf_area.as<float>() + to, s->light_pdf_area.ptr, 4ull * s->n_tris,
                                       hipMemcpyDeviceToDevice, s0));
         }
-        if (dr_status st2 = rebase_nodes(s->nodes.as<WideNode>(), s->n_nodes, (uint32_t)no, (uint32_t)ro,
-                                         m.nodes.as<WideNode>() + no, s0))
+        if (dr_status st2 = rebase_nodes(s->nodes.as<QNode>(), s->n_nodes, (uint32_t)no, (uint32_t)ro,
+                                         m.nodes.as<QNode>() + no, s0))
             return st2;
         if (s->n_emit)
             k_rebase_alias<<<blocks(s->n_emit), kT, 0, s0>>>(s->alias.as<AliasEntryB>(), s->n_emit, (uint32_t)to,

@@ -22,6 +22,7 @@
 // Child boxes are rounded outward by one ulp so every vertex lies strictly
 // inside its leaf box.
 #include "dr_internal.hpp"
+#include "node_quant.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -410,6 +411,29 @@ __global__ void k_single_leaf_root(const float4 *__restrict__ lo, const float4 *
     wn.child[0] = make_leaf_ref(0u, n);
     wn.pad[0] = 1;
     *wide = wn;
+}
+
+// QNode copies of the wide nodes; *err is set if a node's planes are out of
+// range (a vertex coordinate of magnitude 2^126 or more).
+__global__ __launch_bounds__(kT) void k_quantize(const WideNode *__restrict__ src, uint32_t n, QNode *__restrict__ dst,
+                                                 uint32_t *__restrict__ err)
+{
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const WideNode w = src[i];
+    QNode q;
+    bool use[4];
+    for (int c = 0; c < 4; ++c)
+        use[c] = w.child[c] != DR_EMPTY_REF && w.lox[c] <= w.hix[c] && w.loy[c] <= w.hiy[c] && w.loz[c] <= w.hiz[c];
+    uint32_t ex, ey, ez;
+    bool ok = quantize_axis(w.lox, w.hix, use, q.origin[0], ex, q.qlo[0], q.qhi[0]);
+    ok = quantize_axis(w.loy, w.hiy, use, q.origin[1], ey, q.qlo[1], q.qhi[1]) && ok;
+    ok = quantize_axis(w.loz, w.hiz, use, q.origin[2], ez, q.qlo[2], q.qhi[2]) && ok;
+    q.exps = ex | (ey << 8) | (ez << 16);
+    q.pad[0] = q.pad[1] = 0u;
+    for (int c = 0; c < 4; ++c) q.child[c] = w.child[c];
+    dst[i] = q;
+    if (!ok) atomicOr(err, 1u);
 }
 
 __global__ void k_init_counters(uint32_t *c, int32_t *cb, int32_t *rb)
@@ -857,6 +881,7 @@ __global__ void k_init_bounds(int32_t *b)
         b[t] = 0x7FFFFFFF;
         b[3 + t] = (int32_t)0x80000000;
     }
+    if (t == 6) b[6] = 0;  // refit: quantisation error flag
 }
 
 // box of the finite vertex coordinates (ordered ints)
@@ -956,7 +981,7 @@ __global__ __launch_bounds__(kT) void k_refit_level(WideNode *__restrict__ nodes
 dr_status refit_bvh(Scene *s, hipStream_t stream)
 {
     if (!s->built || s->level_ofs.size() < 2) return fail(DR_E_INVALID, "scene BVH not built");
-    if (dr_status st = s->refit_tmp.ensure(6 * sizeof(int32_t))) return st;
+    if (dr_status st = s->refit_tmp.ensure(8 * sizeof(int32_t))) return st;
     int32_t *vb = s->refit_tmp.as<int32_t>();
     k_init_bounds<<<1, 64, 0, stream>>>(vb);
     k_vertex_bounds<<<std::min(blocks(s->n_verts), kBoundsBlocks), kT, 0, stream>>>(s->verts.as<float>(), s->n_verts,
@@ -965,13 +990,17 @@ dr_status refit_bvh(Scene *s, hipStream_t stream)
     for (size_t d = s->level_ofs.size() - 1; d-- > 0;) {
         const uint32_t lo = s->level_ofs[d], hi = s->level_ofs[d + 1];
         if (hi <= lo) continue;
-        k_refit_level<<<blocks(hi - lo), kT, 0, stream>>>(s->nodes.as<WideNode>(), lo, hi, s->verts.as<float>(),
+        k_refit_level<<<blocks(hi - lo), kT, 0, stream>>>(s->bnodes.as<WideNode>(), lo, hi, s->verts.as<float>(),
                                                           s->indices.as<uint32_t>(), s->tri_v.as<float4>(), vb);
         DR_HIP_TRY(hipGetLastError());
     }
-    int32_t b[6];
+    k_quantize<<<blocks(s->n_nodes), kT, 0, stream>>>(s->bnodes.as<WideNode>(), s->n_nodes, s->nodes.as<QNode>(),
+                                                      reinterpret_cast<uint32_t *>(vb + 6));
+    DR_HIP_TRY(hipGetLastError());
+    int32_t b[7];
     DR_HIP_TRY(hipMemcpyAsync(b, vb, sizeof(b), hipMemcpyDeviceToHost, stream));
     DR_HIP_TRY(hipStreamSynchronize(stream));
+    if (b[6]) return fail(DR_E_INVALID, "vertex coordinates must be below 2^126 in magnitude (rebuild or refit again)");
     for (int a = 0; a < 6; ++a) {
         const int32_t i = b[a] >= 0 ? b[a] : b[a] ^ 0x7FFFFFFF;
         float f;
@@ -1085,7 +1114,8 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     // Wide nodes with a wide child have 4 children, the others only leaf
     // children (>= 2), so W <= 2L/3 + 1 <= 2n/3 + 1.
     const uint32_t wide_cap = (uint32_t)(2ull * n / 3 + 4);
-    need(s->nodes, sizeof(WideNode) * (size_t)wide_cap);
+    need(s->bnodes, sizeof(WideNode) * (size_t)wide_cap);
+    need(s->nodes, sizeof(QNode) * (size_t)wide_cap);
     if (st) return st;
 
     k_morton<<<blocks(n), kT, 0, stream>>>(t.rlo, t.rhi, n, ref_bounds, t.keys.as<uint32_t>(), t.vals.as<uint32_t>());
@@ -1099,7 +1129,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
     const float ct = 1.0f, ci = 1.0f;
     uint32_t n_nodes = 1, n_leaves = 1, depth = 1;
     if (n == 1) {
-        k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), 1, s->nodes.as<WideNode>(),
+        k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), 1, s->bnodes.as<WideNode>(),
                                                 nullptr, nullptr);
         DR_HIP_TRY(hipGetLastError());
         float4 lo, hi;
@@ -1139,7 +1169,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
         for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
         if (root_collapse) {
             k_single_leaf_root<<<1, 1, 0, stream>>>(t.leaf_lo.as<float4>(), t.leaf_hi.as<float4>(), n,
-                                                    s->nodes.as<WideNode>(), t.node_lo.as<float4>(), t.node_hi.as<float4>());
+                                                    s->bnodes.as<WideNode>(), t.node_lo.as<float4>(), t.node_hi.as<float4>());
             DR_HIP_TRY(hipGetLastError());
             s->level_ofs = {0u, 1u};
         } else {
@@ -1155,7 +1185,7 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
                                                           t.count.as<uint32_t>(), t.node_lo.as<float4>(),
                                                           t.node_hi.as<float4>(), t.leaf_lo.as<float4>(),
                                                           t.leaf_hi.as<float4>(), t.collapse.as<uint32_t>(),
-                                                          s->nodes.as<WideNode>(), wide_cap);
+                                                          s->bnodes.as<WideNode>(), wide_cap);
                 DR_HIP_TRY(hipGetLastError());
                 uint32_t c[4];
                 DR_HIP_TRY(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, stream));
@@ -1174,8 +1204,15 @@ dr_status build_bvh(Scene *s, const dr_bvh_params *p, hipStream_t stream)
             }
         }
     }
+    // the traversal's quantised copy; counters[4] flags planes out of range
+    k_quantize<<<blocks(n_nodes), kT, 0, stream>>>(s->bnodes.as<WideNode>(), n_nodes, s->nodes.as<QNode>(),
+                                                   counters + 4);
+    DR_HIP_TRY(hipGetLastError());
     DR_HIP_TRY(hipEventRecord(e1, stream));
     DR_HIP_TRY(hipEventSynchronize(e1));
+    uint32_t qerr = 0;
+    DR_HIP_TRY(hipMemcpy(&qerr, counters + 4, 4, hipMemcpyDeviceToHost));
+    if (qerr) return fail(DR_E_INVALID, "vertex coordinates must be below 2^126 in magnitude");
     float ms = 0.0f;
     DR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     hipEventDestroy(e0);

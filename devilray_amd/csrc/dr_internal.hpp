@@ -48,6 +48,22 @@ struct alignas(16) WideNode {
 };
 static_assert(sizeof(WideNode) == 128, "wide node is 128 B");
 
+// The traversal's copy of a wide node: the child boxes quantised to one byte
+// per plane on a grid per axis (node_quant.hpp). Axis a's planes are
+// origin[a] + q * step[a], q = 0..255, step[a] = 2^(exponent byte a - 127);
+// every plane is an exact float, and a child's quantised box contains its
+// WideNode box. A child slot with an empty box (or none) has q = 255 below
+// and 0 above.
+struct alignas(64) QNode {
+    float origin[3];
+    uint32_t exps;     // biased exponents of the steps: x in bits 0-7, y 8-15, z 16-23
+    uint32_t qlo[3];   // per axis x, y, z: child c's lower plane index in byte c
+    uint32_t qhi[3];   // per axis: child c's upper plane index in byte c
+    uint32_t pad[2];
+    int32_t child[4];
+};
+static_assert(sizeof(QNode) == 64, "quantised node is 64 B");
+
 constexpr int32_t DR_EMPTY_REF = -1;  // 0xFFFFFFFF
 constexpr int kLeafMaxCap = 8;
 constexpr uint32_t kMaxLeafRefs = (1u << 28) - 2;  // triangle references per scene or batch
@@ -96,7 +112,8 @@ struct Scene {
     std::vector<dr_material> host_materials;
     // acceleration structure
     DevBuf tri_v;     // 3 float4 per triangle reference, leaf order; w of the first = id bits
-    DevBuf nodes;     // WideNode
+    DevBuf nodes;     // QNode: what the traversal reads
+    DevBuf bnodes;    // WideNode: the same nodes with float boxes (build and refit)
     uint32_t n_refs = 0;  // tri_v entries: triangles plus split pieces
     // two-level forest (instance.hip): nodes [0, n_top) are a tree over the
     // instances whose leaf references name an InstRec, then the instanced
@@ -116,11 +133,10 @@ struct Scene {
     uint32_t sched_closest = DR_SCHED_PERSISTENT_SPECULATIVE;  // traversal schedules
     uint32_t sched_any = DR_SCHED_PERSISTENT_SPECULATIVE;
     uint32_t refill_min = 16;  // persistent waves refill when this many lanes are idle
-    // Persistent grid: resident blocks per CU (0 = occupancy limit, 6 for the
-    // closest-hit kernel). One block fewer than the limit keeps more of each
-    // wave's nodes in L1/L2: 699 against 670 Mrays/s on incoherent rays over
-    // the split 1.3M-triangle icosphere (7 and 8 waves per SIMD were slower).
-    uint32_t blocks_per_cu = 5;
+    // Persistent grid: resident blocks per CU (0 = occupancy limit). On
+    // incoherent rays over the split 1.3M-triangle icosphere (closest hit,
+    // 68 registers, 7 blocks fit): 1.377, 1.344 and 1.347 ms for 5, 6 and 7.
+    uint32_t blocks_per_cu = 6;
     uint32_t sort_key = 0;       // ray reorder key: 0 origin, 1 direction + origin
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (lo, hi), set by the BVH build
     DevBuf ray_keys, ray_perm;             // ray reordering scratch
@@ -296,7 +312,7 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
 
 // batch.hip: copy n wide nodes, moving interior references by node_off and
 // leaf references by ref_off
-dr_status rebase_nodes(const WideNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, WideNode *dst,
+dr_status rebase_nodes(const QNode *src, uint32_t n, uint32_t node_off, uint32_t ref_off, QNode *dst,
                        hipStream_t stream);
 
 // bvh_build.hip

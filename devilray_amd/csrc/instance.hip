@@ -109,12 +109,12 @@ dr_status build_forest(Instances *I, Scene *const *sc, uint32_t n_sc, const dr_i
     bp.flags = DR_BVH_NO_SPLIT;
     if ((st = build_bvh(&top, &bp, nullptr))) return st;
     const uint32_t n_top = top.n_nodes;
-    std::vector<WideNode> tn(n_top);
+    std::vector<QNode> tn(n_top);
     // leaf-ordered references, 12 words each; word 3 holds the id
     std::vector<uint32_t> tref(12ull * top.n_refs);
-    DR_HIP_TRY(hipMemcpy(tn.data(), top.nodes.ptr, sizeof(WideNode) * n_top, hipMemcpyDeviceToHost));
+    DR_HIP_TRY(hipMemcpy(tn.data(), top.nodes.ptr, sizeof(QNode) * n_top, hipMemcpyDeviceToHost));
     DR_HIP_TRY(hipMemcpy(tref.data(), top.tri_v.ptr, sizeof(uint32_t) * tref.size(), hipMemcpyDeviceToHost));
-    for (WideNode &w : tn)
+    for (QNode &w : tn)
         for (int c = 0; c < 4; ++c) {
             const int32_t r = w.child[c];
             if (r >= 0 || r == DR_EMPTY_REF) continue;
@@ -135,17 +135,17 @@ dr_status build_forest(Instances *I, Scene *const *sc, uint32_t n_sc, const dr_i
     f.n_tris = (uint32_t)total_tris;
     f.n_leaves = 0;
     for (uint32_t k = 0; k < n_sc; ++k) f.n_leaves += sc[k]->n_leaves;
-    if (!st) st = f.nodes.ensure(sizeof(WideNode) * (size_t)f.n_nodes);
+    if (!st) st = f.nodes.ensure(sizeof(QNode) * (size_t)f.n_nodes);
     if (!st) st = f.tri_v.ensure(48ull * std::max<uint64_t>(nr, 1));
     if (!st) st = f.inst_recs.ensure(sizeof(InstRec) * (size_t)n_in);
     if (!st) st = f.flags.ensure(2048);
     if (st) return st;
     DR_HIP_TRY(hipMemset(f.flags.ptr, 0, 2048));
-    DR_HIP_TRY(hipMemcpy(f.nodes.ptr, tn.data(), sizeof(WideNode) * n_top, hipMemcpyHostToDevice));
+    DR_HIP_TRY(hipMemcpy(f.nodes.ptr, tn.data(), sizeof(QNode) * n_top, hipMemcpyHostToDevice));
     for (uint32_t k = 0; k < n_sc; ++k) {
         const Scene *s = sc[k];
-        if ((st = rebase_nodes(s->nodes.as<WideNode>(), s->n_nodes, (uint32_t)(n_top + node_off[k]),
-                               (uint32_t)ref_off[k], f.nodes.as<WideNode>() + n_top + node_off[k], nullptr)))
+        if ((st = rebase_nodes(s->nodes.as<QNode>(), s->n_nodes, (uint32_t)(n_top + node_off[k]),
+                               (uint32_t)ref_off[k], f.nodes.as<QNode>() + n_top + node_off[k], nullptr)))
             return st;
         if (s->n_refs)
             DR_HIP_TRY(hipMemcpy(f.tri_v.as<float4>() + 3 * ref_off[k], s->tri_v.ptr, 48ull * s->n_refs,

@@ -568,7 +568,7 @@ struct PersistArgs {
     uint32_t *pview, *proot, *sroot;
     unsigned long long *casts;
     ChunkState *chunk;
-    const WideNode *nodes;
+    const QNode *nodes;
     const float4 *tri_v;
     int32_t *ovf;
     uint32_t *flags;
@@ -765,7 +765,7 @@ dr_status run_persistent(const RenderJob &j, const PathParams &pp, const SceneVi
     a.sroot = j.scenes ? s->rs_sroot.as<uint32_t>() : nullptr;
     a.casts = s->ray_casts.as<unsigned long long>();
     a.chunk = s->rs_chunk.as<ChunkState>();
-    a.nodes = s->nodes.as<WideNode>();
+    a.nodes = s->nodes.as<QNode>();
     a.tri_v = s->tri_v.as<float4>();
     a.ovf = s->ovf_stack.as<int32_t>();
     a.flags = s->flags.as<uint32_t>();

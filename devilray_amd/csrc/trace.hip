@@ -13,7 +13,7 @@ constexpr uint32_t kPerRaySpillMax = 48;
 constexpr uint32_t kCounterSlots = 256;
 
 template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE, bool INST = false>
-__global__ __launch_bounds__(kBlock) void k_trace(const WideNode *__restrict__ nodes,
+__global__ __launch_bounds__(kBlock) void k_trace(const QNode *__restrict__ nodes,
                                                   const float4 *__restrict__ tri_v,
                                                   const float4 *__restrict__ rays, const uint32_t *__restrict__ perm,
                                                   uint32_t n_max, const uint32_t *__restrict__ n_dev,
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const float4 *__restrict__ 
     idx[i] = i;
 }
 
-using TraceKernel = void (*)(const WideNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
+using TraceKernel = void (*)(const QNode *, const float4 *, const float4 *, const uint32_t *, uint32_t,
                             const uint32_t *, dr_hit *,
                             uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *, uint32_t *, uint32_t,
                             const uint32_t *, uint32_t, uint32_t, const InstRec *);
@@ -182,7 +182,7 @@ dr_status launch_trace_noalloc(Scene *s, const dr_ray *rays, uint32_t n, dr_hit 
     uint32_t *flags = reinterpret_cast<uint32_t *>(base);
     uint32_t *counter = flags + 8;
     k_zero_counters8<<<1, 64, 0, stream>>>(counter);
-    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(),
+    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<QNode>(), s->tri_v.as<float4>(),
                                       reinterpret_cast<const float4 *>(rays), nullptr, n, nullptr, hits, nullptr,
                                       nullptr, nullptr, base + kNoAllocHeadWords, flags, counter, s->refill_min,
                                       nullptr, 1u, max_stack, nullptr);
@@ -263,7 +263,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     if (count)
         for (int k = 2; k <= 6; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
     int32_t *ovf = s->ovf_stack.as<int32_t>();
-    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<WideNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
+    kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<QNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
                                       box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices,
                                       max_stack, inst ? s->inst_recs.as<InstRec>() : nullptr);
     DR_HIP_TRY(hipGetLastError());

@@ -21,8 +21,11 @@
 // is a child of a node on the path from the root to the node being visited,
 // so a tree of D wide levels never needs more than 3 D entries: the host sizes
 // the spill region from the built depth, and no ray is ever cut short. At an
-// interior node all four child boxes are tested with the robust slab test; near and far planes are
-// loaded by the ray's octant so no per-slab min/max is needed. Hit children
+// interior node (a 64-byte QNode) the four child boxes are decoded from their
+// plane bytes (one exact fma per plane) and tested with the robust slab test;
+// near and far planes are picked by the ray's octant so no per-slab min/max is
+// needed. The quantised boxes contain the exact ones, so the hits are those
+// of the float boxes; the node is half the bytes of a float node. Hit children
 // are sorted by entry distance, the nearest is taken next and the rest are
 // pushed far to near. Leaves hold up to 8 triangles stored as 3 float4.
 //
@@ -112,7 +115,7 @@ __device__ __forceinline__ void intersect_leaf(const float4 *__restrict__ tri_v,
 template <bool ANY, bool COUNT, bool PERSISTENT, bool SPECULATIVE, bool INST = false>
 __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock], uint32_t tid, uint32_t bid,
                                            uint32_t nblk,
-                                           const WideNode *__restrict__ nodes, const float4 *__restrict__ tri_v,
+                                           const QNode *__restrict__ nodes, const float4 *__restrict__ tri_v,
                                            const float4 *__restrict__ rays, const uint32_t *__restrict__ perm,
                                            uint32_t n_max, const uint32_t *__restrict__ n_dev,
                                            dr_hit *__restrict__ hits, uint32_t *__restrict__ occ,
@@ -145,7 +148,7 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
     int sp = 0;
     int32_t node = kNone, leaf = kNone;
     uint32_t nbox = 0;
-    uint32_t nx = 0, ny = 32, nz = 64;
+    bool negx = false, negy = false, negz = false;  // the ray's octant: the upper plane of the axis is near
     bool overflow = false, any_overflow = false;
     uint32_t high_water = 0;  // counted variant: deepest stack of the lane
     const char *nbase = reinterpret_cast<const char *>(nodes);
@@ -154,11 +157,10 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
     float wox = 0.0f, woy = 0.0f, woz = 0.0f, wdx = 0.0f, wdy = 0.0f, wdz = 0.0f;
     bool in_inst = false;
     uint32_t id_add = 0;
-    // byte offsets of the near slab planes for the ray's octant
     auto set_octant = [&]() {
-        nx = signbit(p.ix) ? 16u : 0u;
-        ny = signbit(p.iy) ? 48u : 32u;
-        nz = signbit(p.iz) ? 80u : 64u;
+        negx = signbit(p.ix);
+        negy = signbit(p.iy);
+        negz = signbit(p.iz);
     };
 
     auto pop = [&]() -> int32_t {
@@ -246,18 +248,32 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
                     ++node_visits;
                     if (leads_wave()) ++wave_node_iters;
                 }
-                const uint32_t nb = (uint32_t)cur * (uint32_t)sizeof(WideNode);
-                const float4 nxv = *reinterpret_cast<const float4 *>(nbase + nb + nx);
-                const float4 fxv = *reinterpret_cast<const float4 *>(nbase + nb + (nx ^ 16u));
-                const float4 nyv = *reinterpret_cast<const float4 *>(nbase + nb + ny);
-                const float4 fyv = *reinterpret_cast<const float4 *>(nbase + nb + (ny ^ 16u));
-                const float4 nzv = *reinterpret_cast<const float4 *>(nbase + nb + nz);
-                const float4 fzv = *reinterpret_cast<const float4 *>(nbase + nb + (nz ^ 16u));
-                const int4 ch = *reinterpret_cast<const int4 *>(nbase + nb + 96u);
+                // a QNode: origin and step exponents, the plane bytes
+                // (lower x y z, upper x | upper y z, pad), the children
+                const char *nd = nbase + (size_t)(uint32_t)cur * sizeof(QNode);
+                const float4 org = *reinterpret_cast<const float4 *>(nd);
+                const uint4 qa = *reinterpret_cast<const uint4 *>(nd + 16);
+                const uint4 qb = *reinterpret_cast<const uint4 *>(nd + 32);
+                const int4 ch = *reinterpret_cast<const int4 *>(nd + 48);
                 const float lim = (ANY ? p.tmax : st.bt) * kBestSlack;
-                const float ax[4] = {nxv.x, nxv.y, nxv.z, nxv.w}, bx[4] = {fxv.x, fxv.y, fxv.z, fxv.w};
-                const float ay[4] = {nyv.x, nyv.y, nyv.z, nyv.w}, by[4] = {fyv.x, fyv.y, fyv.z, fyv.w};
-                const float az[4] = {nzv.x, nzv.y, nzv.z, nzv.w}, bz[4] = {fzv.x, fzv.y, fzv.z, fzv.w};
+                const uint32_t ex = __float_as_uint(org.w);
+                const float sx = __uint_as_float((ex & 0xFFu) << 23);
+                const float sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23);
+                const float sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
+                const uint32_t wnx = negx ? qa.w : qa.x, wfx = negx ? qa.x : qa.w;
+                const uint32_t wny = negy ? qb.x : qa.y, wfy = negy ? qa.y : qb.x;
+                const uint32_t wnz = negz ? qb.y : qa.z, wfz = negz ? qa.z : qb.y;
+                // exact planes: q * step + origin has no rounding
+                float ax[4], bx[4], ay[4], by[4], az[4], bz[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    ax[c] = fmaf((float)((wnx >> (8 * c)) & 0xFFu), sx, org.x);
+                    bx[c] = fmaf((float)((wfx >> (8 * c)) & 0xFFu), sx, org.x);
+                    ay[c] = fmaf((float)((wny >> (8 * c)) & 0xFFu), sy, org.y);
+                    by[c] = fmaf((float)((wfy >> (8 * c)) & 0xFFu), sy, org.y);
+                    az[c] = fmaf((float)((wnz >> (8 * c)) & 0xFFu), sz, org.z);
+                    bz[c] = fmaf((float)((wfz >> (8 * c)) & 0xFFu), sz, org.z);
+                }
                 float tn[4];
                 int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
@@ -268,8 +284,7 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
                     const float nr = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, p.tmin));
                     float fr = fminf(fminf(t1x, t1y), t1z) * kFarScale;
                     fr = fminf(fr, lim);
-                    // empty slots carry boxes at +inf and never pass; the ref
-                    // check keeps that true for any ray
+                    // empty slots never pass (the ref check)
                     tn[c] = (nr <= fr && r[c] != kNone) ? nr : INFINITY;
                 }
                 if (COUNT) nbox += (ch.x != kNone) + (ch.y != kNone) + (ch.z != kNone) + (ch.w != kNone);

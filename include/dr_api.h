@@ -208,7 +208,7 @@ enum {
 dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, uint32_t any_sched);
 /* Tuning knobs (chosen by the autotuner): persistent waves refill their
  * finished lanes once at least refill_min (1..64, default 16) are idle;
- * blocks_per_cu caps the resident grid (default 5; 0 = the occupancy limit);
+ * blocks_per_cu caps the resident grid (default 6; 0 = the occupancy limit);
  * sort_key selects the reorder
  * key (0: dominant axis + origin Morton code, 1: dominant axis + quantised
  * direction + origin). */

@@ -1,0 +1,21 @@
+"""The traversal's node boxes are quantised to one byte per plane
+(devilray_amd/csrc/node_quant.hpp). Hits stay exact only if every quantised
+box contains the float box it replaces and the planes decode without
+rounding; tests/quant_check.cpp checks both, and that the grid step is the
+smallest possible, on random and edge-case boxes, compiled with the host
+compiler."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_quantised_boxes_contain_and_decode_exactly(tmp_path):
+    exe = str(tmp_path / "quant_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "quant_check.cpp")], check=True)
+    out = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.startswith("ok "), out.stdout + out.stderr
