@@ -425,12 +425,9 @@ __global__ __launch_bounds__(kT) void k_quantize(const WideNode *__restrict__ sr
     bool use[4];
     for (int c = 0; c < 4; ++c)
         use[c] = w.child[c] != DR_EMPTY_REF && w.lox[c] <= w.hix[c] && w.loy[c] <= w.hiy[c] && w.loz[c] <= w.hiz[c];
-    uint32_t ex, ey, ez;
-    bool ok = quantize_axis(w.lox, w.hix, use, q.origin[0], ex, q.qlo[0], q.qhi[0]);
-    ok = quantize_axis(w.loy, w.hiy, use, q.origin[1], ey, q.qlo[1], q.qhi[1]) && ok;
-    ok = quantize_axis(w.loz, w.hiz, use, q.origin[2], ez, q.qlo[2], q.qhi[2]) && ok;
-    q.exps = ex | (ey << 8) | (ez << 16);
-    q.pad[0] = q.pad[1] = 0u;
+    bool ok = quantize_axis(w.lox, w.hix, use, q.origin[0], q.step[0], q.qlo[0], q.qhi[0]);
+    ok = quantize_axis(w.loy, w.hiy, use, q.origin[1], q.step[1], q.qlo[1], q.qhi[1]) && ok;
+    ok = quantize_axis(w.loz, w.hiz, use, q.origin[2], q.step[2], q.qlo[2], q.qhi[2]) && ok;
     for (int c = 0; c < 4; ++c) q.child[c] = w.child[c];
     dst[i] = q;
     if (!ok) atomicOr(err, 1u);

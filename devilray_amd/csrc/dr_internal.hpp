@@ -50,16 +50,14 @@ static_assert(sizeof(WideNode) == 128, "wide node is 128 B");
 
 // The traversal's copy of a wide node: the child boxes quantised to one byte
 // per plane on a grid per axis (node_quant.hpp). Axis a's planes are
-// origin[a] + q * step[a], q = 0..255, step[a] = 2^(exponent byte a - 127);
-// every plane is an exact float, and a child's quantised box contains its
-// WideNode box. A child slot with an empty box (or none) has q = 255 below
-// and 0 above.
+// origin[a] + q * step[a], q = 0..255; every plane is an exact float, and a
+// child's quantised box contains its WideNode box. A child slot with an empty
+// box (or none) has q = 255 below and 0 above.
 struct alignas(64) QNode {
     float origin[3];
-    uint32_t exps;     // biased exponents of the steps: x in bits 0-7, y 8-15, z 16-23
+    float step[3];
     uint32_t qlo[3];   // per axis x, y, z: child c's lower plane index in byte c
     uint32_t qhi[3];   // per axis: child c's upper plane index in byte c
-    uint32_t pad[2];
     int32_t child[4];
 };
 static_assert(sizeof(QNode) == 64, "quantised node is 64 B");

@@ -1,13 +1,16 @@
 // Quantisation of a wide node's child boxes to one byte per plane (QNode).
 //
-// Per axis, the children's planes are put on a grid origin + q * 2^e,
-// q = 0..255. e is the smallest exponent (at least -100, so every nonzero
-// plane is a normal float) for which the grid from origin = floor(lo / 2^e) *
-// 2^e spans the highest plane in 255 steps with |origin| below 2^24 steps.
-// Then every grid point is an exact float, and fma(q, 2^e, origin) computes it
-// with no rounding. Lower planes round down and upper planes round up, so the
-// quantised box of a child contains its box. The double arithmetic is exact
-// (scaling by powers of two, floor and ceil of floats).
+// Per axis, the children's planes are put on a grid origin + q * step,
+// q = 0..255, with step = m * 2^e (m an integer below 2^8) and origin =
+// k * 2^e (k an integer). Every grid point is then (k + q m) * 2^e, an exact
+// float while |k| + 255 m < 2^24, so fma(q, step, origin) computes it with no
+// rounding. The step is the smallest of that form whose grid, from the
+// multiple of 2^e at or below the lowest plane, reaches the highest plane in
+// 255 steps, so the grid uses almost the whole byte range; e >= -100 keeps
+// every nonzero plane a normal float. Lower planes round down and upper planes
+// round up, so the quantised box of a child contains its box. The double
+// arithmetic is exact (integers below 2^53 times powers of two, floor and
+// ceil of floats and of their differences with grid points).
 //
 // Only standard headers: the CPU test of this file compiles it with the
 // host compiler.
@@ -21,12 +24,12 @@ namespace dr {
 // Planes of magnitude at or above this are not quantised (the build fails).
 constexpr double kQuantMaxPlane = 0x1p126;
 
-// One axis: lo[c], hi[c] for the children with use[c]; writes the grid origin,
-// the biased exponent (e + 127) and the bytes of each child's lower / upper
-// plane index (byte c). A child without a box gets q = 255 below and 0 above.
-// Returns false when a plane's magnitude is 2^126 or more.
+// One axis: lo[c], hi[c] for the children with use[c]; writes the grid origin
+// and step and the bytes of each child's lower / upper plane index (byte c).
+// A child without a box gets q = 255 below and 0 above. Returns false when a
+// plane's magnitude is 2^126 or more.
 __host__ __device__ inline bool quantize_axis(const float lo[4], const float hi[4], const bool use[4], float &origin,
-                                              uint32_t &ebias, uint32_t &wlo, uint32_t &whi)
+                                              float &step, uint32_t &wlo, uint32_t &whi)
 {
     double plo = INFINITY, phi = -INFINITY;
     for (int c = 0; c < 4; ++c)
@@ -38,30 +41,49 @@ __host__ __device__ inline bool quantize_axis(const float lo[4], const float hi[
     whi = 0u;
     if (!(plo <= phi)) {  // no child with a box
         origin = 0.0f;
-        ebias = 127u;
+        step = 1.0f;
         wlo = 0xFFFFFFFFu;
         return true;
     }
     if (!(fabs(plo) < kQuantMaxPlane && fabs(phi) < kQuantMaxPlane)) return false;
+    // start where m has 8 bits: 2^e <= span / 255 / 128
     int e = -100;
-    const double span = (phi - plo) / 255.0;
+    const double span = phi - plo;
     if (span > 0.0) {
-        frexp(span, &e);  // span < 2^e
-        e = e - 1 > -100 ? e - 1 : -100;
+        int x;
+        frexp(span / 255.0, &x);  // span / 255 in [2^(x-1), 2^x)
+        e = x - 1 - 7 > -100 ? x - 1 - 7 : -100;
     }
-    double st = 0.0, k = 0.0;
+    double u = 0.0, k = 0.0, m = 1.0;
     for (;; ++e) {
         if (e > 126) return false;
-        st = ldexp(1.0, e);
-        k = floor(plo / st);
-        const double kh = ceil(phi / st);
-        if (kh - k <= 255.0 && fabs(k) + 255.0 < 16777216.0) break;
+        u = ldexp(1.0, e);
+        k = floor(plo / u);
+        // the smallest m with k + 255 m reaching the highest plane
+        m = ceil((ceil(phi / u) - k) / 255.0);
+        if (m < 1.0) m = 1.0;
+        if (m < 256.0 && fabs(k) + 255.0 * m < 16777216.0) break;
     }
-    origin = (float)(k * st);
-    ebias = (uint32_t)(e + 127);
+    origin = (float)(k * u);
+    step = (float)(m * u);
+    const double st = m * u;
+    // grid point q: (k + q m) u, exact in double, so the comparisons below
+    // are exact; the quotients only give a starting guess
+    auto point = [&](double q) { return (k + q * m) * u; };
     for (int c = 0; c < 4; ++c) {
-        const uint32_t ql = use[c] ? (uint32_t)(floor((double)lo[c] / st) - k) : 255u;
-        const uint32_t qh = use[c] ? (uint32_t)(ceil((double)hi[c] / st) - k) : 0u;
+        uint32_t ql = 255u, qh = 0u;
+        if (use[c]) {
+            double a = floor(((double)lo[c] - k * u) / st);
+            a = fmin(fmax(a, 0.0), 255.0);
+            while (a > 0.0 && point(a) > (double)lo[c]) a -= 1.0;
+            while (a < 255.0 && point(a + 1.0) <= (double)lo[c]) a += 1.0;
+            double b = ceil(((double)hi[c] - k * u) / st);
+            b = fmin(fmax(b, 0.0), 255.0);
+            while (b < 255.0 && point(b) < (double)hi[c]) b += 1.0;
+            while (b > 0.0 && point(b - 1.0) >= (double)hi[c]) b -= 1.0;
+            ql = (uint32_t)a;
+            qh = (uint32_t)b;
+        }
         wlo |= ql << (8 * c);
         whi |= qh << (8 * c);
     }
@@ -69,11 +91,8 @@ __host__ __device__ inline bool quantize_axis(const float lo[4], const float hi[
 }
 
 // The plane of byte c of word w: what the traversal computes.
-__host__ __device__ inline float quant_plane(uint32_t w, int c, uint32_t ebias, float origin)
+__host__ __device__ inline float quant_plane(uint32_t w, int c, float step, float origin)
 {
-    float step;
-    const uint32_t bits = ebias << 23;
-    __builtin_memcpy(&step, &bits, 4);
     return fmaf((float)((w >> (8 * c)) & 0xFFu), step, origin);
 }
 

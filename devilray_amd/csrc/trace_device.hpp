@@ -248,31 +248,29 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
                     ++node_visits;
                     if (leads_wave()) ++wave_node_iters;
                 }
-                // a QNode: origin and step exponents, the plane bytes
-                // (lower x y z, upper x | upper y z, pad), the children
+                // a QNode: origin xyz and step x | steps y z, lower plane
+                // bytes x y | lower z, upper x y z | the children
                 const char *nd = nbase + (size_t)(uint32_t)cur * sizeof(QNode);
-                const float4 org = *reinterpret_cast<const float4 *>(nd);
-                const uint4 qa = *reinterpret_cast<const uint4 *>(nd + 16);
-                const uint4 qb = *reinterpret_cast<const uint4 *>(nd + 32);
+                const float4 g0 = *reinterpret_cast<const float4 *>(nd);
+                const float4 g1 = *reinterpret_cast<const float4 *>(nd + 16);
+                const uint4 g2 = *reinterpret_cast<const uint4 *>(nd + 32);
                 const int4 ch = *reinterpret_cast<const int4 *>(nd + 48);
                 const float lim = (ANY ? p.tmax : st.bt) * kBestSlack;
-                const uint32_t ex = __float_as_uint(org.w);
-                const float sx = __uint_as_float((ex & 0xFFu) << 23);
-                const float sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23);
-                const float sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
-                const uint32_t wnx = negx ? qa.w : qa.x, wfx = negx ? qa.x : qa.w;
-                const uint32_t wny = negy ? qb.x : qa.y, wfy = negy ? qa.y : qb.x;
-                const uint32_t wnz = negz ? qb.y : qa.z, wfz = negz ? qa.z : qb.y;
+                const float sx = g0.w, sy = g1.x, sz = g1.y;
+                const uint32_t lox = __float_as_uint(g1.z), loy = __float_as_uint(g1.w);
+                const uint32_t wnx = negx ? g2.y : lox, wfx = negx ? lox : g2.y;
+                const uint32_t wny = negy ? g2.z : loy, wfy = negy ? loy : g2.z;
+                const uint32_t wnz = negz ? g2.w : g2.x, wfz = negz ? g2.x : g2.w;
                 // exact planes: q * step + origin has no rounding
                 float ax[4], bx[4], ay[4], by[4], az[4], bz[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    ax[c] = fmaf((float)((wnx >> (8 * c)) & 0xFFu), sx, org.x);
-                    bx[c] = fmaf((float)((wfx >> (8 * c)) & 0xFFu), sx, org.x);
-                    ay[c] = fmaf((float)((wny >> (8 * c)) & 0xFFu), sy, org.y);
-                    by[c] = fmaf((float)((wfy >> (8 * c)) & 0xFFu), sy, org.y);
-                    az[c] = fmaf((float)((wnz >> (8 * c)) & 0xFFu), sz, org.z);
-                    bz[c] = fmaf((float)((wfz >> (8 * c)) & 0xFFu), sz, org.z);
+                    ax[c] = fmaf((float)((wnx >> (8 * c)) & 0xFFu), sx, g0.x);
+                    bx[c] = fmaf((float)((wfx >> (8 * c)) & 0xFFu), sx, g0.x);
+                    ay[c] = fmaf((float)((wny >> (8 * c)) & 0xFFu), sy, g0.y);
+                    by[c] = fmaf((float)((wfy >> (8 * c)) & 0xFFu), sy, g0.y);
+                    az[c] = fmaf((float)((wnz >> (8 * c)) & 0xFFu), sz, g0.z);
+                    bz[c] = fmaf((float)((wfz >> (8 * c)) & 0xFFu), sz, g0.z);
                 }
                 float tn[4];
                 int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};

@@ -1,9 +1,11 @@
 """The traversal's node boxes are quantised to one byte per plane
 (devilray_amd/csrc/node_quant.hpp). Hits stay exact only if every quantised
 box contains the float box it replaces and the planes decode without
-rounding; tests/quant_check.cpp checks both, and that the grid step is the
-smallest possible, on random and edge-case boxes, compiled with the host
-compiler."""
+rounding; tests/quant_check.cpp checks both, that every plane is the
+tightest grid point that bounds it and that the step is within its stated
+slack of span / 255, on random and edge-case boxes (among them planes that
+only the exact grid-point comparisons keep inside their boxes), compiled
+with the host compiler."""
 import os
 import shutil
 import subprocess
