@@ -97,3 +97,53 @@ __host__ __device__ inline float quant_plane(uint32_t w, int c, float step, floa
 }
 
 }  // namespace dr
+
+namespace dr {
+
+// ---- the box test of a QNode's four children --------------------------------
+//
+// Per child, the entry and exit distances nr <= fr of a ray o + t d against
+// the quantised box (near planes picked by the ray's octant, inv = 1 / d per
+// axis); the caller takes the child when nr <= fr. Each plane is decoded
+// exactly (fma(q, step, origin)), then (plane - o) * inv, and the exit
+// distance is scaled by 1 + 2 gamma(3), the robust slab test (Ize, JCGT
+// 2013): inv, the subtraction and the product round once each. So a child
+// whose box the exact ray meets within [tmin, lim] is never dropped
+// (tmin >= 0).
+
+constexpr float kQuantU = 0x1p-24f;
+constexpr float kFarScaleExact = 1.0f + 2.0f * (3.0f * kQuantU) / (1.0f - 3.0f * kQuantU);
+
+// A QNode as the traversal loads it: origin, step, and the lower and upper
+// plane bytes of the three axes.
+struct QNodeWords {
+    float ox, oy, oz, sx, sy, sz;
+    uint32_t lo[3], hi[3];
+};
+
+struct SlabRay {
+    float ox, oy, oz, ix, iy, iz, tmin;
+    bool negx, negy, negz;  // the upper plane of the axis is near
+};
+
+__host__ __device__ inline float qbyte(uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xFFu); }
+
+__host__ __device__ inline void qnode_slabs(const QNodeWords &n, const SlabRay &r, float lim, float (&nr)[4],
+                                            float (&fr)[4])
+{
+    const uint32_t wnx = r.negx ? n.hi[0] : n.lo[0], wfx = r.negx ? n.lo[0] : n.hi[0];
+    const uint32_t wny = r.negy ? n.hi[1] : n.lo[1], wfy = r.negy ? n.lo[1] : n.hi[1];
+    const uint32_t wnz = r.negz ? n.hi[2] : n.lo[2], wfz = r.negz ? n.lo[2] : n.hi[2];
+    for (int c = 0; c < 4; ++c) {
+        const float t0x = (fmaf(qbyte(wnx, c), n.sx, n.ox) - r.ox) * r.ix;
+        const float t1x = (fmaf(qbyte(wfx, c), n.sx, n.ox) - r.ox) * r.ix;
+        const float t0y = (fmaf(qbyte(wny, c), n.sy, n.oy) - r.oy) * r.iy;
+        const float t1y = (fmaf(qbyte(wfy, c), n.sy, n.oy) - r.oy) * r.iy;
+        const float t0z = (fmaf(qbyte(wnz, c), n.sz, n.oz) - r.oz) * r.iz;
+        const float t1z = (fmaf(qbyte(wfz, c), n.sz, n.oz) - r.oz) * r.iz;
+        nr[c] = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, r.tmin));
+        fr[c] = fminf(fminf(fminf(t1x, t1y), t1z) * kFarScaleExact, lim);
+    }
+}
+
+}  // namespace dr

@@ -22,10 +22,11 @@
 // so a tree of D wide levels never needs more than 3 D entries: the host sizes
 // the spill region from the built depth, and no ray is ever cut short. At an
 // interior node (a 64-byte QNode) the four child boxes are decoded from their
-// plane bytes (one exact fma per plane) and tested with the robust slab test;
-// near and far planes are picked by the ray's octant so no per-slab min/max is
-// needed. The quantised boxes contain the exact ones, so the hits are those
-// of the float boxes; the node is half the bytes of a float node. Hit children
+// plane bytes (one exact fma per plane) and tested with the robust slab test
+// (node_quant.hpp); near and far planes are picked by the ray's octant so no
+// per-slab min/max is needed. The quantised boxes
+// contain the exact ones, so the hits are those of the float boxes; the node
+// is half the bytes of a float node. Hit children
 // are sorted by entry distance, the nearest is taken next and the rest are
 // pushed far to near. Leaves hold up to 8 triangles stored as 3 float4.
 //
@@ -42,6 +43,7 @@
 
 #include "dr_device.hpp"
 #include "dr_internal.hpp"
+#include "node_quant.hpp"
 
 namespace dr {
 
@@ -256,35 +258,15 @@ __device__ __forceinline__ void trace_body(int32_t (*__restrict__ stack)[kBlock]
                 const uint4 g2 = *reinterpret_cast<const uint4 *>(nd + 32);
                 const int4 ch = *reinterpret_cast<const int4 *>(nd + 48);
                 const float lim = (ANY ? p.tmax : st.bt) * kBestSlack;
-                const float sx = g0.w, sy = g1.x, sz = g1.y;
-                const uint32_t lox = __float_as_uint(g1.z), loy = __float_as_uint(g1.w);
-                const uint32_t wnx = negx ? g2.y : lox, wfx = negx ? lox : g2.y;
-                const uint32_t wny = negy ? g2.z : loy, wfy = negy ? loy : g2.z;
-                const uint32_t wnz = negz ? g2.w : g2.x, wfz = negz ? g2.x : g2.w;
-                // exact planes: q * step + origin has no rounding
-                float ax[4], bx[4], ay[4], by[4], az[4], bz[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    ax[c] = fmaf((float)((wnx >> (8 * c)) & 0xFFu), sx, g0.x);
-                    bx[c] = fmaf((float)((wfx >> (8 * c)) & 0xFFu), sx, g0.x);
-                    ay[c] = fmaf((float)((wny >> (8 * c)) & 0xFFu), sy, g0.y);
-                    by[c] = fmaf((float)((wfy >> (8 * c)) & 0xFFu), sy, g0.y);
-                    az[c] = fmaf((float)((wnz >> (8 * c)) & 0xFFu), sz, g0.z);
-                    bz[c] = fmaf((float)((wfz >> (8 * c)) & 0xFFu), sz, g0.z);
-                }
+                const QNodeWords nw{g0.x, g0.y, g0.z, g0.w, g1.x, g1.y,
+                                    {__float_as_uint(g1.z), __float_as_uint(g1.w), g2.x}, {g2.y, g2.z, g2.w}};
+                const SlabRay sr{p.ox, p.oy, p.oz, p.ix, p.iy, p.iz, p.tmin, negx, negy, negz};
+                float nr[4], fr[4];
+                qnode_slabs(nw, sr, lim, nr, fr);
                 float tn[4];
                 int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const float t0x = (ax[c] - p.ox) * p.ix, t1x = (bx[c] - p.ox) * p.ix;
-                    const float t0y = (ay[c] - p.oy) * p.iy, t1y = (by[c] - p.oy) * p.iy;
-                    const float t0z = (az[c] - p.oz) * p.iz, t1z = (bz[c] - p.oz) * p.iz;
-                    const float nr = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, p.tmin));
-                    float fr = fminf(fminf(t1x, t1y), t1z) * kFarScale;
-                    fr = fminf(fr, lim);
-                    // empty slots never pass (the ref check)
-                    tn[c] = (nr <= fr && r[c] != kNone) ? nr : INFINITY;
-                }
+                for (int c = 0; c < 4; ++c) tn[c] = (nr[c] <= fr[c] && r[c] != kNone) ? nr[c] : INFINITY;  // empty slots never pass
                 if (COUNT) nbox += (ch.x != kNone) + (ch.y != kNone) + (ch.z != kNone) + (ch.w != kNone);
                 if (!ANY) {  // nearest child first
                     cswap(tn[0], r[0], tn[1], r[1]);
