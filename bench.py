@@ -42,6 +42,8 @@ known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
 --probe-guide: ten guide training steps on 2^18 samples; for rocprofv3.
 --probe-render: three passes of the many-lights render and the calibration
 copy; for rocprofv3.
+--traffic-from FETCH_DIR WRITE_DIR: writes the traffic fixture from the
+counter CSVs of two rocprofv3 runs of --probe (FETCH_SIZE, WRITE_SIZE).
 """
 from __future__ import annotations
 
@@ -591,6 +593,67 @@ def probe_guide() -> int:
     return 0
 
 
+def _counter_rows(d: str, counter: str) -> list[tuple[str, float]]:
+    """(kernel name, value) per dispatch of one counter from the
+    counter_collection CSV files rocprofv3 wrote under directory d."""
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r.get("Dispatch_Id", 0)), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(name, val) for _, name, val in rows]
+
+
+def traffic_from(fetch_dir: str, write_dir: str) -> int:
+    """Write the traversal kernel's traffic fixture from two rocprofv3 runs
+    of `--probe` (FETCH_SIZE, then WRITE_SIZE; both in KiB per dispatch).
+    Each counter is scaled by the factor that makes it read 256 MiB for the
+    probe's 256 MiB device copy, and the fixture records the hash of the
+    sources it was measured on."""
+    copy_bytes = 256 << 20
+    out = {"kernel": "k_trace<closest, persistent, speculative> on the traversal benchmark (1,310,720 tris, "
+                     "2^20 incoherent rays), 10 launches",
+           "sources_hash": sources_hash(), "sources": TRAFFIC_SOURCES}
+    per = {}
+    for key, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        rows = _counter_rows(d, counter)
+        trace = [v for name, v in rows if "k_trace" in name]
+        # the calibration copy is the only work after the last traversal launch
+        last = max((k for k, (name, _) in enumerate(rows) if "k_trace" in name), default=len(rows))
+        copy = [v for name, v in rows[last + 1:] if "copy" in name.lower()]
+        if len(trace) != 10 or not copy:
+            print(f"{counter}: expected 10 k_trace dispatches and a copy kernel, found {len(trace)} and {len(copy)}",
+                  file=sys.stderr)
+            return 1
+        copy_kib = float(sum(copy))
+        factor = copy_bytes / 1024.0 / copy_kib
+        out[f"{key}_size_kib_per_dispatch"] = [round(v, 1) for v in trace]
+        per[key] = (float(np.mean(trace)) * 1024.0 * factor, copy_kib, factor)
+    out["calibration"] = {"copy_bytes": copy_bytes, "fetch_size_kib": per["fetch"][1],
+                          "write_size_kib": per["write"][1], "fetch_factor": round(per["fetch"][2], 4),
+                          "write_factor": round(per["write"][2], 4)}
+    out["fetch_bytes_per_launch"] = int(per["fetch"][0])
+    out["write_bytes_per_launch"] = int(per["write"][0])
+    out["hbm_bytes_per_launch"] = int(per["fetch"][0] + per["write"][0])
+    out["note"] = ("FETCH_SIZE counts L2->fabric reads, so Infinity-Cache hits are included (an upper bound on HBM "
+                   "reads); each counter scaled by the factor that makes it read 256 MiB for the 256 MiB copy")
+    out["commands"] = [
+        "cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv "
+        "-d /root/repo/gpurun_out/prof/trace_fetch -o p -- python3 /root/repo/bench.py --probe",
+        "cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv "
+        "-d /root/repo/gpurun_out/prof/trace_write -o p -- python3 /root/repo/bench.py --probe",
+        "python3 bench.py --traffic-from gpurun_out/prof/trace_fetch gpurun_out/prof/trace_write"]
+    with open(TRAFFIC_FIXTURE, "w") as fh:
+        json.dump(out, fh, indent=1)
+        fh.write("\n")
+    print(f"wrote {TRAFFIC_FIXTURE}: {out['hbm_bytes_per_launch']} B per launch", flush=True)
+    return 0
+
+
 def load_counts() -> dict:
     if not os.path.exists(COUNTS_FIXTURE):
         return {}
@@ -655,7 +718,10 @@ def main() -> int:
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--probe-guide", action="store_true")
     ap.add_argument("--probe-render", action="store_true")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.traffic_from:
+        return traffic_from(*args.traffic_from)
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus": args.gpus}))
