@@ -329,6 +329,15 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
                           const uint32_t *ray_root = nullptr);
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
+// an any-hit queue (rays_a, device count na_dev) and a closest-hit queue
+// (rays_c through list_c, device count nc_dev) traced in one launch, each
+// ray exactly as launch_trace_ex would; both counts at most n_max. Only for
+// the per-ray schedules of a scene without instances (trace_pair_fits).
+bool trace_pair_fits(const Scene *s, uint32_t sched);
+dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_dev, uint32_t *occluded,
+                            const uint32_t *roots_a, const dr_ray *rays_c, const uint32_t *list_c,
+                            const uint32_t *nc_dev, dr_hit *hits, const uint32_t *roots_c, uint32_t n_max,
+                            uint32_t sched, hipStream_t stream);
 // closest hit with every scratch buffer in the static device array (no
 // allocation); noalloc_init looks up this device's copy of it
 dr_status noalloc_init(Context *ctx);

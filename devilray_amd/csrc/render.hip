@@ -188,6 +188,9 @@ __device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
     return V3{sv.verts[3 * (size_t)vi], sv.verts[3 * (size_t)vi + 1], sv.verts[3 * (size_t)vi + 2]};
 }
 
+// GUIDE = false compiles the plain shader (no guide reads, no collection):
+// the same arithmetic in fewer registers.
+template <bool GUIDE>
 __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, const PathParams &pp, uint32_t vertex,
                                            const uint32_t *__restrict__ in_list,
                                            const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
@@ -202,7 +205,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     const uint32_t path = in_list[slot];
     const dr_hit h = hits[path];
     float4 l4 = rad[path];
-    if (gv.smp) {
+    if (GUIDE && gv.smp) {
         // the previous bounce's training sample: radiance before this vertex
         const uint32_t pk = gv.pending[path];
         if (pk != kGuideNoSample) {
@@ -247,7 +250,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     float pdf;
     float hd[kHeadFloats];
     float alpha = 0.0f;
-    const bool guided = gv.head != nullptr && mat.kind == DR_MAT_LAMBERT;
+    const bool guided = GUIDE && gv.head != nullptr && mat.kind == DR_MAT_LAMBERT;
     if (guided) {
         const float4 *H = reinterpret_cast<const float4 *>(gv.head + (size_t)kHeadFloats * path);
 #pragma unroll
@@ -321,15 +324,23 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
             const U4 r2 = path_rng(pixel, sample, vertex, 2u, seed);
             if (u01(r2.x) < alpha) {
                 const float u = u01(r2.y);
-                int j = kGuideLobes - 1;
+                // the first lobe whose cumulative weight exceeds u (else the
+                // last), picked by selects so the head stays in registers
                 float cdf = 0.0f;
+                bool found = false;
+                float mx = hd[5 * (kGuideLobes - 1)], my = hd[5 * (kGuideLobes - 1) + 1];
+                float mz = hd[5 * (kGuideLobes - 1) + 2], kr = hd[5 * (kGuideLobes - 1) + 3];
 #pragma unroll
                 for (int k = 0; k < kGuideLobes - 1; ++k) {
                     cdf += hd[5 * k + 4];
-                    if (u < cdf && j == kGuideLobes - 1) j = k;
+                    const bool take = u < cdf && !found;
+                    mx = take ? hd[5 * k] : mx;
+                    my = take ? hd[5 * k + 1] : my;
+                    mz = take ? hd[5 * k + 2] : mz;
+                    kr = take ? hd[5 * k + 3] : kr;
+                    found = found || take;
                 }
-                const float *lb = hd + 5 * j;
-                nd = vmf_sample(V3{lb[0], lb[1], lb[2]}, lb[3], u01(r2.z), u01(r2.w));
+                nd = vmf_sample(V3{mx, my, mz}, kr, u01(r2.z), u01(r2.w));
             } else {
                 nd = cosine_dir(n, u01(r0.x), u01(r0.y));
             }
@@ -386,7 +397,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     }
     if (!(beta3.x > 0.0f || beta3.y > 0.0f || beta3.z > 0.0f)) alive = false;
     if (!alive) return;
-    if (gv.smp && !spec && u01(r0.w) < gv.frac) {
+    if (GUIDE && gv.smp && !spec && u01(r0.w) < gv.frac) {
         const uint32_t k = atomicAdd(gv.counter, 1u);
         if (k < gv.cap) {
             dr_guide_sample q;
@@ -413,6 +424,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     out_list[atomicAdd(out_count, 1u)] = path;
 }
 
+template <bool GUIDE>
 __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint32_t vertex,
                                               const uint32_t *__restrict__ in_list,
                                               const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
@@ -423,8 +435,8 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                                               GuideView gv, const uint32_t *__restrict__ pview,
                                               uint32_t *__restrict__ sroot)
 {
-    shade_item(blockIdx.x * kT + threadIdx.x, sv, pp, vertex, in_list, in_count, out_list, out_count, rays, hits, beta,
-               rad, srays, scontrib, scount, gv, pview, sroot);
+    shade_item<GUIDE>(blockIdx.x * kT + threadIdx.x, sv, pp, vertex, in_list, in_count, out_list, out_count, rays,
+                      hits, beta, rad, srays, scontrib, scount, gv, pview, sroot);
 }
 
 __device__ __forceinline__ void shadow_apply_item(uint32_t i, const uint32_t *__restrict__ scount,
@@ -506,32 +518,54 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
     k_camera<<<blocks(cap), kT, 0, stream>>>(pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt,
                                              collect ? gv.pending : nullptr, pview, proot);
     DR_HIP_TRY(hipGetLastError());
+    // With a per-ray schedule, the shadow rays of vertex v - 1 are traced in
+    // the same launch as the extension rays of vertex v, and applied before
+    // vertex v is shaded: each radiance gets its terms in the same order as
+    // with separate launches, and two short queues fill one grid.
+    const bool paired = s->n_emit && trace_pair_fits(s, j.sched);
+    auto trace_shadows = [&]() -> dr_status {
+        if (dr_status st = launch_trace_ex(s, srays, nullptr, cap, cnt + 2, nullptr, socc, nullptr, nullptr, true,
+                                           j.sched, stream, sroot))
+            return st;
+        k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
+        DR_HIP_TRY(hipGetLastError());
+        return DR_OK;
+    };
     for (uint32_t v = 0; v < j.prm->max_depth; ++v) {
         uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
         uint32_t *lout = (v & 1u) ? s->rs_list_a.as<uint32_t>() : s->rs_list_b.as<uint32_t>();
         uint32_t *cin = cnt + (v & 1u);
         uint32_t *cout = cnt + ((v + 1u) & 1u);
-        if (dr_status st = launch_trace_ex(s, rays, lin, cap, cin, hits, nullptr, nullptr, nullptr, false, j.sched,
-                                           stream, proot))
+        const bool pair_v = paired && v > 0;
+        if (pair_v) {
+            if (dr_status st = launch_trace_pair(s, srays, cnt + 2, socc, sroot, rays, lin, cin, hits, proot, cap,
+                                                 j.sched, stream))
+                return st;
+        } else if (dr_status st = launch_trace_ex(s, rays, lin, cap, cin, hits, nullptr, nullptr, nullptr, false,
+                                                  j.sched, stream, proot)) {
             return st;
+        }
         const bool guided_v = use_guide && v < j.guide->guided_vertices;
         if (guided_v)
             if (dr_status st = guide_infer_paths(j.guide, s, rays, hits, lin, cin, cap, s->rs_guide_head.as<float>(),
                                                  stream))
                 return st;
         gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
-        k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2, cin, s->ray_casts.as<unsigned long long>());
-        k_shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
-                                                cnt + 2, gv, pview, sroot);
-        DR_HIP_TRY(hipGetLastError());
-        if (s->n_emit) {
-            if (dr_status st = launch_trace_ex(s, srays, nullptr, cap, cnt + 2, nullptr, socc, nullptr, nullptr, true,
-                                               j.sched, stream, sroot))
-                return st;
+        if (pair_v) {
             k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
             DR_HIP_TRY(hipGetLastError());
         }
+        k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2, cin, s->ray_casts.as<unsigned long long>());
+        const auto shade = (gv.head || gv.smp) ? k_shade<true> : k_shade<false>;
+        shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
+                                              cnt + 2, gv, pview, sroot);
+        DR_HIP_TRY(hipGetLastError());
+        if (s->n_emit && !paired)
+            if (dr_status st = trace_shadows()) return st;
     }
+    // the last vertex's shadow rays
+    if (paired && j.prm->max_depth > 0)
+        if (dr_status st = trace_shadows()) return st;
     if (collect)
         if (dr_status st = guide_finalize(j.guide, rad, stream)) return st;
     k_resolve<<<blocks(cap), kT, 0, stream>>>(pp, rad, pview);
@@ -669,7 +703,10 @@ __global__ __launch_bounds__(kPersistThreads) void k_render_persistent(PersistAr
     for (;;) {
         if (gtid == 0) {
             next_chunk(a.chunk);
-            for (int k = 0; k < 8; ++k) ctr_closest[k] = 0u;
+            for (int k = 0; k < 8; ++k) {
+                ctr_any[k] = 0u;
+                ctr_closest[k] = 0u;
+            }
         }
         if (!grid_barrier(a.grid)) return;
         const uint32_t n = load_agent(&a.chunk->n);
@@ -677,41 +714,60 @@ __global__ __launch_bounds__(kPersistThreads) void k_render_persistent(PersistAr
         for (uint32_t p = gtid; p < n; p += nthr)
             camera_item(p, a.pp, a.rays, a.beta, a.rad, a.list_a, a.cnt, nullptr, a.pview, a.proot);
         if (!grid_barrier(a.grid)) return;
-        for (uint32_t v = 0; v < a.pp.max_depth; ++v) {
+        // Vertex v's stage traces the shadow rays of vertex v - 1 and the
+        // extension rays of vertex v (each block drains the shadow queue's
+        // share first, then takes extension rays), applies the shadow terms,
+        // then shades vertex v; a radiance gets its terms in the order of the
+        // launch sequence. Shadow queue counts alternate between words 2 and
+        // 3 by vertex parity, so the shade's count restarts while the
+        // previous vertex's is still read. Vertex max_depth only traces and
+        // applies the last shadow rays.
+        for (uint32_t v = 0; v <= a.pp.max_depth; ++v) {
             uint32_t *lin = (v & 1u) ? a.list_b : a.list_a;
             uint32_t *lout = (v & 1u) ? a.list_a : a.list_b;
             uint32_t *cin = a.cnt + (v & 1u);
             uint32_t *cout = a.cnt + ((v + 1u) & 1u);
-            const uint32_t nin = load_agent(cin);
-            if (nin == 0u) break;  // no path left in this chunk (the same count everywhere)
-            // extension rays; the output queue and the shadow queue restart
-            if (gtid == 0) reset_counts(cout, a.cnt + 2, cin, a.casts);
-            trace_body<false, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
-                                                        reinterpret_cast<const float4 *>(a.rays), lin, a.cap, cin,
-                                                        a.hits, nullptr, nullptr, nullptr, a.ovf, a.flags,
-                                                        ctr_closest, a.refill_min, a.proot, a.slices, a.max_stack,
-                                                        nullptr);
+            uint32_t *scur = a.cnt + 2u + (v & 1u), *sprev = a.cnt + 2u + ((v + 1u) & 1u);
+            // the same counts in every block (read after a barrier)
+            const uint32_t nin = v < a.pp.max_depth ? load_agent(cin) : 0u;
+            const uint32_t ns = (v > 0u && a.has_emit) ? load_agent(sprev) : 0u;
+            if (nin == 0u && ns == 0u) break;  // no path and no shadow ray left in this chunk
+            // the output queue and this vertex's shadow queue restart (the
+            // former was the previous vertex's input, the latter held the
+            // shadow rays of vertex v - 2, applied at v - 1); the extension
+            // rays traced here count as ray casts
+            if (gtid == 0) {
+                *cout = 0u;
+                *scur = 0u;
+                *a.casts += nin;
+            }
+            if (ns)
+                trace_body<true, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
+                                                           reinterpret_cast<const float4 *>(a.srays), nullptr, a.cap,
+                                                           sprev, nullptr, a.socc, nullptr, nullptr, a.ovf, a.flags,
+                                                           ctr_any, a.refill_min, a.sroot, a.slices, a.max_stack,
+                                                           nullptr);
+            if (nin)
+                trace_body<false, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
+                                                            reinterpret_cast<const float4 *>(a.rays), lin, a.cap, cin,
+                                                            a.hits, nullptr, nullptr, nullptr, a.ovf, a.flags,
+                                                            ctr_closest, a.refill_min, a.proot, a.slices, a.max_stack,
+                                                            nullptr);
             if (!grid_barrier(a.grid)) return;
             if (gtid == 0)
                 for (int k = 0; k < 8; ++k) {
                     ctr_any[k] = 0u;
                     ctr_closest[k] = 0u;
                 }
-            for (uint32_t slot = gtid; slot < nin; slot += nthr)
-                shade_item(slot, a.sv, a.pp, v, lin, cin, lout, cout, a.rays, a.hits, a.beta, a.rad, a.srays, a.scon,
-                           a.cnt + 2, gv, a.pview, a.sroot);
-            if (!grid_barrier(a.grid)) return;
-            const uint32_t ns = a.has_emit ? load_agent(a.cnt + 2) : 0u;
             if (ns) {
-                trace_body<true, false, true, SPEC, false>(stack, ptid, vbid, vnblk, a.nodes, a.tri_v,
-                                                           reinterpret_cast<const float4 *>(a.srays), nullptr, a.cap,
-                                                           a.cnt + 2, nullptr, a.socc, nullptr, nullptr, a.ovf,
-                                                           a.flags, ctr_any, a.refill_min, a.sroot, a.slices,
-                                                           a.max_stack, nullptr);
-                if (!grid_barrier(a.grid)) return;
-                for (uint32_t i = gtid; i < ns; i += nthr) shadow_apply_item(i, a.cnt + 2, a.socc, a.scon, a.rad);
+                for (uint32_t i = gtid; i < ns; i += nthr) shadow_apply_item(i, sprev, a.socc, a.scon, a.rad);
                 if (!grid_barrier(a.grid)) return;
             }
+            if (nin == 0u) break;
+            for (uint32_t slot = gtid; slot < nin; slot += nthr)
+                shade_item<false>(slot, a.sv, a.pp, v, lin, cin, lout, cout, a.rays, a.hits, a.beta, a.rad, a.srays,
+                                  a.scon, scur, gv, a.pview, a.sroot);
+            if (!grid_barrier(a.grid)) return;
         }
         for (uint32_t p = gtid; p < n; p += nthr) resolve_item(p, a.pp, a.rad, a.pview);
         // the next chunk rewrites the path state
