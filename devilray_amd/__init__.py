@@ -522,6 +522,32 @@ class Instances:
         self.ctx.synchronize()
         return do.to_numpy(np.uint32, n)
 
+    def lights(self):
+        """(alias entries naming emitters by index, the emitters' instanced
+        triangle ids, their area pdfs) copied to the host."""
+        n = C.c_uint32()
+        check(lib().dr_instances_light_count(self.handle, C.byref(n)))
+        ent = np.zeros(n.value, ALIAS_DTYPE)
+        tri = np.zeros(n.value, np.uint32)
+        pdf = np.zeros(n.value, np.float32)
+        if n.value:
+            check(lib().dr_instances_copy_lights(self.handle, ptr(ent), ptr(tri), ptr(pdf)))
+        return ent, tri, pdf
+
+    def render_dev(self, camera: DrCamera, params: DrRenderParams, accum_ptr: int, stream=None):
+        check(lib().dr_instances_render(self.handle, C.byref(camera), C.byref(params), C.c_void_p(accum_ptr),
+                                        stream))
+
+    def render(self, camera: DrCamera, spp: int, max_depth: int = 8, rr_depth: int = 3, seed: int = 0,
+               fresnel: int = 0, sample_offset: int = 0) -> np.ndarray:
+        """Render into an FP64 RGBW accumulation image (height, width, 4), as Scene.render."""
+        npix = camera.width * camera.height
+        d = self.ctx.alloc(npix * 32)
+        d.copy_from(np.zeros((npix, 4), np.float64))
+        self.render_dev(camera, DrRenderParams(spp, sample_offset, max_depth, rr_depth, seed, fresnel, 0), d.ptr)
+        self.ctx.synchronize()
+        return d.to_numpy(np.float64, npix * 4).reshape(camera.height, camera.width, 4)
+
 
 class Batch:
     """Scenes merged for batched rendering (dr_batch): views of all scenes

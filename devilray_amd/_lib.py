@@ -156,6 +156,9 @@ PROTOTYPES = {
     "dr_instances_trace_any": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "dr_instances_set_trace_schedule": (_st, [_vp, _u32, _u32]),
     "dr_instances_get_info": (_st, [_vp, C.POINTER(DrSceneInfo)]),
+    "dr_instances_render": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _vp]),
+    "dr_instances_light_count": (_st, [_vp, C.POINTER(_u32)]),
+    "dr_instances_copy_lights": (_st, [_vp, _vp, _vp, _vp]),
     "dr_render_guided": (_st, [_vp, C.POINTER(DrCamera), C.POINTER(DrRenderParams), _vp, _u32, _vp, _vp]),
     "dr_guide_create": (_st, [_vp, C.POINTER(DrGuideParams), C.POINTER(_vp)]),
     "dr_guide_destroy": (_st, [_vp]),

@@ -88,6 +88,18 @@ struct InstRec {
 };
 static_assert(sizeof(InstRec) == 32, "instance record is 32 B");
 
+// Shading view of one instance: its transform forward (world = model * s +
+// t, rounded per operation as the flattened copy is) and where its scene's
+// triangles start in the forest's concatenated scene arrays; id_delta as in
+// InstRec.
+struct InstShade {
+    float sx, sy, sz;
+    float tx, ty, tz;
+    uint32_t tri_off;
+    uint32_t id_delta;
+};
+static_assert(sizeof(InstShade) == 32, "instance shading record is 32 B");
+
 struct Context {
     int device = 0;
     int cu_count = 0;
@@ -118,6 +130,16 @@ struct Scene {
     // scenes' trees; 0 for an ordinary scene
     uint32_t n_inst = 0, n_top = 0;
     DevBuf inst_recs;  // InstRec * n_inst
+    // a forest's shading data: verts / indices / tri_mat / materials hold the
+    // instanced scenes' own arrays concatenated (n_cat_tris triangles, indices
+    // and material ids rebased), inst_shade one InstShade per instance, and
+    // the light table names emitters by index: emit_tri holds their instanced
+    // triangle ids (ascending), emit_pdf their selection pdf / world area
+    uint32_t n_cat_tris = 0;
+    DevBuf inst_shade;
+    DevBuf emit_pdf;
+    std::vector<uint32_t> cat_tri_off;  // host: scene k's triangles are [off[k], off[k + 1])
+    std::vector<uint32_t> inst_scene;   // host: the scene of each instance
     uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
     // wide nodes by depth: level d is [level_ofs[d], level_ofs[d + 1]) (refit)
     std::vector<uint32_t> level_ofs;
@@ -303,10 +325,16 @@ dr_status count_ray_casts(Scene *s, uint64_t n, hipStream_t stream, bool set = f
 
 // light.hip
 dr_status build_lights(Scene *s, hipStream_t stream);
+// the light table of an instance forest: every emissive triangle of every
+// instance, weighted by its world area
+dr_status build_instance_lights(Scene *forest, hipStream_t stream);
 
 // render.hip
 dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Guide *guide, uint32_t guide_flags,
                  double *accum, hipStream_t stream);
+// render of an instance forest (plain sampler, the launch sequence)
+dr_status render_instances(Scene *forest, const dr_camera *cam, const dr_render_params *prm, double *accum,
+                           hipStream_t stream);
 
 // batch.hip: copy n wide nodes, moving interior references by node_off and
 // leaf references by ref_off

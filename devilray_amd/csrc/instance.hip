@@ -38,6 +38,81 @@ struct Instances {
 
 namespace {
 
+__global__ void k_offset_u32(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, size_t n, uint32_t off)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + off;
+}
+
+// The shading data of a forest: every scene's vertices, triangles (indices
+// moved by the scene's vertex offset) and materials (ids moved by its
+// material offset) once, and one InstShade per instance.
+dr_status shading_data(Scene &f, Scene *const *sc, uint32_t n_sc, const dr_instance *in, uint32_t n_in)
+{
+    std::vector<uint32_t> voff(n_sc), moff(n_sc);
+    f.cat_tri_off.assign(n_sc + 1, 0u);
+    uint64_t nv = 0, nt = 0, nm = 0;
+    for (uint32_t k = 0; k < n_sc; ++k) {
+        voff[k] = (uint32_t)nv;
+        moff[k] = (uint32_t)nm;
+        f.cat_tri_off[k] = (uint32_t)nt;
+        nv += sc[k]->n_verts;
+        nt += sc[k]->n_tris;
+        nm += sc[k]->n_mats;
+    }
+    if (nv > 0xFFFFFFFFull || nt > 0x7FFFFFFFull || nm > 0xFFFFFFFFull)
+        return fail(DR_E_INVALID, "instanced scenes hold too many vertices, triangles or materials");
+    f.cat_tri_off[n_sc] = (uint32_t)nt;
+    f.n_verts = (uint32_t)nv;
+    f.n_cat_tris = (uint32_t)nt;
+    f.n_mats = (uint32_t)nm;
+    dr_status st = f.verts.ensure(12ull * std::max<uint64_t>(nv, 1));
+    if (!st) st = f.indices.ensure(12ull * std::max<uint64_t>(nt, 1));
+    if (!st) st = f.tri_mat.ensure(4ull * std::max<uint64_t>(nt, 1));
+    if (!st) st = f.materials.ensure(sizeof(dr_material) * std::max<uint64_t>(nm, 1));
+    if (!st) st = f.inst_shade.ensure(sizeof(InstShade) * (size_t)n_in);
+    if (st) return st;
+    f.host_materials.clear();
+    for (uint32_t k = 0; k < n_sc; ++k) {
+        const Scene *s = sc[k];
+        if (s->n_verts)
+            DR_HIP_TRY(hipMemcpy(f.verts.as<float>() + 3ull * voff[k], s->verts.ptr, 12ull * s->n_verts,
+                                 hipMemcpyDeviceToDevice));
+        if (s->n_tris) {
+            const size_t ni = 3ull * s->n_tris;
+            k_offset_u32<<<(unsigned)((ni + 255) / 256), 256>>>(f.indices.as<uint32_t>() + 3ull * f.cat_tri_off[k],
+                                                               s->indices.as<uint32_t>(), ni, voff[k]);
+            k_offset_u32<<<(unsigned)((s->n_tris + 255) / 256), 256>>>(f.tri_mat.as<uint32_t>() + f.cat_tri_off[k],
+                                                                      s->tri_mat.as<uint32_t>(), s->n_tris, moff[k]);
+            DR_HIP_TRY(hipGetLastError());
+        }
+        f.host_materials.insert(f.host_materials.end(), s->host_materials.begin(), s->host_materials.end());
+    }
+    if (nm)
+        DR_HIP_TRY(hipMemcpy(f.materials.ptr, f.host_materials.data(), sizeof(dr_material) * nm,
+                             hipMemcpyHostToDevice));
+    std::vector<InstShade> rec(n_in);
+    f.inst_scene.resize(n_in);
+    uint64_t id_off = 0;
+    for (uint32_t i = 0; i < n_in; ++i) {
+        const uint32_t k = in[i].scene;
+        InstShade &r = rec[i];
+        r.sx = in[i].scale[0];
+        r.sy = in[i].scale[1];
+        r.sz = in[i].scale[2];
+        r.tx = in[i].translate[0];
+        r.ty = in[i].translate[1];
+        r.tz = in[i].translate[2];
+        r.tri_off = f.cat_tri_off[k];
+        r.id_delta = (uint32_t)id_off;
+        f.inst_scene[i] = k;
+        id_off += sc[k]->n_tris;
+    }
+    DR_HIP_TRY(hipMemcpy(f.inst_shade.ptr, rec.data(), sizeof(InstShade) * n_in, hipMemcpyHostToDevice));
+    DR_HIP_TRY(hipDeviceSynchronize());
+    return DR_OK;
+}
+
 // The world box of an instance: the scene's root box through p*s + t, padded
 // by `pad` on every side.
 void world_box(const Scene *s, const dr_instance &in, float pad, float lo[3], float hi[3])
@@ -174,6 +249,7 @@ dr_status build_forest(Instances *I, Scene *const *sc, uint32_t n_sc, const dr_i
     // deepest instanced tree's levels
     f.depth = top.depth + blas_depth;
     f.build_ms = top.build_ms;
+    if ((st = shading_data(f, sc, n_sc, in, n_in))) return st;
     f.built = true;
     I->n_scenes = n_sc;
     return DR_OK;
@@ -260,6 +336,43 @@ dr_status dr_instances_trace_any(dr_instances *ih, const dr_ray *rays, uint32_t 
     auto *I = reinterpret_cast<Instances *>(ih);
     return launch_trace(&I->forest, rays, n, nullptr, occluded, nullptr, nullptr, true,
                         reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_instances_render(dr_instances *ih, const dr_camera *camera, const dr_render_params *params,
+                              double *accum, dr_stream stream)
+{
+    if (!ih) return fail(DR_E_INVALID, "instances handle is null");
+    auto *I = reinterpret_cast<Instances *>(ih);
+    DR_HIP_TRY(hipSetDevice(I->ctx->device));
+    return render_instances(&I->forest, camera, params, accum, reinterpret_cast<hipStream_t>(stream));
+}
+
+dr_status dr_instances_light_count(dr_instances *ih, uint32_t *n_emit)
+{
+    if (!ih || !n_emit) return fail(DR_E_INVALID, "null argument");
+    auto *I = reinterpret_cast<Instances *>(ih);
+    DR_HIP_TRY(hipSetDevice(I->ctx->device));
+    if (!I->forest.lights_built)
+        if (dr_status st = build_instance_lights(&I->forest, nullptr)) return st;
+    *n_emit = I->forest.n_emit;
+    return DR_OK;
+}
+
+dr_status dr_instances_copy_lights(dr_instances *ih, void *entries, uint32_t *emit_tri, float *emit_pdf)
+{
+    if (!ih) return fail(DR_E_INVALID, "instances handle is null");
+    auto *I = reinterpret_cast<Instances *>(ih);
+    Scene &f = I->forest;
+    DR_HIP_TRY(hipSetDevice(I->ctx->device));
+    if (!f.lights_built)
+        if (dr_status st = build_instance_lights(&f, nullptr)) return st;
+    DR_HIP_TRY(hipDeviceSynchronize());
+    if (f.n_emit) {
+        if (entries) DR_HIP_TRY(hipMemcpy(entries, f.alias.ptr, 20ull * f.n_emit, hipMemcpyDeviceToHost));
+        if (emit_tri) DR_HIP_TRY(hipMemcpy(emit_tri, f.emit_tri.ptr, 4ull * f.n_emit, hipMemcpyDeviceToHost));
+        if (emit_pdf) DR_HIP_TRY(hipMemcpy(emit_pdf, f.emit_pdf.ptr, 4ull * f.n_emit, hipMemcpyDeviceToHost));
+    }
+    return DR_OK;
 }
 
 dr_status dr_instances_get_info(dr_instances *ih, dr_scene_info *info)

@@ -165,6 +165,14 @@ struct SceneView {
     const AliasEntry *alias;
     const float *pdf_area;
     uint32_t n_emit;
+    // an instance forest (shaded by shade_item<., true>): hit ids name
+    // instanced triangles, the arrays above are the scenes' own, the alias
+    // entries name emitters, emit_tri their triangle ids (ascending) and
+    // emit_pdf their selection pdf / world area; pdf_area is unused
+    const InstShade *inst;
+    uint32_t n_inst;
+    const uint32_t *emit_tri;
+    const float *emit_pdf;
 };
 
 // Guide state seen by the shading kernel: head != NULL samples from the
@@ -188,9 +196,52 @@ __device__ __forceinline__ V3 vtx(const SceneView &sv, uint32_t vi)
     return V3{sv.verts[3 * (size_t)vi], sv.verts[3 * (size_t)vi + 1], sv.verts[3 * (size_t)vi + 2]};
 }
 
+// World vertices and material id of triangle `tri` (an instanced triangle id
+// when INST: found by the instance whose id range holds it, then moved to
+// world space as p * s + t, one rounding per operation, as a flattened copy
+// of the instance is)
+template <bool INST>
+__device__ __forceinline__ void tri_world(const SceneView &sv, uint32_t tri, V3 &A, V3 &B, V3 &C, uint32_t &mat)
+{
+    uint32_t t = tri;
+    InstShade r{};
+    if (INST) {
+        uint32_t lo = 0, hi = sv.n_inst;  // the last instance whose first id is <= tri
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sv.inst[mid].id_delta <= tri) lo = mid; else hi = mid;
+        }
+        r = sv.inst[lo];
+        t = r.tri_off + (tri - r.id_delta);
+    }
+    A = vtx(sv, sv.idx[3 * (size_t)t]);
+    B = vtx(sv, sv.idx[3 * (size_t)t + 1]);
+    C = vtx(sv, sv.idx[3 * (size_t)t + 2]);
+    if (INST) {
+        A = V3{A.x * r.sx + r.tx, A.y * r.sy + r.ty, A.z * r.sz + r.tz};
+        B = V3{B.x * r.sx + r.tx, B.y * r.sy + r.ty, B.z * r.sz + r.tz};
+        C = V3{C.x * r.sx + r.tx, C.y * r.sy + r.ty, C.z * r.sz + r.tz};
+    }
+    mat = sv.tri_mat[t];
+}
+
+// selection pdf / area of an emitting triangle hit by a BSDF sample (0 for a
+// triangle outside the light table)
+template <bool INST>
+__device__ __forceinline__ float hit_light_pdf(const SceneView &sv, uint32_t tri)
+{
+    if (!INST) return sv.pdf_area[tri];
+    uint32_t lo = 0, hi = sv.n_emit;  // the first emitter whose triangle id is >= tri
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sv.emit_tri[mid] < tri) lo = mid + 1; else hi = mid;
+    }
+    return (lo < sv.n_emit && sv.emit_tri[lo] == tri) ? sv.emit_pdf[lo] : 0.0f;
+}
+
 // GUIDE = false compiles the plain shader (no guide reads, no collection):
-// the same arithmetic in fewer registers.
-template <bool GUIDE>
+// the same arithmetic in fewer registers. INST shades an instance forest.
+template <bool GUIDE, bool INST = false>
 __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, const PathParams &pp, uint32_t vertex,
                                            const uint32_t *__restrict__ in_list,
                                            const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
@@ -230,10 +281,10 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     const float pdf_prev = b4.w;
 
     const uint32_t tri = (uint32_t)h.id;
-    const V3 A = vtx(sv, sv.idx[3 * (size_t)tri]);
-    const V3 B = vtx(sv, sv.idx[3 * (size_t)tri + 1]);
-    const V3 C = vtx(sv, sv.idx[3 * (size_t)tri + 2]);
-    const dr_material mat = sv.mats[sv.tri_mat[tri]];
+    V3 A, B, C;
+    uint32_t mi;
+    tri_world<INST>(sv, tri, A, B, C, mi);
+    const dr_material mat = sv.mats[mi];
     const V3 d = V3{ray.dx, ray.dy, ray.dz};
     const V3 p = bary_point(A, B, C, h.u, h.v);
     const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
@@ -270,7 +321,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
             float w = 1.0f;
             if (!spec_prev) {
                 const float cl = fabsf(dot(d, ng));
-                const float pl = cl < 1e-6f ? 0.0f : sv.pdf_area[tri] * (h.t * h.t) / cl;
+                const float pl = cl < 1e-6f ? 0.0f : hit_light_pdf<INST>(sv, tri) * (h.t * h.t) / cl;
                 w = power_heuristic(pdf_prev, pl);
             }
             L = add(L, V3{beta3.x * E.x * w, beta3.y * E.y * w, beta3.z * E.z * w});
@@ -286,12 +337,13 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
             if (lt >= 0 && psel > 0.0f) {
                 float a1 = u01(r1.z), a2 = u01(r1.w);
                 if (a1 + a2 > 1.0f) { a1 = 1.0f - a1; a2 = 1.0f - a2; }
-                const V3 LA = vtx(sv, sv.idx[3 * (size_t)lt]);
-                const V3 LB = vtx(sv, sv.idx[3 * (size_t)lt + 1]);
-                const V3 LC = vtx(sv, sv.idx[3 * (size_t)lt + 2]);
+                // (an instance forest's entries name emitters)
+                V3 LA, LB, LC;
+                uint32_t lmi;
+                tri_world<INST>(sv, INST ? sv.emit_tri[lt] : (uint32_t)lt, LA, LB, LC, lmi);
                 const V3 x = add(add(LA, mul(sub(LB, LA), a1)), mul(sub(LC, LA), a2));
                 const V3 nl = normalize(cross(sub(LB, LA), sub(LC, LA)));
-                const float pa = sv.pdf_area[lt];
+                const float pa = INST ? sv.emit_pdf[lt] : sv.pdf_area[lt];
                 const V3 toL = sub(x, p);
                 const float dist2 = dot(toL, toL);
                 const float dist = sqrtf(dist2);
@@ -299,7 +351,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
                 const float cs = dot(om, n);
                 const float acl = fabsf(dot(om, nl));
                 if (cs > 0.0f && pa > 0.0f && dist > 0.0f) {
-                    const dr_material lm = sv.mats[sv.tri_mat[lt]];
+                    const dr_material lm = sv.mats[lmi];
                     const float G = cs * acl / dist2;
                     const float pn = acl < 1e-6f ? 0.0f : pa * dist2 / acl;
                     float pb = cs / kPi;
@@ -424,7 +476,7 @@ __device__ __forceinline__ void shade_item(uint32_t slot, const SceneView &sv, c
     out_list[atomicAdd(out_count, 1u)] = path;
 }
 
-template <bool GUIDE>
+template <bool GUIDE, bool INST = false>
 __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint32_t vertex,
                                               const uint32_t *__restrict__ in_list,
                                               const uint32_t *__restrict__ in_count, uint32_t *__restrict__ out_list,
@@ -435,8 +487,8 @@ __global__ __launch_bounds__(kT) void k_shade(SceneView sv, PathParams pp, uint3
                                               GuideView gv, const uint32_t *__restrict__ pview,
                                               uint32_t *__restrict__ sroot)
 {
-    shade_item<GUIDE>(blockIdx.x * kT + threadIdx.x, sv, pp, vertex, in_list, in_count, out_list, out_count, rays,
-                      hits, beta, rad, srays, scontrib, scount, gv, pview, sroot);
+    shade_item<GUIDE, INST>(blockIdx.x * kT + threadIdx.x, sv, pp, vertex, in_list, in_count, out_list, out_count,
+                            rays, hits, beta, rad, srays, scontrib, scount, gv, pview, sroot);
 }
 
 __device__ __forceinline__ void shadow_apply_item(uint32_t i, const uint32_t *__restrict__ scount,
@@ -556,7 +608,7 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
             DR_HIP_TRY(hipGetLastError());
         }
         k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2, cin, s->ray_casts.as<unsigned long long>());
-        const auto shade = (gv.head || gv.smp) ? k_shade<true> : k_shade<false>;
+        const auto shade = sv.inst ? k_shade<false, true> : (gv.head || gv.smp) ? k_shade<true> : k_shade<false>;
         shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
                                               cnt + 2, gv, pview, sroot);
         DR_HIP_TRY(hipGetLastError());
@@ -682,12 +734,13 @@ __device__ bool grid_barrier(uint32_t *g)
 }
 
 // Block size of the persistent launch. The traversal stages treat each
-// kBlock-thread part of a block as a block of their own, so larger blocks
-// work too; measured, 1024-thread blocks (one per CU) were slower than
-// 256-thread ones at the same number of waves (66 against 55 ms on a
-// 1024 x 1024 x 4 spp render): the barrier and the traversal's refill
-// counter see more contention.
-constexpr int kPersistThreads = 256;
+// kBlock-thread part of a block as a block of their own. A grid barrier
+// costs about 0.13 us per resident block (each block's agent-scope fences),
+// so waves are cheapest in few large blocks: on the many-lights view at
+// 1920 x 1080 x 2 spp, one 512-thread block per CU takes 14.3 ms, against
+// 17.3 ms for two 256-thread blocks (the same waves) and 14.5 ms for one
+// 1024-thread block (occupancy 4 waves per SIMD gains nothing over 2).
+constexpr int kPersistThreads = 512;
 constexpr int kPersistParts = kPersistThreads / kBlock;
 
 template <bool SPEC>
@@ -937,6 +990,12 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     sv.alias = s->alias.as<AliasEntry>();
     sv.pdf_area = s->light_pdf_area.as<float>();
     sv.n_emit = s->n_emit;
+    sv.inst = s->n_inst ? s->inst_shade.as<InstShade>() : nullptr;
+    sv.n_inst = s->n_inst;
+    sv.emit_tri = s->emit_tri.as<uint32_t>();
+    sv.emit_pdf = s->emit_pdf.as<float>();
+    if (sv.inst && (use_guide || collect || j.persistent || j.scenes))
+        return fail(DR_E_UNSUPPORTED, "an instance set renders with the plain sampler and the launch sequence");
     PathParams pp;
     pp.views = j.views;
     pp.scenes = j.scenes;
@@ -1008,7 +1067,7 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
     if (dr_status st = check_render_args(cam, prm, accum)) return st;
     if (!s->lights_built)
-        if (dr_status st = build_lights(s, stream)) return st;
+        if (dr_status st = s->n_inst ? build_instance_lights(s, stream) : build_lights(s, stream)) return st;
     const uint32_t npix = cam->width * cam->height;
     const uint64_t items = (uint64_t)npix * prm->spp;
     if (items == 0) return DR_OK;
@@ -1037,6 +1096,13 @@ dr_status render(Scene *s, const dr_camera *cam, const dr_render_params *prm, Gu
     j.sched = DR_SCHED_PER_RAY_SPECULATIVE;
     j.graph = false;
     return render_job(j, stream);
+}
+
+dr_status render_instances(Scene *f, const dr_camera *cam, const dr_render_params *prm, double *accum,
+                           hipStream_t stream)
+{
+    if (!f->n_inst) return fail(DR_E_INVALID, "not an instance set");
+    return render(f, cam, prm, nullptr, 0u, accum, stream);
 }
 
 }  // namespace dr

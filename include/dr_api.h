@@ -429,6 +429,24 @@ dr_status dr_instances_set_trace_schedule(dr_instances *inst, uint32_t closest_s
  * depth: top-level levels plus the deepest scene's levels (the traversal
  * stack holds 3 x depth + 1 entries); overflow_flag as for scenes. */
 dr_status dr_instances_get_info(dr_instances *inst, dr_scene_info *info);
+/* Path-traced render of the instance set as dr_render renders a scene (the
+ * reference renders its objects, each a TriangleMesh view with its own
+ * model_to_world, Renderer::render, src/device/RendererImpl.hpp:22-102):
+ * each instanced triangle keeps its scene's material; hit points, normals
+ * and lights are in world space (model * scale + translate, per axis), and
+ * every emissive triangle of every instance is a light weighted by its world
+ * area (the reference weights an object's lights by its scaled surface area,
+ * src/Renderer.cpp:113-152). Traversal runs in model space through the
+ * two-level forest, so nothing is copied per instance. With exact transforms
+ * the image equals dr_render of the flattened copy bit for bit. Plain
+ * sampler only (no guide); the light table is built on first use. */
+dr_status dr_instances_render(dr_instances *inst, const dr_camera *camera, const dr_render_params *params,
+                              double *accum, dr_stream stream);
+/* The instance set's light table (built if needed): n_emit entries as
+ * dr_scene_copy_lights but naming emitters by index, their instanced
+ * triangle ids (ascending) and their area pdfs; any pointer may be NULL. */
+dr_status dr_instances_light_count(dr_instances *inst, uint32_t *n_emit);
+dr_status dr_instances_copy_lights(dr_instances *inst, void *entries, uint32_t *emit_tri, float *emit_pdf);
 
 /* ---- learned path guide (no reference counterpart: the reference samples
  * Lambert bounces from the BSDF only, PathGeneration.hpp:406-420) ----------

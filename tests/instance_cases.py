@@ -109,3 +109,94 @@ def rays_at_targets(world_v, world_f, lo, hi, n, seed=0, grid=512.0):
     d /= np.maximum(np.linalg.norm(d, axis=1, keepdims=True), 1e-30)
     r["dx"], r["dy"], r["dz"] = d.T.astype(np.float32)
     return r
+
+
+# ---- render workloads --------------------------------------------------------
+
+LAMBERT, DIELECTRIC = dr.DR_MAT_LAMBERT, dr.DR_MAT_DIELECTRIC
+
+
+def _box(half):
+    v, f = cube()
+    return quantize(v * half), f
+
+
+def _quad(center, size):
+    c = np.asarray(center, np.float64)
+    h = size / 2.0
+    v = np.array([c + [-h, 0, -h], c + [h, 0, -h], c + [h, 0, h], c + [-h, 0, h]], np.float64)
+    return quantize(v), np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def _sphere(center, radius, subdiv=2):
+    v, f = dr.gen_icosphere(subdiv, 0.0, 0)
+    v = v.astype(np.float64) / np.linalg.norm(v.astype(np.float64), axis=1, keepdims=True)
+    return quantize(v * radius + np.asarray(center, np.float64)), f
+
+
+def render_scenes():
+    """Model scenes on the 1/256 grid, each (vertices, triangles, per-triangle
+    material, materials as (kind, albedo, emission, ior)): a grey box (the
+    room), an emissive quad, a Lambert sphere off the origin, a glass sphere
+    off the origin and a two-material pair of spheres (one half emits)."""
+    out = []
+    v, f = _box(2.0)
+    out.append((v, f, np.zeros(f.shape[0], np.uint32), [(LAMBERT, (0.7, 0.7, 0.7), (0, 0, 0), 1.0)]))
+    v, f = _quad((0.5, 1.75, 0.5), 0.5)
+    out.append((v, f, np.zeros(2, np.uint32), [(LAMBERT, (0.5, 0.5, 0.5), (6.0, 5.0, 4.0), 1.0)]))
+    v, f = _sphere((2.0, -1.0, 0.0), 0.5)
+    out.append((v, f, np.zeros(f.shape[0], np.uint32), [(LAMBERT, (0.8, 0.3, 0.2), (0, 0, 0), 1.0)]))
+    v, f = _sphere((-2.0, -2.0, 1.0), 0.625)
+    out.append((v, f, np.zeros(f.shape[0], np.uint32), [(DIELECTRIC, (1, 1, 1), (0, 0, 0), 1.5)]))
+    v, f = _sphere((1.0, 1.0, -2.0), 0.375, subdiv=1)
+    m = (np.arange(f.shape[0]) % 2).astype(np.uint32)
+    out.append((v, f, m, [(LAMBERT, (0.3, 0.8, 0.3), (0, 0, 0), 1.0),
+                          (LAMBERT, (0.6, 0.6, 0.6), (1.5, 2.5, 3.0), 1.0)]))
+    return out
+
+
+def render_instances_exact():
+    """Power-of-two scales of one magnitude per instance (signs mirror) and
+    no translation: every vertex and every ray origin moves by an exact
+    power of two, so the instanced traversal and shading see the flattened
+    copy's numbers up to that factor."""
+    rows = [(0, 4.0, (1, 1, 1)), (1, 2.0, (1, 1, 1)), (1, 2.0, (-1, 1, 1)), (2, 1.0, (1, 1, 1)),
+            (2, 2.0, (1, 1, 1)), (2, 2.0, (-1, 1, -1)), (2, 2.0, (1, -1, 1)), (3, 1.0, (1, 1, 1)),
+            (3, 2.0, (1, 1, 1)), (4, 2.0, (1, 1, 1)), (4, 2.0, (-1, -1, 1)), (4, 1.0, (1, 1, 1))]
+    out = np.zeros(len(rows), dr.INSTANCE_DTYPE)
+    for i, (sc, mag, sign) in enumerate(rows):
+        out[i]["scene"] = sc
+        out[i]["scale"] = (np.asarray(sign, np.float64) * mag).astype(np.float32)
+    return out
+
+
+def render_instances_general(seed=5):
+    """The same placements with arbitrary scales and translations."""
+    rng = np.random.default_rng(seed)
+    out = render_instances_exact()
+    for i in range(1, out.shape[0]):
+        out[i]["scale"] = (out[i]["scale"] * rng.uniform(0.8, 1.2, 3)).astype(np.float32)
+        out[i]["translate"] = rng.uniform(-0.4, 0.4, 3).astype(np.float32)
+    return out
+
+
+def flatten_render(scenes, instances):
+    """The flattened copy of an instanced render set as a SceneData: world
+    vertices v * s + t in instance order, each scene's materials once."""
+    from devilray_amd.scenes import SceneData
+    moff = np.cumsum([0] + [len(s[3]) for s in scenes])
+    v, f = flatten([(s[0], s[1]) for s in scenes], instances)
+    tm = np.concatenate([scenes[int(r["scene"])][2] + moff[int(r["scene"])] for r in instances]).astype(np.uint32)
+    mats = [m for s in scenes for m in s[3]]
+    return SceneData(v, f, tm, mats)
+
+
+def build_render_scenes(ctx, scenes):
+    out = []
+    for v, f, tm, mats in scenes:
+        from devilray_amd.scenes import SceneData
+        sd = SceneData(v, f, tm, mats)
+        s = dr.Scene(ctx, v, f, tm, sd.dr_materials())
+        s.build_bvh(leaf_max=4)
+        out.append(s)
+    return out
