@@ -35,7 +35,9 @@ on all host threads allotted to the process (1 warm-up, median of 5).
 sample's ray bytes on the canonical tree plus 64 B of FP64 accumulation,
 from the oracle's counts), and the reference-semantics CPU integrator's
 samples/s on the Cornell box (its full 256 x 256 x 16 spp) and on a
-480 x 270 x 4 spp sample of the many-lights view.
+480 x 270 x 4 spp sample of the many-lights view, and an instanced workload
+(441 placements of three sphere meshes and a room, 8.2M instanced
+triangles) rendered through dr_instances_render against its flattened copy.
 
 --probe: runs only the traversal launches and a 256 MiB device copy (the
 known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
@@ -214,6 +216,54 @@ def render_rate(ctx, sc, spp, max_depth, rr_depth, reps, width=None, height=None
     s.close()
     return {"ms_per_pass": round(ms, 3), "spp_per_pass": spp, "samples_per_s": round(w * h * spp / ms * 1e3, 1),
             "image_mean": round(mean, 6), "resolution": [w, h], "max_depth": max_depth, "rr_depth": rr_depth}
+
+
+def instancing_rate(ctx, spp=4, max_depth=12, rr_depth=3):
+    """The instanced-spheres workload (scenes.instanced_spheres: 400
+    placements of a 20,480-triangle Lambert or glass icosphere, 40 small
+    emitters, a room) rendered at 1920x1080 through dr_instances_render,
+    against dr_render of its flattened world copy (8.2M triangles): samples/s,
+    tree bytes (64 B nodes + 48 B triangle references) and image means."""
+    meshes, inst, c = scenes.instanced_spheres()
+    cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"])
+    w, h = c["width"], c["height"]
+    acc = ctx.alloc(w * h * 32)
+    built = []
+    for m in meshes:
+        s = dr.Scene(ctx, m.vertices, m.indices, m.tri_material, m.dr_materials())
+        s.build_bvh(leaf_max=4)
+        built.append(s)
+    I = dr.Instances(ctx, built, inst)
+    flat_sd = scenes.flatten_instances(meshes, inst)
+    flat = dr.Scene(ctx, flat_sd.vertices, flat_sd.indices, flat_sd.tri_material, flat_sd.dr_materials())
+    flat.build_bvh(leaf_max=4)
+    out = {"instances": int(inst.shape[0]), "instanced_triangles": int(flat_sd.indices.shape[0]),
+           "stored_triangles": int(sum(m.indices.shape[0] for m in meshes)), "resolution": [w, h],
+           "spp_per_pass": spp, "max_depth": max_depth}
+    for label, obj in (("instanced", I), ("flattened", flat)):
+        p = dr.DrRenderParams(spp, 0, max_depth, rr_depth, 0, 0, 0)
+        ctx.fill_zero(acc)
+        obj.render_dev(cam, p, acc.ptr)  # light table, buffers, warm-up
+        ctx.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record()
+        p.sample_offset = spp
+        obj.render_dev(cam, p, acc.ptr)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1)
+        img = acc.to_numpy(np.float64, w * h * 4).reshape(h, w, 4)
+        info = obj.info()
+        out[label] = {"ms_per_pass": round(ms, 3), "samples_per_s": round(w * h * spp / ms * 1e3, 1),
+                      "image_mean": round(float((img[..., :3] / np.maximum(img[..., 3:], 1)).mean()), 6),
+                      "tree_bytes": int(info.n_nodes) * 64 + int(info.n_references) * 48}
+    out["speed_instanced_over_flattened"] = round(out["flattened"]["ms_per_pass"] / out["instanced"]["ms_per_pass"], 3)
+    flat.close()
+    I.close()
+    for s in built:
+        s.close()
+    acc.free()
+    return out
 
 
 def display_rate(ctx, sc):
@@ -780,6 +830,8 @@ def main() -> int:
         renders["many_lights_1080p_64spp"] = ml
         log("[bench] display resolve and debug view at 1920x1080")
         renders["display"] = display_rate(ctx, scenes.many_lights())
+        log("[bench] instanced spheres: 441 instances against their 8.2M-triangle flattened copy")
+        renders["instanced_spheres_1080p"] = instancing_rate(ctx)
 
     guide = {}
     if not args.no_guide:

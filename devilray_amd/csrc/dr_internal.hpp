@@ -360,7 +360,7 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
 // an any-hit queue (rays_a, device count na_dev) and a closest-hit queue
 // (rays_c through list_c, device count nc_dev) traced in one launch, each
 // ray exactly as launch_trace_ex would; both counts at most n_max. Only for
-// the per-ray schedules of a scene without instances (trace_pair_fits).
+// the per-ray schedules (trace_pair_fits); instances run the speculative one.
 bool trace_pair_fits(const Scene *s, uint32_t sched);
 dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_dev, uint32_t *occluded,
                             const uint32_t *roots_a, const dr_ray *rays_c, const uint32_t *list_c,

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const QNode *__restrict__ node
 // each part with its own block indices, grid size and spill region, so every
 // ray is traced exactly as in a launch of its own. Both counts live in device
 // memory; blocks past a count exit at once.
-template <bool SPECULATIVE>
+template <bool SPECULATIVE, bool INST = false>
 __global__ __launch_bounds__(kBlock) void k_trace_pair(const QNode *__restrict__ nodes,
                                                        const float4 *__restrict__ tri_v,
                                                        const float4 *__restrict__ rays_a, const uint32_t *__restrict__ na,
@@ -48,18 +48,19 @@ __global__ __launch_bounds__(kBlock) void k_trace_pair(const QNode *__restrict__
                                                        const uint32_t *__restrict__ nc, dr_hit *__restrict__ hits,
                                                        const uint32_t *__restrict__ roots_c,
                                                        int32_t *__restrict__ ovf_c, uint32_t n_max,
-                                                       uint32_t *__restrict__ flags, uint32_t max_stack)
+                                                       uint32_t *__restrict__ flags, uint32_t max_stack,
+                                                       const InstRec *__restrict__ inst)
 {
     __shared__ int32_t stack[kLdsStack][kBlock];
     if (blockIdx.x < nblk_a)
-        trace_body<true, false, false, SPECULATIVE>(stack, threadIdx.x, blockIdx.x, nblk_a, nodes, tri_v, rays_a,
-                                                    nullptr, n_max, na, nullptr, occ, nullptr, nullptr, ovf_a, flags,
-                                                    nullptr, 0u, roots_a, 1u, max_stack, nullptr);
+        trace_body<true, false, false, SPECULATIVE, INST>(stack, threadIdx.x, blockIdx.x, nblk_a, nodes, tri_v,
+                                                          rays_a, nullptr, n_max, na, nullptr, occ, nullptr, nullptr,
+                                                          ovf_a, flags, nullptr, 0u, roots_a, 1u, max_stack, inst);
     else
-        trace_body<false, false, false, SPECULATIVE>(stack, threadIdx.x, blockIdx.x - nblk_a, gridDim.x - nblk_a,
-                                                     nodes, tri_v, rays_c, list_c, n_max, nc, hits, nullptr, nullptr,
-                                                     nullptr, ovf_c, flags, nullptr, 0u, roots_c, 1u, max_stack,
-                                                     nullptr);
+        trace_body<false, false, false, SPECULATIVE, INST>(stack, threadIdx.x, blockIdx.x - nblk_a,
+                                                           gridDim.x - nblk_a, nodes, tri_v, rays_c, list_c, n_max,
+                                                           nc, hits, nullptr, nullptr, nullptr, ovf_c, flags, nullptr,
+                                                           0u, roots_c, 1u, max_stack, inst);
 }
 
 __global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
@@ -303,8 +304,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
 bool trace_pair_fits(const Scene *s, uint32_t sched)
 {
     const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
-    return s->built && s->n_inst == 0 && (sched & 2u) == 0 && s->depth <= kMaxWideDepth &&
-           max_stack - kLdsStack <= kPerRaySpillMax;
+    return s->built && (sched & 2u) == 0 && s->depth <= kMaxWideDepth && max_stack - kLdsStack <= kPerRaySpillMax;
 }
 
 dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_dev, uint32_t *occluded,
@@ -316,16 +316,20 @@ dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_d
     if (n_max == 0) return DR_OK;
     if (!rays_a || !na_dev || !occluded || !rays_c || !nc_dev || !hits)
         return fail(DR_E_INVALID, "paired trace launch: null ray, count or output buffer");
+    const bool inst = s->n_inst != 0;
+    if (inst && (roots_a || roots_c)) return fail(DR_E_UNSUPPORTED, "instances: no per-ray roots");
     const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
     const uint32_t need = (n_max + kBlock - 1) / kBlock;
     const size_t part = (size_t)(max_stack - kLdsStack) * need * kBlock;  // spill entries of one part
     if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * 2 * part + 4)) return st;
     int32_t *ovf = s->ovf_stack.as<int32_t>();
-    const auto kern = (sched & 1u) ? k_trace_pair<true> : k_trace_pair<false>;
+    // instances run the speculative loop (as launch_trace_ex promotes them)
+    const auto kern = inst ? k_trace_pair<true, true> : (sched & 1u) ? k_trace_pair<true> : k_trace_pair<false>;
     kern<<<2 * need, kBlock, 0, stream>>>(s->nodes.as<QNode>(), s->tri_v.as<float4>(),
                                           reinterpret_cast<const float4 *>(rays_a), na_dev, occluded, roots_a, need,
                                           ovf, reinterpret_cast<const float4 *>(rays_c), list_c, nc_dev, hits, roots_c,
-                                          ovf + part, n_max, s->flags.as<uint32_t>(), max_stack);
+                                          ovf + part, n_max, s->flags.as<uint32_t>(), max_stack,
+                                          inst ? s->inst_recs.as<InstRec>() : nullptr);
     DR_HIP_TRY(hipGetLastError());
     return DR_OK;
 }

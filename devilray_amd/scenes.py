@@ -329,3 +329,67 @@ def batch_c3(n_scenes: int = 64, seed: int = 0, tris_range=(1_000, 2_000_000), p
         spp = int(rng.choice(spp_choices))
         out.append((sd, w, h, spp))
     return out
+
+
+def instanced_spheres(n_spheres: int = 400, n_lights: int = 40, subdiv: int = 5, seed: int = 0,
+                      extent: float = 20.0):
+    """An instanced workload: one Lambert and one glass icosphere mesh
+    (subdivision `subdiv`, 20 * 4^subdiv triangles each) placed n_spheres
+    times (30% glass) with random positions in [-extent, extent]^3 and
+    log-uniform scales in [0.5, 2.5], n_lights placements of a small emissive
+    icosphere, and a grey room around them. Returns (meshes, instances,
+    camera): meshes as SceneData (model space, one material list each),
+    instances as INSTANCE_DTYPE; the camera as for many_lights."""
+    from . import make_instances
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for kind, alb, em, ior, sd in ((DR_MAT_LAMBERT, (0.6, 0.6, 0.6), (0, 0, 0), 1.0, subdiv),
+                                   (DR_MAT_DIELECTRIC, (1, 1, 1), (0, 0, 0), 1.5, subdiv),
+                                   (DR_MAT_LAMBERT, (0.5, 0.5, 0.5), (8.0, 7.0, 6.0), 1.0, 2)):
+        b = _Builder()
+        b.sphere(sd, (0.0, 0.0, 0.0), 1.0, b.material(kind, alb, em, ior))
+        meshes.append(b.build())
+    room = _Builder()
+    grey = room.material(DR_MAT_LAMBERT, (0.7, 0.7, 0.7))
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            c = [0.0, 0.0, 0.0]
+            c[axis] = sgn
+            n = [0.0, 0.0, 0.0]
+            n[axis] = -sgn
+            r = [0.0, 0.0, 0.0]
+            r[(axis + 1) % 3] = 1.0
+            room.quad(c, n, r, 2, 2, grey)
+    meshes.append(room.build())
+    kind = np.where(rng.random(n_spheres) < 0.3, 1, 0)
+    pos = rng.uniform(-extent, extent, (n_spheres, 3))
+    scale = np.exp(rng.uniform(np.log(0.5), np.log(2.5), n_spheres))
+    lpos = rng.uniform(-extent, extent, (n_lights, 3))
+    lscale = rng.uniform(0.3, 0.6, n_lights)
+    R = 3.2 * extent + 8.0
+    inst = np.concatenate([
+        make_instances(kind, np.repeat(scale[:, None], 3, 1), pos),
+        make_instances(np.full(n_lights, 2), np.repeat(lscale[:, None], 3, 1), lpos),
+        make_instances([3], (R, R, R), (0.0, 0.0, 0.0))])
+    cam = dict(eye=(0.0, 0.0, 3.2 * extent), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
+               width=1920, height=1080)
+    return meshes, inst, cam
+
+
+def flatten_instances(meshes, instances) -> SceneData:
+    """The world copy of an instanced set as one scene: vertices v * s + t in
+    instance order (float32, one rounding per operation), materials of every
+    mesh concatenated."""
+    vs, fs, ms, off = [], [], [], 0
+    moff = np.cumsum([0] + [len(m.materials) for m in meshes])
+    for rec in instances:
+        k = int(rec["scene"])
+        m = meshes[k]
+        w = (m.vertices.astype(np.float32) * rec["scale"].astype(np.float32)
+             + rec["translate"].astype(np.float32)).astype(np.float32)
+        vs.append(w)
+        fs.append(m.indices.astype(np.uint32) + off)
+        ms.append(m.tri_material.astype(np.uint32) + moff[k])
+        off += w.shape[0]
+    mats = [x for m in meshes for x in m.materials]
+    return SceneData(np.concatenate(vs), np.concatenate(fs), np.concatenate(ms), mats)
