@@ -249,6 +249,10 @@ class Scene:
     def build_lights(self, stream=None):
         check(lib().dr_scene_build_lights(self.handle, stream))
 
+    def set_render_lanes(self, lanes: int):
+        """Lanes (streams) a render deals its paths to (1..4, default 4)."""
+        check(lib().dr_scene_set_render_lanes(self.handle, lanes))
+
     def lights(self):
         """(alias entries, per-triangle area pdf) copied to the host."""
         n = C.c_uint32()
@@ -574,7 +578,7 @@ class Batch:
             pass
 
     def render_dev(self, views, max_depth: int = 12, rr_depth: int = 3, seed: int = 0, fresnel: int = 0,
-                   chunk_paths: int = 1 << 21, trace_sched: int = 1, refill_min: int = 16, use_graph: bool = True,
+                   chunk_paths: int = 1 << 24, trace_sched: int = 1, refill_min: int = 16, use_graph: bool = False,
                    launch: int = 0, blocks_per_cu: int = 0, stream=None):
         """views: [(scene index, DrCamera, spp, sample_offset, accum device pointer)]. launch:
         DR_BATCH_LAUNCH_SEQUENCE (a launch per stage, per chunk) or DR_BATCH_LAUNCH_PERSISTENT (one

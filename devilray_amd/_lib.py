@@ -137,6 +137,7 @@ PROTOTYPES = {
     "dr_trace_closest_counted": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "dr_scene_set_trace_schedule": (_st, [_vp, _u32, _u32]),
     "dr_scene_set_trace_tuning": (_st, [_vp, _u32, _u32, _u32]),
+    "dr_scene_set_render_lanes": (_st, [_vp, _u32]),
     "dr_scene_build_lights": (_st, [_vp, _vp]),
     "dr_scene_light_count": (_st, [_vp, C.POINTER(_u32)]),
     "dr_scene_copy_lights": (_st, [_vp, _vp, _vp]),

@@ -308,6 +308,14 @@ dr_status dr_scene_set_trace_tuning(dr_scene *sc, uint32_t refill_min, uint32_t 
     return DR_OK;
 }
 
+dr_status dr_scene_set_render_lanes(dr_scene *sc, uint32_t lanes)
+{
+    if (!sc) return fail(DR_E_INVALID, "scene is null");
+    if (lanes < 1 || lanes > kMaxRenderLanes) return fail(DR_E_INVALID, "render lanes must be 1..4");
+    reinterpret_cast<Scene *>(sc)->render_lanes = lanes;
+    return DR_OK;
+}
+
 dr_status dr_trace_closest(dr_scene *sc, const dr_ray *rays, uint32_t n, dr_hit *hits, dr_stream stream)
 {
     if (!sc) return fail(DR_E_INVALID, "scene is null");

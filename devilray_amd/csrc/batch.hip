@@ -244,9 +244,9 @@ dr_status dr_batch_render(dr_batch *bt, const dr_batch_view *views, uint32_t n_v
     if (!bt || !params || (n_views && !views)) return fail(DR_E_INVALID, "null argument");
     auto *b = reinterpret_cast<Batch *>(bt);
     DR_HIP_TRY(hipSetDevice(b->ctx->device));
-    dr_batch_options opt{kPathCap, DR_SCHED_PER_RAY_SPECULATIVE, 16u, 1u, DR_BATCH_LAUNCH_SEQUENCE, 0u};
+    dr_batch_options opt{kPathCap, DR_SCHED_PER_RAY_SPECULATIVE, 16u, 0u, DR_BATCH_LAUNCH_SEQUENCE, 0u};
     if (options) opt = *options;
-    if (opt.chunk_paths < 4096 || opt.chunk_paths > (1u << 23)) return fail(DR_E_INVALID, "chunk_paths out of range");
+    if (opt.chunk_paths < 4096 || opt.chunk_paths > kMaxChunkPaths) return fail(DR_E_INVALID, "chunk_paths out of range");
     if (opt.trace_sched > 15u) return fail(DR_E_INVALID, "unknown trace schedule");
     if (opt.refill_min < 1u || opt.refill_min > 64u) return fail(DR_E_INVALID, "refill_min must be 1..64");
     if (opt.launch > DR_BATCH_LAUNCH_PERSISTENT) return fail(DR_E_INVALID, "unknown launch mode");
@@ -337,7 +337,7 @@ dr_status dr_batch_autotune(dr_batch *bt, const dr_batch_view *views, uint32_t n
     // the resident-grid schedule with one ray counter or one per XCD, refill
     // thresholds and resident blocks per CU (more cost more at its barriers)
     std::vector<dr_batch_options> cand;
-    for (uint32_t chunk : {1u << 20, 1u << 21, 1u << 22, 1u << 23})
+    for (uint32_t chunk : {1u << 21, 1u << 22, 1u << 23, 1u << 24})
         for (uint32_t graph : {0u, 1u}) {
             cand.push_back(dr_batch_options{chunk, DR_SCHED_PER_RAY_SPECULATIVE, 16u, graph,
                                             DR_BATCH_LAUNCH_SEQUENCE, 0u});

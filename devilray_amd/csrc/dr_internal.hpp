@@ -111,6 +111,19 @@ struct Context {
     int32_t *noalloc_scratch = nullptr;
 };
 
+// A lane of the wavefront render: a share of each chunk's paths with its own
+// queues, shadow rays and traversal spill region, on its own stream, so the
+// lanes' launches run side by side and one lane's launch tails are filled by
+// another's work. Lane 0 uses the scene's rs_* queues and the caller's
+// stream; lanes 1.. own these buffers, a non-blocking stream of default
+// priority and an event that ends their part of a chunk.
+struct RenderLane {
+    DevBuf list_a, list_b, counters, srays, socc, scon, sroot, ovf;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+constexpr uint32_t kMaxRenderLanes = 4;
+
 struct Scene {
     Context *ctx = nullptr;
     uint32_t n_tris = 0, n_verts = 0, n_mats = 0;
@@ -183,6 +196,12 @@ struct Scene {
     // the renders of this scene since it was created or the count was reset
     DevBuf ray_casts;
     uint32_t rs_capacity = 0;
+    // lanes of the launch sequence (RenderLane): how many a render uses and
+    // the event that starts them
+    uint32_t render_lanes = 4;
+    RenderLane lanes[kMaxRenderLanes];
+    hipEvent_t lane_fork = nullptr;
+    void release_lanes();
     // The recorded bounce loop of the last graph render and an event after its
     // last replay: an executable graph is freed only once its launches are
     // done (at the next graph render, or with the scene).
@@ -192,7 +211,11 @@ struct Scene {
     Scene() = default;
     Scene(const Scene &) = delete;
     Scene &operator=(const Scene &) = delete;
-    ~Scene() { release_graph(); }
+    ~Scene()
+    {
+        release_graph();
+        release_lanes();
+    }
 };
 
 // ---- learned path guide (guide.hip) ----------------------------------------
@@ -294,7 +317,12 @@ struct SceneRange {
     uint32_t root, alias_off, n_emit, pad;
 };
 
-constexpr uint32_t kPathCap = 1u << 21;  // default paths per chunk
+// default paths per chunk: a larger wavefront hides the traversal's latency
+// and its launch tails better (many-lights at 1920x1080 x 8 spp, one lane:
+// 27.8 ms at 2^22 paths per chunk, 23.4 at 2^23, 21.3 at 2^24); about 150 B
+// of path state and queues per path (2.5 GB at 2^24)
+constexpr uint32_t kPathCap = 1u << 24;
+constexpr uint32_t kMaxChunkPaths = 1u << 24;
 
 // A render call: views (device table) over one scene, or over a batch's
 // merged scene with a per-scene range table.
@@ -351,10 +379,12 @@ dr_status refit_bvh(Scene *s, hipStream_t stream);
 // indices); n_dev (optional): device-resident count, at most n_max.
 // ray_root (optional): per ray, the BVH node its traversal starts at (a
 // batch's merged forest); NULL starts every ray at node 0.
+// ovf (optional): the spill region to use instead of the scene's (a render
+// lane's own, so lanes on several streams never share one).
 dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
                           const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
                           uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream,
-                          const uint32_t *ray_root = nullptr);
+                          const uint32_t *ray_root = nullptr, DevBuf *ovf = nullptr);
 dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, uint32_t *occluded,
                        uint32_t *box_tests, uint32_t *tri_tests, bool any_hit, hipStream_t stream);
 // an any-hit queue (rays_a, device count na_dev) and a closest-hit queue
@@ -365,7 +395,7 @@ bool trace_pair_fits(const Scene *s, uint32_t sched);
 dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_dev, uint32_t *occluded,
                             const uint32_t *roots_a, const dr_ray *rays_c, const uint32_t *list_c,
                             const uint32_t *nc_dev, dr_hit *hits, const uint32_t *roots_c, uint32_t n_max,
-                            uint32_t sched, hipStream_t stream);
+                            uint32_t sched, hipStream_t stream, DevBuf *ovf = nullptr);
 // closest hit with every scratch buffer in the static device array (no
 // allocation); noalloc_init looks up this device's copy of it
 dr_status noalloc_init(Context *ctx);

@@ -104,14 +104,29 @@ __global__ void k_set_view(ViewInfo *dst, ViewInfo v)
 // Each stage is written for one item (path slot, queue slot or shadow ray)
 // and run either as its own kernel, one thread per item, or as a
 // grid-stride loop of the persistent render kernel.
+// The lanes a chunk's paths are dealt to: lane L takes paths [L h, (L + 1) h)
+// with h = ceil(n / n_lanes), each into its own first queue and counters.
+struct CamLanes {
+    uint32_t *list[kMaxRenderLanes];
+    uint32_t *counters[kMaxRenderLanes];
+    uint32_t n_lanes;
+};
+
 __device__ __forceinline__ void camera_item(uint32_t p, const PathParams &pp, dr_ray *__restrict__ rays,
                                             float4 *__restrict__ beta, float4 *__restrict__ rad,
-                                            uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
-                                            uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
-                                            uint32_t *__restrict__ proot)
+                                            const CamLanes &cl, uint32_t *__restrict__ pending,
+                                            uint32_t *__restrict__ pview, uint32_t *__restrict__ proot)
 {
     const uint32_t n = pp.chunk->n;
-    if (p == 0) { counters[0] = n; counters[1] = 0u; counters[2] = 0u; }
+    const uint32_t h = (n + cl.n_lanes - 1u) / cl.n_lanes;
+    if (p < cl.n_lanes) {
+        const uint32_t lo = min(p * h, n);
+        uint32_t *c = cl.counters[p];
+        c[0] = min(lo + h, n) - lo;
+        c[1] = 0u;
+        c[2] = 0u;
+        c[3] = 0u;
+    }
     if (p >= n) return;
     const uint64_t item = pp.chunk->item0 + p;
     const uint32_t view = find_view(pp.views, pp.n_views, item);
@@ -124,27 +139,26 @@ __device__ __forceinline__ void camera_item(uint32_t p, const PathParams &pp, dr
     rays[p] = camera_ray(cam, pixel, sample, V.seed);
     beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);       // w: pdf of the last BSDF sample
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);        // w: 1 = last bounce specular / camera
-    list[p] = p;
+    cl.list[p / h][p % h] = p;
     if (pending) pending[p] = kGuideNoSample;
 }
 
 __global__ __launch_bounds__(kT) void k_camera(PathParams pp, dr_ray *__restrict__ rays,
-                                               float4 *__restrict__ beta, float4 *__restrict__ rad,
-                                               uint32_t *__restrict__ list, uint32_t *__restrict__ counters,
+                                               float4 *__restrict__ beta, float4 *__restrict__ rad, CamLanes cl,
                                                uint32_t *__restrict__ pending, uint32_t *__restrict__ pview,
                                                uint32_t *__restrict__ proot)
 {
-    camera_item(blockIdx.x * kT + threadIdx.x, pp, rays, beta, rad, list, counters, pending, pview, proot);
+    camera_item(blockIdx.x * kT + threadIdx.x, pp, rays, beta, rad, cl, pending, pview, proot);
 }
 
 // also adds this vertex's extension rays (the input queue) to the scene's
-// ray-cast count
+// ray-cast count (atomically: render lanes run on several streams)
 __device__ __forceinline__ void reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
                                              unsigned long long *casts)
 {
     *out_count = 0u;
     *shadow_count = 0u;
-    *casts += *in_count;
+    atomicAdd(casts, (unsigned long long)*in_count);
 }
 __global__ void k_reset_counts(uint32_t *out_count, uint32_t *shadow_count, const uint32_t *in_count,
                                unsigned long long *casts)
@@ -543,58 +557,58 @@ __global__ __launch_bounds__(kT) void k_resolve(PathParams pp, const float4 *__r
 
 inline uint32_t blocks(uint32_t n) { return (n + kT - 1) / kT; }
 
-// One chunk of the bounce loop. Every argument is fixed for the whole call
-// (chunk position and queue counts live in device memory), so the same
-// launches can be recorded once and replayed for every chunk.
-dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &sv, const GuideView &gv_in,
-                    hipStream_t stream)
+// One render lane's view of its buffers (RenderLane; lane 0 holds the
+// scene's rs_* queues) and the stream its launches go to.
+struct LaneView {
+    uint32_t *list_a, *list_b, *cnt;
+    dr_ray *srays;
+    uint32_t *socc;
+    float4 *scon;
+    uint32_t *sroot;
+    DevBuf *ovf;
+    hipStream_t stream;
+};
+
+// The bounce loop of one lane's share of a chunk: vertices 0 .. max_depth - 1
+// of its paths, every queue of at most `cap` entries.
+dr_status bounce_loop(const RenderJob &j, const PathParams &pp, const SceneView &sv, GuideView gv,
+                      const LaneView &L, uint32_t cap)
 {
     Scene *s = j.s;
-    const uint32_t cap = j.cap;
+    const hipStream_t stream = L.stream;
     const bool use_guide = (j.guide_flags & DR_GUIDE_USE) != 0;
-    const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
-    GuideView gv = gv_in;
-    uint32_t *cnt = s->rs_counters.as<uint32_t>();
+    uint32_t *cnt = L.cnt;
     auto *rays = s->rs_rays.as<dr_ray>();
     auto *hits = s->rs_hits.as<dr_hit>();
     auto *beta = s->rs_beta.as<float4>();
     auto *rad = s->rs_radiance.as<float4>();
-    auto *srays = s->rs_shadow_rays.as<dr_ray>();
-    auto *socc = s->rs_shadow_occ.as<uint32_t>();
-    auto *scon = s->rs_shadow_contrib.as<float4>();
     auto *pview = s->rs_view.as<uint32_t>();
     uint32_t *proot = j.scenes ? s->rs_root.as<uint32_t>() : nullptr;
-    uint32_t *sroot = j.scenes ? s->rs_sroot.as<uint32_t>() : nullptr;
-    k_next_chunk<<<1, 64, 0, stream>>>(s->rs_chunk.as<ChunkState>());
-    DR_HIP_TRY(hipGetLastError());
-    k_camera<<<blocks(cap), kT, 0, stream>>>(pp, rays, beta, rad, s->rs_list_a.as<uint32_t>(), cnt,
-                                             collect ? gv.pending : nullptr, pview, proot);
-    DR_HIP_TRY(hipGetLastError());
     // With a per-ray schedule, the shadow rays of vertex v - 1 are traced in
     // the same launch as the extension rays of vertex v, and applied before
     // vertex v is shaded: each radiance gets its terms in the same order as
     // with separate launches, and two short queues fill one grid.
     const bool paired = s->n_emit && trace_pair_fits(s, j.sched);
     auto trace_shadows = [&]() -> dr_status {
-        if (dr_status st = launch_trace_ex(s, srays, nullptr, cap, cnt + 2, nullptr, socc, nullptr, nullptr, true,
-                                           j.sched, stream, sroot))
+        if (dr_status st = launch_trace_ex(s, L.srays, nullptr, cap, cnt + 2, nullptr, L.socc, nullptr, nullptr, true,
+                                           j.sched, stream, L.sroot, L.ovf))
             return st;
-        k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
+        k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, L.socc, L.scon, rad);
         DR_HIP_TRY(hipGetLastError());
         return DR_OK;
     };
     for (uint32_t v = 0; v < j.prm->max_depth; ++v) {
-        uint32_t *lin = (v & 1u) ? s->rs_list_b.as<uint32_t>() : s->rs_list_a.as<uint32_t>();
-        uint32_t *lout = (v & 1u) ? s->rs_list_a.as<uint32_t>() : s->rs_list_b.as<uint32_t>();
+        uint32_t *lin = (v & 1u) ? L.list_b : L.list_a;
+        uint32_t *lout = (v & 1u) ? L.list_a : L.list_b;
         uint32_t *cin = cnt + (v & 1u);
         uint32_t *cout = cnt + ((v + 1u) & 1u);
         const bool pair_v = paired && v > 0;
         if (pair_v) {
-            if (dr_status st = launch_trace_pair(s, srays, cnt + 2, socc, sroot, rays, lin, cin, hits, proot, cap,
-                                                 j.sched, stream))
+            if (dr_status st = launch_trace_pair(s, L.srays, cnt + 2, L.socc, L.sroot, rays, lin, cin, hits, proot,
+                                                 cap, j.sched, stream, L.ovf))
                 return st;
         } else if (dr_status st = launch_trace_ex(s, rays, lin, cap, cin, hits, nullptr, nullptr, nullptr, false,
-                                                  j.sched, stream, proot)) {
+                                                  j.sched, stream, proot, L.ovf)) {
             return st;
         }
         const bool guided_v = use_guide && v < j.guide->guided_vertices;
@@ -604,13 +618,13 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
                 return st;
         gv.head = guided_v ? s->rs_guide_head.as<float>() : nullptr;
         if (pair_v) {
-            k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, socc, scon, rad);
+            k_shadow_apply<<<blocks(cap), kT, 0, stream>>>(cnt + 2, L.socc, L.scon, rad);
             DR_HIP_TRY(hipGetLastError());
         }
         k_reset_counts<<<1, 1, 0, stream>>>(cout, cnt + 2, cin, s->ray_casts.as<unsigned long long>());
         const auto shade = sv.inst ? k_shade<false, true> : (gv.head || gv.smp) ? k_shade<true> : k_shade<false>;
-        shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, srays, scon,
-                                              cnt + 2, gv, pview, sroot);
+        shade<<<blocks(cap), kT, 0, stream>>>(sv, pp, v, lin, cin, lout, cout, rays, hits, beta, rad, L.srays,
+                                              L.scon, cnt + 2, gv, pview, L.sroot);
         DR_HIP_TRY(hipGetLastError());
         if (s->n_emit && !paired)
             if (dr_status st = trace_shadows()) return st;
@@ -618,6 +632,58 @@ dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &s
     // the last vertex's shadow rays
     if (paired && j.prm->max_depth > 0)
         if (dr_status st = trace_shadows()) return st;
+    return DR_OK;
+}
+
+// One chunk: camera rays, each lane's bounce loop on its own stream (lanes
+// 1.. start after the camera rays and end before the resolve, through
+// events), then the resolve. Every argument is fixed for the whole call
+// (chunk position and queue counts live in device memory), so with one lane
+// the same launches can be recorded once and replayed for every chunk.
+dr_status run_chunk(const RenderJob &j, const PathParams &pp, const SceneView &sv, const GuideView &gv,
+                    hipStream_t stream, uint32_t n_lanes)
+{
+    Scene *s = j.s;
+    const uint32_t cap = j.cap;
+    const bool collect = (j.guide_flags & DR_GUIDE_COLLECT) != 0;
+    const uint32_t lane_cap = (cap + n_lanes - 1u) / n_lanes;
+    LaneView lv[kMaxRenderLanes];
+    CamLanes cl{};
+    cl.n_lanes = n_lanes;
+    for (uint32_t k = 0; k < n_lanes; ++k) {
+        LaneView &L = lv[k];
+        if (k == 0) {
+            L = LaneView{s->rs_list_a.as<uint32_t>(), s->rs_list_b.as<uint32_t>(), s->rs_counters.as<uint32_t>(),
+                         s->rs_shadow_rays.as<dr_ray>(), s->rs_shadow_occ.as<uint32_t>(),
+                         s->rs_shadow_contrib.as<float4>(), j.scenes ? s->rs_sroot.as<uint32_t>() : nullptr,
+                         nullptr, stream};
+        } else {
+            RenderLane &R = s->lanes[k];
+            L = LaneView{R.list_a.as<uint32_t>(), R.list_b.as<uint32_t>(), R.counters.as<uint32_t>(),
+                         R.srays.as<dr_ray>(), R.socc.as<uint32_t>(), R.scon.as<float4>(),
+                         j.scenes ? R.sroot.as<uint32_t>() : nullptr, &R.ovf, R.stream};
+        }
+        cl.list[k] = L.list_a;
+        cl.counters[k] = L.cnt;
+    }
+    auto *rad = s->rs_radiance.as<float4>();
+    auto *pview = s->rs_view.as<uint32_t>();
+    k_next_chunk<<<1, 64, 0, stream>>>(s->rs_chunk.as<ChunkState>());
+    DR_HIP_TRY(hipGetLastError());
+    k_camera<<<blocks(cap), kT, 0, stream>>>(pp, s->rs_rays.as<dr_ray>(), s->rs_beta.as<float4>(), rad, cl,
+                                             collect ? gv.pending : nullptr, pview,
+                                             j.scenes ? s->rs_root.as<uint32_t>() : nullptr);
+    DR_HIP_TRY(hipGetLastError());
+    if (n_lanes > 1) {
+        DR_HIP_TRY(hipEventRecord(s->lane_fork, stream));
+        for (uint32_t k = 1; k < n_lanes; ++k) DR_HIP_TRY(hipStreamWaitEvent(lv[k].stream, s->lane_fork, 0));
+    }
+    for (uint32_t k = 0; k < n_lanes; ++k)
+        if (dr_status st = bounce_loop(j, pp, sv, gv, lv[k], lane_cap)) return st;
+    for (uint32_t k = 1; k < n_lanes; ++k) {
+        DR_HIP_TRY(hipEventRecord(s->lanes[k].done, lv[k].stream));
+        DR_HIP_TRY(hipStreamWaitEvent(stream, s->lanes[k].done, 0));
+    }
     if (collect)
         if (dr_status st = guide_finalize(j.guide, rad, stream)) return st;
     k_resolve<<<blocks(cap), kT, 0, stream>>>(pp, rad, pview);
@@ -765,7 +831,8 @@ __global__ __launch_bounds__(kPersistThreads) void k_render_persistent(PersistAr
         const uint32_t n = load_agent(&a.chunk->n);
         if (n == 0u) return;  // every block reads the same count
         for (uint32_t p = gtid; p < n; p += nthr)
-            camera_item(p, a.pp, a.rays, a.beta, a.rad, a.list_a, a.cnt, nullptr, a.pview, a.proot);
+            camera_item(p, a.pp, a.rays, a.beta, a.rad, CamLanes{{a.list_a}, {a.cnt}, 1u}, nullptr, a.pview,
+                        a.proot);
         if (!grid_barrier(a.grid)) return;
         // Vertex v's stage traces the shadow rays of vertex v - 1 and the
         // extension rays of vertex v (each block drains the shadow queue's
@@ -959,6 +1026,30 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         if (st) return st;
         s->rs_capacity = cap;
     }
+    // render lanes: one for a recorded bounce loop, else the scene's setting
+    // (never more lanes than 2^16 paths each)
+    const uint64_t n_chunks_all = (j.total + cap - 1) / cap;
+    uint32_t n_lanes = (j.graph && n_chunks_all > 1) ? 1u : std::max(1u, std::min(s->render_lanes, kMaxRenderLanes));
+    while (n_lanes > 1 && cap / n_lanes < 65536u) --n_lanes;  // (2^17 paths for two lanes)
+    if (n_lanes > 1) {
+        const uint32_t lane_cap = (cap + n_lanes - 1u) / n_lanes;
+        if (!s->lane_fork) DR_HIP_TRY(hipEventCreateWithFlags(&s->lane_fork, hipEventDisableTiming));
+        for (uint32_t k = 1; k < n_lanes; ++k) {
+            RenderLane &R = s->lanes[k];
+            if (!R.stream) DR_HIP_TRY(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+            if (!R.done) DR_HIP_TRY(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
+            dr_status st = DR_OK;
+            auto need = [&](DevBuf &b, size_t bytes) { if (st == DR_OK) st = b.ensure(bytes); };
+            need(R.list_a, 4ull * lane_cap);
+            need(R.list_b, 4ull * lane_cap);
+            need(R.counters, 64);
+            need(R.srays, sizeof(dr_ray) * (size_t)lane_cap);
+            need(R.socc, 4ull * lane_cap);
+            need(R.scon, sizeof(float4) * (size_t)lane_cap);
+            if (j.scenes) need(R.sroot, 4ull * lane_cap);
+            if (st) return st;
+        }
+    }
     if (dr_status st = count_ray_casts(s, 0, stream)) return st;
     if (j.scenes) {
         if (dr_status st = s->rs_root.ensure(4ull * s->rs_capacity)) return st;
@@ -1012,13 +1103,13 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
         if (dr_status st = run_persistent(j, pp, sv, stream)) return st;
     } else if (!j.graph || n_chunks == 1) {
         for (uint64_t c = 0; c < n_chunks; ++c)
-            if (dr_status st = run_chunk(j, pp, sv, gv, stream)) return st;
+            if (dr_status st = run_chunk(j, pp, sv, gv, stream, n_lanes)) return st;
     } else {
         // the first chunk runs eagerly (it sizes every buffer and caches the
         // launch shapes); the loop is then recorded once and replayed
-        if (dr_status st = run_chunk(j, pp, sv, gv, stream)) return st;
+        if (dr_status st = run_chunk(j, pp, sv, gv, stream, 1u)) return st;
         DR_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        dr_status cst = run_chunk(j, pp, sv, gv, stream);
+        dr_status cst = run_chunk(j, pp, sv, gv, stream, 1u);
         hipGraph_t graph = nullptr;
         const hipError_t ce = hipStreamEndCapture(stream, &graph);
         if (cst) {
@@ -1044,6 +1135,25 @@ dr_status render_job(const RenderJob &j, hipStream_t stream)
     if (collect)
         if (dr_status st = guide_compact(j.guide, gv.item_bits + gv.vertex_bits, stream)) return st;
     return DR_OK;
+}
+
+void Scene::release_lanes()
+{
+    for (RenderLane &R : lanes) {
+        if (R.stream) {
+            (void)hipStreamSynchronize(R.stream);
+            (void)hipStreamDestroy(R.stream);
+            R.stream = nullptr;
+        }
+        if (R.done) {
+            (void)hipEventDestroy(R.done);
+            R.done = nullptr;
+        }
+    }
+    if (lane_fork) {
+        (void)hipEventDestroy(lane_fork);
+        lane_fork = nullptr;
+    }
 }
 
 void Scene::release_graph()

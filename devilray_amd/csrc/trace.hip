@@ -231,7 +231,7 @@ dr_status launch_trace(Scene *s, const dr_ray *rays, uint32_t n, dr_hit *hits, u
 dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list, uint32_t n_max,
                           const uint32_t *n_dev, dr_hit *hits, uint32_t *occluded, uint32_t *box_tests,
                           uint32_t *tri_tests, bool any_hit, uint32_t sched, hipStream_t stream,
-                          const uint32_t *ray_root)
+                          const uint32_t *ray_root, DevBuf *ovf_buf)
 {
     const uint32_t n = n_max;
     if (!s->built) return fail(DR_E_INVALID, "scene BVH not built (call dr_scene_build_bvh)");
@@ -268,7 +268,8 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     const uint32_t resident = (uint32_t)(s->ctx->cu_count * bpc);
     const uint32_t grid = persistent ? (need < resident ? need : resident) : need;
     const size_t nlanes = (size_t)grid * kBlock;
-    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * (size_t)(max_stack - kLdsStack) * nlanes + 4)) return st;
+    DevBuf &spill = ovf_buf ? *ovf_buf : s->ovf_stack;
+    if (dr_status st = spill.ensure(sizeof(int32_t) * (size_t)(max_stack - kLdsStack) * nlanes + 4)) return st;
     const auto *r4 = reinterpret_cast<const float4 *>(rays);
     const uint32_t *perm = ray_list;
     if (!ray_list && !n_dev && !ray_root && (sched & DR_SCHED_REORDER) && n >= 4096) {
@@ -293,7 +294,7 @@ dr_status launch_trace_ex(Scene *s, const dr_ray *rays, const uint32_t *ray_list
     const uint32_t slices = (persistent && (sched & DR_SCHED_XCD_SLICES) && grid >= 8u) ? 8u : 1u;
     if (count)
         for (int k = 2; k <= 6; ++k) k_zero_counter<<<1, 1, 0, stream>>>(flags + k);
-    int32_t *ovf = s->ovf_stack.as<int32_t>();
+    int32_t *ovf = spill.as<int32_t>();
     kern<<<grid, kBlock, 0, stream>>>(s->nodes.as<QNode>(), s->tri_v.as<float4>(), r4, perm, n, n_dev, hits, occluded,
                                       box_tests, tri_tests, ovf, flags, counter, s->refill_min, ray_root, slices,
                                       max_stack, inst ? s->inst_recs.as<InstRec>() : nullptr);
@@ -310,7 +311,7 @@ bool trace_pair_fits(const Scene *s, uint32_t sched)
 dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_dev, uint32_t *occluded,
                             const uint32_t *roots_a, const dr_ray *rays_c, const uint32_t *list_c,
                             const uint32_t *nc_dev, dr_hit *hits, const uint32_t *roots_c, uint32_t n_max,
-                            uint32_t sched, hipStream_t stream)
+                            uint32_t sched, hipStream_t stream, DevBuf *ovf_buf)
 {
     if (!trace_pair_fits(s, sched)) return fail(DR_E_INVALID, "paired trace launch: needs a per-ray schedule");
     if (n_max == 0) return DR_OK;
@@ -321,8 +322,9 @@ dr_status launch_trace_pair(Scene *s, const dr_ray *rays_a, const uint32_t *na_d
     const uint32_t max_stack = std::max<uint32_t>((uint32_t)kLdsStack, 3u * s->depth + 1u);
     const uint32_t need = (n_max + kBlock - 1) / kBlock;
     const size_t part = (size_t)(max_stack - kLdsStack) * need * kBlock;  // spill entries of one part
-    if (dr_status st = s->ovf_stack.ensure(sizeof(int32_t) * 2 * part + 4)) return st;
-    int32_t *ovf = s->ovf_stack.as<int32_t>();
+    DevBuf &spill = ovf_buf ? *ovf_buf : s->ovf_stack;
+    if (dr_status st = spill.ensure(sizeof(int32_t) * 2 * part + 4)) return st;
+    int32_t *ovf = spill.as<int32_t>();
     // instances run the speculative loop (as launch_trace_ex promotes them)
     const auto kern = inst ? k_trace_pair<true, true> : (sched & 1u) ? k_trace_pair<true> : k_trace_pair<false>;
     kern<<<2 * need, kBlock, 0, stream>>>(s->nodes.as<QNode>(), s->tri_v.as<float4>(),

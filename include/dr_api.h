@@ -213,6 +213,15 @@ dr_status dr_scene_set_trace_schedule(dr_scene *scene, uint32_t closest_sched, u
  * key (0: dominant axis + origin Morton code, 1: dominant axis + quantised
  * direction + origin). */
 dr_status dr_scene_set_trace_tuning(dr_scene *scene, uint32_t refill_min, uint32_t blocks_per_cu, uint32_t sort_key);
+/* Render lanes (1..4, default 4): dr_render and dr_batch_render deal each
+ * chunk's paths to this many lanes, each running its bounce loop on its own
+ * stream (the caller's and streams the scene owns), joined before the
+ * resolve, so one lane's launch tails are filled by another's work. Images
+ * do not depend on it. A recorded bounce loop (graph replay) uses one lane,
+ * and so does a chunk of fewer than 2^17 paths. No counterpart in the
+ * reference (its megakernel renders a frame in one launch,
+ * src/device/RendererImpl.hpp:49-102). */
+dr_status dr_scene_set_render_lanes(dr_scene *scene, uint32_t lanes);
 
 /* Closest hit with the reference's HitTests counters
  * (include/tracing/Benchmark.hpp:5-14, IntersectionTestsImpl.hpp:228-231):
@@ -347,7 +356,7 @@ typedef struct {
 #define DR_BATCH_LAUNCH_SEQUENCE 0u
 #define DR_BATCH_LAUNCH_PERSISTENT 1u
 typedef struct {
-    uint32_t chunk_paths;   /* paths in flight per chunk (4096 .. 1 << 23) */
+    uint32_t chunk_paths;   /* paths in flight per chunk (4096 .. 1 << 24) */
     uint32_t trace_sched;   /* DR_SCHED_* for the extension and shadow traces */
     uint32_t refill_min;    /* persistent traversal refill threshold (1..64) */
     uint32_t use_graph;     /* sequence: record the bounce loop once, replay it per chunk */
@@ -370,7 +379,7 @@ dr_status dr_batch_render(dr_batch *batch, const dr_batch_view *views, uint32_t 
  * budget_spp into a scratch accumulation buffer (the views' own images are
  * not touched), reps + 1 times per candidate (the first untimed), and writes
  * the candidate of least median time to *best. Candidates: the launch
- * sequence over chunk_paths 2^20 .. 2^23, the per-ray and the resident-grid
+ * sequence over chunk_paths 2^21 .. 2^24, the per-ray and the resident-grid
  * speculative schedules (refill_min 8, 16, 32 for the latter) and graph
  * recording off and on; the persistent launch over chunk_paths 2^22 and
  * 2^23, the resident-grid speculative schedule with one ray counter and with

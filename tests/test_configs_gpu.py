@@ -94,6 +94,12 @@ def test_batch_of_64_uneven_scenes(gpu_ctx):
     a_bat = bat.to_numpy(np.float64, 4 * total)
     print(f"  {len(work)} scenes, {sum(sd.indices.shape[0] for sd, *_ in work)} tris, {total} pixels", flush=True)
     assert np.array_equal(a_sep, a_bat), "batched views differ from their single-scene renders"
+    # the same batch in 2^24-path chunks dealt to four render lanes (streams)
+    print("  render lanes", flush=True)
+    bat.copy_from(np.zeros(4 * total))
+    b.render_dev(views, max_depth=12, rr_depth=3, seed=0, chunk_paths=1 << 24, trace_sched=1, use_graph=False)
+    gpu_ctx.synchronize()
+    assert np.array_equal(a_sep, bat.to_numpy(np.float64, 4 * total)), "render lanes differ"
     # the same batch in one persistent launch
     print("  one persistent launch", flush=True)
     bat.copy_from(np.zeros(4 * total))
