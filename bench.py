@@ -42,8 +42,8 @@ triangles) rendered through dr_instances_render against its flattened copy.
 --probe: runs only the traversal launches and a 256 MiB device copy (the
 known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE); for rocprofv3.
 --probe-guide: ten guide training steps on 2^18 samples; for rocprofv3.
---probe-render: three passes of the many-lights render and the calibration
-copy; for rocprofv3.
+--probe-render: two 8-spp passes of the many-lights render and the
+calibration copy; for rocprofv3.
 --traffic-from FETCH_DIR WRITE_DIR: writes the traffic fixture from the
 counter CSVs of two rocprofv3 runs of --probe (FETCH_SIZE, WRITE_SIZE).
 """
@@ -606,9 +606,10 @@ def probe() -> int:
 
 
 def probe_render() -> int:
-    """Three 2-spp passes of the 2,048,000-triangle many-lights scene at
-    1920x1080 (max depth 12, roulette from 3) and the 256 MiB calibration
-    copy, for rocprofv3 traffic runs on the render kernels."""
+    """Two 8-spp passes of the 2,048,000-triangle many-lights scene at
+    1920x1080 (max depth 12, roulette from 3; each pass one chunk of 16.6M
+    paths in four render lanes, as the bench's render runs) and the 256 MiB
+    calibration copy, for rocprofv3 traffic runs on the render kernels."""
     sc = scenes.many_lights()
     ctx = dr.Context(0)
     s = dr.Scene(ctx, sc.vertices, sc.indices, sc.tri_material, sc.dr_materials())
@@ -617,8 +618,8 @@ def probe_render() -> int:
     cam = dr.camera_lookat(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"])
     acc = ctx.alloc(c["width"] * c["height"] * 32)
     ctx.fill_zero(acc)
-    for k in range(3):
-        s.render_dev(cam, dr.DrRenderParams(2, 2 * k, 12, 3, 0, 0, 0), acc.ptr)
+    for k in range(2):
+        s.render_dev(cam, dr.DrRenderParams(8, 8 * k, 12, 3, 0, 0, 0), acc.ptr)
     ctx.synchronize()
     a, b = ctx.alloc(256 << 20), ctx.alloc(256 << 20)
     dr.check(dr.lib().dr_memcpy_dtod(ctx.handle, dr.C.c_void_p(b.ptr), dr.C.c_void_p(a.ptr), 256 << 20, None))
