@@ -134,3 +134,25 @@ def test_instances_render_arguments(gpu_ctx, rscenes):
     assert np.all(img[..., :3] == 0) and np.all(img[..., 3] == 2)
     assert I.lights()[0].shape[0] == 0
     I.close()
+
+
+def test_exact_instances_render_in_lanes_equals_flattened(gpu_ctx, rscenes):
+    """512 x 384 x 2 spp (393,216 paths, dealt to four render lanes on four
+    streams): the instanced image still equals the flattened copy's bit for
+    bit, and the flattened copy's equals its own one-lane render."""
+    sc, built = rscenes
+    inst = ic.render_instances_exact()
+    print("[case] instanced render in four lanes, 512x384x2", flush=True)
+    I = dr.Instances(gpu_ctx, built, inst)
+    sd = ic.flatten_render(sc, inst)
+    flat = _flat_scene(gpu_ctx, sd)
+    cam = _camera(512, 384)
+    got = I.render(cam, spp=2, max_depth=8, rr_depth=3, seed=5)
+    ref = flat.render(cam, spp=2, max_depth=8, rr_depth=3, seed=5)
+    flat.set_render_lanes(1)
+    one = flat.render(cam, spp=2, max_depth=8, rr_depth=3, seed=5)
+    assert got[..., :3].sum() > 0
+    assert np.array_equal(ref.view(np.uint64), one.view(np.uint64))
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    flat.close()
+    I.close()
