@@ -462,6 +462,20 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
                                 max_depth, train_passes, lr_schedule, guide_after))
         g.close()
     guided = avg(runs)
+    # the plain path at its fastest: the same spp_per_pass x passes samples in
+    # one call (one chunk of paths in its render lanes, no per-pass
+    # launches); the image, so its MSE, is the same bit for bit
+    from devilray_amd.guiding import ProgressiveRenderer
+    best = []
+    for k in range(min(seeds, 8)):
+        r = ProgressiveRenderer(s, cam, None, spp_per_pass=spp_per_pass * passes, max_depth=max_depth, seed=900 + k)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r.step()
+        ctx.synchronize()
+        best.append(time.perf_counter() - t0)
+        r.close()
+    tbest = float(np.median(best))
     s.close()
     target = plain[-1][1]
     tp, tg = plain[-1][0], time_for(guided, target)
@@ -483,6 +497,8 @@ def guided_vs_plain(ctx, sd, width, height, ref_spp, spp_per_pass, passes, check
             "mean_equal_spp_ratio_x_cost": round(float(np.mean(eq[1:])) * cost, 3) if n_ck > 1 else None,
             "target_mse": target, "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1),
             "guided_over_plain_time": round(tg / tp, 3),
+            "plain_one_call_ms": round(tbest * 1e3, 1),
+            "guided_over_plain_one_call_time": round(tg / tbest, 3),
             "mse_plain_curve": [[round(t * 1e3, 1), m] for t, m in plain],
             "mse_guided_curve": [[round(t * 1e3, 1), m] for t, m in guided]}
 
