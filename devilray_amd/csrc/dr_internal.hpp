@@ -264,7 +264,7 @@ struct Guide {
     DevBuf m1, m2;   // Adam moments
     DevBuf frag_f;   // BF16 weight fragments for the forward products
     DevBuf frag_b;   // BF16 transposed fragments for the backward products
-    DevBuf partials; // kGuideTrainBlocks x (N_PARAMS + 1) FP32
+    DevBuf partials; // kGuideTrainBlocks rows of N_PARAMS + 1 FP32, padded to 64
     DevBuf samples;  // staged samples of the last collecting render: dr_guide_sample x stage_cap
     DevBuf aux;      // GuideAux x stage_cap
     DevBuf ring;     // training ring: the latest `capacity` samples of non-zero weight

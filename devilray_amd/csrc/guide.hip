@@ -264,6 +264,49 @@ struct HeadLane {
     float mu[2][3], r[2], kr[2], kap[2], pi[2], sig;
 };
 
+// Combining across the four 16-lane rows with the row-swap instructions
+// (VALU, no LDS round trip): permlane16_swap(x, x) yields x of the even and
+// of the odd row of each row pair, permlane32_swap(x, x) x of the lower and
+// of the upper half wave (checked with an in-bounds probe on the GPU).
+// Sums and maxima are symmetric, so every lane gets what a butterfly of
+// shuffles gives it.
+__device__ __forceinline__ void row_pair16(float x, float &a, float &b)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    a = __builtin_bit_cast(float, (unsigned)p[0]);
+    b = __builtin_bit_cast(float, (unsigned)p[1]);
+}
+__device__ __forceinline__ void row_pair32(float x, float &a, float &b)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    a = __builtin_bit_cast(float, (unsigned)p[0]);
+    b = __builtin_bit_cast(float, (unsigned)p[1]);
+}
+__device__ __forceinline__ float rows_sum(float x)
+{
+    float a, b;
+    row_pair16(x, a, b);
+    row_pair32(a + b, a, b);
+    return a + b;
+}
+__device__ __forceinline__ float rows_max(float x)
+{
+    float a, b;
+    row_pair16(x, a, b);
+    row_pair32(fmaxf(a, b), a, b);
+    return fmaxf(a, b);
+}
+// the value of lane (lane & 15), row 0's, in every row
+__device__ __forceinline__ float row0_bcast(float x)
+{
+    float a, b;
+    row_pair16(x, a, b);
+    row_pair32(a, a, b);
+    return a;
+}
+
 __device__ __forceinline__ void head_decode(const f32x4 (&y)[3], int lane, HeadLane &h)
 {
     float yl[12];
@@ -286,18 +329,14 @@ __device__ __forceinline__ void head_decode(const f32x4 (&y)[3], int lane, HeadL
         h.kap[t] = fexp(fminf(fmaxf(h.kr[t], kLogKappaLo), kLogKappaHi));
         w[t] = yl[q0 + 4];
     }
-    float m = fmaxf(w[0], w[1]);
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
+    const float m = rows_max(fmaxf(w[0], w[1]));
     const float e0 = fexp(w[0] - m), e1 = fexp(w[1] - m);
-    float sum = e0 + e1;
-    sum += __shfl_xor(sum, 16);
-    sum += __shfl_xor(sum, 32);
+    const float sum = rows_sum(e0 + e1);
     const float isum = __builtin_amdgcn_rcpf(sum);
     h.pi[0] = e0 * isum;
     h.pi[1] = e1 * isum;
     const float sl = __builtin_amdgcn_rcpf(1.0f + fexp(-yl[10]));
-    h.sig = __shfl(sl, lane & 15);
+    h.sig = row0_bcast(sl);
 }
 
 struct PathIn {
@@ -465,12 +504,44 @@ constexpr int kTrainWaves = 8;
 constexpr int kTrainThreads = 64 * kTrainWaves;
 constexpr int kGroups = 2;                             // 16-sample groups per wave
 constexpr int kTileCols = 16 * kGroups * kTrainWaves;  // samples per tile
-constexpr int kTilePitch = kTileCols + 8;              // BF16 per tile row: a 16-byte pad shifts each row's banks
-constexpr int kTileElems = 64 * kTilePitch;            // BF16 per X or D tile
+constexpr int kTileElems = kTileCols * 64;             // BF16 per X or D tile: one 128-byte row per sample
 constexpr int kTileKs = kTileCols / 32;                // K steps of a dW product
-// element (row, sample) of a tile; every access is one lane base plus an
-// immediate offset
-__device__ __forceinline__ int tile_idx(int row, int sample) { return row * kTilePitch + sample; }
+
+// Tile layout. Row r holds sample r's 64 features as 16 chunks of 8 bytes
+// (4 features); chunk c sits at position c ^ chunk_swz(r). A lane writes its
+// own sample's values one chunk per store, and the dW products read them back
+// with ds_read_b64_tr_b16, which transposes within each 16-lane group: lane k
+// receives element k % 4 of the chunks addressed by lanes k / 4 + 4 j,
+// j = 0..3 (checked on the GPU with an in-bounds probe). So when lane m
+// addresses row 4 g + m / 4 at chunk C(m % 4), lane k of group g gets one
+// feature of 4 consecutive samples, an operand half of a K = samples product.
+// Features to chunks: a hidden layer's feature 16 b + 4 g + e (the lane
+// layout of the B operands) is element e of chunk 4 g + b; the input's
+// feature 8 g + 4 h + e is element e of chunk 4 g + h. The XOR patterns give
+// each half wave 64 distinct banks both when it writes (16 rows x 2 chunks)
+// and when it reads (8 rows x 4 chunks).
+__device__ __forceinline__ int chunk_swz(int row) { return (int)((0xA381B290u >> (4 * ((row >> 1) & 7))) & 15u); }
+__device__ __forceinline__ int chunk_off(int row, int c) { return row * 64 + ((c ^ chunk_swz(row)) << 2); }
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+__device__ __forceinline__ void put_chunk(__bf16 *t, int row, int c, unsigned long long v)
+{
+    *reinterpret_cast<unsigned long long *>(t + chunk_off(row, c)) = v;
+}
+
+// The operand of one K step (32 samples) at the lane's address p: elements
+// 0..3 are samples 4 g .. 4 g + 3 of the step, elements 4..7 samples
+// 16 + 4 g .. 16 + 4 g + 3 (A and B operands use the same order, so each
+// product sums over every sample of the step once).
+__device__ __forceinline__ bf16x8 tr_operand(const __bf16 *p)
+{
+    const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)p);
+    const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(p + 16 * 64));
+    return __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
 
 // dW tiles of a wave: out block ob = wave % 4 (layer 4 has three, so waves 3
 // and 7 take none there); in blocks 0 .. IB/2 - 1 plus the bias tile for the
@@ -482,14 +553,32 @@ __host__ __device__ constexpr int acc_base(int l) { return l == 4 ? 0 : acc_base
 constexpr int kAcc = acc_base(0) + n_slots(0);
 static_assert(kAcc == 14, "accumulator tiles per wave");
 
+// Row length of the per-block partial gradients: kNP + 1 floats padded to a
+// multiple of 64, so a wave's 64 consecutive floats of one row are one
+// aligned 256-byte segment when the Adam kernel sums them.
+constexpr int kPartStride = (kNP + 1 + 63) / 64 * 64;
+
 struct TrainIn {
     const dr_guide_sample *smp;
     const uint32_t *count;  // device count (NULL: n)
     uint32_t n, cap;
     uint32_t batch, window;  // batch > 0: samples (window * batch + i) mod count, i < batch
-    float *partials;        // gridDim.x x (kNP + 1)
+    float *partials;        // gridDim.x rows of kPartStride floats: kNP + 1 used
     uint32_t sel_variance;  // selection logit: 0 = KL objective, 1 = second-moment objective
 };
+
+// The training sample in slot i of the batch: sample (start + i) mod avail
+// of the ring, or past the batch's end a zero-weight placeholder.
+__device__ __forceinline__ dr_guide_sample train_sample(const dr_guide_sample *smp, uint32_t i, uint32_t n,
+                                                        uint32_t start, uint32_t avail)
+{
+    if (i < n) return smp[start + i < avail ? start + i : start + i - avail];
+    dr_guide_sample z = {};
+    z.normal[2] = 1.0f;
+    z.wo[2] = 1.0f;
+    z.wi[2] = 1.0f;
+    return z;
+}
 
 // The wave's share of dW_L += D_L X_L^T over the tile's samples (K).
 template <int L>
@@ -498,16 +587,22 @@ __device__ __forceinline__ void dw_layer(f32x4 (&acc)[kAcc], const __bf16 *Dt, c
 {
     constexpr int NI = n_in_half(L);
     if (ob >= layer_ob(L)) return;  // wave-uniform
-    const int i = lane & 15, g = lane >> 4;
-    const __bf16 *dp = Dt + tile_idx(ob * 16 + i, 8 * g);
-    const __bf16 *xp = Xt + tile_idx(half * NI * 16 + i, 8 * g);
+    const int m = lane & 15, g = lane >> 4;
+    const int lrow = 4 * g + (m >> 2), sw = chunk_swz(lrow);  // the swizzle of rows lrow + 16 k
+    const __bf16 *dp = Dt + lrow * 64 + (((4 * (m & 3) + ob) ^ sw) << 2);
+    const __bf16 *xp[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int blk = half * NI + u;  // in features 16 blk .. 16 blk + 15
+        const int c = L == 0 ? 8 * blk + 4 * ((m & 3) >> 1) + (m & 1) : 4 * (m & 3) + blk;
+        xp[u] = Xt + lrow * 64 + ((c ^ sw) << 2);
+    }
 #pragma unroll
     for (int ks = 0; ks < kTileKs; ++ks) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dp + 32 * ks);
+        const bf16x8 a = tr_operand(dp + 32 * 64 * ks);
 #pragma unroll
         for (int u = 0; u < NI; ++u)
-            acc[acc_base(L) + u] = mfma(a, *reinterpret_cast<const bf16x8 *>(xp + u * 16 * kTilePitch + 32 * ks),
-                                        acc[acc_base(L) + u]);
+            acc[acc_base(L) + u] = mfma(a, tr_operand(xp[u] + 32 * 64 * ks), acc[acc_base(L) + u]);
         if (half == 0) acc[acc_base(L) + NI] = mfma(a, ones, acc[acc_base(L) + NI]);
     }
 }
@@ -518,6 +613,10 @@ __device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int
     constexpr int NI = n_in_half(L);
     constexpr int kW = w_off(L), kB = b_off(L);
     if (ob >= layer_ob(L)) return;
+    // the empty asm makes the lane index opaque here, so the address
+    // arithmetic happens at the stores and is not hoisted out of the tile
+    // loop (where its registers would be live throughout)
+    asm volatile("" : "+v"(lane));
     const int j = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -529,13 +628,16 @@ __device__ __forceinline__ void dw_store(const f32x4 (&acc)[kAcc], float *P, int
 }
 
 // A hidden layer's values in B-operand order (element j of operand ks is
-// feature (2 ks + j / 4) * 16 + 4 g + j % 4 of the lane's sample) into a tile.
-__device__ __forceinline__ void store_hidden(__bf16 *t, const bf16x8 (&v)[2], int g, int col)
+// feature (2 ks + j / 4) * 16 + 4 g + j % 4 of the lane's sample) into the
+// tile row of that sample: four chunk stores.
+__device__ __forceinline__ void store_hidden(__bf16 *t, const bf16x8 (&v)[2], int g, int row)
 {
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
+    for (int ks = 0; ks < 2; ++ks) {
+        const u64x2 w = __builtin_bit_cast(u64x2, v[ks]);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) t[tile_idx((2 * ks + j / 4) * 16 + 4 * g + j % 4, col)] = v[ks][j];
+        for (int h = 0; h < 2; ++h) put_chunk(t, row, 4 * g + 2 * ks + h, w[h]);
+    }
 }
 
 // Zero the BF16 elements of d whose ReLU activation x is not positive. x
@@ -607,9 +709,7 @@ __device__ __forceinline__ void train_head(const f32x4 (&y)[3], int lane, V3 nn,
         em[t] = fexp(2.0f * k) - 1.0f;
         f[t] = fdiv(k * fexp(k * (c[t] - 1.0f)), kTwoPi * (1.0f - fexp(-2.0f * k)));
     }
-    float q = h.pi[0] * f[0] + h.pi[1] * f[1];
-    q += __shfl_xor(q, 16);
-    q += __shfl_xor(q, 32);
+    const float q = rows_sum(h.pi[0] * f[0] + h.pi[1] * f[1]);
     const float cb = dot(nn, wi);
     const float b = cb > 0.0f ? cb * (1.0f / kPi) : 0.0f;
     // lobes: KL of the fixed half-and-half proposal; selection: KL of the
@@ -655,8 +755,33 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     __shared__ __attribute__((aligned(16))) __bf16 Dt[kTileElems];
     __shared__ float Bs[5][64];
     __shared__ float red[kTrainWaves];
-    for (int k = threadIdx.x; k < kFragF * 64; k += kTrainThreads) Wf[k] = gnet.ff0[k];
-    for (int k = threadIdx.x; k < kFragB * 64; k += kTrainThreads) Wb[k] = gnet.fb[k];
+    {
+        // every load is issued before the first store, so the copy costs one
+        // memory round trip rather than one per fragment
+        constexpr int kCF = (kFragF * 64 + kTrainThreads - 1) / kTrainThreads;
+        constexpr int kCB = (kFragB * 64 + kTrainThreads - 1) / kTrainThreads;
+        bf16x8 tf[kCF], tb[kCB];
+#pragma unroll
+        for (int r = 0; r < kCF; ++r) {
+            const int k = (int)threadIdx.x + r * kTrainThreads;
+            if (k < kFragF * 64) tf[r] = gnet.ff0[k];
+        }
+#pragma unroll
+        for (int r = 0; r < kCB; ++r) {
+            const int k = (int)threadIdx.x + r * kTrainThreads;
+            if (k < kFragB * 64) tb[r] = gnet.fb[k];
+        }
+#pragma unroll
+        for (int r = 0; r < kCF; ++r) {
+            const int k = (int)threadIdx.x + r * kTrainThreads;
+            if (k < kFragF * 64) Wf[k] = tf[r];
+        }
+#pragma unroll
+        for (int r = 0; r < kCB; ++r) {
+            const int k = (int)threadIdx.x + r * kTrainThreads;
+            if (k < kFragB * 64) Wb[k] = tb[r];
+        }
+    }
     for (int k = threadIdx.x; k < 5 * 64; k += kTrainThreads) {
         const int l = k / 64, o = k % 64;
         Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
@@ -684,30 +809,32 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     asm volatile("" : "+v"(ones_w));
     const bf16x8 ones = __builtin_bit_cast(bf16x8, ones_w);
     float wsum = 0.0f;
+    int col[kGroups];
+    // the samples of the block's next tile: loaded one tile ahead, so their
+    // latency hides under the last backward phases of the tile before
+    dr_guide_sample nxt[kGroups];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+        col[q] = (wave * kGroups + q) * 16 + s;
+        nxt[q] = train_sample(in.smp, blockIdx.x * (uint32_t)kTileCols + (uint32_t)col[q], n, start, avail);
+    }
+    // the block's partial gradient; a layer's share is stored as soon as the
+    // block's last tile has added to it, so the stores overlap the remaining
+    // phases instead of all leaving at the end
+    float *P = in.partials + (size_t)blockIdx.x * kPartStride;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        int col[kGroups];
+        const bool last = tile + gridDim.x >= ntiles;  // block-uniform
         bf16x8 x0[kGroups];
         V3 nn[kGroups], wi[kGroups];
         float w[kGroups], ps[kGroups];
 #pragma unroll
         for (int q = 0; q < kGroups; ++q) {
-            col[q] = (wave * kGroups + q) * 16 + s;
-            const uint32_t i = tile * (uint32_t)kTileCols + (uint32_t)col[q];
-            V3 p{0.0f, 0.0f, 0.0f}, wo{0.0f, 0.0f, 1.0f};
-            nn[q] = V3{0.0f, 0.0f, 1.0f};
-            wi[q] = V3{0.0f, 0.0f, 1.0f};
-            w[q] = 0.0f;
-            ps[q] = 0.0f;
-            if (i < n) {
-                const uint32_t src = start + i < avail ? start + i : start + i - avail;
-                const dr_guide_sample smp = in.smp[src];
-                p = V3{smp.pos[0], smp.pos[1], smp.pos[2]};
-                nn[q] = V3{smp.normal[0], smp.normal[1], smp.normal[2]};
-                wo = V3{smp.wo[0], smp.wo[1], smp.wo[2]};
-                wi[q] = V3{smp.wi[0], smp.wi[1], smp.wi[2]};
-                w[q] = smp.weight;
-                ps[q] = smp.pdf;
-            }
+            const dr_guide_sample &smp = nxt[q];
+            const V3 p{smp.pos[0], smp.pos[1], smp.pos[2]}, wo{smp.wo[0], smp.wo[1], smp.wo[2]};
+            nn[q] = V3{smp.normal[0], smp.normal[1], smp.normal[2]};
+            wi[q] = V3{smp.wi[0], smp.wi[1], smp.wi[2]};
+            w[q] = smp.weight;
+            ps[q] = smp.pdf;
             if (g == 0) wsum += w[q];
             x0[q] = encode8(g, p, nn[q], wo, box);
         }
@@ -724,13 +851,17 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kGroups; ++q) {
+            // output deltas: slot qq is feature (qq / 4) * 16 + 4 g + qq % 4
 #pragma unroll
-            for (int qq = 0; qq < 12; ++qq)
-                Dt[tile_idx((qq / 4) * 16 + 4 * g + qq % 4, col[q])] = (__bf16)dy[q][qq];
+            for (int b = 0; b < 3; ++b)
+                put_chunk(Dt, col[q], 4 * g + b,
+                          (unsigned long long)pk_bf16(dy[q][4 * b], dy[q][4 * b + 1]) |
+                              ((unsigned long long)pk_bf16(dy[q][4 * b + 2], dy[q][4 * b + 3]) << 32));
             store_hidden(Xt, xh[q][3], g, col[q]);
         }
         __syncthreads();
         dw_layer<4>(acc, Dt, Xt, ob, half, lane, ones);
+        if (last) dw_store<4>(acc, P, ob, half, lane);
         delta_layer<4>(net, lane, dB, xh);
         __syncthreads();
 #pragma unroll
@@ -740,6 +871,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
         }
         __syncthreads();
         dw_layer<3>(acc, Dt, Xt, ob, half, lane, ones);
+        if (last) dw_store<3>(acc, P, ob, half, lane);
         delta_layer<3>(net, lane, dB, xh);
         __syncthreads();
 #pragma unroll
@@ -749,6 +881,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
         }
         __syncthreads();
         dw_layer<2>(acc, Dt, Xt, ob, half, lane, ones);
+        if (last) dw_store<2>(acc, P, ob, half, lane);
         delta_layer<2>(net, lane, dB, xh);
         __syncthreads();
 #pragma unroll
@@ -756,25 +889,33 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
             store_hidden(Dt, dB[q], g, col[q]);
             store_hidden(Xt, xh[q][0], g, col[q]);
         }
+        const uint32_t next = tile + gridDim.x;
+#pragma unroll
+        for (int q = 0; q < kGroups; ++q)
+            nxt[q] = train_sample(in.smp, next * (uint32_t)kTileCols + (uint32_t)col[q], n, start, avail);
         __syncthreads();
         dw_layer<1>(acc, Dt, Xt, ob, half, lane, ones);
+        if (last) dw_store<1>(acc, P, ob, half, lane);
         delta_layer<1>(net, lane, dB, xh);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kGroups; ++q) {
             store_hidden(Dt, dB[q], g, col[q]);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) Xt[tile_idx(8 * g + j, col[q])] = x0[q][j];
+            const u64x2 w = __builtin_bit_cast(u64x2, x0[q]);  // input features 8 g .. 8 g + 7
+            put_chunk(Xt, col[q], 4 * g, w[0]);
+            put_chunk(Xt, col[q], 4 * g + 1, w[1]);
         }
         __syncthreads();
         dw_layer<0>(acc, Dt, Xt, ob, half, lane, ones);
+        if (last) dw_store<0>(acc, P, ob, half, lane);
     }
-    float *P = in.partials + (size_t)blockIdx.x * (kNP + 1);
-    dw_store<4>(acc, P, ob, half, lane);
-    dw_store<3>(acc, P, ob, half, lane);
-    dw_store<2>(acc, P, ob, half, lane);
-    dw_store<1>(acc, P, ob, half, lane);
-    dw_store<0>(acc, P, ob, half, lane);
+    if (blockIdx.x >= ntiles) {  // no tile for this block: a zero partial
+        dw_store<4>(acc, P, ob, half, lane);
+        dw_store<3>(acc, P, ob, half, lane);
+        dw_store<2>(acc, P, ob, half, lane);
+        dw_store<1>(acc, P, ob, half, lane);
+        dw_store<0>(acc, P, ob, half, lane);
+    }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) wsum += __shfl_xor(wsum, o);
     if (lane == 0) red[wave] = wsum;
@@ -816,6 +957,7 @@ struct AdamIn {
 // loss normalised by the total sample weight, and the BF16 fragments are
 // refreshed.
 constexpr int kAdamSlices = 16;
+static_assert(kGuideTrainBlocks % kAdamSlices == 0 && kGuideTrainBlocks % 64 == 0, "partial slots per slice and lane");
 __global__ __launch_bounds__(64 * kAdamSlices) void k_guide_adam(const float *__restrict__ partials, int nblk,
                                                                  float *master, float *m1, float *m2, __bf16 *ff,
                                                                  __bf16 *fb, AdamIn ad, float *grad)
@@ -824,13 +966,38 @@ __global__ __launch_bounds__(64 * kAdamSlices) void k_guide_adam(const float *__
     __shared__ float s_sw;
     const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
+    // all of the slice's loads are issued before the sum (in block order)
+    constexpr int kPer = kGuideTrainBlocks / kAdamSlices;
+    float pv[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int b = slice + r * kAdamSlices;
+        pv[r] = (p <= kNP && b < nblk) ? partials[(size_t)b * kPartStride + p] : 0.0f;
+    }
+    // the parameter's Adam state, loaded under the partial sums
+    float m1p = 0.0f, m2p = 0.0f, wp = 0.0f;
+    if (slice == 0 && !grad && p < kNP) {
+        m1p = m1[p];
+        m2p = m2[p];
+        wp = master[p];
+    }
     float acc = 0.0f;
-    if (p <= kNP)
-        for (int b = slice; b < nblk; b += kAdamSlices) acc += partials[(size_t)b * (kNP + 1) + p];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+        if (slice + r * kAdamSlices < nblk) acc += pv[r];
     part[slice][lane] = acc;
     if (slice == 1) {  // the total sample weight (column kNP), the same in every block
+        constexpr int kPerLane = kGuideTrainBlocks / 64;
+        float wv[kPerLane];
+#pragma unroll
+        for (int r = 0; r < kPerLane; ++r) {
+            const int b = lane + 64 * r;
+            wv[r] = b < nblk ? partials[(size_t)b * kPartStride + kNP] : 0.0f;
+        }
         float w = 0.0f;
-        for (int b = lane; b < nblk; b += 64) w += partials[(size_t)b * (kNP + 1) + kNP];
+#pragma unroll
+        for (int r = 0; r < kPerLane; ++r)
+            if (lane + 64 * r < nblk) w += wv[r];
         for (int off = 32; off >= 1; off >>= 1) w += __shfl_xor(w, off, 64);
         if (lane == 0) s_sw = w;
     }
@@ -847,11 +1014,11 @@ __global__ __launch_bounds__(64 * kAdamSlices) void k_guide_adam(const float *__
     if (!(sw > 0.0f) || !isfinite(sw)) return;
     const float gr = sum / sw;
     if (!isfinite(gr)) return;
-    const float m = ad.beta1 * m1[p] + (1.0f - ad.beta1) * gr;
-    const float v = ad.beta2 * m2[p] + (1.0f - ad.beta2) * gr * gr;
+    const float m = ad.beta1 * m1p + (1.0f - ad.beta1) * gr;
+    const float v = ad.beta2 * m2p + (1.0f - ad.beta2) * gr * gr;
     m1[p] = m;
     m2[p] = v;
-    const float w = master[p] - ad.lr * (m * ad.c1) / (sqrtf(v * ad.c2) + ad.eps);
+    const float w = wp - ad.lr * (m * ad.c1) / (sqrtf(v * ad.c2) + ad.eps);
     master[p] = w;
     pack_param(p, w, ff, fb);
 }
@@ -1113,7 +1280,7 @@ dr_status dr_guide_create(dr_context *cx, const dr_guide_params *params, dr_guid
     need(g->m2, sizeof(float) * kNP);
     need(g->frag_f, 2ull * kFragElems * kFragF);
     need(g->frag_b, 2ull * kFragElems * kFragB);
-    need(g->partials, sizeof(float) * (size_t)kGuideTrainBlocks * (kNP + 1));
+    need(g->partials, sizeof(float) * (size_t)kGuideTrainBlocks * kPartStride);
     g->stage_cap = p.capacity * 4u;
     need(g->samples, sizeof(dr_guide_sample) * (size_t)g->stage_cap);
     need(g->aux, sizeof(GuideAux) * (size_t)g->stage_cap);
