@@ -151,13 +151,23 @@ __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x
 // 2^(k+1) pi u (k = 0..3; periods 1 to 1/8 of the box) with u the position
 // along axis g normalised to the box; for g = 3 the normal, the outgoing
 // direction, 1 and 0.
-__device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &box)
+// The box's lower corner and extent along the axis that lane group g
+// encodes (g < 3).
+struct BoxAxis {
+    float lo, ext;
+};
+__device__ __forceinline__ BoxAxis box_axis(const Box &box, int g)
+{
+    return BoxAxis{g == 0 ? box.lo[0] : (g == 1 ? box.lo[1] : box.lo[2]),
+                   g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2])};
+}
+
+__device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, BoxAxis ax)
 {
     bf16x8 f;
     if (g < 3) {
         const float x = g == 0 ? p.x : (g == 1 ? p.y : p.z);
-        const float lo = g == 0 ? box.lo[0] : (g == 1 ? box.lo[1] : box.lo[2]);
-        const float ext = g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2]);
+        const float lo = ax.lo, ext = ax.ext;
         float u = (x - lo) / ext;
         u = fminf(fmaxf(u, 0.0f), 1.0f);
         // sin and cos of 2 pi u, then of each doubled angle:
@@ -179,6 +189,11 @@ __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &b
         f[6] = (__bf16)1.0f; f[7] = (__bf16)0.0f;
     }
     return f;
+}
+
+__device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, const Box &box)
+{
+    return encode8(g, p, n, wo, box_axis(box, g));
 }
 
 // Forward pass of one wave's 16 samples. xh[l] = hidden activation l+1 as
@@ -378,6 +393,9 @@ __global__ __launch_bounds__(kT) void k_guide_infer(Net gnet, Box box, PathIn in
     const Net net{Wf, Wf + kFrag0 * 64, nullptr, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 15, g = lane >> 4;
+    // the lane's box axis, selected once (not a per-lane argument load per tile)
+    BoxAxis ax = box_axis(box, g < 3 ? g : 0);
+    asm volatile("" : "+v"(ax.lo), "+v"(ax.ext));
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * 64u < n; tile += gridDim.x) {
         const uint32_t base = tile * 64u + (uint32_t)wave * 16u;
         if (base >= n) continue;  // wave-uniform
@@ -405,7 +423,7 @@ __global__ __launch_bounds__(kT) void k_guide_infer(Net gnet, Box box, PathIn in
                 }
             }
         }
-        const bf16x8 x0 = encode8(g, p, nn, wo, box);
+        const bf16x8 x0 = encode8(g, p, nn, wo, ax);
         bf16x8 xh[4][2];
         f32x4 y[3];
         net_forward(net, lane, x0, xh, y);
@@ -755,6 +773,27 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     __shared__ __attribute__((aligned(16))) __bf16 Dt[kTileElems];
     __shared__ float Bs[5][64];
     __shared__ float red[kTrainWaves];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform, so the wave's dW tiles are chosen by scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // waves go to the SIMDs round-robin: out block = wave % 4, so each SIMD
+    // runs one wave of each half (the first half also takes the bias tiles)
+    const int ob = wave & 3, half = wave >> 2;
+    const int s = lane & 15, g = lane >> 4;
+    const uint32_t avail = in.count ? min(*in.count, in.cap) : in.n;
+    const uint32_t n = in.batch ? min(avail, in.batch) : avail;
+    const uint32_t start = (in.batch && avail) ? (uint32_t)(((uint64_t)in.window * in.batch) % avail) : 0u;
+    const uint32_t ntiles = (n + (kTileCols - 1)) / kTileCols;
+    int col[kGroups];
+    // the samples of the block's next tile: loaded one tile ahead, so their
+    // latency hides under the last backward phases of the tile before (the
+    // first tile's under the weight copy)
+    dr_guide_sample nxt[kGroups];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+        col[q] = (wave * kGroups + q) * 16 + s;
+        nxt[q] = train_sample(in.smp, blockIdx.x * (uint32_t)kTileCols + (uint32_t)col[q], n, start, avail);
+    }
     {
         // every load is issued before the first store, so the copy costs one
         // memory round trip rather than one per fragment
@@ -788,17 +827,6 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     }
     __syncthreads();
     const Net net{Wf, Wf + kFrag0 * 64, Wb, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
-    const int lane = threadIdx.x & 63;
-    // wave-uniform, so the wave's dW tiles are chosen by scalar branches
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    // waves go to the SIMDs round-robin: out block = wave % 4, so each SIMD
-    // runs one wave of each half (the first half also takes the bias tiles)
-    const int ob = wave & 3, half = wave >> 2;
-    const int s = lane & 15, g = lane >> 4;
-    const uint32_t avail = in.count ? min(*in.count, in.cap) : in.n;
-    const uint32_t n = in.batch ? min(avail, in.batch) : avail;
-    const uint32_t start = (in.batch && avail) ? (uint32_t)(((uint64_t)in.window * in.batch) % avail) : 0u;
-    const uint32_t ntiles = (n + (kTileCols - 1)) / kTileCols;
     f32x4 acc[kAcc];
 #pragma unroll
     for (int k = 0; k < kAcc; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -809,15 +837,10 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
     asm volatile("" : "+v"(ones_w));
     const bf16x8 ones = __builtin_bit_cast(bf16x8, ones_w);
     float wsum = 0.0f;
-    int col[kGroups];
-    // the samples of the block's next tile: loaded one tile ahead, so their
-    // latency hides under the last backward phases of the tile before
-    dr_guide_sample nxt[kGroups];
-#pragma unroll
-    for (int q = 0; q < kGroups; ++q) {
-        col[q] = (wave * kGroups + q) * 16 + s;
-        nxt[q] = train_sample(in.smp, blockIdx.x * (uint32_t)kTileCols + (uint32_t)col[q], n, start, avail);
-    }
+    // the lane's box axis, selected once: inside the loop the compiler turned
+    // the selection into a per-lane load from the kernel arguments every tile
+    BoxAxis ax = box_axis(box, g < 3 ? g : 0);
+    asm volatile("" : "+v"(ax.lo), "+v"(ax.ext));
     // the block's partial gradient; a layer's share is stored as soon as the
     // block's last tile has added to it, so the stores overlap the remaining
     // phases instead of all leaving at the end
@@ -836,7 +859,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(Net gnet, Box box
             w[q] = smp.weight;
             ps[q] = smp.pdf;
             if (g == 0) wsum += w[q];
-            x0[q] = encode8(g, p, nn[q], wo, box);
+            x0[q] = encode8(g, p, nn[q], wo, ax);
         }
         bf16x8 xh[kGroups][4][2];
         f32x4 y[kGroups][3];
