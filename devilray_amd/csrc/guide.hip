@@ -158,8 +158,12 @@ struct BoxAxis {
 };
 __device__ __forceinline__ BoxAxis box_axis(const Box &box, int g)
 {
-    return BoxAxis{g == 0 ? box.lo[0] : (g == 1 ? box.lo[1] : box.lo[2]),
-                   g == 0 ? box.ext[0] : (g == 1 ? box.ext[1] : box.ext[2])};
+    // the six values are made opaque first, so the selections stay register
+    // selects (left alone, the compiler turns them into a per-lane indexed
+    // load from a copy of the argument)
+    float l0 = box.lo[0], l1 = box.lo[1], l2 = box.lo[2], e0 = box.ext[0], e1 = box.ext[1], e2 = box.ext[2];
+    asm volatile("" : "+s"(l0), "+s"(l1), "+s"(l2), "+s"(e0), "+s"(e1), "+s"(e2));
+    return BoxAxis{g == 0 ? l0 : (g == 1 ? l1 : l2), g == 0 ? e0 : (g == 1 ? e1 : e2)};
 }
 
 __device__ __forceinline__ bf16x8 encode8(int g, V3 p, V3 n, V3 wo, BoxAxis ax)
@@ -378,51 +382,96 @@ __device__ __forceinline__ V3 vload(const float *v, uint32_t i)
 // tile read the 34 KB of weights from L2 for every 64 paths).
 constexpr uint32_t kInferBlocks = 1024;
 
+// A path's shading point for the guide: position, geometric normal facing the
+// incoming ray, and the direction back along it; valid for a hit on a
+// Lambert triangle.
+struct PathGeo {
+    uint32_t path;
+    bool valid;
+    V3 p, nn, wo;
+};
+
+__device__ __forceinline__ PathGeo path_geo(const PathIn &in, uint32_t slot, uint32_t n)
+{
+    PathGeo q{0u, slot < n, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 1.0f}, V3{0.0f, 0.0f, 1.0f}};
+    if (q.valid) {
+        q.path = in.list[slot];
+        const dr_hit h = in.hits[q.path];
+        q.valid = h.id >= 0;
+        if (q.valid) {
+            const uint32_t tri = (uint32_t)h.id;
+            q.valid = in.mats[in.tri_mat[tri]].kind == DR_MAT_LAMBERT;
+            if (q.valid) {
+                const V3 A = vload(in.verts, in.idx[3 * (size_t)tri]);
+                const V3 B = vload(in.verts, in.idx[3 * (size_t)tri + 1]);
+                const V3 C = vload(in.verts, in.idx[3 * (size_t)tri + 2]);
+                const dr_ray r = in.rays[q.path];
+                const V3 d{r.dx, r.dy, r.dz};
+                q.p = bary_point(A, B, C, h.u, h.v);
+                const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
+                q.nn = dot(ng, d) > 0.0f ? mul(ng, -1.0f) : ng;
+                q.wo = mul(d, -1.0f);
+            }
+        }
+    }
+    return q;
+}
+
 __global__ __launch_bounds__(kT) void k_guide_infer(Net gnet, Box box, PathIn in)
 {
     __shared__ bf16x8 Wf[kFragF * 64];
     __shared__ float Bs[5][64];
+    // Most launches give a block one tile, so its time is the chain of
+    // memory round trips before the network runs. The weight loads are
+    // issued first (they need no count), and the first tile's path data is
+    // gathered before the weights are stored and the block meets, so those
+    // latencies overlap.
+    constexpr int kCF = (kFragF * 64 + kT - 1) / kT;
+    constexpr int kCB = (5 * 64 + kT - 1) / kT;
+    bf16x8 tf[kCF];
+    float tb[kCB];
+#pragma unroll
+    for (int r = 0; r < kCF; ++r) {
+        const int k = (int)threadIdx.x + r * kT;
+        if (k < kFragF * 64) tf[r] = gnet.ff0[k];
+    }
+#pragma unroll
+    for (int r = 0; r < kCB; ++r) {
+        const int k = (int)threadIdx.x + r * kT;
+        const int l = k / 64, o = k % 64;
+        // the layer's bias array by selects: indexing the argument's pointer
+        // array with a per-lane value would copy it to scratch memory
+        const float *b = l == 0 ? gnet.bias[0] : l == 1 ? gnet.bias[1] : l == 2 ? gnet.bias[2]
+                       : l == 3 ? gnet.bias[3] : gnet.bias[4];
+        tb[r] = (k < 5 * 64 && o < layer_out(l)) ? b[o] : 0.0f;
+    }
     const uint32_t n = *in.count;
     if ((uint64_t)blockIdx.x * 64u >= n) return;  // block-uniform: no tile for this block
-    for (int k = threadIdx.x; k < kFragF * 64; k += kT) Wf[k] = gnet.ff0[k];
-    for (int k = threadIdx.x; k < 5 * 64; k += kT) {
-        const int l = k / 64, o = k % 64;
-        Bs[l][o] = o < layer_out(l) ? gnet.bias[l][o] : 0.0f;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & 15, g = lane >> 4;
+    const PathGeo first = path_geo(in, blockIdx.x * 64u + (uint32_t)wave * 16u + (uint32_t)s, n);
+#pragma unroll
+    for (int r = 0; r < kCF; ++r) {
+        const int k = (int)threadIdx.x + r * kT;
+        if (k < kFragF * 64) Wf[k] = tf[r];
+    }
+#pragma unroll
+    for (int r = 0; r < kCB; ++r) {
+        const int k = (int)threadIdx.x + r * kT;
+        if (k < 5 * 64) Bs[k / 64][k % 64] = tb[r];
     }
     __syncthreads();
     const Net net{Wf, Wf + kFrag0 * 64, nullptr, {Bs[0], Bs[1], Bs[2], Bs[3], Bs[4]}};
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int s = lane & 15, g = lane >> 4;
     // the lane's box axis, selected once (not a per-lane argument load per tile)
     BoxAxis ax = box_axis(box, g < 3 ? g : 0);
     asm volatile("" : "+v"(ax.lo), "+v"(ax.ext));
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * 64u < n; tile += gridDim.x) {
         const uint32_t base = tile * 64u + (uint32_t)wave * 16u;
         if (base >= n) continue;  // wave-uniform
-        const uint32_t slot = base + (uint32_t)s;
-        bool valid = slot < n;
-        uint32_t path = 0;
-        V3 p{0.0f, 0.0f, 0.0f}, nn{0.0f, 0.0f, 1.0f}, wo{0.0f, 0.0f, 1.0f};
-        if (valid) {
-            path = in.list[slot];
-            const dr_hit h = in.hits[path];
-            valid = h.id >= 0;
-            if (valid) {
-                const uint32_t tri = (uint32_t)h.id;
-                valid = in.mats[in.tri_mat[tri]].kind == DR_MAT_LAMBERT;
-                if (valid) {
-                    const V3 A = vload(in.verts, in.idx[3 * (size_t)tri]);
-                    const V3 B = vload(in.verts, in.idx[3 * (size_t)tri + 1]);
-                    const V3 C = vload(in.verts, in.idx[3 * (size_t)tri + 2]);
-                    const dr_ray r = in.rays[path];
-                    const V3 d{r.dx, r.dy, r.dz};
-                    p = bary_point(A, B, C, h.u, h.v);
-                    const V3 ng = normalize(cross(sub(B, A), sub(C, A)));
-                    nn = dot(ng, d) > 0.0f ? mul(ng, -1.0f) : ng;
-                    wo = mul(d, -1.0f);
-                }
-            }
-        }
+        const PathGeo q = tile == blockIdx.x ? first : path_geo(in, base + (uint32_t)s, n);
+        const bool valid = q.valid;
+        const uint32_t path = q.path;
+        const V3 p = q.p, nn = q.nn, wo = q.wo;
         const bf16x8 x0 = encode8(g, p, nn, wo, ax);
         bf16x8 xh[4][2];
         f32x4 y[3];
