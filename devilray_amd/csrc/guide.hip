@@ -560,8 +560,8 @@ __global__ __launch_bounds__(kT) void k_guide_eval(Net net, Box box, const dr_gu
 // takes two groups of 16 samples and applies every weight fragment it loads
 // to both: half the fragment reads per sample, and two independent chains of
 // matrix products to cover each other's latency. LDS holds all weight
-// fragments, the biases and one 64-row x 256-sample BF16 tile each of a
-// layer's input activations X and output deltas D (136,480 B). The
+// fragments, the biases and one 256-sample x 64-feature BF16 tile each of a
+// layer's input activations X and output deltas D (134,432 B). The
 // activations stay in registers from the forward pass; the backward pass
 // visits one layer per phase: the waves write X_L and D_L, meet at a barrier,
 // accumulate their share of dW_L = D_L X_L^T on the matrix cores while they
